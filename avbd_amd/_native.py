@@ -1,0 +1,316 @@
+"""ctypes binding of libavbd_hip.so (the C ABI declared in include/avbd.h).
+
+The structs below mirror ``avbd_params``, ``avbd_scene`` and ``avbd_report``
+field for field. ``NativeHandle`` wraps one solver handle; every call checks
+the status code and raises ``AvbdError`` with the library's message.
+
+Loading fails loudly: there is no CPU fallback for the solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("AVBD_HIP_LIB", str(_PKG / "libavbd_hip.so")))
+
+F32P = C.POINTER(C.c_float)
+I32P = C.POINTER(C.c_int)
+F64P = C.POINTER(C.c_double)
+I64P = C.POINTER(C.c_longlong)
+
+COLOR_JACOBI = 0
+COLOR_JONES_PLASSMANN = 1
+
+
+class AvbdError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("iterations", C.c_int),
+        ("gravity", C.c_float * 3),
+        ("alpha", C.c_float),
+        ("beta", C.c_float),
+        ("gamma", C.c_float),
+        ("post_stabilize", C.c_int),
+        ("max_linear_speed", C.c_float),
+        ("max_angular_speed", C.c_float),
+        ("friction_static_mult", C.c_float),
+        ("coloring_mode", C.c_int),
+        ("self_collide", C.c_int),
+        ("self_friction", C.c_float),
+        ("fixed_capacity", C.c_int),
+        ("margin", C.c_float),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("n_bodies", C.c_int),
+        ("x", F32P), ("q", F32P), ("v", F32P), ("w", F32P), ("prev_v", F32P), ("prev_w", F32P),
+        ("mass", F32P), ("inertia_local", F32P), ("half_extents", F32P), ("friction", F32P),
+        ("priority", F32P),
+        ("n_static_rows", C.c_int),
+        ("type", I32P), ("body_a", I32P), ("body_b", I32P), ("sibling", I32P), ("partner", I32P),
+        ("anchor", F32P), ("off_a", F32P), ("off_b", F32P), ("stiffness", F32P), ("fmin", F32P),
+        ("fmax", F32P), ("fracture", F32P), ("friction_d", F32P), ("friction_s", F32P),
+        ("lambda_", F32P), ("penalty", F32P),
+        ("active", I32P), ("was_static", I32P),
+        ("max_pairs", C.c_int),
+        ("n_dyn_capacity", C.c_int),
+        ("hash_cap", C.c_int),
+        ("pool_max", C.c_int),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("recolored", C.c_int),
+        ("n_active_colors", C.c_int),
+        ("n_active_rows", C.c_int),
+        ("row_capacity", C.c_int),
+        ("row_overflow", C.c_int),
+        ("new_row_capacity", C.c_int),
+        ("pair_count", C.c_int),
+        ("pair_capacity", C.c_int),
+        ("pair_grew", C.c_int),
+        ("pool_count", C.c_int),
+        ("n_contacts", C.c_int),
+        ("color_conflicts", C.c_int),
+        ("broadphase_ms", C.c_float),
+    ]
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(F32P)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(I32P)
+
+
+# Every exported symbol and its ctypes signature (argtypes, restype).
+SIGNATURES = {
+    "last_error": ([], C.c_char_p),
+    "device_count": ([I32P], C.c_int),
+    "create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "destroy": ([C.c_void_p], C.c_int),
+    "flush": ([C.c_void_p, C.POINTER(Scene)], C.c_int),
+    "substep": ([C.c_void_p, C.POINTER(Params), C.c_int, C.POINTER(Report)], C.c_int),
+    "read_bodies": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
+    "write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
+    "read_rows": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P, I32P, I32P, I32P, F32P], C.c_int),
+    "write_static_rows": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P], C.c_int),
+    "read_colors": ([C.c_void_p, I32P], C.c_int),
+    "write_colors": ([C.c_void_p, I32P, C.c_int], C.c_int),
+    "read_counters": ([C.c_void_p, I32P, I32P, I32P], C.c_int),
+    "count_color_conflicts": ([C.c_void_p, I32P], C.c_int),
+    "pack_state": ([C.c_void_p, F32P, C.c_int, F32P], C.c_int),
+    "export_hash": ([C.c_void_p, C.c_int, I32P, F32P, I32P], C.c_int),
+    "import_hash": ([C.c_void_p, C.c_int, I32P, F32P], C.c_int),
+    "read_capacities": ([C.c_void_p, I32P, I32P, I32P, I32P], C.c_int),
+    "read_pairs": ([C.c_void_p, C.c_int, I32P, I32P, I32P, I32P, F32P, F32P, I32P], C.c_int),
+    "read_manifold": ([C.c_void_p, C.c_int, I32P, I32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
+    "profile_enable": ([C.c_void_p, C.c_int], C.c_int),
+    "profile_read": ([C.c_void_p, C.c_int, F64P, I64P, F64P], C.c_int),
+}
+
+_LIB = None
+
+
+def bind(lib: C.CDLL, prefix: str, names=None) -> C.CDLL:
+    for name, (argt, rest) in SIGNATURES.items():
+        if names is not None and name not in names:
+            continue
+        fn = getattr(lib, prefix + name)
+        fn.argtypes = argt
+        fn.restype = rest
+    return lib
+
+
+def load_library() -> C.CDLL:
+    """Load libavbd_hip.so once; raise with build instructions if it is absent."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"avbd_amd: native library {LIB_PATH} is missing. Build it with "
+                "`python avbd_amd/build.py` (or __graft_entry__.build()). There is no "
+                "CPU fallback for the solver.")
+        _LIB = bind(C.CDLL(str(LIB_PATH)), "avbd_")
+    return _LIB
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int(0)
+    lib.avbd_device_count(C.byref(n))
+    return int(n.value)
+
+
+class NativeHandle:
+    """One solver handle of libavbd_hip.so; all state lives on the GPU."""
+
+    prefix = "avbd_"
+
+    def __init__(self, device: int = 0, lib: C.CDLL | None = None):
+        self.lib = lib if lib is not None else load_library()
+        self.h = C.c_void_p()
+        self._create(device)
+
+    def _create(self, device: int) -> None:
+        self._check(self._fn("create")(int(device), C.byref(self.h)))
+
+    def _fn(self, name: str):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self._fn("last_error")()
+            raise AvbdError(f"{self.prefix}: {msg.decode() if msg else 'error'} (code {rc})")
+
+    def close(self) -> None:
+        if self.h:
+            self._fn("destroy")(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- calls ----
+    def flush(self, scene: Scene) -> None:
+        self._check(self._fn("flush")(self.h, C.byref(scene)))
+
+    def substep(self, params: Params, color_dirty: bool) -> Report:
+        rep = Report()
+        self._check(self._fn("substep")(self.h, C.byref(params), int(bool(color_dirty)), C.byref(rep)))
+        return rep
+
+    def read_bodies(self, n: int) -> dict[str, np.ndarray]:
+        out = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "w", "prev_v", "prev_w")}
+        out["q"] = np.zeros((n, 4), np.float32)
+        self._check(self._fn("read_bodies")(self.h, fptr(out["x"]), fptr(out["q"]), fptr(out["v"]),
+                                            fptr(out["w"]), fptr(out["prev_v"]), fptr(out["prev_w"])))
+        return out
+
+    def write_bodies(self, field: int, first: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        n = v.shape[0] if v.ndim > 1 else 1
+        self._check(self._fn("write_bodies")(self.h, int(field), int(first), int(n), fptr(v)))
+
+    def read_rows(self, n: int) -> dict[str, np.ndarray]:
+        out = {
+            "lambda": np.zeros(n, np.float32), "penalty": np.zeros(n, np.float32),
+            "active": np.zeros(n, np.int32), "was_static": np.zeros(n, np.int32),
+            "type": np.zeros(n, np.int32), "body_a": np.zeros(n, np.int32),
+            "body_b": np.zeros(n, np.int32), "alpha_c0": np.zeros(n, np.float32),
+        }
+        self._check(self._fn("read_rows")(
+            self.h, int(n), fptr(out["lambda"]), fptr(out["penalty"]), iptr(out["active"]),
+            iptr(out["was_static"]), iptr(out["type"]), iptr(out["body_a"]), iptr(out["body_b"]),
+            fptr(out["alpha_c0"])))
+        return out
+
+    def write_static_rows(self, lam, pen, act, was) -> None:
+        lam = np.ascontiguousarray(lam, np.float32)
+        pen = np.ascontiguousarray(pen, np.float32)
+        act = np.ascontiguousarray(act, np.int32)
+        was = np.ascontiguousarray(was, np.int32)
+        self._check(self._fn("write_static_rows")(self.h, int(lam.shape[0]), fptr(lam), fptr(pen),
+                                                  iptr(act), iptr(was)))
+
+    def read_colors(self, n: int) -> np.ndarray:
+        c = np.zeros(max(n, 1), np.int32)
+        self._check(self._fn("read_colors")(self.h, iptr(c)))
+        return c[:n]
+
+    def write_colors(self, colors: np.ndarray, n_active_colors: int) -> None:
+        c = np.ascontiguousarray(colors, np.int32)
+        self._check(self._fn("write_colors")(self.h, iptr(c), int(n_active_colors)))
+
+    def read_counters(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)
+        self._check(self._fn("read_counters")(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def count_color_conflicts(self) -> int:
+        n = C.c_int(0)
+        self._check(self._fn("count_color_conflicts")(self.h, C.byref(n)))
+        return int(n.value)
+
+    def pack_state(self, nb: int, n_rows: int) -> tuple[np.ndarray, np.ndarray]:
+        bod = np.zeros(max(nb, 1) * 10, np.float32)
+        rows = np.zeros(max(n_rows, 1) * 3, np.float32)
+        self._check(self._fn("pack_state")(self.h, fptr(bod), int(n_rows), fptr(rows)))
+        return bod[: nb * 10].reshape(nb, 10), rows[: n_rows * 3].reshape(n_rows, 3)
+
+    def export_hash(self) -> tuple[np.ndarray, np.ndarray]:
+        n = C.c_int(0)
+        _, _, _, hc = self.read_capacities()
+        keys = np.zeros(max(hc, 1), np.int32)
+        st = np.zeros((max(hc, 1), 8), np.float32)
+        self._check(self._fn("export_hash")(self.h, int(hc), iptr(keys), fptr(st), C.byref(n)))
+        m = int(n.value)
+        return keys[:m].copy(), st[:m].copy()
+
+    def import_hash(self, keys: np.ndarray, state: np.ndarray) -> None:
+        k = np.ascontiguousarray(keys, np.int32)
+        s = np.ascontiguousarray(state, np.float32).reshape(-1, 8)
+        self._check(self._fn("import_hash")(self.h, int(k.shape[0]), iptr(k), fptr(s)))
+
+    def read_capacities(self) -> tuple[int, int, int, int]:
+        v = [C.c_int(0) for _ in range(4)]
+        self._check(self._fn("read_capacities")(self.h, *[C.byref(x) for x in v]))
+        return tuple(int(x.value) for x in v)
+
+    def read_pairs(self) -> dict[str, np.ndarray]:
+        mp, _, _, _ = self.read_capacities()
+        cap = max(mp, 1)
+        out = {"a": np.zeros(cap, np.int32), "b": np.zeros(cap, np.int32),
+               "overlap": np.zeros(cap, np.int32), "sat_idx": np.zeros(cap, np.int32),
+               "n_hat": np.zeros((cap, 3), np.float32), "depth": np.zeros(cap, np.float32)}
+        cnt = C.c_int(0)
+        self._check(self._fn("read_pairs")(self.h, cap, iptr(out["a"]), iptr(out["b"]),
+                                           iptr(out["overlap"]), iptr(out["sat_idx"]),
+                                           fptr(out["n_hat"]), fptr(out["depth"]), C.byref(cnt)))
+        k = min(int(cnt.value), mp)
+        res = {key: val[:k] for key, val in out.items()}
+        res["count"] = int(cnt.value)
+        return res
+
+    def read_manifold(self) -> dict[str, np.ndarray]:
+        mp, _, _, _ = self.read_capacities()
+        cap = max(mp, 1)
+        out = {"count": np.zeros(cap, np.int32), "ref_is_a": np.zeros(cap, np.int32),
+               "n_hat": np.zeros((cap, 3), np.float32), "t_hat": np.zeros((cap, 3), np.float32),
+               "b_hat": np.zeros((cap, 3), np.float32),
+               "off_ref": np.zeros((cap, 4, 3), np.float32),
+               "off_inc": np.zeros((cap, 4, 3), np.float32)}
+        self._check(self._fn("read_manifold")(self.h, cap, iptr(out["count"]), iptr(out["ref_is_a"]),
+                                              fptr(out["n_hat"]), fptr(out["t_hat"]),
+                                              fptr(out["b_hat"]), fptr(out["off_ref"]),
+                                              fptr(out["off_inc"])))
+        _, pc, _ = self.read_counters()
+        k = min(pc, mp)
+        return {key: val[:k] for key, val in out.items()}
+
+    def profile_enable(self, on: bool) -> None:
+        self._check(self._fn("profile_enable")(self.h, int(bool(on))))
+
+    def profile_read(self, n: int = 7):
+        ms = np.zeros(n, np.float64)
+        launches = np.zeros(n, np.int64)
+        nbytes = np.zeros(n, np.float64)
+        self._check(self._fn("profile_read")(self.h, n, ms.ctypes.data_as(F64P),
+                                             launches.ctypes.data_as(I64P),
+                                             nbytes.ctypes.data_as(F64P)))
+        return ms, launches, nbytes

@@ -1,0 +1,135 @@
+// avbd_internal.h — device data layout and kernel launchers of libavbd_hip.
+//
+// HBM layout (all arrays owned by the solver handle):
+//   bodies  pose[2*i] = {x, y, z, 0}, pose[2*i+1] = {qx, qy, qz, qw}   (32 B)
+//           pose0 / poseI: same layout for x_initial / x_inertial
+//           vel[2*i] = {v, 0}, vel[2*i+1] = {w, 0}; pvel: previous v, w
+//           inert[i] = 12 floats: I_world row-major in 3 float4 (w unused)
+//           bstat[i] = {mass, he.x, he.y, he.z}; Ilocal[9*i]; bfric[i]
+//   contact groups (one record per normal row plus its tangent rows):
+//           GroupGeo (64 B, rewritten only at emit) + GroupSt (48 B, mutable)
+//   A reference row j maps to (group g, slot r) with j = geo[g].hdr.w + r.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "avbd_fp.h"
+
+namespace avbd {
+
+// Group kinds.
+constexpr int kGFloor = 0;   // FLOOR normal [+ world x / z tangents]
+constexpr int kGBox = 1;     // BOX_BOX normal [+ t, b tangents]
+constexpr int kGPin = 2;     // one PIN row
+
+// hdr.x bit fields
+__host__ __device__ inline int g_kind(int h) { return h & 3; }
+__host__ __device__ inline int g_nrows(int h) { return ((h >> 2) & 3) + 1; }
+__host__ __device__ inline int g_cidx(int h) { return (h >> 4) & 3; }
+__host__ __device__ inline int g_make(int kind, int nrows, int cidx) {
+  return kind | ((nrows - 1) << 2) | (cidx << 4);
+}
+
+// bit casts usable on host and device
+__host__ __device__ inline float i2f(int v) { float f; __builtin_memcpy(&f, &v, 4); return f; }
+__host__ __device__ inline int f2i(float f) { int v; __builtin_memcpy(&v, &f, 4); return v; }
+
+// state bits (GroupSt.s2.y)
+constexpr int kActN = 1, kActT = 2, kActB = 4, kWasN = 8, kWasT = 16, kWasB = 32;
+
+struct __align__(16) GroupGeo {
+  int4 hdr;     // {kind|nrows|cidx, body_a, body_b (axis for PIN, -1 floor), row_base}
+  float4 g0;    // {anchor (floor: (0, y, 0); box: n; pin: world pt), stiffness}
+  float4 g1;    // {off_a, mu_d}
+  float4 g2;    // {off_b, mu_s (pin: fracture)}
+};
+struct __align__(16) GroupSt {
+  float4 s0;    // {lam_n, lam_t, lam_b, ac0_n}
+  float4 s1;    // {k_n, k_t, k_b, ac0_t}
+  float4 s2;    // {ac0_b, bits (int), -, -}
+};
+
+struct LaunchCfg {
+  hipStream_t stream;
+};
+
+// ---- solve kernels (kernels_solve.hip) ----
+void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose, float4* pose0,
+                          float4* poseI, float4* inert, const float4* vel, const float4* pvel,
+                          const float4* bstat, const float* Ilocal, GroupGeo* geo, GroupSt* gst,
+                          float dt, float gx, float gy, float gz, float alpha, float gamma,
+                          int post, int do_prelude);
+void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
+                         float4* pose, const float4* poseI, const float4* inert,
+                         const float4* bstat, const int* csr_start, const int* csr_idx,
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass);
+void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
+                 GroupSt* gst, float beta);
+void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
+                     float4* vel, float4* pvel, const float4* bstat, float dt, float max_lin,
+                     float max_ang);
+
+// ---- contact kernels (kernels_contacts.hip) ----
+void launch_aabb(hipStream_t st, int nb, const float4* pose, const float4* bstat, float margin,
+                 float4* lo, float4* hi);
+int bvh_build(hipStream_t st, int nb, const float4* lo, const float4* hi, void* ws,
+              size_t ws_bytes);
+size_t bvh_workspace_bytes(int nb);
+void bvh_count_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
+                     const float4* bstat, const void* ws, int* counts);
+void bvh_write_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
+                     const float4* bstat, const void* ws, const int* counts, const int* offsets,
+                     int cap, int* pair_a, int* pair_b);
+void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                      int* vals_tmp, int n, void* ws);
+size_t radix_workspace_bytes(int n);
+void launch_narrowphase(hipStream_t st, int n_pairs, const float4* pose, const float4* bstat,
+                        const int* pair_a, const int* pair_b, float margin, int* overlap,
+                        int* sat_idx, float4* n_hat, float* depth, int* mf_count, int* mf_refa,
+                        float4* mf_off /* 8 per pair: 4 ref, 4 inc */, float4* mf_n);
+void launch_emit_demand(hipStream_t st, int n_pairs, const int* pair_a, const int* pair_b,
+                        const int* mf_count, const int* mf_refa, const float* bfric,
+                        float default_mu, int* demand, int* rpc_out);
+void launch_emit_succ(hipStream_t st, int n_pairs, const int* mf_count, const int* rpc,
+                      const int* demand_scan, int cap_rows_after_static, int* succ,
+                      int* reserve);
+void launch_emit_write(hipStream_t st, int n_pairs, const int* pair_a, const int* pair_b,
+                       const int* mf_refa, const float4* mf_n, const float4* mf_off,
+                       const float* bfric, float default_mu, float mult, const int* rpc,
+                       const int* succ, const int* demand_scan, const int* succ_scan,
+                       int n_static_rows, int g_static, GroupGeo* geo, GroupSt* gst,
+                       int hash_cap, const int* hkeys, const float4* hstate);
+void launch_fill_int(hipStream_t st, int* p, int n, int v);
+void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const GroupGeo* geo,
+                         const GroupSt* gst, int hash_cap, int* hkeys, int* howner,
+                         int* slot_of, float4* hstate);
+
+// ---- graph kernels (kernels_graph.hip) ----
+// Exclusive scan of n ints; returns the total through *total_dev (device int).
+size_t scan_workspace_bytes(int n);
+void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws);
+void launch_csr_count(hipStream_t st, int nb, const int* st_count, int n_dyn, int g_first,
+                      const GroupGeo* geo, int* counts);
+void launch_csr_fill(hipStream_t st, int nb, const int* st_start, const int* st_idx,
+                     const int* starts, int* cursor, int n_dyn, int g_first,
+                     const GroupGeo* geo, int* csr_idx);
+void launch_csr_sort(hipStream_t st, int nb, const int* starts, const int* st_count,
+                     int* csr_idx);
+void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                            const int* color, int* out);
+void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                      int* counts);
+void launch_adj_fill(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                     const int* starts, int* cursor, int* nbr);
+void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* nbr_start,
+                         const int* nbr, const float* prio, int* color_in_out, int* pub,
+                         int* err, int* ticket, int mode_jp, int n_blocks);
+void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
+                          const float* prio, const int* color, int* color_next, int* uncolored);
+void launch_color_finish(hipStream_t st, int nb, int* color, int* counts);
+void launch_color_scatter(hipStream_t st, int nb, const int* color, const int* starts,
+                          int* cursor, int* bodies);
+void launch_sort_segments(hipStream_t st, int nb, const int* starts, int* idx);
+int color_greedy_blocks(int device);
+
+}  // namespace avbd

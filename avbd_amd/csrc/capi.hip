@@ -1,0 +1,1077 @@
+// capi.hip — host runtime and C ABI of libavbd_hip.so (declarations in
+// include/avbd.h). Owns every device buffer of a solver, turns the static rows
+// of a flush into contact groups, and issues the substep on one HIP stream in
+// the order of Solver6DOF._step_one (src/avbd3d/solver_6dof.py:1232-1464),
+// readback path: the pair count, the row reservation count and the
+// color-conflict count are read back each substep and drive the same
+// regrow / recolor decisions as the reference.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd.h"
+#include "avbd_internal.h"
+
+using namespace avbd;
+
+namespace avbd {
+void color_bucket(hipStream_t st, int nb, const int* color, unsigned* keys, int* vals,
+                  unsigned* keys_tmp, int* vals_tmp, void* ws);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCK(expr)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return set_err(AVBD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define CK(expr)               \
+  do {                         \
+    int r_ = (expr);           \
+    if (r_ != AVBD_OK) return r_; \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    size_t c = count ? count : 1;
+    if (hipMalloc(&p, c * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return set_err(AVBD_E_NOMEM, "hipMalloc of " + std::to_string(c * sizeof(T)) + " bytes failed");
+    }
+    n = c;
+    return AVBD_OK;
+  }
+  int ensure(size_t count) { return count <= n && p ? AVBD_OK : alloc(count); }
+  // Grow to `count`, keeping the first `keep` elements.
+  int grow_keep(size_t count, size_t keep, hipStream_t st) {
+    if (count <= n && p) return AVBD_OK;
+    T* q = nullptr;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess)
+      return set_err(AVBD_E_NOMEM, "hipMalloc (grow) failed");
+    if (p && keep) {
+      hipError_t e = hipMemcpyAsync(q, p, std::min(keep, n) * sizeof(T), hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return set_err(AVBD_E_HIP, hipGetErrorString(e));
+      (void)hipStreamSynchronize(st);
+    }
+    release();
+    p = q;
+    n = count;
+    return AVBD_OK;
+  }
+};
+
+enum ProfClass { kPrimal = 0, kDual, kPrelude, kContacts, kColoring, kCsr, kOther, kNumProf };
+
+}  // namespace
+
+struct avbd_solver {
+  int device = 0;
+  hipStream_t st = nullptr;
+  bool flushed = false;
+
+  // bodies
+  int nb = 0;
+  DBuf<float4> pose[2];
+  int cur = 0;
+  DBuf<float4> poseI, inert, vel, pvel, bstat, lo, hi;
+  DBuf<float> Ilocal, bfric, prio;
+  DBuf<int> order_id, order_jp;
+
+  // rows / groups
+  int n_static_rows = 0, g_static = 0, row_cap = 0, n_dyn_cap = 0, g_cap = 0;
+  int n_floor_groups = 0;
+  DBuf<GroupGeo> geo;
+  DBuf<GroupSt> gst;
+  std::vector<int> h_st_count, h_st_start;
+  DBuf<int> st_count, st_start, st_idx;
+  int st_total = 0;
+
+  // pairs
+  int pair_cap = 0;
+  DBuf<int> pa, pb, pov, psat, mfc, mfr, demand, rpc, dscan, succ, sscan, reserve, rscan;
+  DBuf<float4> pn, mfoff, mfn;
+  DBuf<float> pdepth;
+  DBuf<int> bcount, boff;
+
+  // hash
+  int hash_cap = 0, pool_max = 0;
+  DBuf<int> hkeys, howner, slot_of;
+  DBuf<float4> hstate;
+
+  // CSR
+  DBuf<int> csr_count, csr_start, csr_cursor, csr_idx;
+
+  // coloring
+  DBuf<int> color, color_next, nbr_count, nbr_start, nbr_cursor, nbr, pub, ticket, ccounts;
+  DBuf<unsigned> ckeys, ckeys_tmp;
+  DBuf<int> cvals, cvals_tmp;
+  int cstart[kMaxColors + 1] = {0};
+  int n_active_colors = kMaxColors;
+
+  // workspaces + counters
+  DBuf<char> ws_bvh, ws_scan, ws_color;
+  size_t ws_scan_elems = 0;
+  DBuf<int> dcnt;   // device counters
+  int* hcnt = nullptr;  // pinned mirror
+
+  // host mirrors of the last substep
+  int n_active_rows = 0, pair_count = 0, pool_count = 0, n_contacts = 0;
+  int last_post = 0;
+
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pending;  // (class, start event index)
+  size_t ev_used = 0;
+  double prof_ms[kNumProf] = {0};
+  long long prof_launch[kNumProf] = {0};
+  double prof_bytes[kNumProf] = {0};
+};
+
+namespace {
+
+int ensure_scan_ws(avbd_solver* s, size_t n) {
+  if (n <= s->ws_scan_elems && s->ws_scan.p) return AVBD_OK;
+  size_t want = std::max(n, (size_t)1024);
+  CK(s->ws_scan.alloc(scan_workspace_bytes((int)want)));
+  s->ws_scan_elems = want;
+  return AVBD_OK;
+}
+
+int scan(avbd_solver* s, const int* in, int* out, int n, int* total_dev) {
+  CK(ensure_scan_ws(s, (size_t)n));
+  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p);
+  return AVBD_OK;
+}
+
+int readback(avbd_solver* s, int first, int count) {
+  HIPCK(hipMemcpyAsync(s->hcnt + first, s->dcnt.p + first, count * sizeof(int),
+                       hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  return AVBD_OK;
+}
+
+// ---- profiling ----
+int prof_begin(avbd_solver* s, int cls) {
+  if (!s->prof) return AVBD_OK;
+  if (s->ev_used + 2 > s->ev_pool.size()) {
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      HIPCK(hipEventCreate(&e));
+      s->ev_pool.push_back(e);
+    }
+  }
+  HIPCK(hipEventRecord(s->ev_pool[s->ev_used], s->st));
+  s->ev_pending.push_back({cls, (int)s->ev_used});
+  s->ev_used += 2;
+  return AVBD_OK;
+}
+int prof_end(avbd_solver* s, double bytes) {
+  if (!s->prof) return AVBD_OK;
+  auto& pe = s->ev_pending.back();
+  HIPCK(hipEventRecord(s->ev_pool[pe.second + 1], s->st));
+  s->prof_bytes[pe.first] += bytes;
+  s->prof_launch[pe.first] += 1;
+  return AVBD_OK;
+}
+int prof_collect(avbd_solver* s) {
+  if (!s->prof || s->ev_pending.empty()) return AVBD_OK;
+  HIPCK(hipStreamSynchronize(s->st));
+  for (auto& pe : s->ev_pending) {
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, s->ev_pool[pe.second], s->ev_pool[pe.second + 1]));
+    s->prof_ms[pe.first] += ms;
+  }
+  s->ev_pending.clear();
+  s->ev_used = 0;
+  return AVBD_OK;
+}
+
+int ensure_pairs(avbd_solver* s, int cap) {
+  if (cap <= s->pair_cap && s->pa.p) return AVBD_OK;
+  CK(s->pa.alloc(cap)); CK(s->pb.alloc(cap)); CK(s->pov.alloc(cap)); CK(s->psat.alloc(cap));
+  CK(s->mfc.alloc(cap)); CK(s->mfr.alloc(cap)); CK(s->demand.alloc(cap)); CK(s->rpc.alloc(cap));
+  CK(s->dscan.alloc(cap)); CK(s->succ.alloc(cap)); CK(s->sscan.alloc(cap));
+  CK(s->reserve.alloc(cap)); CK(s->rscan.alloc(cap));
+  CK(s->pn.alloc(cap)); CK(s->mfn.alloc(cap)); CK(s->mfoff.alloc(8 * (size_t)cap));
+  CK(s->pdepth.alloc(cap));
+  s->pair_cap = cap;
+  return ensure_scan_ws(s, (size_t)cap);
+}
+
+// _grow_row_pool (solver_6dof.py:1698-1772): row capacity grows; the groups
+// written this substep are kept.
+int grow_row_pool(avbd_solver* s, int new_dyn, int live_groups) {
+  int new_cap = s->n_static_rows + new_dyn;
+  if (new_cap <= s->row_cap) return AVBD_OK;
+  int new_g = s->g_static + new_dyn;
+  CK(s->geo.grow_keep(new_g, live_groups, s->st));
+  CK(s->gst.grow_keep(new_g, live_groups, s->st));
+  s->g_cap = new_g;
+  s->row_cap = new_cap;
+  s->n_dyn_cap = new_dyn;
+  int new_pool = std::max(s->pool_max, s->pair_cap * 4);
+  if (new_pool > s->pool_max) {
+    CK(s->slot_of.alloc(new_pool));
+    s->pool_max = new_pool;
+  }
+  return AVBD_OK;
+}
+
+// Row-side byte model per launch (algorithmic bytes on a minimal record).
+double primal_sweep_bytes(const avbd_solver* s) {
+  return (double)s->nb * 120.0 + 2.0 * s->n_contacts * 112.0 + (double)s->n_floor_groups * 76.0;
+}
+double dual_sweep_bytes(const avbd_solver* s) {
+  return (double)s->n_contacts * 136.0 + (double)s->n_floor_groups * 100.0;
+}
+
+// _gpu_recolor (solver_6dof.py:680-844)
+int recolor(avbd_solver* s, int mode) {
+  const int nb = s->nb;
+  hipStream_t st = s->st;
+  const int nd = s->n_contacts;
+  CK(prof_begin(s, kColoring));
+  launch_fill_int(st, s->nbr_count.p, nb, 0);
+  launch_adj_count(st, nd, s->g_static, s->geo.p, s->nbr_count.p);
+  CK(scan(s, s->nbr_count.p, s->nbr_start.p, nb, s->nbr_start.p + nb));
+  launch_fill_int(st, s->nbr_cursor.p, nb, 0);
+  CK(s->nbr.ensure(2 * (size_t)nd + 1));
+  launch_adj_fill(st, nd, s->g_static, s->geo.p, s->nbr_start.p, s->nbr_cursor.p, s->nbr.p);
+  launch_fill_int(st, s->color.p, nb, -1);
+  launch_fill_int(st, s->dcnt.p + 6, 1, 0);
+  if (mode == AVBD_COLOR_JACOBI) {
+    for (int round = 0; round < kMaxColors; ++round) {
+      launch_color_greedy(st, nb, s->order_id.p, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
+                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0, 0);
+      launch_fill_int(st, s->dcnt.p + 5, 1, 0);
+      launch_color_resolve(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
+                           s->color_next.p, s->dcnt.p + 5);
+      std::swap(s->color.p, s->color_next.p);
+      CK(readback(s, 5, 1));
+      if (s->hcnt[5] == 0) break;
+    }
+  } else {
+    launch_color_greedy(st, nb, s->order_jp.p, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
+                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, 0);
+  }
+  launch_color_finish(st, nb, s->color.p, s->ccounts.p);
+  color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
+               s->ws_color.p);
+  CK(prof_end(s, 0.0));
+  HIPCK(hipMemcpyAsync(s->hcnt + 16, s->ccounts.p, kMaxColors * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCK(hipMemcpyAsync(s->hcnt + 6, s->dcnt.p + 6, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCK(hipStreamSynchronize(st));
+  if (s->hcnt[6] != 0) return set_err(AVBD_E_STATE, "coloring dataflow did not converge");
+  int mx = 0;
+  s->cstart[0] = 0;
+  for (int c = 0; c < kMaxColors; ++c) {
+    s->cstart[c + 1] = s->cstart[c] + s->hcnt[16 + c];
+    if (s->hcnt[16 + c] > 0) mx = c;
+  }
+  s->n_active_colors = std::max(1, std::min(mx + 1, kMaxColors));
+  return AVBD_OK;
+}
+
+int count_conflicts(avbd_solver* s, int* out) {
+  launch_fill_int(s->st, s->dcnt.p + 4, 1, 0);
+  launch_count_conflicts(s->st, s->n_contacts, s->g_static, s->geo.p, s->color.p, s->dcnt.p + 4);
+  CK(readback(s, 4, 1));
+  *out = s->hcnt[4];
+  return AVBD_OK;
+}
+
+int build_csr(avbd_solver* s) {
+  const int nb = s->nb, nd = s->n_contacts;
+  hipStream_t st = s->st;
+  CK(prof_begin(s, kCsr));
+  launch_csr_count(st, nb, s->st_count.p, nd, s->g_static, s->geo.p, s->csr_count.p);
+  CK(scan(s, s->csr_count.p, s->csr_start.p, nb, s->csr_start.p + nb));
+  CK(s->csr_idx.ensure((size_t)s->st_total + 2 * (size_t)nd + 1));
+  launch_csr_fill(st, nb, s->st_start.p, s->st_idx.p, s->csr_start.p, s->csr_cursor.p, nd,
+                  s->g_static, s->geo.p, s->csr_idx.p);
+  launch_csr_sort(st, nb, s->csr_start.p, s->st_count.p, s->csr_idx.p);
+  CK(prof_end(s, 0.0));
+  return AVBD_OK;
+}
+
+int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report* rep) {
+  const int nb = s->nb;
+  hipStream_t st = s->st;
+  auto t0 = std::chrono::steady_clock::now();
+  CK(prof_begin(s, kContacts));
+  CK(ensure_pairs(s, std::max(256, 16 * nb)));
+  float4* pc = s->pose[s->cur].p;
+  launch_aabb(st, nb, pc, s->bstat.p, p->margin, s->lo.p, s->hi.p);
+  bvh_build(st, nb, s->lo.p, s->hi.p, s->ws_bvh.p, s->ws_bvh.n);
+  bvh_count_pairs(st, nb, s->lo.p, s->hi.p, s->bstat.p, s->ws_bvh.p, s->bcount.p);
+  CK(scan(s, s->bcount.p, s->boff.p, nb, s->dcnt.p + 0));
+  CK(prof_end(s, 0.0));
+  CK(readback(s, 0, 1));
+  int total = s->hcnt[0];
+  if (!fixed && total > s->pair_cap) {
+    CK(ensure_pairs(s, std::max(2 * s->pair_cap, 2 * total)));
+    rep->pair_grew = 1;
+  }
+  s->pair_count = total;
+  rep->pair_count = total;
+  rep->pair_capacity = s->pair_cap;
+  int kept = std::min(total, s->pair_cap);
+  if (fixed || total > 0) {
+    CK(prof_begin(s, kContacts));
+    bvh_write_pairs(st, nb, s->lo.p, s->hi.p, s->bstat.p, s->ws_bvh.p, s->bcount.p, s->boff.p,
+                    s->pair_cap, s->pa.p, s->pb.p);
+    launch_narrowphase(st, kept, pc, s->bstat.p, s->pa.p, s->pb.p, p->margin, s->pov.p, s->psat.p,
+                       s->pn.p, s->pdepth.p, s->mfc.p, s->mfr.p, s->mfoff.p, s->mfn.p);
+    launch_emit_demand(st, kept, s->pa.p, s->pb.p, s->mfc.p, s->mfr.p, s->bfric.p,
+                       p->self_friction, s->demand.p, s->rpc.p);
+    CK(scan(s, s->demand.p, s->dscan.p, kept, s->dcnt.p + 1));
+    launch_emit_succ(st, kept, s->mfc.p, s->rpc.p, s->dscan.p, s->row_cap - s->n_static_rows,
+                     s->succ.p, s->reserve.p);
+    CK(scan(s, s->succ.p, s->sscan.p, kept, s->dcnt.p + 2));
+    CK(scan(s, s->reserve.p, s->rscan.p, kept, s->dcnt.p + 3));
+    launch_emit_write(st, kept, s->pa.p, s->pb.p, s->mfr.p, s->mfn.p, s->mfoff.p, s->bfric.p,
+                      p->self_friction, p->friction_static_mult, s->rpc.p, s->succ.p, s->dscan.p,
+                      s->sscan.p, s->n_static_rows, s->g_static, s->geo.p, s->gst.p, s->hash_cap,
+                      s->hkeys.p, s->hstate.p);
+    CK(prof_end(s, 0.0));
+    if (kept > 0) {
+      CK(readback(s, 2, 2));
+      s->n_contacts = s->hcnt[2];
+      s->n_active_rows = s->n_static_rows + s->hcnt[3];
+      s->pool_count = s->n_contacts;
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  rep->broadphase_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return AVBD_OK;
+}
+
+int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep) {
+  std::memset(rep, 0, sizeof(*rep));
+  rep->n_active_colors = s->n_active_colors;
+  if (s->nb == 0) return AVBD_OK;
+  const int nb = s->nb;
+  const bool fixed = p->fixed_capacity != 0;
+  hipStream_t st = s->st;
+  s->n_active_rows = s->n_static_rows;
+  s->pool_count = 0;
+  s->pair_count = 0;
+  s->n_contacts = 0;
+  int n_active = s->n_static_rows;
+  rep->row_capacity = s->row_cap;
+  if (p->self_collide && nb >= 2) {
+    CK(emit_contacts(s, p, fixed, rep));
+    if (fixed) {
+      n_active = s->row_cap;
+    } else {
+      int raw = s->n_active_rows, cap = s->row_cap;
+      if (raw > cap) {
+        rep->row_overflow = 1;
+        CK(grow_row_pool(s, std::max(2 * s->n_dyn_cap, (raw - s->n_static_rows) * 2),
+                         s->g_static + s->n_contacts));
+      }
+      n_active = std::min(raw, cap);
+    }
+  }
+  rep->n_active_rows = s->n_active_rows;
+  rep->new_row_capacity = s->row_cap;
+  rep->pool_count = s->pool_count;
+  rep->pair_capacity = s->pair_cap;
+  rep->n_contacts = std::min(s->n_contacts, s->pool_max);
+
+  bool need = color_dirty != 0;
+  if (!need) {
+    CK(count_conflicts(s, &rep->color_conflicts));
+    need = rep->color_conflicts > 0;
+  }
+  if (need) {
+    CK(recolor(s, p->coloring_mode));
+    rep->recolored = 1;
+  }
+  rep->n_active_colors = s->n_active_colors;
+
+  if (n_active > 0) CK(build_csr(s));
+
+  const int n_groups = s->g_static + s->n_contacts;
+  float4* p0 = s->pose[s->cur].p;
+  float4* pw = s->pose[1 - s->cur].p;
+  const float dt = p->dt;
+  const int post = p->post_stabilize ? 1 : 0;
+  CK(prof_begin(s, kPrelude));
+  launch_substep_begin(st, nb, n_groups, p0, pw, s->poseI.p, s->inert.p, s->vel.p, s->pvel.p,
+                       s->bstat.p, s->Ilocal.p, s->geo.p, s->gst.p, dt, p->gravity[0],
+                       p->gravity[1], p->gravity[2], p->alpha, p->gamma, post, n_active > 0 ? 1 : 0);
+  CK(prof_end(s, (double)nb * 188.0 + (double)s->n_contacts * 124.0 + (double)s->n_floor_groups * 88.0));
+
+  float max_lin = p->max_linear_speed, max_ang = p->max_angular_speed;
+  if (!(max_lin > 0.0f && max_lin < kInf && max_ang > 0.0f && max_ang < kInf)) {
+    max_lin = kInf;
+    max_ang = kInf;
+  }
+  const int I = p->iterations;
+  const int total = I + post;
+  const double pbytes = primal_sweep_bytes(s) / std::max(1, s->n_active_colors);
+  for (int it = 0; it < total; ++it) {
+    const int post_pass = (post && it == I) ? 1 : 0;
+    for (int c = 0; c < s->n_active_colors; ++c) {
+      int cnt = s->cstart[c + 1] - s->cstart[c];
+      CK(prof_begin(s, kPrimal));
+      launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
+                          s->bstat.p, s->csr_start.p, s->csr_idx.p, s->geo.p, s->gst.p, dt,
+                          post_pass);
+      CK(prof_end(s, pbytes));
+    }
+    if (it < I) {
+      CK(prof_begin(s, kDual));
+      launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta);
+      CK(prof_end(s, dual_sweep_bytes(s)));
+    }
+    if (it == I - 1) {
+      CK(prof_begin(s, kOther));
+      launch_finalize(st, nb, pw, p0, s->vel.p, s->pvel.p, s->bstat.p, dt, max_lin, max_ang);
+      CK(prof_end(s, (double)nb * 132.0));
+    }
+  }
+  if (p->self_collide && s->hash_cap > 0) {
+    CK(prof_begin(s, kOther));
+    launch_fill_int(st, s->hkeys.p, s->hash_cap, kHashEmpty);
+    launch_fill_int(st, s->howner.p, s->hash_cap, -1);
+    int entries = std::min(s->n_contacts, s->pool_max);
+    launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_cap, s->hkeys.p,
+                        s->howner.p, s->slot_of.p, s->hstate.p);
+    CK(prof_end(s, 0.0));
+  }
+  HIPCK(hipGetLastError());
+  s->cur = 1 - s->cur;
+  s->last_post = post;
+  CK(prof_collect(s));
+  return AVBD_OK;
+}
+
+bool isinf_pos(float v) { return v == kInf; }
+bool isinf_neg(float v) { return v == -kInf; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* avbd_last_error(void) { return g_err.c_str(); }
+
+int avbd_device_count(int* out) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *out = (e == hipSuccess) ? n : 0;
+  return AVBD_OK;
+}
+
+int avbd_create(int device, avbd_solver** out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return set_err(AVBD_E_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return set_err(AVBD_E_ARG, "device ordinal out of range");
+  HIPCK(hipSetDevice(device));
+  avbd_solver* s = new avbd_solver();
+  s->device = device;
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return set_err(AVBD_E_HIP, "hipStreamCreate failed");
+  }
+  if (hipHostMalloc((void**)&s->hcnt, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(s->st);
+    delete s;
+    return set_err(AVBD_E_NOMEM, "hipHostMalloc failed");
+  }
+  std::memset(s->hcnt, 0, 64 * sizeof(int));
+  int r = s->dcnt.alloc(64);
+  if (r != AVBD_OK) {
+    (void)hipHostFree(s->hcnt);
+    (void)hipStreamDestroy(s->st);
+    delete s;
+    return r;
+  }
+  *out = s;
+  return AVBD_OK;
+}
+
+int avbd_destroy(avbd_solver* s) {
+  if (!s) return AVBD_OK;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->st);
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->hcnt) (void)hipHostFree(s->hcnt);
+  (void)hipStreamDestroy(s->st);
+  delete s;
+  return AVBD_OK;
+}
+
+int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
+  if (!s || !sc) return set_err(AVBD_E_ARG, "null argument");
+  HIPCK(hipSetDevice(s->device));
+  const int nb = sc->n_bodies, ns = sc->n_static_rows;
+  if (nb < 0 || ns < 0 || sc->n_dyn_capacity < 0 || sc->hash_cap < 0 || sc->pool_max < 0 ||
+      sc->max_pairs < 0)
+    return set_err(AVBD_E_ARG, "negative size in scene");
+  HIPCK(hipStreamSynchronize(s->st));
+  s->nb = nb;
+  s->cur = 0;
+  const size_t nbb = std::max(nb, 1);
+  // ---- bodies ----
+  std::vector<float4> pose(2 * nbb), vel(2 * nbb), pvel(2 * nbb), bst(nbb);
+  for (int i = 0; i < nb; ++i) {
+    pose[2 * i] = make_float4(sc->x[3 * i], sc->x[3 * i + 1], sc->x[3 * i + 2], 0.f);
+    pose[2 * i + 1] = make_float4(sc->q[4 * i], sc->q[4 * i + 1], sc->q[4 * i + 2], sc->q[4 * i + 3]);
+    vel[2 * i] = make_float4(sc->v[3 * i], sc->v[3 * i + 1], sc->v[3 * i + 2], 0.f);
+    vel[2 * i + 1] = make_float4(sc->w[3 * i], sc->w[3 * i + 1], sc->w[3 * i + 2], 0.f);
+    pvel[2 * i] = make_float4(sc->prev_v[3 * i], sc->prev_v[3 * i + 1], sc->prev_v[3 * i + 2], 0.f);
+    pvel[2 * i + 1] = make_float4(sc->prev_w[3 * i], sc->prev_w[3 * i + 1], sc->prev_w[3 * i + 2], 0.f);
+    bst[i] = make_float4(sc->mass[i], sc->half_extents[3 * i], sc->half_extents[3 * i + 1],
+                         sc->half_extents[3 * i + 2]);
+  }
+  CK(s->pose[0].alloc(2 * nbb)); CK(s->pose[1].alloc(2 * nbb));
+  CK(s->poseI.alloc(2 * nbb)); CK(s->inert.alloc(3 * nbb)); CK(s->vel.alloc(2 * nbb));
+  CK(s->pvel.alloc(2 * nbb)); CK(s->bstat.alloc(nbb)); CK(s->lo.alloc(nbb)); CK(s->hi.alloc(nbb));
+  CK(s->Ilocal.alloc(9 * nbb)); CK(s->bfric.alloc(nbb)); CK(s->prio.alloc(nbb));
+  CK(s->order_id.alloc(nbb)); CK(s->order_jp.alloc(nbb));
+  if (nb > 0) {
+    HIPCK(hipMemcpy(s->pose[0].p, pose.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->pose[1].p, pose.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->vel.p, vel.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->pvel.p, pvel.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->bstat.p, bst.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->Ilocal.p, sc->inertia_local, 9 * (size_t)nb * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->bfric.p, sc->friction, nb * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->prio.p, sc->priority, nb * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<int> ord(nb);
+    std::iota(ord.begin(), ord.end(), 0);
+    HIPCK(hipMemcpy(s->order_id.p, ord.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+    // JP processing order: priority descending, index ascending on ties.
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return sc->priority[a] > sc->priority[b]; });
+    HIPCK(hipMemcpy(s->order_jp.p, ord.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+  }
+  // ---- static rows -> groups ----
+  std::vector<GroupGeo> hg;
+  std::vector<GroupSt> hs;
+  int n_floor = 0;
+  auto v3at = [](const float* a, int j) { return mk3(a[3 * j], a[3 * j + 1], a[3 * j + 2]); };
+  for (int j = 0; j < ns;) {
+    int t = sc->type[j];
+    GroupGeo G;
+    GroupSt S;
+    std::memset(&G, 0, sizeof(G));
+    std::memset(&S, 0, sizeof(S));
+    if (t == kFloor) {
+      if (!(isinf_neg(sc->fmin[j]) && sc->fmax[j] == 0.0f && isinf_pos(sc->fracture[j])))
+        return set_err(AVBD_E_ARG, "floor row " + std::to_string(j) + " has non-default bounds");
+      int nr = 1;
+      if (j + 2 < ns && sc->type[j + 1] == kTangent &&
+          sc->type[j + 2] == kTangent && sc->sibling[j + 1] == j && sc->sibling[j + 2] == j &&
+          sc->partner[j + 1] == j + 2 && sc->partner[j + 2] == j + 1)
+        nr = 3;
+      v3 off = v3at(sc->off_a, j);
+      if (nr == 3) {
+        for (int r = 1; r < 3; ++r) {
+          int q = j + r;
+          v3 o2 = v3at(sc->off_a, q), an = v3at(sc->anchor, q);
+          v3 want = r == 1 ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 0.f, 1.f);
+          if (sc->body_a[q] != sc->body_a[j] || sc->body_b[q] >= 0 || o2.x != off.x ||
+              o2.y != off.y || o2.z != off.z || an.x != want.x || an.y != want.y ||
+              an.z != want.z || !(sc->stiffness[q] == sc->stiffness[j] ||
+                                   (std::isnan(sc->stiffness[q]) && std::isnan(sc->stiffness[j]))) ||
+              !isinf_pos(sc->fracture[q]) || sc->friction_d[q] != sc->friction_d[j + 1] ||
+              sc->friction_s[q] != sc->friction_s[j + 1])
+            return set_err(AVBD_E_ARG, "floor tangent row " + std::to_string(q) +
+                                           " does not follow add_floor_contact_box layout");
+        }
+      }
+      if (sc->body_b[j] >= 0)
+        return set_err(AVBD_E_ARG, "floor row " + std::to_string(j) + " names a second body");
+      G.hdr = make_int4(g_make(kGFloor, nr, 0), sc->body_a[j], -1, j);
+      v3 an = v3at(sc->anchor, j);
+      G.g0 = make_float4(an.x, an.y, an.z, sc->stiffness[j]);
+      G.g1 = make_float4(off.x, off.y, off.z, nr == 3 ? sc->friction_d[j + 1] : 0.f);
+      G.g2 = make_float4(0.f, 0.f, 0.f, nr == 3 ? sc->friction_s[j + 1] : 0.f);
+      float lam[3] = {0, 0, 0}, pen[3] = {0, 0, 0};
+      int bits = 0;
+      for (int r = 0; r < nr; ++r) {
+        lam[r] = sc->lambda[j + r];
+        pen[r] = sc->penalty[j + r];
+        if (sc->active[j + r]) bits |= (r == 0 ? kActN : (r == 1 ? kActT : kActB));
+        if (sc->was_static[j + r]) bits |= (r == 0 ? kWasN : (r == 1 ? kWasT : kWasB));
+      }
+      S.s0 = make_float4(lam[0], lam[1], lam[2], 0.f);
+      S.s1 = make_float4(pen[0], pen[1], pen[2], 0.f);
+      S.s2 = make_float4(0.f, i2f(bits), 0.f, 0.f);
+      ++n_floor;
+      j += nr;
+    } else if (t == kPin) {
+      int axis = sc->body_b[j];
+      if (axis < 0 || axis > 2) return set_err(AVBD_E_ARG, "pin row axis out of range");
+      if (!(isinf_neg(sc->fmin[j]) && isinf_pos(sc->fmax[j])))
+        return set_err(AVBD_E_ARG, "pin row " + std::to_string(j) + " has non-default bounds");
+      v3 an = v3at(sc->anchor, j), off = v3at(sc->off_a, j);
+      G.hdr = make_int4(g_make(kGPin, 1, 0), sc->body_a[j], axis, j);
+      G.g0 = make_float4(an.x, an.y, an.z, sc->stiffness[j]);
+      G.g1 = make_float4(off.x, off.y, off.z, 0.f);
+      G.g2 = make_float4(0.f, 0.f, 0.f, sc->fracture[j]);
+      int bits = (sc->active[j] ? kActN : 0) | (sc->was_static[j] ? kWasN : 0);
+      S.s0 = make_float4(sc->lambda[j], 0.f, 0.f, 0.f);
+      S.s1 = make_float4(sc->penalty[j], 0.f, 0.f, 0.f);
+      S.s2 = make_float4(0.f, i2f(bits), 0.f, 0.f);
+      j += 1;
+    } else {
+      return set_err(AVBD_E_ARG, "static row " + std::to_string(j) + " of type " +
+                                     std::to_string(t) + " is not supported");
+    }
+    if (G.hdr.y < 0 || G.hdr.y >= nb) return set_err(AVBD_E_ARG, "row body index out of range");
+    hg.push_back(G);
+    hs.push_back(S);
+  }
+  s->n_static_rows = ns;
+  s->g_static = (int)hg.size();
+  s->n_floor_groups = n_floor;
+  s->n_dyn_cap = sc->n_dyn_capacity;
+  s->row_cap = ns + sc->n_dyn_capacity;
+  s->g_cap = s->g_static + sc->n_dyn_capacity;
+  CK(s->geo.alloc(std::max(s->g_cap, 1)));
+  CK(s->gst.alloc(std::max(s->g_cap, 1)));
+  if (s->g_static > 0) {
+    HIPCK(hipMemcpy(s->geo.p, hg.data(), hg.size() * sizeof(GroupGeo), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(s->gst.p, hs.data(), hs.size() * sizeof(GroupSt), hipMemcpyHostToDevice));
+  }
+  // static CSR (each body's static groups in ascending order)
+  s->h_st_count.assign(nbb, 0);
+  for (auto& G : hg) s->h_st_count[G.hdr.y]++;
+  s->h_st_start.assign(nbb + 1, 0);
+  for (int i = 0; i < nb; ++i) s->h_st_start[i + 1] = s->h_st_start[i] + s->h_st_count[i];
+  s->st_total = nb > 0 ? s->h_st_start[nb] : 0;
+  std::vector<int> sidx(std::max(s->st_total, 1)), cur(s->h_st_start.begin(), s->h_st_start.end() - 1);
+  for (int g = 0; g < (int)hg.size(); ++g) sidx[cur[hg[g].hdr.y]++] = g;
+  CK(s->st_count.alloc(nbb)); CK(s->st_start.alloc(nbb + 1)); CK(s->st_idx.alloc(sidx.size()));
+  HIPCK(hipMemcpy(s->st_count.p, s->h_st_count.data(), nbb * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(s->st_start.p, s->h_st_start.data(), (nbb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(s->st_idx.p, sidx.data(), sidx.size() * sizeof(int), hipMemcpyHostToDevice));
+  // ---- pairs / hash / pools ----
+  s->pair_cap = 0;
+  s->pa.release();
+  if (sc->max_pairs > 0) CK(ensure_pairs(s, sc->max_pairs));
+  CK(s->bcount.alloc(nbb)); CK(s->boff.alloc(nbb));
+  s->hash_cap = sc->hash_cap;
+  s->pool_max = sc->pool_max;
+  CK(s->hkeys.alloc(std::max(s->hash_cap, 1)));
+  CK(s->howner.alloc(std::max(s->hash_cap, 1)));
+  CK(s->hstate.alloc(2 * (size_t)std::max(s->hash_cap, 1)));
+  CK(s->slot_of.alloc(std::max(s->pool_max, 1)));
+  launch_fill_int(s->st, s->hkeys.p, std::max(s->hash_cap, 1), kHashEmpty);
+  HIPCK(hipMemsetAsync(s->hstate.p, 0, 2 * (size_t)std::max(s->hash_cap, 1) * sizeof(float4), s->st));
+  // ---- CSR + coloring ----
+  CK(s->csr_count.alloc(nbb)); CK(s->csr_start.alloc(nbb + 1)); CK(s->csr_cursor.alloc(nbb));
+  CK(s->csr_idx.alloc(std::max(s->st_total, 1)));
+  HIPCK(hipMemsetAsync(s->csr_start.p, 0, (nbb + 1) * sizeof(int), s->st));
+  CK(s->color.alloc(nbb)); CK(s->color_next.alloc(nbb)); CK(s->nbr_count.alloc(nbb));
+  CK(s->nbr_start.alloc(nbb + 1)); CK(s->nbr_cursor.alloc(nbb)); CK(s->nbr.alloc(1));
+  CK(s->pub.alloc(nbb)); CK(s->ticket.alloc(1)); CK(s->ccounts.alloc(kMaxColors));
+  CK(s->ckeys.alloc(nbb)); CK(s->ckeys_tmp.alloc(nbb)); CK(s->cvals.alloc(nbb));
+  CK(s->cvals_tmp.alloc(nbb));
+  CK(s->ws_bvh.alloc(bvh_workspace_bytes((int)nbb)));
+  CK(s->ws_color.alloc(radix_workspace_bytes((int)nbb)));
+  s->ws_scan_elems = 0;
+  CK(ensure_scan_ws(s, std::max<size_t>(nbb, 16 * ((nbb + 1023) / 1024) + 16)));
+  launch_fill_int(s->st, s->color.p, (int)nbb, -1);
+  s->n_active_colors = kMaxColors;
+  for (int c = 0; c <= kMaxColors; ++c) s->cstart[c] = 0;
+  s->n_active_rows = ns;
+  s->pair_count = s->pool_count = s->n_contacts = 0;
+  s->last_post = 0;
+  HIPCK(hipStreamSynchronize(s->st));
+  s->flushed = true;
+  return AVBD_OK;
+}
+
+int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out) {
+  if (!s || !p || !out) return set_err(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
+  HIPCK(hipSetDevice(s->device));
+  return do_substep(s, p, color_dirty, out);
+}
+
+int avbd_read_bodies(avbd_solver* s, float* x, float* q, float* v, float* w, float* pv,
+                     float* pw) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  int nb = s->nb;
+  if (nb == 0) return AVBD_OK;
+  std::vector<float4> a(2 * nb), b(2 * nb), c(2 * nb);
+  HIPCK(hipMemcpyAsync(a.data(), s->pose[s->cur].p, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipMemcpyAsync(b.data(), s->vel.p, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipMemcpyAsync(c.data(), s->pvel.p, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  for (int i = 0; i < nb; ++i) {
+    if (x) { x[3 * i] = a[2 * i].x; x[3 * i + 1] = a[2 * i].y; x[3 * i + 2] = a[2 * i].z; }
+    if (q) { q[4 * i] = a[2 * i + 1].x; q[4 * i + 1] = a[2 * i + 1].y; q[4 * i + 2] = a[2 * i + 1].z; q[4 * i + 3] = a[2 * i + 1].w; }
+    if (v) { v[3 * i] = b[2 * i].x; v[3 * i + 1] = b[2 * i].y; v[3 * i + 2] = b[2 * i].z; }
+    if (w) { w[3 * i] = b[2 * i + 1].x; w[3 * i + 1] = b[2 * i + 1].y; w[3 * i + 2] = b[2 * i + 1].z; }
+    if (pv) { pv[3 * i] = c[2 * i].x; pv[3 * i + 1] = c[2 * i].y; pv[3 * i + 2] = c[2 * i].z; }
+    if (pw) { pw[3 * i] = c[2 * i + 1].x; pw[3 * i + 1] = c[2 * i + 1].y; pw[3 * i + 2] = c[2 * i + 1].z; }
+  }
+  return AVBD_OK;
+}
+
+int avbd_write_bodies(avbd_solver* s, int field, int first, int n, const float* vals) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "write before flush");
+  if (first < 0 || n < 0 || first + n > s->nb) return set_err(AVBD_E_ARG, "body range");
+  if (n == 0) return AVBD_OK;
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  std::vector<float4> h(n);
+  float4* base;
+  int off;
+  switch (field) {
+    case 0: base = s->pose[s->cur].p; off = 0; break;
+    case 1: base = s->pose[s->cur].p; off = 1; break;
+    case 2: base = s->vel.p; off = 0; break;
+    case 3: base = s->vel.p; off = 1; break;
+    case 4: base = s->pvel.p; off = 0; break;
+    case 5: base = s->pvel.p; off = 1; break;
+    default: return set_err(AVBD_E_ARG, "unknown body field");
+  }
+  for (int k = 0; k < n; ++k) {
+    if (field == 1) h[k] = make_float4(vals[4 * k], vals[4 * k + 1], vals[4 * k + 2], vals[4 * k + 3]);
+    else h[k] = make_float4(vals[3 * k], vals[3 * k + 1], vals[3 * k + 2], 0.f);
+  }
+  HIPCK(hipMemcpy2DAsync(base + 2 * first + off, 2 * sizeof(float4), h.data(), sizeof(float4),
+                         sizeof(float4), n, hipMemcpyHostToDevice, s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  return AVBD_OK;
+}
+
+static int download_groups(avbd_solver* s, int ng, std::vector<GroupGeo>& G,
+                           std::vector<GroupSt>& S) {
+  G.resize(std::max(ng, 1));
+  S.resize(std::max(ng, 1));
+  if (ng > 0) {
+    HIPCK(hipMemcpyAsync(G.data(), s->geo.p, ng * sizeof(GroupGeo), hipMemcpyDeviceToHost, s->st));
+    HIPCK(hipMemcpyAsync(S.data(), s->gst.p, ng * sizeof(GroupSt), hipMemcpyDeviceToHost, s->st));
+  }
+  HIPCK(hipStreamSynchronize(s->st));
+  return AVBD_OK;
+}
+
+int avbd_read_rows(avbd_solver* s, int n, float* lam, float* pen, int* act, int* was, int* type,
+                   int* ba, int* bb, float* ac0) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  for (int j = 0; j < n; ++j) {
+    if (lam) lam[j] = 0.f;
+    if (pen) pen[j] = 0.f;
+    if (act) act[j] = 0;
+    if (was) was[j] = 0;
+    if (type) type[j] = 0;
+    if (ba) ba[j] = 0;
+    if (bb) bb[j] = 0;
+    if (ac0) ac0[j] = 0.f;
+  }
+  int ng = s->g_static + s->n_contacts;
+  std::vector<GroupGeo> G;
+  std::vector<GroupSt> S;
+  CK(download_groups(s, ng, G, S));
+  for (int g = 0; g < ng; ++g) {
+    int h = G[g].hdr.x, kind = g_kind(h), nr = g_nrows(h);
+    int bits = f2i(S[g].s2.y);
+    const float L[3] = {S[g].s0.x, S[g].s0.y, S[g].s0.z};
+    const float K[3] = {S[g].s1.x, S[g].s1.y, S[g].s1.z};
+    const float A[3] = {S[g].s0.w, S[g].s1.w, S[g].s2.x};
+    for (int r = 0; r < nr; ++r) {
+      int j = G[g].hdr.w + r;
+      if (j < 0 || j >= n) continue;
+      int rt = r == 0 ? (kind == kGFloor ? kFloor : (kind == kGBox ? kBoxBox : kPin)) : kTangent;
+      if (lam) lam[j] = L[r];
+      if (pen) pen[j] = K[r];
+      if (act) act[j] = (bits >> r) & 1;
+      if (was) was[j] = (bits >> (3 + r)) & 1;
+      if (type) type[j] = rt;
+      if (ba) ba[j] = G[g].hdr.y;
+      if (bb) bb[j] = kind == kGFloor ? -1 : G[g].hdr.z;
+      if (ac0) ac0[j] = (s->last_post && rt != kTangent) ? 0.0f : A[r];
+    }
+  }
+  return AVBD_OK;
+}
+
+int avbd_write_static_rows(avbd_solver* s, int n, const float* lam, const float* pen,
+                           const int* act, const int* was) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "write before flush");
+  if (n > s->n_static_rows) return set_err(AVBD_E_ARG, "more rows than static rows");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  std::vector<GroupGeo> G;
+  std::vector<GroupSt> S;
+  CK(download_groups(s, s->g_static, G, S));
+  for (int g = 0; g < s->g_static; ++g) {
+    int nr = g_nrows(G[g].hdr.x);
+    int bits = f2i(S[g].s2.y);
+    float* L[3] = {&S[g].s0.x, &S[g].s0.y, &S[g].s0.z};
+    float* K[3] = {&S[g].s1.x, &S[g].s1.y, &S[g].s1.z};
+    for (int r = 0; r < nr; ++r) {
+      int j = G[g].hdr.w + r;
+      if (j >= n) continue;
+      if (lam) *L[r] = lam[j];
+      if (pen) *K[r] = pen[j];
+      if (act) bits = act[j] ? (bits | (1 << r)) : (bits & ~(1 << r));
+      if (was) bits = was[j] ? (bits | (8 << r)) : (bits & ~(8 << r));
+    }
+    S[g].s2.y = i2f(bits);
+  }
+  if (s->g_static > 0)
+    HIPCK(hipMemcpy(s->gst.p, S.data(), s->g_static * sizeof(GroupSt), hipMemcpyHostToDevice));
+  return AVBD_OK;
+}
+
+int avbd_read_colors(avbd_solver* s, int* c) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  if (s->nb == 0) return AVBD_OK;
+  HIPCK(hipMemcpyAsync(c, s->color.p, s->nb * sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  return AVBD_OK;
+}
+
+int avbd_write_colors(avbd_solver* s, const int* c, int n_active_colors) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "write before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  if (s->nb == 0) return AVBD_OK;
+  for (int i = 0; i < s->nb; ++i)
+    if (c[i] < 0 || c[i] >= kMaxColors) return set_err(AVBD_E_ARG, "color out of range");
+  HIPCK(hipMemcpy(s->color.p, c, s->nb * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> cnt(kMaxColors, 0);
+  for (int i = 0; i < s->nb; ++i) cnt[c[i]]++;
+  s->cstart[0] = 0;
+  for (int k = 0; k < kMaxColors; ++k) s->cstart[k + 1] = s->cstart[k] + cnt[k];
+  color_bucket(s->st, s->nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
+               s->ws_color.p);
+  HIPCK(hipStreamSynchronize(s->st));
+  s->n_active_colors = n_active_colors;
+  return AVBD_OK;
+}
+
+int avbd_read_counters(avbd_solver* s, int* nar, int* pc, int* poc) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  if (nar) *nar = s->n_active_rows;
+  if (pc) *pc = s->pair_count;
+  if (poc) *poc = s->pool_count;
+  return AVBD_OK;
+}
+
+int avbd_count_color_conflicts(avbd_solver* s, int* out) {
+  if (!s || !s->flushed) {
+    *out = 0;
+    return AVBD_OK;
+  }
+  HIPCK(hipSetDevice(s->device));
+  return count_conflicts(s, out);
+}
+
+int avbd_pack_state(avbd_solver* s, float* bodies10, int n_rows, float* rows3) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  int nb = s->nb;
+  std::vector<float> x(3 * (size_t)nb + 1), q(4 * (size_t)nb + 1), w(3 * (size_t)nb + 1);
+  CK(avbd_read_bodies(s, x.data(), q.data(), nullptr, w.data(), nullptr, nullptr));
+  for (int i = 0; i < nb; ++i) {
+    float* o = bodies10 + 10 * (size_t)i;
+    o[0] = x[3 * i]; o[1] = x[3 * i + 1]; o[2] = x[3 * i + 2];
+    o[3] = q[4 * i]; o[4] = q[4 * i + 1]; o[5] = q[4 * i + 2]; o[6] = q[4 * i + 3];
+    o[7] = w[3 * i]; o[8] = w[3 * i + 1]; o[9] = w[3 * i + 2];
+  }
+  if (n_rows > 0 && rows3) {
+    std::vector<float> lam(n_rows);
+    std::vector<int> act(n_rows), was(n_rows), ty(n_rows);
+    CK(avbd_read_rows(s, n_rows, lam.data(), nullptr, act.data(), was.data(), ty.data(), nullptr,
+                      nullptr, nullptr));
+    for (int j = 0; j < n_rows; ++j) {
+      rows3[3 * (size_t)j] = lam[j];
+      rows3[3 * (size_t)j + 1] = (float)act[j];
+      rows3[3 * (size_t)j + 2] = (float)(was[j] * 16 + ty[j]);
+    }
+  }
+  return AVBD_OK;
+}
+
+int avbd_export_hash(avbd_solver* s, int cap, int* keys, float* st8, int* n) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  int hc = s->hash_cap;
+  *n = 0;
+  if (hc <= 0) return AVBD_OK;
+  std::vector<int> k(hc);
+  std::vector<float4> v(2 * (size_t)hc);
+  HIPCK(hipMemcpyAsync(k.data(), s->hkeys.p, hc * sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipMemcpyAsync(v.data(), s->hstate.p, 2 * (size_t)hc * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  int m = 0;
+  for (int sl = 0; sl < hc; ++sl) {
+    if (k[sl] == kHashEmpty) continue;
+    if (m < cap) {
+      keys[m] = k[sl];
+      float4 a = v[2 * (size_t)sl], b = v[2 * (size_t)sl + 1];
+      float* o = st8 + 8 * (size_t)m;
+      o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+    }
+    ++m;
+  }
+  *n = m;
+  return AVBD_OK;
+}
+
+int avbd_import_hash(avbd_solver* s, int n, const int* keys, const float* st8) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "write before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  int hc = s->hash_cap;
+  if (hc <= 0) return n == 0 ? AVBD_OK : set_err(AVBD_E_ARG, "no hash table");
+  std::vector<int> k(hc, kHashEmpty);
+  std::vector<float4> v(2 * (size_t)hc, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int e = 0; e < n; ++e) {
+    int h = hash_pair_slot(keys[e], hc);
+    for (int probe = 0; probe < hc; ++probe) {
+      int sl = (h + probe) % hc;
+      if (k[sl] == kHashEmpty || k[sl] == keys[e]) {
+        k[sl] = keys[e];
+        const float* o = st8 + 8 * (size_t)e;
+        v[2 * (size_t)sl] = make_float4(o[0], o[1], o[2], o[3]);
+        v[2 * (size_t)sl + 1] = make_float4(o[4], o[5], o[6], o[7]);
+        break;
+      }
+    }
+  }
+  HIPCK(hipMemcpy(s->hkeys.p, k.data(), hc * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(s->hstate.p, v.data(), 2 * (size_t)hc * sizeof(float4), hipMemcpyHostToDevice));
+  return AVBD_OK;
+}
+
+int avbd_read_capacities(avbd_solver* s, int* mp, int* rc, int* pm, int* hc) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  if (mp) *mp = s->pair_cap;
+  if (rc) *rc = s->row_cap;
+  if (pm) *pm = s->pool_max;
+  if (hc) *hc = s->hash_cap;
+  return AVBD_OK;
+}
+
+int avbd_read_pairs(avbd_solver* s, int cap, int* a, int* b, int* ov, int* sat, float* n,
+                    float* depth, int* count) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  *count = s->pair_count;
+  int k = std::min(std::min(s->pair_count, s->pair_cap), cap);
+  if (k <= 0) return AVBD_OK;
+  std::vector<int> A(k), B(k), O(k), T(k);
+  std::vector<float4> N(k);
+  std::vector<float> D(k);
+  HIPCK(hipMemcpy(A.data(), s->pa.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(B.data(), s->pb.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(O.data(), s->pov.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(T.data(), s->psat.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(N.data(), s->pn.p, k * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(D.data(), s->pdepth.p, k * sizeof(float), hipMemcpyDeviceToHost));
+  for (int p = 0; p < k; ++p) {
+    if (a) a[p] = A[p];
+    if (b) b[p] = B[p];
+    if (ov) ov[p] = O[p];
+    if (sat) sat[p] = O[p] ? T[p] : -1;
+    if (n) { n[3 * p] = O[p] ? N[p].x : 0.f; n[3 * p + 1] = O[p] ? N[p].y : 0.f; n[3 * p + 2] = O[p] ? N[p].z : 0.f; }
+    if (depth) depth[p] = O[p] ? D[p] : 0.f;
+  }
+  return AVBD_OK;
+}
+
+int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* refa, float* n, float* t,
+                       float* b, float* offr, float* offi) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  int k = std::min(std::min(s->pair_count, s->pair_cap), cap);
+  if (k <= 0) return AVBD_OK;
+  std::vector<int> C(k), R(k);
+  std::vector<float4> N(k), O(8 * (size_t)k);
+  HIPCK(hipMemcpy(C.data(), s->mfc.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(R.data(), s->mfr.p, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(N.data(), s->mfn.p, k * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(O.data(), s->mfoff.p, 8 * (size_t)k * sizeof(float4), hipMemcpyDeviceToHost));
+  for (int p = 0; p < k; ++p) {
+    int c = C[p];
+    if (count) count[p] = c;
+    if (refa) refa[p] = c ? R[p] : 0;
+    v3 nn = c ? mk3(N[p].x, N[p].y, N[p].z) : mk3(0.f, 0.f, 0.f);
+    v3 tt = c ? orthonormal_basis(nn) : mk3(0.f, 0.f, 0.f);
+    v3 bb = c ? cross(nn, tt) : mk3(0.f, 0.f, 0.f);
+    if (n) { n[3 * p] = nn.x; n[3 * p + 1] = nn.y; n[3 * p + 2] = nn.z; }
+    if (t) { t[3 * p] = tt.x; t[3 * p + 1] = tt.y; t[3 * p + 2] = tt.z; }
+    if (b) { b[3 * p] = bb.x; b[3 * p + 1] = bb.y; b[3 * p + 2] = bb.z; }
+    for (int sl = 0; sl < 4; ++sl) {
+      float4 r = sl < c ? O[8 * (size_t)p + sl] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 i = sl < c ? O[8 * (size_t)p + 4 + sl] : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (offr) { offr[3 * (4 * p + sl)] = r.x; offr[3 * (4 * p + sl) + 1] = r.y; offr[3 * (4 * p + sl) + 2] = r.z; }
+      if (offi) { offi[3 * (4 * p + sl)] = i.x; offi[3 * (4 * p + sl) + 1] = i.y; offi[3 * (4 * p + sl) + 2] = i.z; }
+    }
+  }
+  return AVBD_OK;
+}
+
+int avbd_profile_enable(avbd_solver* s, int on) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  s->prof = on != 0;
+  for (int k = 0; k < kNumProf; ++k) {
+    s->prof_ms[k] = 0;
+    s->prof_launch[k] = 0;
+    s->prof_bytes[k] = 0;
+  }
+  return AVBD_OK;
+}
+
+int avbd_profile_read(avbd_solver* s, int nc, double* ms, long long* launches, double* bytes) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  for (int k = 0; k < nc && k < kNumProf; ++k) {
+    if (ms) ms[k] = s->prof_ms[k];
+    if (launches) launches[k] = s->prof_launch[k];
+    if (bytes) bytes[k] = s->prof_bytes[k];
+  }
+  return AVBD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,856 @@
+// kernels_contacts.hip — contact generation for gfx950.
+//
+//  k_aabb           compute_body_aabb_6dof (kernels_6dof.py:1922-1943)
+//  LBVH             replaces wp.Bvh + bvh_broadphase_pairs (solver_6dof.py:1819-1848,
+//                   kernels_6dof.py:1946-1978): 30-bit Morton codes, LSD radix
+//                   sort, Karras (2012) hierarchy, bottom-up refit, escape-link
+//                   (stackless) traversal. The pair list comes out in canonical
+//                   (a asc, b asc) order: per-body counts, exclusive scan,
+//                   per-body segment sort.
+//  k_narrowphase    obb_sat_pairs + obb_contact_manifold_6dof fused
+//                   (kernels_6dof.py:1991-2130, 2221-2537); the clip polygons
+//                   live in registers instead of a global scratch buffer.
+//  k_emit_*         gpu_pool_emit_rows (kernels_6dof.py:2695-2912) with the
+//                   serial row-reservation order restated as scans.
+//  k_hash_*         gpu_pool_hash_clear / collect (kernels_6dof.py:2915-2995);
+//                   duplicate keys resolve to the highest pool index, the
+//                   entry a serial upsert leaves behind.
+#include "avbd_internal.h"
+
+namespace avbd {
+
+void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                      int* vals_tmp, int n, void* ws);
+size_t radix_workspace_bytes(int n);
+
+__device__ __forceinline__ v3 f3(float4 a) { return mk3(a.x, a.y, a.z); }
+__device__ __forceinline__ q4 f4q(float4 a) { return mk4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ float4 p4(v3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+__device__ __forceinline__ v3 colm(const m33& R, int k) { return mk3(R.m[0][k], R.m[1][k], R.m[2][k]); }
+
+// ---------------------------------------------------------------------------
+// AABB
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_aabb(int nb, const float4* __restrict__ pose, const float4* __restrict__ bstat, float margin,
+       float4* __restrict__ lo, float4* __restrict__ hi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  v3 c = f3(pose[2 * i]);
+  float4 bs = bstat[i];
+  m33 R = qtomat(f4q(pose[2 * i + 1]));
+  float ex = ((fabs_(R.m[0][0]) * bs.y + fabs_(R.m[0][1]) * bs.z) + fabs_(R.m[0][2]) * bs.w) + margin;
+  float ey = ((fabs_(R.m[1][0]) * bs.y + fabs_(R.m[1][1]) * bs.z) + fabs_(R.m[1][2]) * bs.w) + margin;
+  float ez = ((fabs_(R.m[2][0]) * bs.y + fabs_(R.m[2][1]) * bs.z) + fabs_(R.m[2][2]) * bs.w) + margin;
+  lo[i] = p4(sub(c, mk3(ex, ey, ez)));
+  hi[i] = p4(add(c, mk3(ex, ey, ez)));
+}
+
+void launch_aabb(hipStream_t st, int nb, const float4* pose, const float4* bstat, float margin,
+                 float4* lo, float4* hi) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_aabb, dim3((nb + 255) / 256), dim3(256), 0, st, nb, pose, bstat, margin,
+                     lo, hi);
+}
+
+// ---------------------------------------------------------------------------
+// LBVH
+// ---------------------------------------------------------------------------
+struct BvhWs {
+  int n;
+  int* bounds;          // 6 ordered ints: min xyz, max xyz of centroids
+  unsigned* keys;       // n
+  int* ids;             // n  (sorted leaf -> body)
+  unsigned* keys_tmp;   // n
+  int* ids_tmp;         // n
+  int* left;            // n-1
+  int* right;           // n-1
+  int* parent;          // 2n-1
+  int* escape;          // 2n-1
+  int* flags;           // n-1
+  float4* nlo;          // 2n-1
+  float4* nhi;          // 2n-1
+  void* radix_ws;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static BvhWs bvh_carve(void* base, int n) {
+  BvhWs w;
+  w.n = n;
+  char* p = (char*)base;
+  auto take = [&](size_t bytes) { char* r = p; p += align256(bytes); return (void*)r; };
+  int nn = 2 * n - 1 > 0 ? 2 * n - 1 : 1;
+  int ni = n - 1 > 0 ? n - 1 : 1;
+  w.bounds = (int*)take(8 * sizeof(int));
+  w.keys = (unsigned*)take(n * sizeof(unsigned));
+  w.ids = (int*)take(n * sizeof(int));
+  w.keys_tmp = (unsigned*)take(n * sizeof(unsigned));
+  w.ids_tmp = (int*)take(n * sizeof(int));
+  w.left = (int*)take(ni * sizeof(int));
+  w.right = (int*)take(ni * sizeof(int));
+  w.parent = (int*)take(nn * sizeof(int));
+  w.escape = (int*)take(nn * sizeof(int));
+  w.flags = (int*)take(ni * sizeof(int));
+  w.nlo = (float4*)take(nn * sizeof(float4));
+  w.nhi = (float4*)take(nn * sizeof(float4));
+  w.radix_ws = (void*)p;
+  return w;
+}
+
+size_t bvh_workspace_bytes(int n) {
+  size_t nn = 2 * (size_t)n, b = 0;
+  b += align256(8 * sizeof(int));
+  b += 4 * align256(n * sizeof(int));
+  b += 3 * align256(n * sizeof(int));
+  b += 2 * align256(nn * sizeof(int));
+  b += 2 * align256(nn * sizeof(float4));
+  b += radix_workspace_bytes(n);
+  return b + 4096;
+}
+
+__device__ __forceinline__ int ord_f(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float unord_f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void k_bounds_init(int* b) {
+  if (threadIdx.x < 3) b[threadIdx.x] = 0x7fffffff;
+  else if (threadIdx.x < 6) b[threadIdx.x] = (int)0x80000000;
+}
+
+__global__ void __launch_bounds__(256)
+k_bounds(int n, const float4* __restrict__ lo, const float4* __restrict__ hi, int* b) {
+  __shared__ int smin[3][256], smax[3][256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int mx[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
+  if (i < n) {
+    float4 l = lo[i], h = hi[i];
+    float c[3] = {(l.x + h.x) * 0.5f, (l.y + h.y) * 0.5f, (l.z + h.z) * 0.5f};
+    for (int k = 0; k < 3; ++k) { mn[k] = ord_f(c[k]); mx[k] = mn[k]; }
+  }
+  for (int k = 0; k < 3; ++k) { smin[k][threadIdx.x] = mn[k]; smax[k][threadIdx.x] = mx[k]; }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int k = 0; k < 3; ++k) {
+        smin[k][threadIdx.x] = min(smin[k][threadIdx.x], smin[k][threadIdx.x + s]);
+        smax[k][threadIdx.x] = max(smax[k][threadIdx.x], smax[k][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&b[k], smin[k][0]);
+      atomicMax(&b[3 + k], smax[k][0]);
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned expand10(unsigned v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+k_morton(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+         const int* __restrict__ b, unsigned* __restrict__ keys, int* __restrict__ ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 l = lo[i], h = hi[i];
+  float c[3] = {(l.x + h.x) * 0.5f, (l.y + h.y) * 0.5f, (l.z + h.z) * 0.5f};
+  unsigned q[3];
+  for (int k = 0; k < 3; ++k) {
+    float mn = unord_f(b[k]), mx = unord_f(b[3 + k]);
+    float ext = mx - mn;
+    float u = ext > 0.0f ? (c[k] - mn) / ext : 0.0f;
+    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    q[k] = (unsigned)u;
+  }
+  keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  ids[i] = i;
+}
+
+__device__ __forceinline__ int bvh_delta(const unsigned* keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  unsigned a = keys[i], b = keys[j];
+  if (a == b) return 32 + __clz((unsigned)(i ^ j));
+  return __clz(a ^ b);
+}
+
+// Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees
+// and k-d trees": internal node i covers a key range found by a binary search.
+// Node ids: internal 0..n-2 (root 0), leaf k -> n-1+k.
+__global__ void __launch_bounds__(256)
+k_karras(int n, const unsigned* __restrict__ keys, int* __restrict__ left,
+         int* __restrict__ right, int* __restrict__ parent, int* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  flags[i] = 0;
+  int d = bvh_delta(keys, n, i, i + 1) - bvh_delta(keys, n, i, i - 1) > 0 ? 1 : -1;
+  int dmin = bvh_delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (bvh_delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (bvh_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = bvh_delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (bvh_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int gamma = i + s * d + (d < 0 ? -1 : 0);
+  int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
+  int L = (lo_i == gamma) ? (n - 1 + gamma) : gamma;
+  int R = (hi_i == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+  left[i] = L;
+  right[i] = R;
+  parent[L] = i;
+  parent[R] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+__device__ __forceinline__ float4 ld_coherent(const float4* p) {
+  const int* q = reinterpret_cast<const int*>(p);
+  float x = __int_as_float(__hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  float y = __int_as_float(__hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  float z = __int_as_float(__hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  return make_float4(x, y, z, 0.0f);
+}
+
+// Bottom-up union of child boxes; the second arrival at a node continues.
+__global__ void __launch_bounds__(256)
+k_refit(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+        const int* __restrict__ ids, const int* __restrict__ left, const int* __restrict__ right,
+        const int* __restrict__ parent, int* flags, float4* nlo, float4* nhi) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int node = n - 1 + k;
+  int body = ids[k];
+  nlo[node] = lo[body];
+  nhi[node] = hi[body];
+  if (n == 1) return;
+  __threadfence();
+  int p = parent[node];
+  for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
+    int prev = atomicAdd(&flags[p], 1);
+    if (prev == 0) return;
+    __threadfence();
+    int L = left[p], R = right[p];
+    // Children may have been written by other CUs: read through L2.
+    float4 a = ld_coherent(&nlo[L]), b = ld_coherent(&nlo[R]);
+    float4 c = ld_coherent(&nhi[L]), e = ld_coherent(&nhi[R]);
+    nlo[p] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+    nhi[p] = make_float4(fmaxf(c.x, e.x), fmaxf(c.y, e.y), fmaxf(c.z, e.z), 0.0f);
+    __threadfence();
+    p = parent[p];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_escape(int n, const int* __restrict__ left, const int* __restrict__ right,
+         const int* __restrict__ parent, int* __restrict__ escape) {
+  int node = blockIdx.x * blockDim.x + threadIdx.x;
+  int nn = 2 * n - 1;
+  if (node >= nn) return;
+  int e = node, out = -1;
+  for (int guard = 0; e != 0 && guard < nn; ++guard) {
+    int p = parent[e];
+    if (p < 0 || p >= n - 1) break;
+    if (left[p] == e) { out = right[p]; break; }
+    e = p;
+  }
+  escape[node] = out;
+}
+
+int bvh_build(hipStream_t st, int n, const float4* lo, const float4* hi, void* ws, size_t) {
+  BvhWs w = bvh_carve(ws, n);
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, w.bounds);
+  hipLaunchKernelGGL(k_bounds, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.bounds);
+  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.bounds, w.keys,
+                     w.ids);
+  radix_sort_pairs(st, w.keys, w.ids, w.keys_tmp, w.ids_tmp, n, w.radix_ws);
+  if (n > 1)
+    hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, w.keys, w.left,
+                       w.right, w.parent, w.flags);
+  hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.ids, w.left,
+                     w.right, w.parent, w.flags, w.nlo, w.nhi);
+  if (n > 1)
+    hipLaunchKernelGGL(k_escape, dim3((2 * n - 1 + 255) / 256), dim3(256), 0, st, n, w.left,
+                       w.right, w.parent, w.escape);
+  return 0;
+}
+
+__device__ __forceinline__ bool box_overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  return alo.x <= bhi.x && blo.x <= ahi.x && alo.y <= bhi.y && blo.y <= ahi.y &&
+         alo.z <= bhi.z && blo.z <= ahi.z;
+}
+
+// Traverse for body i; visit(j) for every body j > i with overlapping AABB that
+// is not static-static. Returns the number visited.
+template <bool kWrite>
+__device__ int bvh_query(int n, int i, const float4* __restrict__ lo, const float4* __restrict__ hi,
+                         const float4* __restrict__ bstat, const BvhWs& w, int* out, int cap_left) {
+  float4 qlo = lo[i], qhi = hi[i];
+  bool st_i = bstat[i].x <= 0.0f;
+  int cnt = 0;
+  if (n == 1) return 0;
+  int node = 0;
+  int steps = 0, max_steps = 4 * n + 8;   // a DFS visits 2n-1 nodes once
+  while (node != -1 && ++steps <= max_steps) {
+    if (node >= n - 1) {
+      int j = w.ids[node - (n - 1)];
+      if (j > i && box_overlap(qlo, qhi, lo[j], hi[j]) && !(st_i && bstat[j].x <= 0.0f)) {
+        if (kWrite && cnt < cap_left) out[cnt] = j;
+        ++cnt;
+      }
+      node = w.escape[node];
+    } else if (box_overlap(qlo, qhi, w.nlo[node], w.nhi[node])) {
+      node = w.left[node];
+    } else {
+      node = w.escape[node];
+    }
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(256)
+k_bvh_count(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+            const float4* __restrict__ bstat, BvhWs w, int* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  counts[i] = bvh_query<false>(n, i, lo, hi, bstat, w, nullptr, 0);
+}
+
+// Smallest overlapping partner of body i that is greater than `after`.
+__device__ int bvh_next_partner(int n, int i, int after, const float4* __restrict__ lo,
+                                const float4* __restrict__ hi, const float4* __restrict__ bstat,
+                                const BvhWs& w) {
+  float4 qlo = lo[i], qhi = hi[i];
+  bool st_i = bstat[i].x <= 0.0f;
+  int best = 0x7fffffff;
+  int node = 0;
+  int steps = 0, max_steps = 4 * n + 8;
+  while (node != -1 && ++steps <= max_steps) {
+    if (node >= n - 1) {
+      int j = w.ids[node - (n - 1)];
+      if (j > i && j > after && j < best && box_overlap(qlo, qhi, lo[j], hi[j]) &&
+          !(st_i && bstat[j].x <= 0.0f))
+        best = j;
+      node = w.escape[node];
+    } else if (box_overlap(qlo, qhi, w.nlo[node], w.nhi[node])) {
+      node = w.left[node];
+    } else {
+      node = w.escape[node];
+    }
+  }
+  return best;
+}
+
+__global__ void __launch_bounds__(256)
+k_bvh_write(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+            const float4* __restrict__ bstat, BvhWs w, const int* __restrict__ counts,
+            const int* __restrict__ offsets, int cap, int* __restrict__ pa,
+            int* __restrict__ pb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int off = offsets[i];
+  int room = cap - off;
+  if (room <= 0 || n == 1) return;
+  int cnt = counts[i];
+  int* seg = pb + off;
+  if (cnt <= room) {
+    bvh_query<true>(n, i, lo, hi, bstat, w, seg, cnt);
+    // Canonical order: this body's partners ascending (insertion sort).
+    for (int a = 1; a < cnt; ++a) {
+      int v = seg[a];
+      int b = a - 1;
+      while (b >= 0 && seg[b] > v) { seg[b + 1] = seg[b]; --b; }
+      seg[b + 1] = v;
+    }
+    for (int a = 0; a < cnt; ++a) pa[off + a] = i;
+  } else {
+    // Pair pool ends inside this body's segment: keep its `room` smallest
+    // partners, the prefix of the canonical list.
+    int last = -1;
+    for (int a = 0; a < room; ++a) {
+      last = bvh_next_partner(n, i, last, lo, hi, bstat, w);
+      seg[a] = last;
+      pa[off + a] = i;
+    }
+  }
+}
+
+void bvh_count_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
+                     const float4* bstat, const void* ws, int* counts) {
+  BvhWs w = bvh_carve(const_cast<void*>(ws), n);
+  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, bstat, w,
+                     counts);
+}
+
+void bvh_write_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
+                     const float4* bstat, const void* ws, const int* counts, const int* offsets,
+                     int cap, int* pa, int* pb) {
+  BvhWs w = bvh_carve(const_cast<void*>(ws), n);
+  hipLaunchKernelGGL(k_bvh_write, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, bstat, w,
+                     counts, offsets, cap, pa, pb);
+}
+
+// ---------------------------------------------------------------------------
+// Narrow phase: SAT + manifold per pair
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float proj_radius(v3 L, v3 c0, v3 c1, v3 c2, v3 e) {
+  return (e.x * fabs_(dot(L, c0)) + e.y * fabs_(dot(L, c1))) + e.z * fabs_(dot(L, c2));
+}
+
+__global__ void __launch_bounds__(128)
+k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __restrict__ bstat,
+              const int* __restrict__ pair_a, const int* __restrict__ pair_b, float margin,
+              int* __restrict__ overlap, int* __restrict__ sat_out, float4* __restrict__ n_out,
+              float* __restrict__ depth_out, int* __restrict__ mf_count,
+              int* __restrict__ mf_refa, float4* __restrict__ mf_off, float4* __restrict__ mf_n) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  int i = pair_a[p], j = pair_b[p];
+  v3 cA = f3(pose[2 * i]), cB = f3(pose[2 * j]);
+  float4 bsA = bstat[i], bsB = bstat[j];
+  v3 eA = mk3(bsA.y, bsA.z, bsA.w), eB = mk3(bsB.y, bsB.z, bsB.w);
+  m33 RA = qtomat(f4q(pose[2 * i + 1])), RB = qtomat(f4q(pose[2 * j + 1]));
+  v3 t = sub(cB, cA);
+  v3 A[3] = {colm(RA, 0), colm(RA, 1), colm(RA, 2)};
+  v3 Bc[3] = {colm(RB, 0), colm(RB, 1), colm(RB, 2)};
+  float best = 1.0e20f;
+  int best_idx = -1;
+  v3 best_axis = mk3(0.f, 0.f, 0.f);
+  mf_count[p] = 0;
+  bool sep = false;
+  for (int k = 0; k < 3 && !sep; ++k) {
+    v3 L = A[k];
+    float ov = (comp(eA, k) + proj_radius(L, Bc[0], Bc[1], Bc[2], eB)) - fabs_(dot(t, L));
+    float tdl = dot(t, L);
+    if (ov < -margin) { sep = true; break; }
+    if (ov < best) { best = ov; best_idx = k; best_axis = tdl > 0.0f ? neg(L) : L; }
+  }
+  for (int k = 0; k < 3 && !sep; ++k) {
+    v3 L = Bc[k];
+    float ov = (proj_radius(L, A[0], A[1], A[2], eA) + comp(eB, k)) - fabs_(dot(t, L));
+    float tdl = dot(t, L);
+    if (ov < -margin) { sep = true; break; }
+    if (ov < best) { best = ov; best_idx = 3 + k; best_axis = tdl > 0.0f ? neg(L) : L; }
+  }
+  for (int ie = 0; ie < 3 && !sep; ++ie) {
+    for (int je = 0; je < 3; ++je) {
+      v3 L = cross(A[ie], Bc[je]);
+      float nl = length(L);
+      if (nl < 1.0e-6f) continue;
+      L = divs(L, nl);
+      float ov = (proj_radius(L, A[0], A[1], A[2], eA) + proj_radius(L, Bc[0], Bc[1], Bc[2], eB)) -
+                 fabs_(dot(t, L));
+      float tdl = dot(t, L);
+      if (ov < 0.0f) { sep = true; break; }
+      if (ov < best) { best = ov; best_idx = 6 + 3 * ie + je; best_axis = tdl > 0.0f ? neg(L) : L; }
+    }
+  }
+  if (sep) {
+    overlap[p] = 0;
+    return;
+  }
+  overlap[p] = 1;
+  sat_out[p] = best_idx;
+  n_out[p] = p4(best_axis);
+  depth_out[p] = best;
+
+  const int sat = best_idx;
+  const v3 n_in = best_axis;
+  float4* off = mf_off + 8 * (size_t)p;
+  if (sat >= 6) {
+    int e = sat - 6, kA = e / 3, kB = e % 3;
+    v3 dA = colm(RA, kA), dB = colm(RB, kB);
+    if (length(cross(dA, dB)) < 1.0e-4f) return;
+    int kA1 = (kA + 1) % 3, kA2 = (kA + 2) % 3;
+    v3 A1 = colm(RA, kA1), A2 = colm(RA, kA2);
+    v3 toB = sub(cB, cA);
+    float sbA = dot(toB, A1) >= 0.0f ? 1.0f : -1.0f;
+    float scA = dot(toB, A2) >= 0.0f ? 1.0f : -1.0f;
+    v3 midA = add(add(cA, mul(A1, sbA * comp(eA, kA1))), mul(A2, scA * comp(eA, kA2)));
+    v3 P1 = sub(midA, mul(dA, comp(eA, kA)));
+    v3 Q1 = add(midA, mul(dA, comp(eA, kA)));
+    int kB1 = (kB + 1) % 3, kB2 = (kB + 2) % 3;
+    v3 B1 = colm(RB, kB1), B2 = colm(RB, kB2);
+    v3 toA = sub(cA, cB);
+    float sbB = dot(toA, B1) >= 0.0f ? 1.0f : -1.0f;
+    float scB = dot(toA, B2) >= 0.0f ? 1.0f : -1.0f;
+    v3 midB = add(add(cB, mul(B1, sbB * comp(eB, kB1))), mul(B2, scB * comp(eB, kB2)));
+    v3 P2 = sub(midB, mul(dB, comp(eB, kB)));
+    v3 Q2 = add(midB, mul(dB, comp(eB, kB)));
+    v3 d1 = sub(Q1, P1), d2 = sub(Q2, P2), r = sub(P1, P2);
+    float a = dot(d1, d1), ev = dot(d2, d2), f = dot(d2, r);
+    const float eps = 1.0e-12f;
+    float sp = 0.0f, tp = 0.0f;
+    if (a <= eps && ev <= eps) {
+      sp = 0.0f; tp = 0.0f;
+    } else if (a <= eps) {
+      sp = 0.0f;
+      tp = clampf_(f / fmax_(ev, eps), 0.0f, 1.0f);
+    } else if (ev <= eps) {
+      tp = 0.0f;
+      float c = dot(d1, r);
+      sp = clampf_(-c / a, 0.0f, 1.0f);
+    } else {
+      float c = dot(d1, r);
+      float b = dot(d1, d2);
+      float denom = a * ev - b * b;
+      if (denom != 0.0f) sp = clampf_((b * f - c * ev) / denom, 0.0f, 1.0f);
+      else sp = 0.0f;
+      tp = (b * sp + f) / ev;
+      if (tp < 0.0f) {
+        tp = 0.0f;
+        sp = clampf_(-c / a, 0.0f, 1.0f);
+      } else if (tp > 1.0f) {
+        tp = 1.0f;
+        sp = clampf_((b - c) / a, 0.0f, 1.0f);
+      }
+    }
+    v3 pA = add(P1, mul(d1, sp));
+    v3 pB = add(P2, mul(d2, tp));
+    float gap = dot(sub(pA, pB), n_in);
+    if (gap > margin) return;
+    off[0] = p4(mvmul(mT(RA), sub(pA, cA)));
+    off[4] = p4(mvmul(mT(RB), sub(pB, cB)));
+    mf_refa[p] = 1;
+    mf_n[p] = p4(n_in);
+    mf_count[p] = 1;
+    return;
+  }
+  int ref_is_a = sat < 3 ? 1 : 0;
+  int ref_axis;
+  v3 n_used, ref_c, inc_c, e_ref, e_inc;
+  m33 R_ref, R_inc;
+  if (sat < 3) {
+    ref_axis = sat; n_used = n_in; ref_c = cA; R_ref = RA; e_ref = eA; inc_c = cB; R_inc = RB; e_inc = eB;
+  } else {
+    ref_axis = sat - 3; n_used = neg(n_in); ref_c = cB; R_ref = RB; e_ref = eB; inc_c = cA; R_inc = RA; e_inc = eA;
+  }
+  v3 rcol = colm(R_ref, ref_axis);
+  float rsign = dot(rcol, n_used) > 0.0f ? -1.0f : 1.0f;
+  v3 rfn = mul(rcol, rsign);
+  v3 rfc = add(ref_c, mul(rfn, comp(e_ref, ref_axis)));
+  int ax1 = (ref_axis + 1) % 3, ax2 = (ref_axis + 2) % 3;
+  v3 u_ref = colm(R_ref, ax1), v_ref = colm(R_ref, ax2);
+  float eu = comp(e_ref, ax1), evv = comp(e_ref, ax2);
+  int inc_axis = 0;
+  float inc_sign = 1.0f, best_dot = -1.0e20f;
+  for (int k = 0; k < 3; ++k) {
+    float dpos = dot(colm(R_inc, k), n_used);
+    if (dpos > best_dot) { best_dot = dpos; inc_axis = k; inc_sign = 1.0f; }
+    float dneg = -dpos;
+    if (dneg > best_dot) { best_dot = dneg; inc_axis = k; inc_sign = -1.0f; }
+  }
+  v3 ifn = mul(colm(R_inc, inc_axis), inc_sign);
+  v3 ifc = add(inc_c, mul(ifn, comp(e_inc, inc_axis)));
+  int iax1 = (inc_axis + 1) % 3, iax2 = (inc_axis + 2) % 3;
+  v3 iu = colm(R_inc, iax1), iv = colm(R_inc, iax2);
+  float ieu = comp(e_inc, iax1), iev = comp(e_inc, iax2);
+  v3 buf[2][8];
+  buf[0][0] = add(add(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][1] = add(sub(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][2] = sub(sub(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][3] = sub(add(ifc, mul(iu, ieu)), mul(iv, iev));
+  int cur = 0, len = 4;
+  for (int pl = 0; pl < 4; ++pl) {
+    v3 ppt, pn;
+    if (pl == 0) { ppt = add(rfc, mul(u_ref, eu)); pn = u_ref; }
+    else if (pl == 1) { ppt = sub(rfc, mul(u_ref, eu)); pn = neg(u_ref); }
+    else if (pl == 2) { ppt = add(rfc, mul(v_ref, evv)); pn = v_ref; }
+    else { ppt = sub(rfc, mul(v_ref, evv)); pn = neg(v_ref); }
+    if (len == 0) return;
+    int out = 0;
+    for (int iv2 = 0; iv2 < len; ++iv2) {
+      v3 ap = buf[cur][iv2];
+      v3 bp = buf[cur][(iv2 + 1) % len];
+      float da = dot(sub(ap, ppt), pn);
+      float db = dot(sub(bp, ppt), pn);
+      if (da <= 0.0f) {
+        if (out < 8) buf[1 - cur][out++] = ap;
+        if (db > 0.0f) {
+          float tt = da / (da - db);
+          if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+        }
+      } else if (db <= 0.0f) {
+        float tt = da / (da - db);
+        if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+      }
+    }
+    cur = 1 - cur;
+    len = out;
+  }
+  if (len == 0) return;
+  float bd[4] = {-1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
+  int bi[4] = {-1, -1, -1, -1};
+  int nbest = 0;
+  for (int iv2 = 0; iv2 < len; ++iv2) {
+    float ds = dot(sub(buf[cur][iv2], rfc), rfn);
+    if (ds < margin) {
+      float depth = -ds;
+      if (nbest < 4) {
+        bd[nbest] = depth; bi[nbest] = iv2; ++nbest;
+      } else {
+        int mp = 0;
+        float mv = bd[0];
+        for (int k = 1; k < 4; ++k) if (bd[k] < mv) { mv = bd[k]; mp = k; }
+        if (depth > mv) { bd[mp] = depth; bi[mp] = iv2; }
+      }
+    }
+  }
+  if (nbest == 0) return;
+  for (int is = 0; is < nbest - 1; ++is) {
+    int mp = is;
+    for (int k = is + 1; k < nbest; ++k) if (bd[k] > bd[mp]) mp = k;
+    if (mp != is) {
+      float td = bd[is]; bd[is] = bd[mp]; bd[mp] = td;
+      int ti = bi[is]; bi[is] = bi[mp]; bi[mp] = ti;
+    }
+  }
+  m33 Rtr = mT(R_ref), Rti = mT(R_inc);
+  for (int c = 0; c < nbest; ++c) {
+    v3 pinc = buf[cur][bi[c]];
+    float ds = dot(sub(pinc, rfc), rfn);
+    v3 pref = sub(pinc, mul(rfn, ds));
+    off[c] = p4(mvmul(Rtr, sub(pref, ref_c)));
+    off[4 + c] = p4(mvmul(Rti, sub(pinc, inc_c)));
+  }
+  mf_refa[p] = ref_is_a;
+  mf_n[p] = p4(n_used);
+  mf_count[p] = nbest;
+}
+
+void launch_narrowphase(hipStream_t st, int n_pairs, const float4* pose, const float4* bstat,
+                        const int* pair_a, const int* pair_b, float margin, int* overlap,
+                        int* sat_idx, float4* n_hat, float* depth, int* mf_count, int* mf_refa,
+                        float4* mf_off, float4* mf_n) {
+  if (n_pairs <= 0) return;
+  hipLaunchKernelGGL(k_narrowphase, dim3((n_pairs + 127) / 128), dim3(128), 0, st, n_pairs, pose,
+                     bstat, pair_a, pair_b, margin, overlap, sat_idx, n_hat, depth, mf_count,
+                     mf_refa, mf_off, mf_n);
+}
+
+// ---------------------------------------------------------------------------
+// Row emission
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pair_mu(float mr, float mi, float default_mu) {
+  float mu = default_mu;
+  if (mr > 0.0f || mi > 0.0f) mu = sqrt_(mr * mi);
+  return mu;
+}
+
+__global__ void __launch_bounds__(256)
+k_emit_demand(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
+              const int* __restrict__ mf_count, const int* __restrict__ mf_refa,
+              const float* __restrict__ bfric, float default_mu, int* __restrict__ demand,
+              int* __restrict__ rpc_out) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  int nc = mf_count[p];
+  int rpc = 3;
+  if (nc > 0) {
+    int ref = mf_refa[p] == 1 ? pa[p] : pb[p];
+    int inc = mf_refa[p] == 1 ? pb[p] : pa[p];
+    float mu = pair_mu(bfric[ref], bfric[inc], default_mu);
+    rpc = mu <= 0.0f ? 1 : 3;
+  }
+  rpc_out[p] = rpc;
+  demand[p] = nc * rpc;
+}
+
+void launch_emit_demand(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                        const int* mf_count, const int* mf_refa, const float* bfric,
+                        float default_mu, int* demand, int* rpc_out) {
+  if (n_pairs <= 0) return;
+  hipLaunchKernelGGL(k_emit_demand, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, pa, pb,
+                     mf_count, mf_refa, bfric, default_mu, demand, rpc_out);
+}
+
+// Serial reservation order restated: a contact (p, c) fits iff every earlier
+// contact fit and its rows end within capacity; after the first failure every
+// later pair with contacts still reserves the rows of one failed attempt.
+__global__ void __launch_bounds__(256)
+k_emit_succ(int n_pairs, const int* __restrict__ mf_count, const int* __restrict__ rpc,
+            const int* __restrict__ dscan, int avail, int* __restrict__ succ,
+            int* __restrict__ reserve) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  int nc = mf_count[p], r = rpc[p];
+  int s = 0;
+  if (nc > 0) {
+    long long room = (long long)avail - (long long)dscan[p];
+    if (room > 0) {
+      long long k = room / r;
+      s = k < nc ? (int)k : nc;
+    }
+  }
+  succ[p] = s;
+  reserve[p] = s * r + (s < nc ? r : 0);
+}
+
+void launch_emit_succ(hipStream_t st, int n_pairs, const int* mf_count, const int* rpc,
+                      const int* dscan, int avail, int* succ, int* reserve) {
+  if (n_pairs <= 0) return;
+  hipLaunchKernelGGL(k_emit_succ, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, mf_count,
+                     rpc, dscan, avail, succ, reserve);
+}
+
+__global__ void __launch_bounds__(256)
+k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
+             const int* __restrict__ mf_refa, const float4* __restrict__ mf_n,
+             const float4* __restrict__ mf_off, const float* __restrict__ bfric, float default_mu,
+             float mult, const int* __restrict__ rpc, const int* __restrict__ succ,
+             const int* __restrict__ dscan, const int* __restrict__ sscan, int n_static_rows,
+             int g_static, GroupGeo* __restrict__ geo, GroupSt* __restrict__ gst, int hash_cap,
+             const int* __restrict__ hkeys, const float4* __restrict__ hstate) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  int ns = succ[p];
+  if (ns == 0) return;
+  int ref = mf_refa[p] == 1 ? pa[p] : pb[p];
+  int inc = mf_refa[p] == 1 ? pb[p] : pa[p];
+  float mu = pair_mu(bfric[ref], bfric[inc], default_mu);
+  float mus = mu * mult;
+  int r = rpc[p];
+  int ka = ref < inc ? ref : inc, kb = ref < inc ? inc : ref;
+  float4 n = mf_n[p];
+  const float4* off = mf_off + 8 * (size_t)p;
+  int gbase = g_static + sscan[p];
+  int rbase = n_static_rows + dscan[p];
+  for (int c = 0; c < ns; ++c) {
+    GroupGeo G;
+    G.hdr = make_int4(g_make(kGBox, r, c), ref, inc, rbase + c * r);
+    G.g0 = make_float4(n.x, n.y, n.z, kInf);
+    float4 oa = off[c], ob = off[4 + c];
+    G.g1 = make_float4(oa.x, oa.y, oa.z, mu);
+    G.g2 = make_float4(ob.x, ob.y, ob.z, mus);
+    float ln = 0.f, lt = 0.f, lb = 0.f, kn = kPenaltyMin, kt = kPenaltyMinTangent,
+          kbb = kPenaltyMinTangent;
+    int wt = 0, wb = 0;
+    if (hash_cap > 0) {
+      int key = encode_pair_key(ka, kb, c);
+      int h = hash_pair_slot(key, hash_cap);
+      for (int probe = 0; probe < hash_cap; ++probe) {
+        int slot = (h + probe) % hash_cap;
+        int sk = hkeys[slot];
+        if (sk == key) {
+          float4 s0 = hstate[2 * (size_t)slot], s1 = hstate[2 * (size_t)slot + 1];
+          ln = s0.x; lt = s0.y; lb = s0.z; kn = s0.w; kt = s1.x; kbb = s1.y;
+          wt = (int)s1.z; wb = (int)s1.w;
+          break;
+        }
+        if (sk == kHashEmpty) break;
+      }
+    }
+    int bits = kActN;
+    if (r == 3) {
+      bits |= kActT | kActB;
+      if (wt) bits |= kWasT;
+      if (wb) bits |= kWasB;
+    }
+    GroupSt S;
+    S.s0 = make_float4(ln, r == 3 ? lt : 0.0f, r == 3 ? lb : 0.0f, 0.0f);
+    S.s1 = make_float4(kn, r == 3 ? kt : 0.0f, r == 3 ? kbb : 0.0f, 0.0f);
+    S.s2 = make_float4(0.0f, __int_as_float(bits), 0.0f, 0.0f);
+    geo[gbase + c] = G;
+    gst[gbase + c] = S;
+  }
+}
+
+void launch_emit_write(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                       const int* mf_refa, const float4* mf_n, const float4* mf_off,
+                       const float* bfric, float default_mu, float mult, const int* rpc,
+                       const int* succ, const int* dscan, const int* sscan, int n_static_rows,
+                       int g_static, GroupGeo* geo, GroupSt* gst, int hash_cap, const int* hkeys,
+                       const float4* hstate) {
+  if (n_pairs <= 0) return;
+  hipLaunchKernelGGL(k_emit_write, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, pa, pb,
+                     mf_refa, mf_n, mf_off, bfric, default_mu, mult, rpc, succ, dscan, sscan,
+                     n_static_rows, g_static, geo, gst, hash_cap, hkeys, hstate);
+}
+
+// ---------------------------------------------------------------------------
+// Warm-start cache refresh
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fill_int(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_int(hipStream_t st, int* p, int n, int v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+__global__ void __launch_bounds__(256)
+k_hash_insert(int n, int g_first, const GroupGeo* __restrict__ geo,
+              const GroupSt* __restrict__ gst, int hash_cap, int* hkeys, int* howner,
+              int* slot_of) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  slot_of[e] = -1;
+  GroupSt S = gst[g_first + e];
+  int bits = __float_as_int(S.s2.y);
+  if (!(bits & kActN)) return;
+  int4 hd = geo[g_first + e].hdr;
+  int a = hd.y, b = hd.z;
+  int ka = a < b ? a : b, kb = a < b ? b : a;
+  int key = encode_pair_key(ka, kb, g_cidx(hd.x));
+  int h = hash_pair_slot(key, hash_cap);
+  for (int probe = 0; probe < hash_cap; ++probe) {
+    int slot = (h + probe) % hash_cap;
+    int prev = atomicCAS(&hkeys[slot], kHashEmpty, key);
+    if (prev == kHashEmpty || prev == key) {
+      slot_of[e] = slot;
+      atomicMax(&howner[slot], e);
+      return;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_hash_store(int n, int g_first, const GroupGeo* __restrict__ geo,
+             const GroupSt* __restrict__ gst, const int* __restrict__ howner,
+             const int* __restrict__ slot_of, float4* __restrict__ hstate) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int slot = slot_of[e];
+  if (slot < 0 || howner[slot] != e) return;
+  GroupSt S = gst[g_first + e];
+  int nr = g_nrows(geo[g_first + e].hdr.x);
+  int bits = __float_as_int(S.s2.y);
+  float lt = 0.f, lb = 0.f, kt = kPenaltyMinTangent, kb = kPenaltyMinTangent;
+  float wt = 0.f, wb = 0.f;
+  if (nr == 3) {
+    lt = S.s0.y; lb = S.s0.z; kt = S.s1.y; kb = S.s1.z;
+    wt = (bits & kWasT) ? 1.0f : 0.0f;
+    wb = (bits & kWasB) ? 1.0f : 0.0f;
+  }
+  hstate[2 * (size_t)slot] = make_float4(S.s0.x, lt, lb, S.s1.x);
+  hstate[2 * (size_t)slot + 1] = make_float4(kt, kb, wt, wb);
+}
+
+void launch_hash_collect(hipStream_t st, int n, int g_first, const GroupGeo* geo,
+                         const GroupSt* gst, int hash_cap, int* hkeys, int* howner, int* slot_of,
+                         float4* hstate) {
+  if (n <= 0 || hash_cap <= 0) return;
+  hipLaunchKernelGGL(k_hash_insert, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
+                     hash_cap, hkeys, howner, slot_of);
+  hipLaunchKernelGGL(k_hash_store, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
+                     howner, slot_of, hstate);
+}
+
+}  // namespace avbd

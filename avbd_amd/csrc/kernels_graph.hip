@@ -1,0 +1,537 @@
+// kernels_graph.hip — scans, radix sort, CSR and coloring for gfx950.
+//
+//  exclusive_scan       device-wide int scan (block scan + carry pass)
+//  radix_sort_pairs     LSD radix sort, 4-bit digits, stable ranks from
+//                       64-lane ballots (used for Morton codes and colors)
+//  csr_*                body -> contact-group CSR with each body's groups in
+//                       ascending order (gpu_csr_count/starts/scatter,
+//                       kernels_6dof.py:2626-2692, ascending as a serial
+//                       scatter leaves it)
+//  count_conflicts      gpu_count_color_conflicts (kernels_6dof.py:3313-3336)
+//  adj_*                gpu_body_adj_count/scatter (kernels_6dof.py:3045-3096)
+//  k_color_greedy       the serial effect of gpu_color_spec_assign
+//                       (kernels_6dof.py:3233-3262, greedy in body order) or
+//                       of JP_ROUNDS gpu_color_round passes (:3099-3148,
+//                       greedy in priority order, round index
+//                       r(i) = max_blockers r(j) + [j > i]) computed as a
+//                       dataflow: bodies are taken in processing order from a
+//                       ticket counter and each waits only on bodies earlier
+//                       in that order, so every wait ends.
+//  k_color_resolve      gpu_color_spec_resolve (kernels_6dof.py:3265-3296)
+#include "avbd_internal.h"
+
+namespace avbd {
+
+static size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------------------
+// scan
+// ---------------------------------------------------------------------------
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  int lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one int per thread (256 threads); returns the
+// exclusive prefix and writes the block total to *total.
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  __shared__ int wsum[4];
+  int lane = __lane_id(), wid = threadIdx.x >> 6;
+  int inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int pre = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wid) pre += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + inc - v;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_local(const int* __restrict__ in, int* __restrict__ out, int n, int* __restrict__ bsum) {
+  int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int tot;
+  int pre = block_excl_scan(s, &tot);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_sums(int* __restrict__ bsum, int nblk, int* __restrict__ total) {
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nblk; b0 += 256) {
+    int i = b0 + threadIdx.x;
+    int v = i < nblk ? bsum[i] : 0;
+    int tot;
+    int pre = block_excl_scan(v, &tot);
+    if (i < nblk) bsum[i] = pre + carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_add(int* __restrict__ out, int n, const int* __restrict__ bsum) {
+  int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+  int add = bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+size_t scan_workspace_bytes(int n) { return a256(((size_t)n / kScanTile + 2) * sizeof(int)); }
+
+void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws) {
+  if (n <= 0) {
+    if (total_dev) launch_fill_int(st, total_dev, 1, 0);
+    return;
+  }
+  int nblk = (n + kScanTile - 1) / kScanTile;
+  int* bsum = (int*)ws;
+  hipLaunchKernelGGL(k_scan_local, dim3(nblk), dim3(256), 0, st, in, out, n, bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, bsum, nblk, total_dev);
+  if (nblk > 1) hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(256), 0, st, out, n, bsum);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort (keys unsigned, values int), 4-bit digits
+// ---------------------------------------------------------------------------
+constexpr int kRadixTile = 1024;
+
+__global__ void __launch_bounds__(256)
+k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, int nblk,
+             int* __restrict__ hist) {
+  __shared__ int cnt[16];
+  if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kRadixTile;
+  for (int r = 0; r < 4; ++r) {
+    int idx = base + r * 256 + threadIdx.x;
+    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 15u], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256)
+k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin, int n, int shift,
+                int nblk, const int* __restrict__ hscan, unsigned* __restrict__ kout,
+                int* __restrict__ vout) {
+  __shared__ int running[16];
+  __shared__ int wcnt[4][16];
+  int lane = __lane_id(), wid = threadIdx.x >> 6;
+  if (threadIdx.x < 16) running[threadIdx.x] = hscan[threadIdx.x * nblk + blockIdx.x];
+  __syncthreads();
+  unsigned long long lt = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
+  int base = blockIdx.x * kRadixTile;
+  for (int r = 0; r < 4; ++r) {
+    int idx = base + r * 256 + threadIdx.x;
+    bool valid = idx < n;
+    unsigned key = valid ? kin[idx] : 0u;
+    int val = valid ? vin[idx] : 0;
+    int digit = valid ? (int)((key >> shift) & 15u) : 16;
+    int rank = 0;
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+      unsigned long long m = __ballot(digit == d);
+      if (digit == d) rank = __popcll(m & lt);
+      if (lane == 0) wcnt[wid][d] = __popcll(m);
+    }
+    __syncthreads();
+    if (valid) {
+      int pre = 0;
+      for (int w = 0; w < wid; ++w) pre += wcnt[w][digit];
+      int pos = running[digit] + pre + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      int d = threadIdx.x;
+      running[d] += wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d];
+    }
+    __syncthreads();
+  }
+}
+
+size_t radix_workspace_bytes(int n) {
+  int nblk = (n + kRadixTile - 1) / kRadixTile;
+  size_t h = (size_t)16 * nblk;
+  return 2 * a256(h * sizeof(int)) + scan_workspace_bytes((int)h) + 256;
+}
+
+static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                            int* vals_tmp, int n, int n_passes, void* ws) {
+  if (n <= 1) return;
+  int nblk = (n + kRadixTile - 1) / kRadixTile;
+  size_t h = (size_t)16 * nblk;
+  int* hist = (int*)ws;
+  int* hscan = (int*)((char*)ws + a256(h * sizeof(int)));
+  void* sws = (char*)ws + 2 * a256(h * sizeof(int));
+  unsigned* ka = keys;
+  int* va = vals;
+  unsigned* kb = keys_tmp;
+  int* vb = vals_tmp;
+  for (int pass = 0; pass < n_passes; ++pass) {
+    int shift = 4 * pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, nblk, hist);
+    exclusive_scan(st, hist, hscan, (int)h, nullptr, sws);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, st, ka, va, n, shift, nblk,
+                       hscan, kb, vb);
+    unsigned* tk = ka; ka = kb; kb = tk;
+    int* tv = va; va = vb; vb = tv;
+  }
+  if (ka != keys) {
+    hipMemcpyAsync(keys, ka, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(vals, va, n * sizeof(int), hipMemcpyDeviceToDevice, st);
+  }
+}
+
+void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                      int* vals_tmp, int n, void* ws) {
+  radix_sort_bits(st, keys, vals, keys_tmp, vals_tmp, n, 8, ws);
+}
+
+void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                      int* vals_tmp, int n, int key_bits, void* ws) {
+  radix_sort_bits(st, keys, vals, keys_tmp, vals_tmp, n, (key_bits + 3) / 4, ws);
+}
+
+// ---------------------------------------------------------------------------
+// body -> group CSR
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_csr_init(int nb, const int* __restrict__ st_count, int* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) counts[i] = st_count[i];
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_count_dyn(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, int* counts) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  int4 hd = geo[g_first + e].hdr;
+  atomicAdd(&counts[hd.y], 1);
+  atomicAdd(&counts[hd.z], 1);
+}
+
+void launch_csr_count(hipStream_t st, int nb, const int* st_count, int n_dyn, int g_first,
+                      const GroupGeo* geo, int* counts) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_csr_init, dim3((nb + 255) / 256), dim3(256), 0, st, nb, st_count, counts);
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_csr_count_dyn, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
+                       g_first, geo, counts);
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_fill_static(int nb, const int* __restrict__ st_start, const int* __restrict__ st_idx,
+                  const int* __restrict__ starts, int* __restrict__ cursor, int* __restrict__ csr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  int s0 = st_start[i], s1 = st_start[i + 1], o = starts[i];
+  for (int k = s0; k < s1; ++k) csr[o + (k - s0)] = st_idx[k];
+  cursor[i] = o + (s1 - s0);
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_fill_dyn(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, int* cursor, int* csr) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  int g = g_first + e;
+  int4 hd = geo[g].hdr;
+  csr[atomicAdd(&cursor[hd.y], 1)] = g;
+  csr[atomicAdd(&cursor[hd.z], 1)] = g;
+}
+
+void launch_csr_fill(hipStream_t st, int nb, const int* st_start, const int* st_idx,
+                     const int* starts, int* cursor, int n_dyn, int g_first, const GroupGeo* geo,
+                     int* csr_idx) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_csr_fill_static, dim3((nb + 255) / 256), dim3(256), 0, st, nb, st_start,
+                       st_idx, starts, cursor, csr_idx);
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_csr_fill_dyn, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn, g_first,
+                       geo, cursor, csr_idx);
+}
+
+// Sort each body's dynamic tail [starts[i] + st_count[i], starts[i+1]) ascending.
+__global__ void __launch_bounds__(256)
+k_csr_sort(int nb, const int* __restrict__ starts, const int* __restrict__ st_count,
+           int* __restrict__ csr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  int b0 = starts[i] + st_count[i], b1 = starts[i + 1];
+  for (int a = b0 + 1; a < b1; ++a) {
+    int v = csr[a];
+    int b = a - 1;
+    while (b >= b0 && csr[b] > v) { csr[b + 1] = csr[b]; --b; }
+    csr[b + 1] = v;
+  }
+}
+
+void launch_csr_sort(hipStream_t st, int nb, const int* starts, const int* st_count,
+                     int* csr_idx) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_csr_sort, dim3((nb + 255) / 256), dim3(256), 0, st, nb, starts, st_count,
+                       csr_idx);
+}
+
+// ---------------------------------------------------------------------------
+// coloring
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_count_conflicts(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+                  const int* __restrict__ color, int* out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = 0;
+  if (e < n_dyn) {
+    int4 hd = geo[g_first + e].hdr;
+    if (hd.y >= 0 && hd.z >= 0 && hd.y != hd.z && color[hd.y] == color[hd.z])
+      c = g_nrows(hd.x);
+  }
+  // wave-level sum, one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if (__lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                            const int* color, int* out) {
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_count_conflicts, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
+                       g_first, geo, color, out);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_count(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, int* counts) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  int4 hd = geo[g_first + e].hdr;
+  if (hd.y < 0 || hd.z < 0 || hd.y == hd.z) return;
+  atomicAdd(&counts[hd.y], 1);
+  atomicAdd(&counts[hd.z], 1);
+}
+
+void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo, int* counts) {
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_adj_count, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn, g_first,
+                       geo, counts);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_fill(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, const int* __restrict__ starts,
+           int* cursor, int* nbr) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  int4 hd = geo[g_first + e].hdr;
+  int a = hd.y, b = hd.z;
+  if (a < 0 || b < 0 || a == b) return;
+  nbr[starts[a] + atomicAdd(&cursor[a], 1)] = b;
+  nbr[starts[b] + atomicAdd(&cursor[b], 1)] = a;
+}
+
+void launch_adj_fill(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                     const int* starts, int* cursor, int* nbr) {
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_adj_fill, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn, g_first, geo,
+                       starts, cursor, nbr);
+}
+
+__device__ __forceinline__ int lowest_free_color(unsigned used) {
+  unsigned fr = ~used;
+  return fr ? (__ffs((int)fr) - 1) : (kMaxColors - 1);
+}
+
+__device__ __forceinline__ int ld_acq(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// pub[i]: -1 until body i is resolved. Jacobi: the chosen color. JP:
+// (min(round, 24) << 8) | (color + 1), color = -1 when round >= 24.
+__global__ void __launch_bounds__(256)
+k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ nbr_start,
+               const int* __restrict__ nbr, const float* __restrict__ prio,
+               const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp,
+               int* err) {
+  __shared__ int base_s;
+  for (;;) {
+    if (threadIdx.x == 0) base_s = atomicAdd(ticket, 256);
+    __syncthreads();
+    int base = base_s;
+    __syncthreads();
+    if (base >= nb) return;
+    int k = base + threadIdx.x;
+    int i = k < nb ? order[k] : -1;
+    bool done = (i < 0) || (!mode_jp && color_in[i] != -1);
+    int s0 = 0, s1 = 0;
+    float pi = 0.0f;
+    if (!done) {
+      s0 = nbr_start[i];
+      s1 = nbr_start[i + 1];
+      pi = prio[i];
+    }
+    int spins = 0;
+    while (!done) {
+      bool ready = true;
+      unsigned used = 0u;
+      int rmax = 0;
+      for (int q = s0; q < s1 && ready; ++q) {
+        int j = nbr[q];
+        if (!mode_jp) {
+          int cj = color_in[j];
+          if (cj >= 0) {
+            if (cj < kMaxColors) used |= 1u << cj;
+          } else if (j < i) {
+            int pj = ld_acq(&pub[j]);
+            if (pj < 0) ready = false;
+            else if (pj < kMaxColors) used |= 1u << pj;
+          }
+        } else {
+          float pj_ = prio[j];
+          if (pj_ > pi || (pj_ == pi && j < i)) {
+            int pj = ld_acq(&pub[j]);
+            if (pj < 0) {
+              ready = false;
+            } else {
+              int cj = (pj & 255) - 1;
+              int rj = (pj >> 8) + (j > i ? 1 : 0);
+              if (cj >= 0) used |= 1u << cj;
+              rmax = rj > rmax ? rj : rmax;
+            }
+          }
+        }
+      }
+      if (ready) {
+        if (!mode_jp) {
+          st_rel(&pub[i], lowest_free_color(used));
+        } else {
+          int r = rmax < kJpRounds ? rmax : kJpRounds;
+          int c = rmax < kJpRounds ? lowest_free_color(used) : -1;
+          st_rel(&pub[i], (r << 8) | (c + 1));
+        }
+        done = true;
+      } else if (++spins > (1 << 20)) {
+        // A dependency never resolved: publish color 0 so waiters drain,
+        // and flag the failure for the host.
+        atomicExch(err, 1);
+        st_rel(&pub[i], mode_jp ? 1 : 0);
+        done = true;
+      } else {
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  if (mode_jp) color[i] = (pub[i] & 255) - 1;
+  else if (color[i] == -1) color[i] = pub[i];
+}
+
+void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* nbr_start,
+                         const int* nbr, const float* prio, int* color_in_out, int* pub,
+                         int* err, int* ticket, int mode_jp, int /*n_blocks*/) {
+  if (nb <= 0) return;
+  launch_fill_int(st, pub, nb, -1);
+  launch_fill_int(st, ticket, 1, 0);
+  hipLaunchKernelGGL(k_color_greedy, dim3((nb + 255) / 256), dim3(256), 0, st, nb, order,
+                     nbr_start, nbr, prio, color_in_out, pub, ticket, mode_jp, err);
+  hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
+                     mode_jp);
+}
+
+__global__ void __launch_bounds__(256)
+k_color_resolve(int nb, const int* __restrict__ nbr_start, const int* __restrict__ nbr,
+                const float* __restrict__ prio, const int* __restrict__ color,
+                int* __restrict__ color_next, int* uncolored) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  int ci = color[i];
+  int out = ci;
+  if (ci >= 0) {
+    float mp = prio[i];
+    for (int q = nbr_start[i]; q < nbr_start[i + 1]; ++q) {
+      int j = nbr[q];
+      if (color[j] == ci && (prio[j] > mp || (prio[j] == mp && j < i))) {
+        out = -1;
+        break;
+      }
+    }
+  }
+  color_next[i] = out;
+  if (out == -1) atomicAdd(uncolored, 1);
+}
+
+void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
+                          const float* prio, const int* color, int* color_next, int* uncolored) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_color_resolve, dim3((nb + 255) / 256), dim3(256), 0, st, nb, nbr_start, nbr,
+                       prio, color, color_next, uncolored);
+}
+
+__global__ void __launch_bounds__(256) k_color_finish(int nb, int* color, int* counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  int c = color[i];
+  if (c == -1) { c = 0; color[i] = 0; }
+  if (c >= 0 && c < kMaxColors) atomicAdd(&counts[c], 1);
+}
+
+void launch_color_finish(hipStream_t st, int nb, int* color, int* counts) {
+  launch_fill_int(st, counts, kMaxColors, 0);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_color_finish, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, counts);
+}
+
+__global__ void __launch_bounds__(256)
+k_color_keys(int nb, const int* __restrict__ color, unsigned* keys, int* vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  keys[i] = (unsigned)color[i];
+  vals[i] = i;
+}
+
+void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
+                      int* vals_tmp, int n, int key_bits, void* ws);
+
+// color_bodies: bodies bucketed by color, ascending within a color
+// (gpu_color_bodies_scatter, kernels_6dof.py:3182-3199, in serial order).
+void color_bucket(hipStream_t st, int nb, const int* color, unsigned* keys, int* vals,
+                  unsigned* keys_tmp, int* vals_tmp, void* ws) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_color_keys, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, keys, vals);
+  radix_sort_small(st, keys, vals, keys_tmp, vals_tmp, nb, 8, ws);
+}
+
+}  // namespace avbd

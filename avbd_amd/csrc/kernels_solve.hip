@@ -1,0 +1,523 @@
+// kernels_solve.hip — per-substep solve kernels for gfx950.
+//
+//  k_substep_begin  predict_inertial_6dof + substep_prelude_6dof (+ the
+//                   main-pass cache_alpha_C0) in one launch
+//                   (kernels_6dof.py:122-187, 1565-1672)
+//  k_primal_color   primal_update_6dof for one color (kernels_6dof.py:322-582)
+//  k_dual           dual_update_6dof (kernels_6dof.py:1451-1548)
+//  k_finalize       finalize_and_cap_6dof (kernels_6dof.py:1871-1901)
+//
+// Body poses are double-buffered: the substep reads x_initial from the
+// buffer that holds the state at substep start and iterates on the other
+// one, so predict (writer of the warm start) and the row prelude (reader of
+// x_initial) share a launch without a race.
+//
+// Per body, constraint rows are visited in ascending row order (groups in
+// CSR order, rows n -> t -> b inside a group) and accumulated serially, the
+// order the reference's serial CPU launch uses; every float operation is the
+// one in avbd_fp.h, so results round like the host restatement.
+#include "avbd_internal.h"
+
+namespace avbd {
+
+__device__ __forceinline__ v3 f3(float4 a) { return mk3(a.x, a.y, a.z); }
+__device__ __forceinline__ q4 f4q(float4 a) { return mk4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ float4 p4(v3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+__device__ __forceinline__ float4 p4q(q4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+
+__device__ __forceinline__ m33 load_m33(const float4* p) {
+  float4 r0 = p[0], r1 = p[1], r2 = p[2];
+  m33 m;
+  m.m[0][0] = r0.x; m.m[0][1] = r0.y; m.m[0][2] = r0.z;
+  m.m[1][0] = r1.x; m.m[1][1] = r1.y; m.m[1][2] = r1.z;
+  m.m[2][0] = r2.x; m.m[2][1] = r2.y; m.m[2][2] = r2.z;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// predict + prelude
+// ---------------------------------------------------------------------------
+__device__ void predict_body(int i, const float4* __restrict__ pose_cur,
+                             float4* __restrict__ pose_work, float4* __restrict__ poseI,
+                             float4* __restrict__ inert, const float4* __restrict__ vel,
+                             const float4* __restrict__ pvel, const float4* __restrict__ bstat,
+                             const float* __restrict__ Ilocal, float dt, v3 g) {
+  float m = bstat[i].x;
+  v3 x = f3(pose_cur[2 * i]);
+  q4 q = f4q(pose_cur[2 * i + 1]);
+  m33 R = qtomat(q);
+  m33 Il;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Il.m[r][c] = Ilocal[9 * i + 3 * r + c];
+  m33 Iw = mmul(mmul(R, Il), mT(R));
+  inert[3 * i + 0] = make_float4(Iw.m[0][0], Iw.m[0][1], Iw.m[0][2], 0.0f);
+  inert[3 * i + 1] = make_float4(Iw.m[1][0], Iw.m[1][1], Iw.m[1][2], 0.0f);
+  inert[3 * i + 2] = make_float4(Iw.m[2][0], Iw.m[2][1], Iw.m[2][2], 0.0f);
+  if (m <= 0.0f) {
+    poseI[2 * i] = p4(x);
+    poseI[2 * i + 1] = p4q(q);
+    pose_work[2 * i] = p4(x);
+    pose_work[2 * i + 1] = p4q(q);
+    return;
+  }
+  v3 g_dt2 = mul(g, dt * dt);
+  v3 v = f3(vel[2 * i]);
+  v3 w = f3(vel[2 * i + 1]);
+  v3 pv = f3(pvel[2 * i]);
+  v3 xin = add(add(x, mul(v, dt)), g_dt2);
+  q4 dq = quat_from_rotvec(mul(w, dt));
+  q4 qin = qmul(dq, q);
+  v3 accel = divs(sub(v, pv), dt);
+  float g_norm = length(g);
+  float wgt = 0.0f;
+  if (g_norm > 0.0f) {
+    v3 g_hat = divs(g, g_norm);
+    float a_ext = dot(accel, g_hat);
+    wgt = clampf_(a_ext / g_norm, 0.0f, 1.0f);
+  }
+  v3 xw = add(add(x, mul(v, dt)), mul(g_dt2, wgt));
+  poseI[2 * i] = p4(xin);
+  poseI[2 * i + 1] = p4q(qin);
+  pose_work[2 * i] = p4(xw);
+  pose_work[2 * i + 1] = p4q(qin);
+}
+
+__device__ void prelude_group(int g, const float4* __restrict__ pose0, GroupGeo* __restrict__ geo,
+                              GroupSt* __restrict__ gst, float alpha, float gamma, int post) {
+  GroupGeo G = geo[g];
+  GroupSt S = gst[g];
+  const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
+  const int a = G.hdr.y, b = G.hdr.z;
+  const float stiff = G.g0.w;
+  int bits = __float_as_int(S.s2.y);
+  float lam[3] = {S.s0.x, S.s0.y, S.s0.z};
+  float kk[3] = {S.s1.x, S.s1.y, S.s1.z};
+  float ac0[3] = {S.s0.w, S.s1.w, S.s2.x};
+  const float main_alpha = post ? 1.0f : alpha;
+  const bool finite_s = !(stiff != stiff) && stiff < kInf;
+  // (1) warm-start decay of lambda and penalty (Eq. 19)
+  for (int r = 0; r < nr; ++r) {
+    float kf = r == 0 ? kPenaltyMin : kPenaltyMinTangent;
+    float p = clampf_(kk[r] * gamma, kf, kPenaltyMax);
+    if (finite_s) p = fmin_(p, stiff);
+    kk[r] = p;
+  }
+  float lam_dec[3];
+  for (int r = 0; r < nr; ++r) lam_dec[r] = post == 0 ? lam[r] * alpha * gamma : lam[r];
+  // (2) static-friction state, evaluated on the lower tangent row with the
+  // values serial row order gives it: sibling and itself already decayed,
+  // partner not yet.
+  if (nr == 3 && (bits & kActT) && (bits & kActN)) {
+    float lam_n = fabs_(lam_dec[0]);
+    int was;
+    if (lam_n < 1.0e-9f) {
+      was = 0;
+    } else {
+      float lt = lam_dec[1];
+      float lb = lam[2];
+      float ltb = sqrt_(lt * lt + lb * lb);
+      was = ltb <= G.g2.w * lam_n ? 1 : 0;
+    }
+    bits = was ? (bits | kWasN) : (bits & ~kWasN);
+  }
+  // (3) alpha * C0 at x_initial (Eq. 18)
+  v3 off_a = f3(G.g1);
+  if (kind == kGFloor) {
+    v3 x0 = f3(pose0[2 * a]);
+    q4 q0 = f4q(pose0[2 * a + 1]);
+    v3 rw = add(x0, qrotate(q0, off_a));
+    float C0 = rw.y - G.g0.y;
+    ac0[0] = (bits & kActN) ? (C0 > 0.0f ? 0.0f : main_alpha * C0) : 0.0f;
+    if (nr == 3) {
+      ac0[1] = (bits & kActT) ? dot(mk3(1.0f, 0.0f, 0.0f), rw) : 0.0f;
+      ac0[2] = (bits & kActB) ? dot(mk3(0.0f, 0.0f, 1.0f), rw) : 0.0f;
+    }
+  } else if (kind == kGBox) {
+    v3 xa = f3(pose0[2 * a]);
+    q4 qa = f4q(pose0[2 * a + 1]);
+    v3 xb = f3(pose0[2 * b]);
+    q4 qb = f4q(pose0[2 * b + 1]);
+    v3 ra = add(xa, qrotate(qa, off_a));
+    v3 rb = add(xb, qrotate(qb, f3(G.g2)));
+    v3 d = sub(ra, rb);
+    v3 n = f3(G.g0);
+    float C0 = dot(n, d);
+    ac0[0] = (bits & kActN) ? (C0 > 0.0f ? 0.0f : main_alpha * C0) : 0.0f;
+    if (nr == 3) {
+      v3 t = orthonormal_basis(n);
+      v3 bb = cross(n, t);
+      ac0[1] = (bits & kActT) ? dot(t, d) : 0.0f;
+      ac0[2] = (bits & kActB) ? dot(bb, d) : 0.0f;
+    }
+  } else {  // PIN
+    v3 x0 = f3(pose0[2 * a]);
+    q4 q0 = f4q(pose0[2 * a + 1]);
+    v3 rw = add(x0, qrotate(q0, off_a));
+    v3 anc = f3(G.g0);
+    float C0 = b == 0 ? rw.x - anc.x : (b == 1 ? rw.y - anc.y : rw.z - anc.z);
+    ac0[0] = (bits & kActN) ? main_alpha * C0 : 0.0f;
+  }
+  S.s0 = make_float4(lam_dec[0], nr > 1 ? lam_dec[1] : S.s0.y, nr > 2 ? lam_dec[2] : S.s0.z, ac0[0]);
+  S.s1 = make_float4(kk[0], nr > 1 ? kk[1] : S.s1.y, nr > 2 ? kk[2] : S.s1.z, ac0[1]);
+  S.s2 = make_float4(ac0[2], __int_as_float(bits), S.s2.z, S.s2.w);
+  gst[g] = S;
+}
+
+__global__ void __launch_bounds__(256)
+k_substep_begin(int nb, int n_groups, const float4* __restrict__ pose_cur,
+                float4* __restrict__ pose_work, float4* __restrict__ poseI,
+                float4* __restrict__ inert, const float4* __restrict__ vel,
+                const float4* __restrict__ pvel, const float4* __restrict__ bstat,
+                const float* __restrict__ Ilocal, GroupGeo* __restrict__ geo,
+                GroupSt* __restrict__ gst, float dt, float gx, float gy, float gz, float alpha,
+                float gamma, int post, int do_prelude) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < nb) {
+    predict_body(tid, pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, dt,
+                 mk3(gx, gy, gz));
+    return;
+  }
+  int g = tid - nb;
+  if (!do_prelude || g >= n_groups) return;
+  prelude_group(g, pose_cur, geo, gst, alpha, gamma, post);
+}
+
+void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose_cur,
+                          float4* pose_work, float4* poseI, float4* inert, const float4* vel,
+                          const float4* pvel, const float4* bstat, const float* Ilocal,
+                          GroupGeo* geo, GroupSt* gst, float dt, float gx, float gy, float gz,
+                          float alpha, float gamma, int post, int do_prelude) {
+  int total = nb + (do_prelude ? n_groups : 0);
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_substep_begin, dim3((total + 255) / 256), dim3(256), 0, st, nb, n_groups,
+                     pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, geo, gst, dt,
+                     gx, gy, gz, alpha, gamma, post, do_prelude);
+}
+
+// ---------------------------------------------------------------------------
+// primal
+// ---------------------------------------------------------------------------
+struct Acc {
+  m33 A, B, D;
+  v3 rl, ra;
+};
+
+// One constraint row's contribution (primal_update_6dof loop body,
+// kernels_6dof.py:507-564). `is_t` selects the friction-cone bounds.
+__device__ __forceinline__ void acc_row(Acc& acc, v3 jl, v3 ja, float C, v3 nG, v3 r_self,
+                                        bool hard, float kp, float lam, float ac0, bool is_t,
+                                        float lam_sib, bool was_sib, float mu_d, float mu_s,
+                                        float f_lo_n, float f_hi_n) {
+  if (hard) C = C - ac0;
+  float lam_eff = hard ? lam : 0.0f;
+  float lam_plus = kp * C + lam_eff;
+  float f_lo, f_hi;
+  if (is_t) {
+    float mu = was_sib ? mu_s : mu_d;
+    float bound = mu * fabs_(lam_sib);
+    f_lo = -bound;
+    f_hi = bound;
+  } else {
+    f_lo = f_lo_n;
+    f_hi = f_hi_n;
+  }
+  float f = clampf_(lam_plus, f_lo, f_hi);
+  float k_lhs = kp;
+  float absC = fabs_(C);
+  if (absC > 1.0e-12f) {
+    if (lam_plus < f_lo) k_lhs = fabs_(f_lo - lam_plus) / absC;
+    else if (lam_plus > f_hi) k_lhs = fabs_(f_hi - lam_plus) / absC;
+  }
+  float l[3] = {jl.x, jl.y, jl.z};
+  float an[3] = {ja.x, ja.y, ja.z};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      acc.A.m[r][c] = acc.A.m[r][c] + (l[r] * l[c]) * k_lhs;
+      acc.B.m[r][c] = acc.B.m[r][c] + (an[r] * l[c]) * k_lhs;
+      acc.D.m[r][c] = acc.D.m[r][c] + (an[r] * an[c]) * k_lhs;
+    }
+  }
+  float fmag = fabs_(f);
+  if (fmag > 0.0f) {
+    v3 gd = mul(geom_stiffness_diag(nG, r_self), fmag);
+    acc.D.m[0][0] = acc.D.m[0][0] + gd.x;
+    acc.D.m[1][1] = acc.D.m[1][1] + gd.y;
+    acc.D.m[2][2] = acc.D.m[2][2] + gd.z;
+  }
+  acc.rl = add(acc.rl, mul(jl, f));
+  acc.ra = add(acc.ra, mul(ja, f));
+}
+
+__device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, const GroupGeo& G,
+                                             const GroupSt& S, const float4* __restrict__ pose,
+                                             int post_pass) {
+  const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
+  const int bits = __float_as_int(S.s2.y);
+  const float stiff = G.g0.w;
+  const bool hard = stiff >= kInf;
+  const bool wasN = (bits & kWasN) != 0;
+  const float lam_n = S.s0.x;
+  v3 off_a = f3(G.g1);
+  if (kind == kGFloor) {
+    v3 r_self = qrotate(qi, off_a);
+    if (bits & kActN) {
+      v3 n = mk3(0.0f, 1.0f, 0.0f);
+      float C = (xi.y + r_self.y) - G.g0.y;
+      acc_row(acc, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
+              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+    }
+    if (nr == 3) {
+      v3 xr = add(xi, r_self);
+      if (bits & kActT) {
+        v3 t = mk3(1.0f, 0.0f, 0.0f);
+        acc_row(acc, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w,
+                true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+      }
+      if (bits & kActB) {
+        v3 t = mk3(0.0f, 0.0f, 1.0f);
+        acc_row(acc, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.z, S.s0.z, S.s2.x,
+                true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+      }
+    }
+  } else if (kind == kGBox) {
+    const int a = G.hdr.y, b = G.hdr.z;
+    const bool self_a = (a == i);
+    v3 off_b = f3(G.g2);
+    v3 r_self, wa, wb;
+    if (self_a) {
+      r_self = qrotate(qi, off_a);
+      v3 r_o = qrotate(f4q(pose[2 * b + 1]), off_b);
+      wa = add(xi, r_self);
+      wb = add(f3(pose[2 * b]), r_o);
+    } else {
+      v3 r_o = qrotate(f4q(pose[2 * a + 1]), off_a);
+      r_self = qrotate(qi, off_b);
+      wa = add(f3(pose[2 * a]), r_o);
+      wb = add(xi, r_self);
+    }
+    v3 d = sub(wa, wb);
+    v3 n = f3(G.g0);
+    if (bits & kActN) {
+      v3 jl = self_a ? n : neg(n);
+      v3 ja = self_a ? cross(r_self, n) : neg(cross(r_self, n));
+      acc_row(acc, jl, ja, dot(n, d), n, r_self, hard, S.s1.x, S.s0.x,
+              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+    }
+    if (nr == 3) {
+      v3 t = orthonormal_basis(n);
+      v3 bb = cross(n, t);
+      if (bits & kActT) {
+        v3 jl = self_a ? t : neg(t);
+        v3 ja = self_a ? cross(r_self, t) : neg(cross(r_self, t));
+        acc_row(acc, jl, ja, dot(t, d), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w, true, lam_n,
+                wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+      }
+      if (bits & kActB) {
+        v3 jl = self_a ? bb : neg(bb);
+        v3 ja = self_a ? cross(r_self, bb) : neg(cross(r_self, bb));
+        acc_row(acc, jl, ja, dot(bb, d), bb, r_self, hard, S.s1.z, S.s0.z, S.s2.x, true, lam_n,
+                wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+      }
+    }
+  } else {  // PIN
+    if (bits & kActN) {
+      const int axis = G.hdr.z;
+      v3 r_self = qrotate(qi, off_a);
+      v3 wp = add(xi, r_self);
+      v3 anc = f3(G.g0);
+      v3 n;
+      float C;
+      if (axis == 0) { n = mk3(1.0f, 0.0f, 0.0f); C = wp.x - anc.x; }
+      else if (axis == 1) { n = mk3(0.0f, 1.0f, 0.0f); C = wp.y - anc.y; }
+      else { n = mk3(0.0f, 0.0f, 1.0f); C = wp.z - anc.z; }
+      acc_row(acc, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
+              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, kInf);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_primal_color(int count, const int* __restrict__ color_bodies, int base,
+               float4* __restrict__ pose, const float4* __restrict__ poseI,
+               const float4* __restrict__ inert, const float4* __restrict__ bstat,
+               const int* __restrict__ csr_start, const int* __restrict__ csr_idx,
+               const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
+               int post_pass) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= count) return;
+  int i = color_bodies[base + tid];
+  float m = bstat[i].x;
+  if (m <= 0.0f) return;
+  float inv_dt2 = 1.0f / (dt * dt);
+  m33 Iw = load_m33(inert + 3 * i);
+  Acc acc;
+  acc.A = mzero();
+  acc.A.m[0][0] = m * inv_dt2;
+  acc.A.m[1][1] = m * inv_dt2;
+  acc.A.m[2][2] = m * inv_dt2;
+  acc.D = mscale(Iw, inv_dt2);
+  acc.B = mzero();
+  v3 xi = f3(pose[2 * i]);
+  q4 qi = f4q(pose[2 * i + 1]);
+  acc.rl = mul(sub(xi, f3(poseI[2 * i])), m * inv_dt2);
+  q4 dq = qmul(qi, qconj(f4q(poseI[2 * i + 1])));
+  acc.ra = mvmul(Iw, mul(quat_to_rotvec(dq), inv_dt2));
+  const int k0 = csr_start[i], k1 = csr_start[i + 1];
+  for (int k = k0; k < k1; ++k) {
+    int g = csr_idx[k];
+    GroupGeo G = geo[g];
+    GroupSt S = gst[g];
+    primal_group(acc, i, xi, qi, G, S, pose, post_pass);
+  }
+  m33 A_inv = minverse(acc.A);
+  m33 BAinv = mmul(acc.B, A_inv);
+  v3 rhs = sub(acc.ra, mvmul(BAinv, acc.rl));
+  m33 Sm = msub(acc.D, mmul(BAinv, mT(acc.B)));
+  m33 S_inv = minverse(Sm);
+  v3 d_th = mvmul(S_inv, rhs);
+  v3 d_x = mvmul(A_inv, sub(acc.rl, mvmul(mT(acc.B), d_th)));
+  pose[2 * i] = p4(sub(xi, d_x));
+  q4 dq2 = quat_from_rotvec(neg(d_th));
+  pose[2 * i + 1] = p4q(qnormalize(qmul(dq2, qi)));
+}
+
+void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
+                         float4* pose, const float4* poseI, const float4* inert,
+                         const float4* bstat, const int* csr_start, const int* csr_idx,
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_primal_color, dim3((count + 63) / 64), dim3(64), 0, st, count,
+                     color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
+                     dt, post_pass);
+}
+
+// ---------------------------------------------------------------------------
+// dual: one lane per group, rows n -> t -> b in order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float& lam, float& kp,
+                                         float ac0, float lo, float hi, float fracture,
+                                         bool is_t, bool used_static, int act_bit, int& bits,
+                                         float beta) {
+  if (hard) C = C - ac0;
+  float lam_eff = hard ? lam : 0.0f;
+  float nl = clampf_(kp * C + lam_eff, lo, hi);
+  lam = nl;
+  if (fabs_(nl) >= fracture) {
+    bits &= ~act_bit;
+    lam = 0.0f;
+    kp = 0.0f;
+    return;
+  }
+  if (nl > lo && nl < hi) {
+    float upper = fmin_(kPenaltyMax, stiff);
+    kp = fmin_(kp + beta * fabs_(C), upper);
+  } else if (is_t && used_static) {
+    bits &= ~kWasN;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict__ geo,
+       GroupSt* __restrict__ gst, float beta) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  GroupGeo G = geo[g];
+  GroupSt S = gst[g];
+  const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
+  const int a = G.hdr.y, b = G.hdr.z;
+  int bits = __float_as_int(S.s2.y);
+  if (!(bits & (kActN | kActT | kActB))) return;
+  const float stiff = G.g0.w;
+  const bool hard = stiff >= kInf;
+  float lam[3] = {S.s0.x, S.s0.y, S.s0.z};
+  float kk[3] = {S.s1.x, S.s1.y, S.s1.z};
+  const float ac0[3] = {S.s0.w, S.s1.w, S.s2.x};
+  float C[3] = {0.f, 0.f, 0.f};
+  v3 off_a = f3(G.g1);
+  if (kind == kGFloor) {
+    v3 rw = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
+    C[0] = rw.y - G.g0.y;
+    C[1] = dot(mk3(1.0f, 0.0f, 0.0f), rw);
+    C[2] = dot(mk3(0.0f, 0.0f, 1.0f), rw);
+  } else if (kind == kGBox) {
+    v3 ra = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
+    v3 rb = add(f3(pose[2 * b]), qrotate(f4q(pose[2 * b + 1]), f3(G.g2)));
+    v3 d = sub(ra, rb);
+    v3 n = f3(G.g0);
+    C[0] = dot(n, d);
+    if (nr == 3) {
+      v3 t = orthonormal_basis(n);
+      C[1] = dot(t, d);
+      C[2] = dot(cross(n, t), d);
+    }
+  } else {
+    v3 rw = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
+    v3 anc = f3(G.g0);
+    C[0] = b == 0 ? rw.x - anc.x : (b == 1 ? rw.y - anc.y : rw.z - anc.z);
+  }
+  // normal / pin row
+  if (bits & kActN) {
+    float hi0 = kind == kGPin ? kInf : 0.0f;
+    float frac0 = kind == kGPin ? G.g2.w : kInf;
+    dual_row(C[0], hard, stiff, lam[0], kk[0], ac0[0], -kInf, hi0, frac0, false, false, kActN,
+             bits, beta);
+  }
+  // tangent rows: bounds from this launch's normal lambda and was_static
+  for (int r = 1; r < nr; ++r) {
+    int act = r == 1 ? kActT : kActB;
+    if (!(bits & act)) continue;
+    bool used_static = (bits & kWasN) != 0;
+    float mu = used_static ? G.g2.w : G.g1.w;
+    float bound = mu * fabs_(lam[0]);
+    dual_row(C[r], hard, stiff, lam[r], kk[r], ac0[r], -bound, bound, kInf, true, used_static,
+             act, bits, beta);
+  }
+  S.s0.x = lam[0]; S.s0.y = lam[1]; S.s0.z = lam[2];
+  S.s1.x = kk[0]; S.s1.y = kk[1]; S.s1.z = kk[2];
+  S.s2.y = __int_as_float(bits);
+  gst[g] = S;
+}
+
+void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
+                 GroupSt* gst, float beta) {
+  if (n_groups <= 0) return;
+  hipLaunchKernelGGL(k_dual, dim3((n_groups + 255) / 256), dim3(256), 0, st, n_groups, pose, geo,
+                     gst, beta);
+}
+
+// ---------------------------------------------------------------------------
+// finalize_and_cap
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_finalize(int nb, const float4* __restrict__ pose, const float4* __restrict__ pose0,
+           float4* __restrict__ vel, float4* __restrict__ pvel, const float4* __restrict__ bstat,
+           float dt, float max_lin, float max_ang) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  float4 v4 = vel[2 * i], w4 = vel[2 * i + 1];
+  pvel[2 * i] = v4;
+  pvel[2 * i + 1] = w4;
+  if (bstat[i].x > 0.0f) {
+    v3 nv = divs(sub(f3(pose[2 * i]), f3(pose0[2 * i])), dt);
+    q4 dq = qmul(f4q(pose[2 * i + 1]), qconj(f4q(pose0[2 * i + 1])));
+    v3 nw = divs(quat_to_rotvec(dq), dt);
+    float sl = length(nv);
+    if (sl > max_lin) nv = mul(nv, max_lin / sl);
+    float sa = length(nw);
+    if (sa > max_ang) nw = mul(nw, max_ang / sa);
+    vel[2 * i] = p4(nv);
+    vel[2 * i + 1] = p4(nw);
+  }
+}
+
+void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
+                     float4* vel, float4* pvel, const float4* bstat, float dt, float max_lin,
+                     float max_ang) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_finalize, dim3((nb + 255) / 256), dim3(256), 0, st, nb, pose, pose0, vel,
+                     pvel, bstat, dt, max_lin, max_ang);
+}
+
+}  // namespace avbd

@@ -1,0 +1,137 @@
+"""Throughput of the Solver6DOF substep on one MI355X.
+
+Workload: the square pyramid of 98,021 cubes (66 layers, half-extent 0.25 m,
+125 kg, mu_d 0.5 / mu_s 0.6, seeded 1 mm jitter), dt = 1/60, 4 substeps,
+20 iterations, FP32. One step = one frame (4 substeps: contacts, coloring,
+CSR, prelude, 21 primal sweeps, 20 dual sweeps, finalize, cache refresh).
+Metric: body-substeps/s = bodies x substeps x frames / wall time, with the
+scene resident in HBM before the timed region starts.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
+    """The C++ restatement of the reference substep (one host thread) on the
+    same scene, timed over `substeps_sample` substeps from the initial state."""
+    from avbd_amd import scenes
+    from oracle.oracle import oracle_solver_class
+    from tests.parity import one_substep
+
+    s = scenes.pyramid(layers, seed=seed, solver_cls=oracle_solver_class())
+    s._flush()
+    n_b = len(s._x)
+    t0 = time.perf_counter()
+    for _ in range(substeps_sample):
+        one_substep(s)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_b * substeps_sample / dt,
+        "unit": "body-substeps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{substeps_sample} substep(s) of the {n_b}-cube pyramid from its initial "
+                   f"state ({dt:.1f} s), C++ restatement of the reference substep, 1 thread"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--layers", type=int, default=66)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-substeps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the substep is not sharded across GPUs.", file=sys.stderr)
+        return 2
+
+    from avbd_amd import scenes
+
+    t_build = time.perf_counter()
+    s = scenes.pyramid(args.layers, seed=args.seed)
+    s._flush()
+    n_b = len(s._x)
+    print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
+          f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        s.step()
+        print(f"# warmup frame {w}: {1e3 * (time.perf_counter() - t0):.1f} ms", file=sys.stderr,
+              flush=True)
+    s._h.read_counters()
+    s.positions()   # synchronises the stream
+    s._h.profile_enable(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step()
+    s.positions()
+    elapsed = time.perf_counter() - t0
+    ms, launches, nbytes = s._h.profile_read(7)
+    s._h.profile_enable(False)
+    n_rows, n_pairs, n_contacts = s._h.read_counters()
+    substeps = s.substeps * args.steps
+    value = n_b * substeps / elapsed
+    prim_ms, prim_n, prim_b = float(ms[0]), int(launches[0]), float(nbytes[0])
+    achieved = (prim_b / prim_n) / (prim_ms / prim_n * 1e-3) / 1e9 if prim_n else 0.0
+    classes = ["primal", "dual", "prelude", "contacts", "coloring", "csr", "other"]
+    breakdown = {c: {"ms": round(float(ms[i]), 3), "launches": int(launches[i])}
+                 for i, c in enumerate(classes)}
+    print(f"# kernel time by class over the timed region: {breakdown}", file=sys.stderr,
+          flush=True)
+    out = {
+        "metric": "body-substeps/s @100k boxes",
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"pyramid of {n_b} cubes ({args.layers} layers), dt 1/60, "
+                               f"{s.substeps} substeps, {s.iterations} iterations",
+                   "bodies": n_b, "substeps_per_step": s.substeps, "iterations": s.iterations,
+                   "contacts": n_contacts, "pairs": n_pairs, "rows": n_rows,
+                   "colors": s.num_active_colors, "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_primal_color (per-color primal sweep)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": 1e3 * prim_ms / prim_n if prim_n else None,
+            "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
+        },
+        "kernel_ms": breakdown,
+    }
+    if args.cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.layers, args.seed, args.cpu_substeps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

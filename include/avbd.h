@@ -1,0 +1,196 @@
+/* avbd.h — C ABI of the MI355X Solver6DOF substep library (libavbd_hip.so).
+ *
+ * Plain C: pointers, sizes and status codes only. Every entry point returns
+ * 0 on success and a negative AVBD_E_* code on failure; avbd_last_error()
+ * returns a message for the most recent failure on the calling thread.
+ *
+ * The Python host (avbd_amd/solver_6dof.py) mirrors the reference
+ * Solver6DOF API (src/avbd3d/solver_6dof.py:177-2052) and binds these
+ * symbols through ctypes; the reference itself launches Warp kernels
+ * (src/avbd3d/kernels_6dof.py) through wp.launch at the call sites cited on
+ * each declaration.
+ *
+ * Threading: one solver handle is driven by one host thread at a time; all
+ * device work of a handle is issued on the handle's own HIP stream.
+ */
+#ifndef AVBD_H_
+#define AVBD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_OK 0
+#define AVBD_E_ARG (-1)        /* invalid argument / shape */
+#define AVBD_E_HIP (-2)        /* HIP runtime error */
+#define AVBD_E_STATE (-3)      /* call out of order (e.g. step before flush) */
+#define AVBD_E_NOMEM (-4)      /* device allocation failed */
+
+/* Coloring modes (solver_6dof.py:597-624). */
+#define AVBD_COLOR_JACOBI 0
+#define AVBD_COLOR_JONES_PLASSMANN 1
+
+/* Scalars that the reference bakes into each launch
+ * (solver_6dof.py:180-201 constructor kwargs, _run_iter_loop :1514-1522). */
+typedef struct avbd_params {
+  float dt;                    /* substep dt (step() divides by substeps) */
+  int iterations;
+  float gravity[3];
+  float alpha, beta, gamma;
+  int post_stabilize;
+  float max_linear_speed;      /* +inf disables the cap */
+  float max_angular_speed;     /* +inf disables the cap */
+  float friction_static_mult;
+  int coloring_mode;           /* AVBD_COLOR_* */
+  int self_collide;            /* enable_self_collision (solver_6dof.py:570) */
+  float self_friction;         /* default_friction for box-box contacts */
+  int fixed_capacity;          /* unsafe_fixed_capacity semantics */
+  float margin;                /* contact margin, 0.005 (solver_6dof.py:1789) */
+} avbd_params;
+
+/* Host-side scene handed over at _flush (solver_6dof.py:904-1174).
+ * Body arrays have n_bodies entries (vec3 = 3 floats, quat = 4 floats xyzw,
+ * mat33 = 9 floats row-major). Row arrays hold the n_static_rows static rows
+ * built by add_floor_contact_box / add_pin_corner. */
+typedef struct avbd_scene {
+  int n_bodies;
+  const float* x;
+  const float* q;
+  const float* v;
+  const float* w;
+  const float* prev_v;
+  const float* prev_w;
+  const float* mass;
+  const float* inertia_local;  /* 9 per body */
+  const float* half_extents;
+  const float* friction;
+  const float* priority;       /* default_rng(0xA5BD).uniform -> f32 */
+
+  int n_static_rows;
+  const int* type;
+  const int* body_a;
+  const int* body_b;
+  const int* sibling;
+  const int* partner;
+  const float* anchor;         /* 3 per row */
+  const float* off_a;          /* 3 per row */
+  const float* off_b;          /* 3 per row */
+  const float* stiffness;
+  const float* fmin;
+  const float* fmax;
+  const float* fracture;
+  const float* friction_d;
+  const float* friction_s;
+  const float* lambda;         /* initial row state */
+  const float* penalty;
+  const int* active;
+  const int* was_static;
+
+  /* Capacities sized by the host with the reference formulas
+   * (solver_6dof.py:937-943, 1153, 1158, 1811). */
+  int max_pairs;
+  int n_dyn_capacity;          /* rows */
+  int hash_cap;
+  int pool_max;
+} avbd_scene;
+
+/* What one _step_one did; the host turns these into the reference's
+ * warnings and attribute updates (solver_6dof.py:1296-1356). */
+typedef struct avbd_report {
+  int recolored;               /* 1 if the coloring was rebuilt */
+  int n_active_colors;         /* achieved colors, clamped to [1, 32] */
+  int n_active_rows;           /* raw row reservation counter */
+  int row_capacity;            /* capacity before any regrow this substep */
+  int row_overflow;            /* 1 if raw > capacity (pool regrown) */
+  int new_row_capacity;
+  int pair_count;              /* broadphase pair counter */
+  int pair_capacity;
+  int pair_grew;               /* pair buffers regrown this substep */
+  int pool_count;
+  int n_contacts;              /* dynamic contact groups emitted */
+  int color_conflicts;         /* conflict rows seen by the recolor gate */
+  float broadphase_ms;         /* host wall time of contact generation */
+} avbd_report;
+
+const char* avbd_last_error(void);
+int avbd_device_count(int* out);
+
+/* ---- lifecycle ------------------------------------------------------ */
+typedef struct avbd_solver avbd_solver;
+/* Solver6DOF.__init__ (solver_6dof.py:180-442). */
+int avbd_create(int device, avbd_solver** out);
+int avbd_destroy(avbd_solver* s);
+/* Solver6DOF._flush (solver_6dof.py:904-1174): upload bodies and static rows,
+ * size every pool. Replaces the previous scene of the handle. */
+int avbd_flush(avbd_solver* s, const avbd_scene* scene);
+
+/* ---- the substep ---------------------------------------------------- */
+/* Solver6DOF._step_one (solver_6dof.py:1232-1464): reset counters, contact
+ * generation (:1277-1309), recolor gate (:1333-1356), body->constraint CSR
+ * (:1366-1385), predict + prelude (:1388-1417), iteration loop
+ * (_run_iter_loop :1507-1679) and warm-start cache refresh (:1450-1464).
+ * color_dirty: the host's _color_dirty flag. */
+int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out);
+
+/* ---- readback / write-back ------------------------------------------ */
+/* positions()/orientations()/velocities()/angular_velocities()
+ * (solver_6dof.py:1920-1938); any pointer may be NULL. */
+int avbd_read_bodies(avbd_solver* s, float* x, float* q, float* v, float* w,
+                     float* prev_v, float* prev_w);
+/* set_position/orientation/velocity/angular_velocity (solver_6dof.py:878-900):
+ * field 0 = x, 1 = q, 2 = v, 3 = w, 4 = prev_v, 5 = prev_w; writes n entries
+ * starting at body `first`, in place. */
+int avbd_write_bodies(avbd_solver* s, int field, int first, int n, const float* values);
+/* lambdas()/active() and the c_* row arrays (solver_6dof.py:1940-1948):
+ * per-row values for rows [0, n_rows); rows past the live count read as 0.
+ * Any pointer may be NULL. */
+int avbd_read_rows(avbd_solver* s, int n_rows, float* lambda, float* penalty, int* active,
+                   int* was_static, int* type, int* body_a, int* body_b, float* alpha_c0);
+/* Write static-row state (lambda, penalty, was_static, active) for rows
+ * [0, n) -- used to load a snapshot. */
+int avbd_write_static_rows(avbd_solver* s, int n, const float* lambda, const float* penalty,
+                           const int* active, const int* was_static);
+/* body_color (solver_6dof.py:1011) */
+int avbd_read_colors(avbd_solver* s, int* body_color);
+int avbd_write_colors(avbd_solver* s, const int* body_color, int n_active_colors);
+/* n_active_rows / _bp_pair_count / pool_count device counters. */
+int avbd_read_counters(avbd_solver* s, int* n_active_rows, int* pair_count, int* pool_count);
+/* count_color_conflicts (solver_6dof.py:846-862 -> kernels_6dof.py:3313-3336). */
+int avbd_count_color_conflicts(avbd_solver* s, int* out);
+/* read_state_batched packing (kernels_6dof.py:2154-2177, 2540-2555):
+ * bodies -> 10 floats [x, q, w]; rows [0, n_rows) -> 3 floats
+ * [lambda, active, was_static*16 + type]. */
+int avbd_pack_state(avbd_solver* s, float* bodies10, int n_rows, float* rows3);
+
+/* Warm-start cache contents (hash_keys / hash_state, solver_6dof.py:1150-1156):
+ * export writes up to `cap` occupied entries (key + 8 floats each) and sets
+ * *n; import clears the table and inserts n entries. */
+int avbd_export_hash(avbd_solver* s, int cap, int* keys, float* state8, int* n);
+int avbd_import_hash(avbd_solver* s, int n, const int* keys, const float* state8);
+/* Capacities currently allocated: max_pairs, row capacity, pool_max, hash_cap. */
+int avbd_read_capacities(avbd_solver* s, int* max_pairs, int* row_cap, int* pool_max, int* hash_cap);
+
+/* ---- per-stage views for tests ---------------------------------------- */
+/* Broadphase pairs of the last substep in canonical (a asc, b asc) order
+ * (bvh_broadphase_pairs, kernels_6dof.py:1946-1978); copies min(count, cap). */
+int avbd_read_pairs(avbd_solver* s, int cap, int* a, int* b, int* overlap, int* sat_idx,
+                    float* n_hat, float* depth, int* count);
+/* Contact manifold of the last substep (obb_contact_manifold_6dof,
+ * kernels_6dof.py:2221-2537): per pair count/ref_is_a/n/t/b and 4 offsets. */
+int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float* n_hat,
+                       float* t_hat, float* b_hat, float* off_ref, float* off_inc);
+
+/* ---- profiling ------------------------------------------------------- */
+/* When enabled, the primal, dual and prelude launches are bracketed by HIP
+ * events on the solver stream; avbd_profile_read returns accumulated
+ * milliseconds, launch counts and algorithmic bytes per kernel class
+ * (0 = primal sweep, 1 = dual sweep, 2 = substep prelude, 3 = contacts,
+ * 4 = coloring, 5 = csr, 6 = finalize/hash). */
+int avbd_profile_enable(avbd_solver* s, int on);
+int avbd_profile_read(avbd_solver* s, int n_classes, double* ms, long long* launches,
+                      double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AVBD_H_ */

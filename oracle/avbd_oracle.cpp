@@ -1,0 +1,1482 @@
+// avbd_oracle.cpp — CPU restatement of the reference Solver6DOF substep.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker the HIP library is
+// compared against; nothing in the avbd_amd product path loads or calls it.
+// It may be used by tests/, by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg, and nowhere else.
+//
+// What it follows: src/avbd3d/solver_6dof.py (_step_one :1232-1464,
+// _run_iter_loop :1507-1679, _gpu_emit_dynamic_contacts :1774-1916,
+// _gpu_recolor :680-844, _grow_row_pool :1698-1772) and every live kernel of
+// src/avbd3d/kernels_6dof.py, executed with Warp-CPU semantics: each launch
+// is a serial loop over its thread index on one host thread, so in-place
+// updates made by lower thread ids are visible to higher ones. Data is kept
+// in the reference's own per-row c_* layout (capacity n_static + dynamic).
+//
+// Choices that the reference leaves to third-party code (warp-lang 1.13.0)
+// are fixed as follows:
+//  * the broadphase pair list is the exact AABB-overlap set in canonical
+//    (a asc, b asc) order instead of Warp's SAH traversal order;
+//  * quaternion / mat33 / sin / cos / atan2 come from avbd_fp.h;
+//  * the pair-hash key arithmetic wraps in 32 bits (kernels_6dof.py:2588).
+//
+// Build: g++ -O2 -ffp-contract=off -fno-fast-math (see oracle/Makefile).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/avbd.h"
+#include "../avbd_amd/csrc/avbd_fp.h"
+
+using namespace avbd;
+
+namespace {
+
+struct Orc {
+  avbd_params P{};
+  int nb = 0;
+  std::vector<v3> x, v, w, pv, pw, x0, xin, he, lo, hi;
+  std::vector<q4> q, q0, qin;
+  std::vector<float> mass, bfric, prio;
+  std::vector<m33> Il, Iw;
+
+  // rows (reference c_* arrays)
+  int n_static = 0, n_cap = 0, n_dyn_cap = 0;
+  std::vector<int> ctype, ca, cb, csib, cpart, cact, cwas;
+  std::vector<v3> canc, coffa, coffb;
+  std::vector<float> cstiff, cfmin, cfmax, cfrac, cfric, cfrics, clam, cpen, caC0;
+  int n_active_rows = 0;
+
+  // broadphase / SAT
+  int max_pairs = 0, pair_count = 0;
+  std::vector<int> pa, pb, pov, psat;
+  std::vector<v3> pn;
+  std::vector<float> pdepth;
+  // manifold
+  std::vector<int> mf_count, mf_refa;
+  std::vector<v3> mf_n, mf_t, mf_b, mf_offr, mf_offi;
+  // pool + hash
+  int pool_max = 0, pool_count = 0;
+  std::vector<int> pool_n, pool_t, pool_b, pool_c;
+  int hash_cap = 0;
+  std::vector<int> hkeys;
+  std::vector<float> hstate;
+  // body -> row CSR
+  std::vector<int> con_counts, con_starts, con_idx;
+  // coloring
+  std::vector<int> color, color_next, nbr_counts, nbr_starts, nbr_cursor, nbr_idx;
+  std::vector<int> color_counts, color_starts, color_bodies;
+  int n_active_colors = kMaxColors;
+  bool flushed = false;
+};
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+v3 ld3(const float* p, int i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+q4 ld4(const float* p, int i) { return mk4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]); }
+void st3(float* p, int i, v3 a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+void st4(float* p, int i, q4 a) { p[4 * i] = a.x; p[4 * i + 1] = a.y; p[4 * i + 2] = a.z; p[4 * i + 3] = a.w; }
+
+void resize_rows(Orc& S, int cap) {
+  // New tail entries: zeros, sibling/partner = -1 (solver_6dof.py:1711-1756).
+  S.ctype.resize(cap, 0); S.ca.resize(cap, 0); S.cb.resize(cap, 0);
+  S.csib.resize(cap, -1); S.cpart.resize(cap, -1);
+  S.cact.resize(cap, 0); S.cwas.resize(cap, 0);
+  S.canc.resize(cap, mk3(0, 0, 0)); S.coffa.resize(cap, mk3(0, 0, 0)); S.coffb.resize(cap, mk3(0, 0, 0));
+  S.cstiff.resize(cap, 0.f); S.cfmin.resize(cap, 0.f); S.cfmax.resize(cap, 0.f);
+  S.cfrac.resize(cap, 0.f); S.cfric.resize(cap, 0.f); S.cfrics.resize(cap, 0.f);
+  S.clam.resize(cap, 0.f); S.cpen.resize(cap, 0.f); S.caC0.resize(cap, 0.f);
+  S.con_idx.resize(std::max(2 * cap, 1), 0);
+  S.n_cap = cap;
+}
+
+void ensure_pairs(Orc& S, int cap) {
+  // _ensure_pair_buffers (solver_6dof.py:1681-1696) + manifold buffers (:580-595)
+  if ((int)S.pa.size() >= cap && S.max_pairs >= cap) return;
+  S.pa.assign(cap, 0); S.pb.assign(cap, 0); S.pov.assign(cap, 0); S.psat.assign(cap, 0);
+  S.pn.assign(cap, mk3(0, 0, 0)); S.pdepth.assign(cap, 0.f);
+  S.mf_count.assign(cap, 0); S.mf_refa.assign(cap, 0);
+  S.mf_n.assign(cap, mk3(0, 0, 0)); S.mf_t.assign(cap, mk3(0, 0, 0)); S.mf_b.assign(cap, mk3(0, 0, 0));
+  S.mf_offr.assign(4 * (size_t)cap, mk3(0, 0, 0)); S.mf_offi.assign(4 * (size_t)cap, mk3(0, 0, 0));
+  S.max_pairs = cap;
+}
+
+// ---------------------------------------------------------------------------
+// Row C / J helpers (kernels_6dof.py:219-277)
+// ---------------------------------------------------------------------------
+float eval_floor_C(v3 x, q4 q, v3 off, float floor_y) {
+  v3 r = add(x, qrotate(q, off));
+  return r.y - floor_y;
+}
+float eval_pin_axis(v3 x, q4 q, v3 off, v3 anchor, int axis) {
+  v3 r = add(x, qrotate(q, off));
+  if (axis == 0) return r.x - anchor.x;
+  if (axis == 1) return r.y - anchor.y;
+  return r.z - anchor.z;
+}
+float eval_box_box_C(v3 xa, q4 qa, v3 oa, v3 xb, q4 qb, v3 ob, v3 n) {
+  v3 ra = add(xa, qrotate(qa, oa));
+  v3 rb = add(xb, qrotate(qb, ob));
+  return dot(n, sub(ra, rb));
+}
+
+// ---------------------------------------------------------------------------
+// predict_inertial_6dof (kernels_6dof.py:122-187)
+// ---------------------------------------------------------------------------
+void predict(Orc& S, float dt) {
+  v3 g = mk3(S.P.gravity[0], S.P.gravity[1], S.P.gravity[2]);
+  for (int i = 0; i < S.nb; ++i) {
+    float m = S.mass[i];
+    S.x0[i] = S.x[i];
+    S.q0[i] = S.q[i];
+    m33 R = qtomat(S.q[i]);
+    S.Iw[i] = mmul(mmul(R, S.Il[i]), mT(R));
+    if (m <= 0.0f) {
+      S.xin[i] = S.x[i];
+      S.qin[i] = S.q[i];
+      continue;
+    }
+    v3 g_dt2 = mul(g, dt * dt);
+    S.xin[i] = add(add(S.x[i], mul(S.v[i], dt)), g_dt2);
+    q4 dq = quat_from_rotvec(mul(S.w[i], dt));
+    S.qin[i] = qmul(dq, S.q[i]);
+    v3 accel = divs(sub(S.v[i], S.pv[i]), dt);
+    float g_norm = length(g);
+    float wgt = 0.0f;
+    if (g_norm > 0.0f) {
+      v3 g_hat = divs(g, g_norm);
+      float a_ext = dot(accel, g_hat);
+      wgt = clampf_(a_ext / g_norm, 0.0f, 1.0f);
+    }
+    S.x[i] = add(add(S.x[i], mul(S.v[i], dt)), mul(g_dt2, wgt));
+    S.q[i] = qmul(dq, S.q[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// substep_prelude_6dof (kernels_6dof.py:1565-1672), serial over rows.
+// ---------------------------------------------------------------------------
+void prelude(Orc& S, int n_rows) {
+  const float alpha = S.P.alpha, gamma = S.P.gamma;
+  const int post = S.P.post_stabilize;
+  const float main_alpha = post ? 1.0f : alpha;
+  for (int j = 0; j < n_rows; ++j) {
+    float k_floor = S.ctype[j] == kTangent ? kPenaltyMinTangent : kPenaltyMin;
+    float p = clampf_(S.cpen[j] * gamma, k_floor, kPenaltyMax);
+    if (post == 0) S.clam[j] = S.clam[j] * alpha * gamma;
+    float s = S.cstiff[j];
+    if (!std::isnan(s) && s < kInf) p = fmin_(p, s);
+    S.cpen[j] = p;
+
+    if (S.ctype[j] == kTangent && S.cact[j] != 0) {
+      int sib = S.csib[j];
+      if (sib >= 0 && S.cact[sib] != 0) {
+        int partner = S.cpart[j];
+        if (partner < 0 || partner >= j) {
+          float lam_n = fabs_(S.clam[sib]);
+          if (lam_n < 1.0e-9f) {
+            S.cwas[sib] = 0;
+          } else {
+            float lt = S.clam[j];
+            float lb = 0.0f;
+            if (partner >= 0) lb = S.clam[partner];
+            float ltb = sqrt_(lt * lt + lb * lb);
+            S.cwas[sib] = (ltb <= S.cfrics[j] * lam_n) ? 1 : 0;
+          }
+        }
+      }
+    }
+
+    if (S.cact[j] == 0) {
+      S.caC0[j] = 0.0f;
+      continue;
+    }
+    int t = S.ctype[j];
+    int a = S.ca[j];
+    float C0 = 0.0f;
+    bool unilateral = false;
+    if (t == kFloor) {
+      C0 = eval_floor_C(S.x0[a], S.q0[a], S.coffa[j], S.canc[j].y);
+      unilateral = true;
+    } else if (t == kTangent) {
+      v3 tan = S.canc[j];
+      int bb = S.cb[j];
+      if (bb < 0) {
+        v3 r = add(S.x0[a], qrotate(S.q0[a], S.coffa[j]));
+        S.caC0[j] = dot(tan, r);
+      } else {
+        v3 ra = add(S.x0[a], qrotate(S.q0[a], S.coffa[j]));
+        v3 rb = add(S.x0[bb], qrotate(S.q0[bb], S.coffb[j]));
+        S.caC0[j] = dot(tan, sub(ra, rb));
+      }
+      continue;
+    } else if (t == kPin) {
+      C0 = eval_pin_axis(S.x0[a], S.q0[a], S.coffa[j], S.canc[j], S.cb[j]);
+    } else if (t == kBoxBox) {
+      int b = S.cb[j];
+      C0 = eval_box_box_C(S.x0[a], S.q0[a], S.coffa[j], S.x0[b], S.q0[b], S.coffb[j], S.canc[j]);
+      unilateral = true;
+    }
+    if (unilateral && C0 > 0.0f) S.caC0[j] = 0.0f;
+    else S.caC0[j] = main_alpha * C0;
+  }
+}
+
+// cache_alpha_C0_6dof with alpha = 0 (kernels_6dof.py:1733-1811)
+void cache_alpha_c0_post(Orc& S, int n_rows) {
+  for (int j = 0; j < n_rows; ++j) {
+    if (S.cact[j] == 0) { S.caC0[j] = 0.0f; continue; }
+    int t = S.ctype[j];
+    int a = S.ca[j];
+    float C0 = 0.0f;
+    bool unilateral = false;
+    if (t == kFloor) {
+      C0 = eval_floor_C(S.x0[a], S.q0[a], S.coffa[j], S.canc[j].y);
+      unilateral = true;
+    } else if (t == kTangent) {
+      v3 tan = S.canc[j];
+      int bb = S.cb[j];
+      if (bb < 0) {
+        v3 r = add(S.x0[a], qrotate(S.q0[a], S.coffa[j]));
+        S.caC0[j] = dot(tan, r);
+      } else {
+        v3 ra = add(S.x0[a], qrotate(S.q0[a], S.coffa[j]));
+        v3 rb = add(S.x0[bb], qrotate(S.q0[bb], S.coffb[j]));
+        S.caC0[j] = dot(tan, sub(ra, rb));
+      }
+      continue;
+    } else if (t == kPin) {
+      C0 = eval_pin_axis(S.x0[a], S.q0[a], S.coffa[j], S.canc[j], S.cb[j]);
+    } else if (t == kBoxBox) {
+      int b = S.cb[j];
+      C0 = eval_box_box_C(S.x0[a], S.q0[a], S.coffa[j], S.x0[b], S.q0[b], S.coffb[j], S.canc[j]);
+      unilateral = true;
+    }
+    if (unilateral && C0 > 0.0f) S.caC0[j] = 0.0f;
+    else S.caC0[j] = 0.0f * C0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// primal_update_6dof (kernels_6dof.py:322-582), one color, serial over slot.
+// ---------------------------------------------------------------------------
+m33 outer3(v3 a, v3 b) {
+  m33 r;
+  float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = av[i] * bv[j];
+  return r;
+}
+
+void primal_body(Orc& S, int i, float dt) {
+  float m = S.mass[i];
+  if (m <= 0.0f) return;
+  float inv_dt2 = 1.0f / (dt * dt);
+  m33 Iw = S.Iw[i];
+  m33 A = mzero();
+  A.m[0][0] = m * inv_dt2; A.m[1][1] = m * inv_dt2; A.m[2][2] = m * inv_dt2;
+  m33 D = mscale(Iw, inv_dt2);
+  m33 B = mzero();
+  v3 r_lin = mul(sub(S.x[i], S.xin[i]), m * inv_dt2);
+  q4 dq_iner = qmul(S.q[i], qconj(S.qin[i]));
+  v3 dth = quat_to_rotvec(dq_iner);
+  v3 r_ang = mvmul(Iw, mul(dth, inv_dt2));
+  q4 qi = S.q[i];
+  v3 xi = S.x[i];
+  for (int k = S.con_starts[i]; k < S.con_starts[i + 1]; ++k) {
+    int cj = S.con_idx[k];
+    if (S.cact[cj] == 0) continue;
+    int t = S.ctype[cj];
+    float s = S.cstiff[cj];
+    float kp = S.cpen[cj];
+    v3 off_a = S.coffa[cj];
+    v3 anchor = S.canc[cj];
+    v3 jl = mk3(0, 0, 0), ja = mk3(0, 0, 0), r_self = mk3(0, 0, 0), nG = mk3(0, 0, 0);
+    float C = 0.0f;
+    bool haveG = false;
+    if (t == kFloor) {
+      r_self = qrotate(qi, off_a);
+      v3 n = mk3(0.0f, 1.0f, 0.0f);
+      jl = n;
+      ja = cross(r_self, n);
+      C = (xi.y + r_self.y) - anchor.y;
+      nG = n;
+      haveG = true;
+    } else if (t == kTangent) {
+      v3 tan = anchor;
+      int bb = S.cb[cj];
+      if (bb < 0) {
+        r_self = qrotate(qi, off_a);
+        jl = tan;
+        ja = cross(r_self, tan);
+        C = dot(tan, add(xi, r_self));
+      } else {
+        int ba = S.ca[cj];
+        v3 off_b = S.coffb[cj];
+        if (ba == i) {
+          r_self = qrotate(qi, off_a);
+          v3 r_o = qrotate(S.q[bb], off_b);
+          C = dot(tan, sub(add(xi, r_self), add(S.x[bb], r_o)));
+          jl = tan;
+          ja = cross(r_self, tan);
+        } else {
+          v3 r_o = qrotate(S.q[ba], off_a);
+          r_self = qrotate(qi, off_b);
+          C = dot(tan, sub(add(S.x[ba], r_o), add(xi, r_self)));
+          jl = neg(tan);
+          ja = neg(cross(r_self, tan));
+        }
+      }
+      nG = tan;
+      haveG = true;
+    } else if (t == kPin) {
+      int axis = S.cb[cj];
+      r_self = qrotate(qi, off_a);
+      v3 wp = add(xi, r_self);
+      v3 n;
+      if (axis == 0) { n = mk3(1, 0, 0); C = wp.x - anchor.x; }
+      else if (axis == 1) { n = mk3(0, 1, 0); C = wp.y - anchor.y; }
+      else { n = mk3(0, 0, 1); C = wp.z - anchor.z; }
+      jl = n;
+      ja = cross(r_self, n);
+      nG = n;
+      haveG = true;
+    } else if (t == kBoxBox) {
+      int ba = S.ca[cj], bb = S.cb[cj];
+      v3 n = anchor;
+      v3 off_b = S.coffb[cj];
+      if (ba == i) {
+        r_self = qrotate(qi, off_a);
+        v3 r_o = qrotate(S.q[bb], off_b);
+        C = dot(n, sub(add(xi, r_self), add(S.x[bb], r_o)));
+        jl = n;
+        ja = cross(r_self, n);
+      } else {
+        v3 r_o = qrotate(S.q[ba], off_a);
+        r_self = qrotate(qi, off_b);
+        C = dot(n, sub(add(S.x[ba], r_o), add(xi, r_self)));
+        jl = neg(n);
+        ja = neg(cross(r_self, n));
+      }
+      nG = n;
+      haveG = true;
+    }
+    bool hard = s >= kInf;
+    if (hard) C = C - S.caC0[cj];
+    float lam_eff = S.clam[cj];
+    if (!hard) lam_eff = 0.0f;
+    float lam_plus = kp * C + lam_eff;
+    float f_lo, f_hi;
+    if (t == kTangent) {
+      int sib = S.csib[cj];
+      float mu = S.cfric[cj];
+      if (S.cwas[sib] != 0) mu = S.cfrics[cj];
+      float bound = mu * fabs_(S.clam[sib]);
+      f_lo = -bound;
+      f_hi = bound;
+    } else {
+      f_lo = S.cfmin[cj];
+      f_hi = S.cfmax[cj];
+    }
+    float f = clampf_(lam_plus, f_lo, f_hi);
+    float k_lhs = kp;
+    float absC = fabs_(C);
+    if (absC > 1.0e-12f) {
+      if (lam_plus < f_lo) k_lhs = fabs_(f_lo - lam_plus) / absC;
+      else if (lam_plus > f_hi) k_lhs = fabs_(f_hi - lam_plus) / absC;
+    }
+    A = madd(A, mscale(outer3(jl, jl), k_lhs));
+    B = madd(B, mscale(outer3(ja, jl), k_lhs));
+    D = madd(D, mscale(outer3(ja, ja), k_lhs));
+    float fmag = fabs_(f);
+    if (fmag > 0.0f && haveG) {
+      v3 g = mul(geom_stiffness_diag(nG, r_self), fmag);
+      m33 Gd = mzero();
+      Gd.m[0][0] = g.x; Gd.m[1][1] = g.y; Gd.m[2][2] = g.z;
+      D = madd(D, Gd);
+    }
+    r_lin = add(r_lin, mul(jl, f));
+    r_ang = add(r_ang, mul(ja, f));
+  }
+  m33 A_inv = minverse(A);
+  m33 BAinv = mmul(B, A_inv);
+  v3 rhs_th = sub(r_ang, mvmul(BAinv, r_lin));
+  m33 Sm = msub(D, mmul(BAinv, mT(B)));
+  m33 S_inv = minverse(Sm);
+  v3 d_th = mvmul(S_inv, rhs_th);
+  v3 d_x = mvmul(A_inv, sub(r_lin, mvmul(mT(B), d_th)));
+  S.x[i] = sub(S.x[i], d_x);
+  q4 dq = quat_from_rotvec(neg(d_th));
+  S.q[i] = qnormalize(qmul(dq, S.q[i]));
+}
+
+// ---------------------------------------------------------------------------
+// dual_update_6dof (kernels_6dof.py:1451-1548), serial over rows.
+// ---------------------------------------------------------------------------
+void dual(Orc& S, int n_rows) {
+  const float beta = S.P.beta;
+  for (int j = 0; j < n_rows; ++j) {
+    if (S.cact[j] == 0) continue;
+    int t = S.ctype[j];
+    int a = S.ca[j];
+    float C = 0.0f;
+    if (t == kFloor) {
+      C = eval_floor_C(S.x[a], S.q[a], S.coffa[j], S.canc[j].y);
+    } else if (t == kTangent) {
+      v3 tan = S.canc[j];
+      int bb = S.cb[j];
+      if (bb < 0) {
+        v3 r = add(S.x[a], qrotate(S.q[a], S.coffa[j]));
+        C = dot(tan, r);
+      } else {
+        v3 ra = add(S.x[a], qrotate(S.q[a], S.coffa[j]));
+        v3 rb = add(S.x[bb], qrotate(S.q[bb], S.coffb[j]));
+        C = dot(tan, sub(ra, rb));
+      }
+    } else if (t == kPin) {
+      C = eval_pin_axis(S.x[a], S.q[a], S.coffa[j], S.canc[j], S.cb[j]);
+    } else if (t == kBoxBox) {
+      int b = S.cb[j];
+      C = eval_box_box_C(S.x[a], S.q[a], S.coffa[j], S.x[b], S.q[b], S.coffb[j], S.canc[j]);
+    }
+    float s = S.cstiff[j];
+    if (s >= kInf) C = C - S.caC0[j];
+    float lam_eff = S.clam[j];
+    if (!(s >= kInf)) lam_eff = 0.0f;
+    float lo = S.cfmin[j], hi = S.cfmax[j];
+    bool is_t = (t == kTangent);
+    int sib = -1;
+    bool used_static = false;
+    if (is_t) {
+      sib = S.csib[j];
+      float mu = S.cfric[j];
+      if (S.cwas[sib] != 0) { mu = S.cfrics[j]; used_static = true; }
+      float bound = mu * fabs_(S.clam[sib]);
+      lo = -bound;
+      hi = bound;
+    }
+    float nl = clampf_(S.cpen[j] * C + lam_eff, lo, hi);
+    S.clam[j] = nl;
+    if (fabs_(nl) >= S.cfrac[j]) {
+      S.cact[j] = 0;
+      S.clam[j] = 0.0f;
+      S.cpen[j] = 0.0f;
+      continue;
+    }
+    if (nl > lo && nl < hi) {
+      float upper = fmin_(kPenaltyMax, s);
+      S.cpen[j] = fmin_(S.cpen[j] + beta * fabs_(C), upper);
+    } else if (is_t && used_static) {
+      S.cwas[sib] = 0;
+    }
+  }
+}
+
+// finalize_and_cap_6dof (kernels_6dof.py:1871-1901)
+void finalize(Orc& S, float dt, float max_lin, float max_ang) {
+  for (int i = 0; i < S.nb; ++i) {
+    S.pv[i] = S.v[i];
+    S.pw[i] = S.w[i];
+    if (S.mass[i] > 0.0f) {
+      v3 nv = divs(sub(S.x[i], S.x0[i]), dt);
+      q4 dq = qmul(S.q[i], qconj(S.q0[i]));
+      v3 nw = divs(quat_to_rotvec(dq), dt);
+      float sl = length(nv);
+      if (sl > max_lin) nv = mul(nv, max_lin / sl);
+      float sa = length(nw);
+      if (sa > max_ang) nw = mul(nw, max_ang / sa);
+      S.v[i] = nv;
+      S.w[i] = nw;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Contact generation
+// ---------------------------------------------------------------------------
+// compute_body_aabb_6dof (kernels_6dof.py:1922-1943)
+void compute_aabb(Orc& S, float margin) {
+  for (int i = 0; i < S.nb; ++i) {
+    v3 c = S.x[i], e = S.he[i];
+    m33 R = qtomat(S.q[i]);
+    float ex = ((fabs_(R.m[0][0]) * e.x + fabs_(R.m[0][1]) * e.y) + fabs_(R.m[0][2]) * e.z) + margin;
+    float ey = ((fabs_(R.m[1][0]) * e.x + fabs_(R.m[1][1]) * e.y) + fabs_(R.m[1][2]) * e.z) + margin;
+    float ez = ((fabs_(R.m[2][0]) * e.x + fabs_(R.m[2][1]) * e.y) + fabs_(R.m[2][2]) * e.z) + margin;
+    S.lo[i] = sub(c, mk3(ex, ey, ez));
+    S.hi[i] = add(c, mk3(ex, ey, ez));
+  }
+}
+
+bool aabb_overlap(v3 alo, v3 ahi, v3 blo, v3 bhi) {
+  return alo.x <= bhi.x && blo.x <= ahi.x && alo.y <= bhi.y && blo.y <= ahi.y &&
+         alo.z <= bhi.z && blo.z <= ahi.z;
+}
+
+// Exact pair set of bvh_broadphase_pairs (kernels_6dof.py:1946-1978): every
+// (i, j), i < j, whose inclusive AABBs overlap, minus static-static pairs.
+// Returned in canonical order. Sweep-and-prune on x, then sort.
+void all_pairs(Orc& S, std::vector<std::pair<int, int>>& out) {
+  out.clear();
+  std::vector<int> order(S.nb);
+  for (int i = 0; i < S.nb; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    if (S.lo[a].x != S.lo[b].x) return S.lo[a].x < S.lo[b].x;
+    return a < b;
+  });
+  for (int ii = 0; ii < S.nb; ++ii) {
+    int i = order[ii];
+    for (int jj = ii + 1; jj < S.nb; ++jj) {
+      int j = order[jj];
+      if (S.lo[j].x > S.hi[i].x) break;
+      if (!aabb_overlap(S.lo[i], S.hi[i], S.lo[j], S.hi[j])) continue;
+      int a = std::min(i, j), b = std::max(i, j);
+      if (S.mass[a] <= 0.0f && S.mass[b] <= 0.0f) continue;
+      out.push_back({a, b});
+    }
+  }
+  std::sort(out.begin(), out.end());
+}
+
+float proj_radius(v3 L, v3 c0, v3 c1, v3 c2, v3 e) {
+  return (e.x * fabs_(dot(L, c0)) + e.y * fabs_(dot(L, c1))) + e.z * fabs_(dot(L, c2));
+}
+
+v3 col(const m33& R, int k) { return mk3(R.m[0][k], R.m[1][k], R.m[2][k]); }
+
+// obb_sat_pairs (kernels_6dof.py:1991-2130) for one pair.
+void sat_pair(Orc& S, int p, float margin) {
+  int i = S.pa[p], j = S.pb[p];
+  v3 cA = S.x[i], cB = S.x[j], eA = S.he[i], eB = S.he[j];
+  m33 RA = qtomat(S.q[i]), RB = qtomat(S.q[j]);
+  v3 t = sub(cB, cA);
+  v3 A[3] = {col(RA, 0), col(RA, 1), col(RA, 2)};
+  v3 Bc[3] = {col(RB, 0), col(RB, 1), col(RB, 2)};
+  float best = 1.0e20f;
+  int best_idx = -1;
+  v3 best_axis = mk3(0, 0, 0);
+  for (int k = 0; k < 3; ++k) {
+    v3 L = A[k];
+    float rA = comp(eA, k);
+    float rB = proj_radius(L, Bc[0], Bc[1], Bc[2], eB);
+    float tdl = dot(t, L);
+    float ov = (rA + rB) - fabs_(tdl);
+    if (ov < -margin) { S.pov[p] = 0; return; }
+    if (ov < best) { best = ov; best_idx = k; best_axis = tdl > 0.0f ? neg(L) : L; }
+  }
+  for (int k = 0; k < 3; ++k) {
+    v3 L = Bc[k];
+    float rA = proj_radius(L, A[0], A[1], A[2], eA);
+    float rB = comp(eB, k);
+    float tdl = dot(t, L);
+    float ov = (rA + rB) - fabs_(tdl);
+    if (ov < -margin) { S.pov[p] = 0; return; }
+    if (ov < best) { best = ov; best_idx = 3 + k; best_axis = tdl > 0.0f ? neg(L) : L; }
+  }
+  for (int ie = 0; ie < 3; ++ie) {
+    for (int je = 0; je < 3; ++je) {
+      v3 L = cross(A[ie], Bc[je]);
+      float nl = length(L);
+      if (nl < 1.0e-6f) continue;
+      L = divs(L, nl);
+      float rA = proj_radius(L, A[0], A[1], A[2], eA);
+      float rB = proj_radius(L, Bc[0], Bc[1], Bc[2], eB);
+      float tdl = dot(t, L);
+      float ov = (rA + rB) - fabs_(tdl);
+      if (ov < 0.0f) { S.pov[p] = 0; return; }
+      if (ov < best) { best = ov; best_idx = 6 + 3 * ie + je; best_axis = tdl > 0.0f ? neg(L) : L; }
+    }
+  }
+  S.pov[p] = 1;
+  S.psat[p] = best_idx;
+  S.pn[p] = best_axis;
+  S.pdepth[p] = best;
+}
+
+// obb_contact_manifold_6dof (kernels_6dof.py:2221-2537) for one pair.
+void manifold_pair(Orc& S, int p, float margin) {
+  S.mf_count[p] = 0;
+  if (S.pov[p] == 0) return;
+  int sat = S.psat[p];
+  int i = S.pa[p], j = S.pb[p];
+  v3 cA = S.x[i], cB = S.x[j], eA = S.he[i], eB = S.he[j];
+  m33 RA = qtomat(S.q[i]), RB = qtomat(S.q[j]);
+  v3 n_in = S.pn[p];
+  if (sat >= 6) {
+    int e = sat - 6, kA = e / 3, kB = e % 3;
+    v3 dA = col(RA, kA), dB = col(RB, kB);
+    if (length(cross(dA, dB)) < 1.0e-4f) return;
+    int kA1 = (kA + 1) % 3, kA2 = (kA + 2) % 3;
+    v3 A1 = col(RA, kA1), A2 = col(RA, kA2);
+    v3 toB = sub(cB, cA);
+    float sbA = dot(toB, A1) >= 0.0f ? 1.0f : -1.0f;
+    float scA = dot(toB, A2) >= 0.0f ? 1.0f : -1.0f;
+    v3 midA = add(add(cA, mul(A1, sbA * comp(eA, kA1))), mul(A2, scA * comp(eA, kA2)));
+    v3 P1 = sub(midA, mul(dA, comp(eA, kA)));
+    v3 Q1 = add(midA, mul(dA, comp(eA, kA)));
+    int kB1 = (kB + 1) % 3, kB2 = (kB + 2) % 3;
+    v3 B1 = col(RB, kB1), B2 = col(RB, kB2);
+    v3 toA = sub(cA, cB);
+    float sbB = dot(toA, B1) >= 0.0f ? 1.0f : -1.0f;
+    float scB = dot(toA, B2) >= 0.0f ? 1.0f : -1.0f;
+    v3 midB = add(add(cB, mul(B1, sbB * comp(eB, kB1))), mul(B2, scB * comp(eB, kB2)));
+    v3 P2 = sub(midB, mul(dB, comp(eB, kB)));
+    v3 Q2 = add(midB, mul(dB, comp(eB, kB)));
+    v3 d1 = sub(Q1, P1), d2 = sub(Q2, P2), r = sub(P1, P2);
+    float a = dot(d1, d1), ev = dot(d2, d2), f = dot(d2, r);
+    const float eps = 1.0e-12f;
+    float sp = 0.0f, tp = 0.0f;
+    if (a <= eps && ev <= eps) {
+      sp = 0.0f; tp = 0.0f;
+    } else if (a <= eps) {
+      sp = 0.0f;
+      tp = clampf_(f / fmax_(ev, eps), 0.0f, 1.0f);
+    } else if (ev <= eps) {
+      tp = 0.0f;
+      float c = dot(d1, r);
+      sp = clampf_(-c / a, 0.0f, 1.0f);
+    } else {
+      float c = dot(d1, r);
+      float b = dot(d1, d2);
+      float denom = a * ev - b * b;
+      if (denom != 0.0f) sp = clampf_((b * f - c * ev) / denom, 0.0f, 1.0f);
+      else sp = 0.0f;
+      tp = (b * sp + f) / ev;
+      if (tp < 0.0f) {
+        tp = 0.0f;
+        sp = clampf_(-c / a, 0.0f, 1.0f);
+      } else if (tp > 1.0f) {
+        tp = 1.0f;
+        sp = clampf_((b - c) / a, 0.0f, 1.0f);
+      }
+    }
+    v3 pA = add(P1, mul(d1, sp));
+    v3 pB = add(P2, mul(d2, tp));
+    float gap = dot(sub(pA, pB), n_in);
+    if (gap > margin) return;
+    v3 oa = mvmul(mT(RA), sub(pA, cA));
+    v3 ob = mvmul(mT(RB), sub(pB, cB));
+    v3 th = orthonormal_basis(n_in);
+    v3 bh = cross(n_in, th);
+    S.mf_count[p] = 1;
+    S.mf_refa[p] = 1;
+    S.mf_n[p] = n_in;
+    S.mf_t[p] = th;
+    S.mf_b[p] = bh;
+    S.mf_offr[4 * (size_t)p] = oa;
+    S.mf_offi[4 * (size_t)p] = ob;
+    return;
+  }
+  int ref_is_a = sat < 3 ? 1 : 0;
+  S.mf_refa[p] = ref_is_a;
+  int ref_axis;
+  v3 n_used, ref_c, inc_c, e_ref, e_inc;
+  m33 R_ref, R_inc;
+  if (sat < 3) {
+    ref_axis = sat; n_used = n_in; ref_c = cA; R_ref = RA; e_ref = eA; inc_c = cB; R_inc = RB; e_inc = eB;
+  } else {
+    ref_axis = sat - 3; n_used = neg(n_in); ref_c = cB; R_ref = RB; e_ref = eB; inc_c = cA; R_inc = RA; e_inc = eA;
+  }
+  v3 rcol = col(R_ref, ref_axis);
+  float rsign = dot(rcol, n_used) > 0.0f ? -1.0f : 1.0f;
+  v3 rfn = mul(rcol, rsign);
+  v3 rfc = add(ref_c, mul(rfn, comp(e_ref, ref_axis)));
+  int ax1 = (ref_axis + 1) % 3, ax2 = (ref_axis + 2) % 3;
+  v3 u_ref = col(R_ref, ax1), v_ref = col(R_ref, ax2);
+  float eu = comp(e_ref, ax1), evv = comp(e_ref, ax2);
+  int inc_axis = 0;
+  float inc_sign = 1.0f, best_dot = -1.0e20f;
+  for (int k = 0; k < 3; ++k) {
+    v3 c = col(R_inc, k);
+    float dpos = dot(c, n_used);
+    if (dpos > best_dot) { best_dot = dpos; inc_axis = k; inc_sign = 1.0f; }
+    float dneg = -dpos;
+    if (dneg > best_dot) { best_dot = dneg; inc_axis = k; inc_sign = -1.0f; }
+  }
+  v3 ifn = mul(col(R_inc, inc_axis), inc_sign);
+  v3 ifc = add(inc_c, mul(ifn, comp(e_inc, inc_axis)));
+  int iax1 = (inc_axis + 1) % 3, iax2 = (inc_axis + 2) % 3;
+  v3 iu = col(R_inc, iax1), iv = col(R_inc, iax2);
+  float ieu = comp(e_inc, iax1), iev = comp(e_inc, iax2);
+  v3 buf[2][8];
+  int cur = 0;
+  buf[0][0] = add(add(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][1] = add(sub(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][2] = sub(sub(ifc, mul(iu, ieu)), mul(iv, iev));
+  buf[0][3] = sub(add(ifc, mul(iu, ieu)), mul(iv, iev));
+  int len = 4;
+  for (int pl = 0; pl < 4; ++pl) {
+    v3 ppt, pn;
+    if (pl == 0) { ppt = add(rfc, mul(u_ref, eu)); pn = u_ref; }
+    else if (pl == 1) { ppt = sub(rfc, mul(u_ref, eu)); pn = neg(u_ref); }
+    else if (pl == 2) { ppt = add(rfc, mul(v_ref, evv)); pn = v_ref; }
+    else { ppt = sub(rfc, mul(v_ref, evv)); pn = neg(v_ref); }
+    if (len == 0) return;
+    int out = 0;
+    for (int iv2 = 0; iv2 < len; ++iv2) {
+      v3 ap = buf[cur][iv2];
+      v3 bp = buf[cur][(iv2 + 1) % len];
+      float da = dot(sub(ap, ppt), pn);
+      float db = dot(sub(bp, ppt), pn);
+      if (da <= 0.0f) {
+        if (out < 8) buf[1 - cur][out++] = ap;
+        if (db > 0.0f) {
+          float tt = da / (da - db);
+          if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+        }
+      } else if (db <= 0.0f) {
+        float tt = da / (da - db);
+        if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+      }
+    }
+    cur = 1 - cur;
+    len = out;
+  }
+  if (len == 0) return;
+  float bd[4] = {-1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
+  int bi[4] = {-1, -1, -1, -1};
+  int nbest = 0;
+  for (int iv2 = 0; iv2 < len; ++iv2) {
+    float ds = dot(sub(buf[cur][iv2], rfc), rfn);
+    if (ds < margin) {
+      float depth = -ds;
+      if (nbest < 4) {
+        bd[nbest] = depth; bi[nbest] = iv2; ++nbest;
+      } else {
+        int mp = 0;
+        float mv = bd[0];
+        for (int k = 1; k < 4; ++k) if (bd[k] < mv) { mv = bd[k]; mp = k; }
+        if (depth > mv) { bd[mp] = depth; bi[mp] = iv2; }
+      }
+    }
+  }
+  if (nbest == 0) return;
+  for (int is = 0; is < nbest - 1; ++is) {
+    int mp = is;
+    for (int k = is + 1; k < nbest; ++k) if (bd[k] > bd[mp]) mp = k;
+    if (mp != is) {
+      std::swap(bd[is], bd[mp]);
+      std::swap(bi[is], bi[mp]);
+    }
+  }
+  S.mf_n[p] = n_used;
+  v3 th = orthonormal_basis(n_used);
+  S.mf_t[p] = th;
+  S.mf_b[p] = cross(n_used, th);
+  S.mf_count[p] = nbest;
+  m33 Rtr = mT(R_ref), Rti = mT(R_inc);
+  for (int c = 0; c < nbest; ++c) {
+    v3 pinc = buf[cur][bi[c]];
+    float ds = dot(sub(pinc, rfc), rfn);
+    v3 pref = sub(pinc, mul(rfn, ds));
+    S.mf_offr[4 * (size_t)p + c] = mvmul(Rtr, sub(pref, ref_c));
+    S.mf_offi[4 * (size_t)p + c] = mvmul(Rti, sub(pinc, inc_c));
+  }
+}
+
+// gpu_pool_emit_rows (kernels_6dof.py:2695-2912), serial over pairs.
+void emit_rows(Orc& S, int n_pairs) {
+  const float default_mu = S.P.self_friction, mult = S.P.friction_static_mult;
+  for (int p = 0; p < n_pairs; ++p) {
+    int nc = S.mf_count[p];
+    if (nc == 0) continue;
+    int ain = S.pa[p], bin = S.pb[p];
+    int ref_i, inc_i;
+    if (S.mf_refa[p] == 1) { ref_i = ain; inc_i = bin; } else { ref_i = bin; inc_i = ain; }
+    v3 n = S.mf_n[p], th = S.mf_t[p], bh = S.mf_b[p];
+    float mr = S.bfric[ref_i], mi = S.bfric[inc_i];
+    float mu = default_mu;
+    if (mr > 0.0f || mi > 0.0f) mu = sqrt_(mr * mi);
+    float mus = mu * mult;
+    int ka = ref_i, kb = inc_i;
+    if (kb < ka) { ka = inc_i; kb = ref_i; }
+    for (int c = 0; c < nc; ++c) {
+      v3 oref = S.mf_offr[4 * (size_t)p + c], oinc = S.mf_offi[4 * (size_t)p + c];
+      int rpc = mu <= 0.0f ? 1 : 3;
+      int rb = S.n_active_rows;
+      S.n_active_rows += rpc;
+      if (rb + rpc > S.n_cap) break;
+      int ni = rb;
+      S.ctype[ni] = kBoxBox; S.ca[ni] = ref_i; S.cb[ni] = inc_i; S.canc[ni] = n;
+      S.coffa[ni] = oref; S.coffb[ni] = oinc; S.cstiff[ni] = kInf; S.cfmin[ni] = -kInf;
+      S.cfmax[ni] = 0.0f; S.caC0[ni] = 0.0f; S.cact[ni] = 1; S.cfrac[ni] = kInf;
+      S.csib[ni] = -1; S.cpart[ni] = -1; S.cfric[ni] = 0.0f; S.cfrics[ni] = 0.0f;
+      int ti = -1, bi = -1;
+      if (mu > 0.0f) {
+        ti = rb + 1; bi = rb + 2;
+        const int rows[2] = {ti, bi};
+        const v3 dirs[2] = {th, bh};
+        for (int r = 0; r < 2; ++r) {
+          int q = rows[r];
+          S.ctype[q] = kTangent; S.ca[q] = ref_i; S.cb[q] = inc_i; S.canc[q] = dirs[r];
+          S.coffa[q] = oref; S.coffb[q] = oinc; S.cstiff[q] = kInf; S.cfmin[q] = -kInf;
+          S.cfmax[q] = kInf; S.caC0[q] = 0.0f; S.cact[q] = 1; S.cfrac[q] = kInf;
+          S.csib[q] = ni; S.cpart[q] = rows[1 - r]; S.cfric[q] = mu; S.cfrics[q] = mus;
+        }
+      }
+      float ln = 0.f, lt = 0.f, lb = 0.f, kn = kPenaltyMin, kt = kPenaltyMinTangent, kb2 = kPenaltyMinTangent;
+      int wt = 0, wb = 0;
+      int key = encode_pair_key(ka, kb, c);
+      int h = hash_pair_slot(key, S.hash_cap);
+      for (int probe = 0; probe < S.hash_cap; ++probe) {
+        int slot = (h + probe) % S.hash_cap;
+        int sk = S.hkeys[slot];
+        if (sk == key) {
+          const float* st = &S.hstate[8 * (size_t)slot];
+          ln = st[0]; lt = st[1]; lb = st[2]; kn = st[3]; kt = st[4]; kb2 = st[5];
+          wt = (int)st[6]; wb = (int)st[7];
+          break;
+        }
+        if (sk == kHashEmpty) break;
+      }
+      S.clam[ni] = ln; S.cpen[ni] = kn; S.cwas[ni] = 0;
+      if (ti >= 0) { S.clam[ti] = lt; S.cpen[ti] = kt; S.cwas[ti] = wt; }
+      if (bi >= 0) { S.clam[bi] = lb; S.cpen[bi] = kb2; S.cwas[bi] = wb; }
+      int pid = S.pool_count++;
+      if (pid >= S.pool_max) continue;
+      S.pool_n[pid] = ni; S.pool_t[pid] = ti; S.pool_b[pid] = bi; S.pool_c[pid] = c;
+    }
+  }
+}
+
+// gpu_pool_hash_clear + gpu_pool_hash_collect (kernels_6dof.py:2915-2995)
+void hash_refresh(Orc& S) {
+  std::fill(S.hkeys.begin(), S.hkeys.end(), kHashEmpty);
+  int n = std::min(S.pool_count, S.pool_max);
+  for (int p = 0; p < n; ++p) {
+    int ni = S.pool_n[p];
+    if (S.cact[ni] == 0) continue;
+    int ar = S.ca[ni], br = S.cb[ni];
+    int ka = ar, kb = br;
+    if (kb < ka) { ka = br; kb = ar; }
+    int c = S.pool_c[p];
+    float ln = S.clam[ni], kn = S.cpen[ni];
+    float lt = 0.f, lb = 0.f, kt = kPenaltyMinTangent, kbb = kPenaltyMinTangent;
+    int wt = 0, wb = 0;
+    int ti = S.pool_t[p], bi = S.pool_b[p];
+    if (ti >= 0) { lt = S.clam[ti]; kt = S.cpen[ti]; wt = S.cwas[ti]; }
+    if (bi >= 0) { lb = S.clam[bi]; kbb = S.cpen[bi]; wb = S.cwas[bi]; }
+    int key = encode_pair_key(ka, kb, c);
+    int h = hash_pair_slot(key, S.hash_cap);
+    for (int probe = 0; probe < S.hash_cap; ++probe) {
+      int slot = (h + probe) % S.hash_cap;
+      int prev = S.hkeys[slot];
+      if (prev == kHashEmpty) S.hkeys[slot] = key;
+      if (prev == kHashEmpty || prev == key) {
+        float* st = &S.hstate[8 * (size_t)slot];
+        st[0] = ln; st[1] = lt; st[2] = lb; st[3] = kn; st[4] = kt; st[5] = kbb;
+        st[6] = (float)wt; st[7] = (float)wb;
+        break;
+      }
+    }
+  }
+}
+
+// _grow_row_pool (solver_6dof.py:1698-1772)
+void grow_row_pool(Orc& S, int new_dyn) {
+  int new_cap = S.n_static + new_dyn;
+  if (new_cap <= S.n_cap) return;
+  resize_rows(S, new_cap);
+  S.n_dyn_cap = new_dyn;
+  int new_pool = std::max(S.pool_max, S.max_pairs * 4);
+  if (new_pool > S.pool_max) {
+    S.pool_n.assign(new_pool, 0); S.pool_t.assign(new_pool, 0);
+    S.pool_b.assign(new_pool, 0); S.pool_c.assign(new_pool, 0);
+    S.pool_max = new_pool;
+  }
+}
+
+// _gpu_emit_dynamic_contacts (solver_6dof.py:1774-1916)
+void emit_rows(Orc& S, int n_pairs);
+void emit_dynamic_impl(Orc& S, bool fixed_cap, avbd_report* rep);
+void emit_dynamic(Orc& S, bool fixed_cap, avbd_report* rep) {
+  auto t0 = std::chrono::steady_clock::now();
+  emit_dynamic_impl(S, fixed_cap, rep);
+  rep->broadphase_ms = (float)std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t0).count();
+}
+void emit_dynamic_impl(Orc& S, bool fixed_cap, avbd_report* rep) {
+  const float margin = S.P.margin;
+  ensure_pairs(S, std::max(256, 16 * S.nb));
+  compute_aabb(S, margin);
+  std::vector<std::pair<int, int>> pairs;
+  all_pairs(S, pairs);
+  int n = (int)pairs.size();
+  if (!fixed_cap && n > S.max_pairs) {
+    ensure_pairs(S, std::max(2 * S.max_pairs, 2 * n));
+    rep->pair_grew = 1;
+  }
+  S.pair_count = n;
+  int kept = std::min(n, S.max_pairs);
+  for (int p = 0; p < kept; ++p) { S.pa[p] = pairs[p].first; S.pb[p] = pairs[p].second; }
+  rep->pair_count = n;
+  rep->pair_capacity = S.max_pairs;
+  if (!fixed_cap && n == 0) return;
+  for (int p = 0; p < kept; ++p) sat_pair(S, p, margin);
+  for (int p = 0; p < kept; ++p) manifold_pair(S, p, margin);
+  emit_rows(S, kept);
+}
+
+// ---------------------------------------------------------------------------
+// Coloring (solver_6dof.py:680-844, kernels_6dof.py:3022-3336)
+// ---------------------------------------------------------------------------
+int live_rows(const Orc& S) { return std::min(S.n_active_rows, S.n_cap); }
+
+int count_conflicts(const Orc& S) {
+  int n = live_rows(S), c = 0;
+  for (int j = 0; j < n; ++j) {
+    if (S.ctype[j] == kPin) continue;
+    int a = S.ca[j], b = S.cb[j];
+    if (a < 0 || b < 0 || a == b) continue;
+    if (S.color[a] == S.color[b]) ++c;
+  }
+  return c;
+}
+
+unsigned used_mask(const Orc& S, int i) {
+  unsigned used = 0u;
+  int st = S.nbr_starts[i], en = st + S.nbr_counts[i];
+  for (int k = st; k < en; ++k) {
+    int c = S.color[S.nbr_idx[k]];
+    if (c >= 0 && c < kMaxColors) used |= (1u << c);
+  }
+  return used;
+}
+int lowest_free(unsigned used) {
+  for (int c = 0; c < kMaxColors; ++c) if ((used & (1u << c)) == 0u) return c;
+  return kMaxColors - 1;
+}
+
+void recolor(Orc& S) {
+  int nb = S.nb;
+  if (nb < 1) return;
+  std::fill(S.color_counts.begin(), S.color_counts.end(), 0);
+  std::fill(S.nbr_counts.begin(), S.nbr_counts.end(), 0);
+  std::fill(S.nbr_cursor.begin(), S.nbr_cursor.end(), 0);
+  std::fill(S.color.begin(), S.color.end(), -1);
+  int n = live_rows(S);
+  for (int j = 0; j < n; ++j) {
+    if (S.ctype[j] == kPin) continue;
+    int a = S.ca[j], b = S.cb[j];
+    if (a < 0 || b < 0 || a == b) continue;
+    S.nbr_counts[a]++; S.nbr_counts[b]++;
+  }
+  S.nbr_starts[0] = 0;
+  for (int i = 0; i < nb; ++i) S.nbr_starts[i + 1] = S.nbr_starts[i] + S.nbr_counts[i];
+  S.nbr_idx.assign(std::max(S.nbr_starts[nb], 1), 0);
+  for (int j = 0; j < n; ++j) {
+    if (S.ctype[j] == kPin) continue;
+    int a = S.ca[j], b = S.cb[j];
+    if (a < 0 || b < 0 || a == b) continue;
+    S.nbr_idx[S.nbr_starts[a] + S.nbr_cursor[a]++] = b;
+    S.nbr_idx[S.nbr_starts[b] + S.nbr_cursor[b]++] = a;
+  }
+  if (S.P.coloring_mode == AVBD_COLOR_JACOBI) {
+    for (int round = 0; round < kMaxColors; ++round) {
+      // gpu_color_spec_assign: serial, in place
+      for (int i = 0; i < nb; ++i) {
+        if (S.color[i] != -1) continue;
+        S.color[i] = lowest_free(used_mask(S, i));
+      }
+      // gpu_color_spec_resolve into the second buffer, then swap
+      for (int i = 0; i < nb; ++i) {
+        int ci = S.color[i];
+        S.color_next[i] = ci;
+        if (ci < 0) continue;
+        float mp = S.prio[i];
+        int st = S.nbr_starts[i], en = st + S.nbr_counts[i];
+        for (int k = st; k < en; ++k) {
+          int nbi = S.nbr_idx[k];
+          if (S.color[nbi] == ci && (S.prio[nbi] > mp || (S.prio[nbi] == mp && nbi < i))) {
+            S.color_next[i] = -1;
+            break;
+          }
+        }
+      }
+      std::swap(S.color, S.color_next);
+      int unc = 0;
+      for (int i = 0; i < nb; ++i) if (S.color[i] == -1) ++unc;
+      if (unc == 0) break;
+    }
+  } else {
+    for (int round = 0; round < kJpRounds; ++round) {
+      for (int i = 0; i < nb; ++i) {
+        if (S.color[i] != -1) continue;
+        float mp = S.prio[i];
+        int st = S.nbr_starts[i], en = st + S.nbr_counts[i];
+        bool lose = false;
+        for (int k = st; k < en; ++k) {
+          int nbi = S.nbr_idx[k];
+          if (S.color[nbi] == -1 && S.prio[nbi] >= mp) {
+            if (S.prio[nbi] > mp || nbi < i) { lose = true; break; }
+          }
+        }
+        if (lose) continue;
+        S.color[i] = lowest_free(used_mask(S, i));
+      }
+    }
+  }
+  for (int i = 0; i < nb; ++i) if (S.color[i] == -1) S.color[i] = 0;
+  for (int i = 0; i < nb; ++i) {
+    int c = S.color[i];
+    if (c >= 0 && c < kMaxColors) S.color_counts[c]++;
+  }
+  S.color_starts[0] = 0;
+  for (int c = 0; c < kMaxColors; ++c) S.color_starts[c + 1] = S.color_starts[c] + S.color_counts[c];
+  std::vector<int> cursor(kMaxColors, 0);
+  for (int i = 0; i < nb; ++i) {
+    int c = S.color[i];
+    if (c < 0 || c >= kMaxColors) continue;
+    S.color_bodies[S.color_starts[c] + cursor[c]++] = i;
+  }
+  int mx = 0;
+  for (int i = 0; i < nb; ++i) mx = std::max(mx, S.color[i]);
+  S.n_active_colors = std::max(1, std::min(mx + 1, kMaxColors));
+}
+
+// gpu_csr_count / starts / scatter (kernels_6dof.py:2626-2692)
+void build_csr(Orc& S) {
+  int nb = S.nb, n = live_rows(S);
+  std::fill(S.con_counts.begin(), S.con_counts.end(), 0);
+  for (int j = 0; j < n; ++j) {
+    int a = S.ca[j];
+    if (a >= 0) S.con_counts[a]++;
+    if (S.ctype[j] != kPin) {
+      int b = S.cb[j];
+      if (b >= 0) S.con_counts[b]++;
+    }
+  }
+  S.con_starts[0] = 0;
+  for (int i = 0; i < nb; ++i) S.con_starts[i + 1] = S.con_starts[i] + S.con_counts[i];
+  std::vector<int> cursor(S.con_starts.begin(), S.con_starts.begin() + nb);
+  for (int j = 0; j < n; ++j) {
+    int a = S.ca[j];
+    if (a >= 0) S.con_idx[cursor[a]++] = j;
+    if (S.ctype[j] != kPin) {
+      int b = S.cb[j];
+      if (b >= 0) S.con_idx[cursor[b]++] = j;
+    }
+  }
+}
+
+// Solver6DOF._step_one (solver_6dof.py:1232-1464), readback (non-resident) path.
+int substep(Orc& S, const avbd_params* p, int color_dirty, avbd_report* rep) {
+  std::memset(rep, 0, sizeof(*rep));
+  S.P = *p;
+  const float dt = p->dt;
+  rep->n_active_colors = S.n_active_colors;
+  if (S.nb == 0) return AVBD_OK;
+  const bool fixed = p->fixed_capacity != 0;
+  S.n_active_rows = S.n_static;
+  S.pool_count = 0;
+  S.pair_count = 0;
+  int n_active = S.n_static;
+  rep->row_capacity = S.n_cap;
+  if (p->self_collide && S.nb >= 2) {
+    emit_dynamic(S, fixed, rep);
+    if (fixed) {
+      n_active = S.n_cap;
+    } else {
+      int raw = S.n_active_rows, cap = S.n_cap;
+      if (raw > cap) {
+        rep->row_overflow = 1;
+        grow_row_pool(S, std::max(2 * S.n_dyn_cap, (raw - S.n_static) * 2));
+      }
+      n_active = std::min(raw, cap);
+    }
+  }
+  rep->n_active_rows = S.n_active_rows;
+  rep->new_row_capacity = S.n_cap;
+  rep->pool_count = S.pool_count;
+  rep->pair_capacity = S.max_pairs;
+  rep->n_contacts = std::min(S.pool_count, S.pool_max);
+
+  bool need = color_dirty != 0;
+  if (!need) {
+    rep->color_conflicts = count_conflicts(S);
+    need = rep->color_conflicts > 0;
+  }
+  if (need) {
+    recolor(S);
+    rep->recolored = 1;
+  }
+  rep->n_active_colors = S.n_active_colors;
+
+  int rows = live_rows(S);
+  if (n_active > 0) build_csr(S);
+  predict(S, dt);
+  if (n_active > 0) prelude(S, rows);
+
+  float max_lin = p->max_linear_speed, max_ang = p->max_angular_speed;
+  if (!(max_lin > 0.0f && max_lin < kInf && max_ang > 0.0f && max_ang < kInf)) {
+    max_lin = kInf;
+    max_ang = kInf;
+  }
+  const int I = p->iterations;
+  const int total = I + (p->post_stabilize ? 1 : 0);
+  for (int it = 0; it < total; ++it) {
+    if (p->post_stabilize && it == I) cache_alpha_c0_post(S, rows);
+    for (int c = 0; c < S.n_active_colors; ++c) {
+      for (int k = S.color_starts[c]; k < S.color_starts[c + 1]; ++k) primal_body(S, S.color_bodies[k], dt);
+    }
+    if (it < I) dual(S, rows);
+    if (it == I - 1) finalize(S, dt, max_lin, max_ang);
+  }
+  if (p->self_collide && S.hash_cap > 0) hash_refresh(S);
+  return AVBD_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API (names mirror include/avbd.h with an orc_ prefix)
+// ===========================================================================
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+int orc_create(void** out) {
+  *out = new Orc();
+  return AVBD_OK;
+}
+int orc_destroy(void* h) {
+  delete static_cast<Orc*>(h);
+  return AVBD_OK;
+}
+
+int orc_flush(void* h, const avbd_scene* sc) {
+  Orc& S = *static_cast<Orc*>(h);
+  int nb = sc->n_bodies;
+  if (nb < 0 || sc->n_static_rows < 0) return fail(AVBD_E_ARG, "negative size");
+  S.nb = nb;
+  S.x.resize(nb); S.q.resize(nb); S.v.resize(nb); S.w.resize(nb); S.pv.resize(nb); S.pw.resize(nb);
+  S.x0.resize(nb); S.q0.resize(nb); S.xin.resize(nb); S.qin.resize(nb); S.he.resize(nb);
+  S.lo.resize(nb); S.hi.resize(nb);
+  S.mass.resize(nb); S.bfric.resize(nb); S.prio.resize(nb); S.Il.resize(nb); S.Iw.resize(nb);
+  for (int i = 0; i < nb; ++i) {
+    S.x[i] = ld3(sc->x, i); S.q[i] = ld4(sc->q, i); S.v[i] = ld3(sc->v, i); S.w[i] = ld3(sc->w, i);
+    S.pv[i] = ld3(sc->prev_v, i); S.pw[i] = ld3(sc->prev_w, i);
+    S.x0[i] = mk3(0, 0, 0); S.q0[i] = mk4(0, 0, 0, 0); S.xin[i] = mk3(0, 0, 0); S.qin[i] = mk4(0, 0, 0, 0);
+    S.he[i] = ld3(sc->half_extents, i);
+    S.mass[i] = sc->mass[i]; S.bfric[i] = sc->friction[i]; S.prio[i] = sc->priority[i];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) S.Il[i].m[r][c] = sc->inertia_local[9 * i + 3 * r + c];
+    S.Iw[i] = mzero();
+  }
+  int ns = sc->n_static_rows;
+  S.n_static = ns;
+  S.n_dyn_cap = sc->n_dyn_capacity;
+  S.ctype.clear(); S.ca.clear(); S.cb.clear(); S.csib.clear(); S.cpart.clear(); S.cact.clear(); S.cwas.clear();
+  S.canc.clear(); S.coffa.clear(); S.coffb.clear(); S.cstiff.clear(); S.cfmin.clear(); S.cfmax.clear();
+  S.cfrac.clear(); S.cfric.clear(); S.cfrics.clear(); S.clam.clear(); S.cpen.clear(); S.caC0.clear();
+  resize_rows(S, ns + sc->n_dyn_capacity);
+  // dynamic region: penalty 1.0, active 0 (solver_6dof.py:1118-1129)
+  std::fill(S.cpen.begin(), S.cpen.end(), 1.0f);
+  for (int j = 0; j < ns; ++j) {
+    S.ctype[j] = sc->type[j]; S.ca[j] = sc->body_a[j]; S.cb[j] = sc->body_b[j];
+    S.csib[j] = sc->sibling[j]; S.cpart[j] = sc->partner[j];
+    S.canc[j] = ld3(sc->anchor, j); S.coffa[j] = ld3(sc->off_a, j); S.coffb[j] = ld3(sc->off_b, j);
+    S.cstiff[j] = sc->stiffness[j]; S.cfmin[j] = sc->fmin[j]; S.cfmax[j] = sc->fmax[j];
+    S.cfrac[j] = sc->fracture[j]; S.cfric[j] = sc->friction_d[j]; S.cfrics[j] = sc->friction_s[j];
+    S.clam[j] = sc->lambda[j]; S.cpen[j] = sc->penalty[j]; S.cact[j] = sc->active[j];
+    S.cwas[j] = sc->was_static[j];
+  }
+  S.n_active_rows = ns;
+  S.max_pairs = 0;
+  S.pa.clear();
+  ensure_pairs(S, std::max(sc->max_pairs, 1));
+  S.max_pairs = sc->max_pairs;
+  S.pool_max = sc->pool_max;
+  S.pool_n.assign(std::max(S.pool_max, 1), 0); S.pool_t.assign(std::max(S.pool_max, 1), 0);
+  S.pool_b.assign(std::max(S.pool_max, 1), 0); S.pool_c.assign(std::max(S.pool_max, 1), 0);
+  S.hash_cap = sc->hash_cap;
+  S.hkeys.assign(std::max(S.hash_cap, 1), kHashEmpty);
+  S.hstate.assign(8 * (size_t)std::max(S.hash_cap, 1), 0.0f);
+  S.con_counts.assign(std::max(nb, 1), 0);
+  S.con_starts.assign(std::max(nb + 1, 2), 0);
+  S.color.assign(std::max(nb, 1), -1);
+  S.color_next.assign(std::max(nb, 1), 0);
+  S.nbr_counts.assign(std::max(nb, 1), 0);
+  S.nbr_starts.assign(std::max(nb + 1, 2), 0);
+  S.nbr_cursor.assign(std::max(nb, 1), 0);
+  S.nbr_idx.assign(1, 0);
+  S.color_counts.assign(kMaxColors, 0);
+  S.color_starts.assign(kMaxColors + 1, 0);
+  S.color_bodies.assign(std::max(nb, 1), 0);
+  S.n_active_colors = kMaxColors;
+  S.pair_count = 0;
+  S.pool_count = 0;
+  S.flushed = true;
+  return AVBD_OK;
+}
+
+int orc_substep(void* h, const avbd_params* p, int color_dirty, avbd_report* out) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (!S.flushed) return fail(AVBD_E_STATE, "substep before flush");
+  return substep(S, p, color_dirty, out);
+}
+
+int orc_read_bodies(void* h, float* x, float* q, float* v, float* w, float* pv, float* pw) {
+  Orc& S = *static_cast<Orc*>(h);
+  for (int i = 0; i < S.nb; ++i) {
+    if (x) st3(x, i, S.x[i]);
+    if (q) st4(q, i, S.q[i]);
+    if (v) st3(v, i, S.v[i]);
+    if (w) st3(w, i, S.w[i]);
+    if (pv) st3(pv, i, S.pv[i]);
+    if (pw) st3(pw, i, S.pw[i]);
+  }
+  return AVBD_OK;
+}
+
+int orc_write_bodies(void* h, int field, int first, int n, const float* vals) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (first < 0 || n < 0 || first + n > S.nb) return fail(AVBD_E_ARG, "body range");
+  for (int k = 0; k < n; ++k) {
+    int i = first + k;
+    switch (field) {
+      case 0: S.x[i] = ld3(vals, k); break;
+      case 1: S.q[i] = ld4(vals, k); break;
+      case 2: S.v[i] = ld3(vals, k); break;
+      case 3: S.w[i] = ld3(vals, k); break;
+      case 4: S.pv[i] = ld3(vals, k); break;
+      case 5: S.pw[i] = ld3(vals, k); break;
+      default: return fail(AVBD_E_ARG, "field");
+    }
+  }
+  return AVBD_OK;
+}
+
+int orc_read_rows(void* h, int n, float* lam, float* pen, int* act, int* was, int* type,
+                  int* ba, int* bb, float* ac0) {
+  Orc& S = *static_cast<Orc*>(h);
+  int live = live_rows(S);
+  for (int j = 0; j < n; ++j) {
+    bool ok = j < live;
+    if (lam) lam[j] = ok ? S.clam[j] : 0.0f;
+    if (pen) pen[j] = ok ? S.cpen[j] : 0.0f;
+    if (act) act[j] = ok ? S.cact[j] : 0;
+    if (was) was[j] = ok ? S.cwas[j] : 0;
+    if (type) type[j] = ok ? S.ctype[j] : 0;
+    if (ba) ba[j] = ok ? S.ca[j] : 0;
+    if (bb) bb[j] = ok ? S.cb[j] : 0;
+    if (ac0) ac0[j] = ok ? S.caC0[j] : 0.0f;
+  }
+  return AVBD_OK;
+}
+
+int orc_write_static_rows(void* h, int n, const float* lam, const float* pen, const int* act,
+                          const int* was) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (n > S.n_static) return fail(AVBD_E_ARG, "more rows than static rows");
+  for (int j = 0; j < n; ++j) {
+    if (lam) S.clam[j] = lam[j];
+    if (pen) S.cpen[j] = pen[j];
+    if (act) S.cact[j] = act[j];
+    if (was) S.cwas[j] = was[j];
+  }
+  return AVBD_OK;
+}
+
+int orc_read_colors(void* h, int* c) {
+  Orc& S = *static_cast<Orc*>(h);
+  for (int i = 0; i < S.nb; ++i) c[i] = S.color[i];
+  return AVBD_OK;
+}
+
+int orc_write_colors(void* h, const int* c, int n_active_colors) {
+  Orc& S = *static_cast<Orc*>(h);
+  for (int i = 0; i < S.nb; ++i) S.color[i] = c[i];
+  // rebuild buckets the way _gpu_recolor's tail does
+  std::fill(S.color_counts.begin(), S.color_counts.end(), 0);
+  for (int i = 0; i < S.nb; ++i) if (S.color[i] >= 0 && S.color[i] < kMaxColors) S.color_counts[S.color[i]]++;
+  S.color_starts[0] = 0;
+  for (int k = 0; k < kMaxColors; ++k) S.color_starts[k + 1] = S.color_starts[k] + S.color_counts[k];
+  std::vector<int> cur(kMaxColors, 0);
+  for (int i = 0; i < S.nb; ++i) {
+    int k = S.color[i];
+    if (k < 0 || k >= kMaxColors) continue;
+    S.color_bodies[S.color_starts[k] + cur[k]++] = i;
+  }
+  S.n_active_colors = n_active_colors;
+  return AVBD_OK;
+}
+
+int orc_read_counters(void* h, int* nar, int* pc, int* poc) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (nar) *nar = S.n_active_rows;
+  if (pc) *pc = S.pair_count;
+  if (poc) *poc = S.pool_count;
+  return AVBD_OK;
+}
+
+int orc_count_color_conflicts(void* h, int* out) {
+  Orc& S = *static_cast<Orc*>(h);
+  *out = count_conflicts(S);
+  return AVBD_OK;
+}
+
+int orc_export_hash(void* h, int cap, int* keys, float* st, int* n) {
+  Orc& S = *static_cast<Orc*>(h);
+  int k = 0;
+  for (int s = 0; s < S.hash_cap; ++s) {
+    if (S.hkeys[s] == kHashEmpty) continue;
+    if (k < cap) {
+      keys[k] = S.hkeys[s];
+      for (int f = 0; f < 8; ++f) st[8 * (size_t)k + f] = S.hstate[8 * (size_t)s + f];
+    }
+    ++k;
+  }
+  *n = k;
+  return AVBD_OK;
+}
+
+int orc_import_hash(void* h, int n, const int* keys, const float* st) {
+  Orc& S = *static_cast<Orc*>(h);
+  std::fill(S.hkeys.begin(), S.hkeys.end(), kHashEmpty);
+  for (int k = 0; k < n; ++k) {
+    int key = keys[k];
+    int hs = hash_pair_slot(key, S.hash_cap);
+    for (int probe = 0; probe < S.hash_cap; ++probe) {
+      int slot = (hs + probe) % S.hash_cap;
+      if (S.hkeys[slot] == kHashEmpty || S.hkeys[slot] == key) {
+        S.hkeys[slot] = key;
+        for (int f = 0; f < 8; ++f) S.hstate[8 * (size_t)slot + f] = st[8 * (size_t)k + f];
+        break;
+      }
+    }
+  }
+  return AVBD_OK;
+}
+
+int orc_read_capacities(void* h, int* mp, int* rc, int* pm, int* hc) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (mp) *mp = S.max_pairs;
+  if (rc) *rc = S.n_cap;
+  if (pm) *pm = S.pool_max;
+  if (hc) *hc = S.hash_cap;
+  return AVBD_OK;
+}
+
+int orc_set_capacities(void* h, int max_pairs, int row_cap, int pool_max) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (max_pairs > S.max_pairs) ensure_pairs(S, max_pairs);
+  if (row_cap > S.n_cap) {
+    resize_rows(S, row_cap);
+    S.n_dyn_cap = row_cap - S.n_static;
+  }
+  if (pool_max > S.pool_max) {
+    S.pool_n.resize(pool_max, 0); S.pool_t.resize(pool_max, 0);
+    S.pool_b.resize(pool_max, 0); S.pool_c.resize(pool_max, 0);
+    S.pool_max = pool_max;
+  }
+  return AVBD_OK;
+}
+
+int orc_read_pairs(void* h, int cap, int* a, int* b, int* ov, int* sat, float* n, float* depth,
+                   int* count) {
+  Orc& S = *static_cast<Orc*>(h);
+  int k = std::min(S.pair_count, S.max_pairs);
+  *count = S.pair_count;
+  for (int p = 0; p < std::min(k, cap); ++p) {
+    if (a) a[p] = S.pa[p];
+    if (b) b[p] = S.pb[p];
+    if (ov) ov[p] = S.pov[p];
+    if (sat) sat[p] = S.pov[p] ? S.psat[p] : -1;
+    if (n) st3(n, p, S.pov[p] ? S.pn[p] : mk3(0, 0, 0));
+    if (depth) depth[p] = S.pov[p] ? S.pdepth[p] : 0.0f;
+  }
+  return AVBD_OK;
+}
+
+int orc_read_manifold(void* h, int cap, int* count, int* refa, float* n, float* t, float* b,
+                      float* offr, float* offi) {
+  Orc& S = *static_cast<Orc*>(h);
+  int k = std::min(std::min(S.pair_count, S.max_pairs), cap);
+  for (int p = 0; p < k; ++p) {
+    int c = S.mf_count[p];
+    if (count) count[p] = c;
+    if (refa) refa[p] = c ? S.mf_refa[p] : 0;
+    if (n) st3(n, p, c ? S.mf_n[p] : mk3(0, 0, 0));
+    if (t) st3(t, p, c ? S.mf_t[p] : mk3(0, 0, 0));
+    if (b) st3(b, p, c ? S.mf_b[p] : mk3(0, 0, 0));
+    for (int s = 0; s < 4; ++s) {
+      if (offr) st3(offr, 4 * p + s, s < c ? S.mf_offr[4 * (size_t)p + s] : mk3(0, 0, 0));
+      if (offi) st3(offi, 4 * p + s, s < c ? S.mf_offi[4 * (size_t)p + s] : mk3(0, 0, 0));
+    }
+  }
+  return AVBD_OK;
+}
+
+// Scalar probes used by the closed-form tests of avbd_fp.h.
+void orc_fp_sincos(int n, const float* x, float* s, float* c) {
+  for (int i = 0; i < n; ++i) sincos_(x[i], &s[i], &c[i]);
+}
+void orc_fp_atan2(int n, const float* y, const float* x, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = atan2_(y[i], x[i]);
+}
+void orc_fp_geom_stiffness(const float* n, const float* r, float* out) {
+  v3 g = geom_stiffness_diag(mk3(n[0], n[1], n[2]), mk3(r[0], r[1], r[2]));
+  out[0] = g.x; out[1] = g.y; out[2] = g.z;
+}
+void orc_fp_quat(int n, const float* q, const float* v, float* rot, float* mat) {
+  for (int i = 0; i < n; ++i) {
+    q4 qq = ld4(q, i);
+    st3(rot, i, qrotate(qq, ld3(v, i)));
+    m33 R = qtomat(qq);
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) mat[9 * i + 3 * r + c] = R.m[r][c];
+  }
+}
+void orc_fp_rotvec(int n, const float* rv, float* q_out, float* rv_back) {
+  for (int i = 0; i < n; ++i) {
+    q4 qq = quat_from_rotvec(ld3(rv, i));
+    st4(q_out, i, qq);
+    st3(rv_back, i, quat_to_rotvec(qq));
+  }
+}
+void orc_fp_inverse(int n, const float* m, float* out) {
+  for (int i = 0; i < n; ++i) {
+    m33 a;
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) a.m[r][c] = m[9 * i + 3 * r + c];
+    m33 b = minverse(a);
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) out[9 * i + 3 * r + c] = b.m[r][c];
+  }
+}
+int orc_fp_hash(int a, int b, int c, int cap, int* slot) {
+  int key = encode_pair_key(a, b, c);
+  *slot = hash_pair_slot(key, cap);
+  return key;
+}
+// One-pair SAT + manifold probe on explicit geometry (pair 0 of a 2-body scratch).
+int orc_probe_pair(const float* xa, const float* qa, const float* ea, const float* xb,
+                   const float* qb, const float* eb, float margin, int* overlap, int* sat,
+                   float* n, float* depth, int* count, int* refa, float* offr, float* offi) {
+  Orc S;
+  S.nb = 2;
+  S.x = {ld3(xa, 0), ld3(xb, 0)};
+  S.q = {ld4(qa, 0), ld4(qb, 0)};
+  S.he = {ld3(ea, 0), ld3(eb, 0)};
+  ensure_pairs(S, 1);
+  S.pa[0] = 0; S.pb[0] = 1;
+  sat_pair(S, 0, margin);
+  manifold_pair(S, 0, margin);
+  *overlap = S.pov[0];
+  *sat = S.pov[0] ? S.psat[0] : -1;
+  st3(n, 0, S.pov[0] ? S.pn[0] : mk3(0, 0, 0));
+  *depth = S.pov[0] ? S.pdepth[0] : 0.0f;
+  *count = S.mf_count[0];
+  *refa = S.mf_refa[0];
+  for (int s = 0; s < 4; ++s) {
+    st3(offr, s, S.mf_offr[s]);
+    st3(offi, s, S.mf_offi[s]);
+  }
+  return AVBD_OK;
+}
+
+}  // extern "C"

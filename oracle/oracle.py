@@ -1,0 +1,110 @@
+"""Python driver of the CPU checker (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the avbd_amd package.
+
+``OracleHandle`` exposes the same methods as ``avbd_amd._native.NativeHandle``
+so a scene can be stepped through both side by side.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from avbd_amd import _native as N
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "build" / "liboracle.so"
+_L = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not LIB.exists():
+            build()
+        lib = C.CDLL(str(LIB))
+        N.bind(lib, "orc_", names=[k for k in N.SIGNATURES if k not in ("create", "device_count",
+                                                                        "pack_state",
+                                                                        "profile_enable",
+                                                                        "profile_read")])
+        lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
+        lib.orc_create.restype = C.c_int
+        lib.orc_set_capacities.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        lib.orc_set_capacities.restype = C.c_int
+        F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C")
+        lib.orc_fp_sincos.argtypes = [C.c_int, F, F, F]
+        lib.orc_fp_atan2.argtypes = [C.c_int, F, F, F]
+        lib.orc_fp_geom_stiffness.argtypes = [F, F, F]
+        lib.orc_fp_quat.argtypes = [C.c_int, F, F, F, F]
+        lib.orc_fp_rotvec.argtypes = [C.c_int, F, F, F]
+        lib.orc_fp_inverse.argtypes = [C.c_int, F, F]
+        lib.orc_fp_hash.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        lib.orc_fp_hash.restype = C.c_int
+        lib.orc_probe_pair.argtypes = [F, F, F, F, F, F, C.c_float, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), F, F, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), F, F]
+        lib.orc_probe_pair.restype = C.c_int
+        _L = lib
+    return _L
+
+
+class OracleHandle(N.NativeHandle):
+    """The CPU restatement behind the NativeHandle method set."""
+
+    prefix = "orc_"
+
+    def __init__(self, device: int = 0):
+        super().__init__(device, lib=load())
+
+    def _create(self, device: int) -> None:
+        self._check(self.lib.orc_create(C.byref(self.h)))
+
+    def set_capacities(self, max_pairs: int, row_cap: int, pool_max: int) -> None:
+        self._check(self.lib.orc_set_capacities(self.h, int(max_pairs), int(row_cap),
+                                                int(pool_max)))
+
+    def pack_state(self, nb, n_rows):
+        b = self.read_bodies(nb)
+        bod = np.concatenate([b["x"], b["q"], b["w"]], axis=1)
+        r = self.read_rows(n_rows)
+        rows = np.stack([r["lambda"], r["active"].astype(np.float32),
+                         (r["was_static"] * 16 + r["type"]).astype(np.float32)], axis=1)
+        return bod, rows
+
+    def profile_enable(self, on):
+        pass
+
+
+def oracle_solver_class():
+    """Solver6DOF whose handle is the CPU restatement (for side-by-side runs)."""
+    from avbd_amd.solver_6dof import Solver6DOF
+
+    class OracleSolver6DOF(Solver6DOF):
+        def _make_handle(self):
+            return OracleHandle()
+
+    return OracleSolver6DOF
+
+
+def probe_pair(xa, qa, ea, xb, qb, eb, margin=0.005):
+    """SAT + manifold of one explicit box pair through the restatement."""
+    lib = load()
+    f = lambda v: np.ascontiguousarray(v, np.float32)
+    ov, sat, cnt, refa = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    n = np.zeros(3, np.float32)
+    depth = np.zeros(1, np.float32)
+    offr = np.zeros((4, 3), np.float32)
+    offi = np.zeros((4, 3), np.float32)
+    lib.orc_probe_pair(f(xa), f(qa), f(ea), f(xb), f(qb), f(eb), float(margin), C.byref(ov),
+                       C.byref(sat), n, depth, C.byref(cnt), C.byref(refa), offr, offi)
+    return {"overlap": ov.value, "sat_idx": sat.value, "n_hat": n, "depth": float(depth[0]),
+            "count": cnt.value, "ref_is_a": refa.value, "off_ref": offr, "off_inc": offi}
