@@ -24,6 +24,7 @@ I64P = C.POINTER(C.c_longlong)
 
 COLOR_JACOBI = 0
 COLOR_JONES_PLASSMANN = 1
+OPT_PRIMAL_VARIANT = 0
 
 
 class AvbdError(RuntimeError):
@@ -117,6 +118,7 @@ SIGNATURES = {
     "read_capacities": ([C.c_void_p, I32P, I32P, I32P, I32P], C.c_int),
     "read_pairs": ([C.c_void_p, C.c_int, I32P, I32P, I32P, I32P, F32P, F32P, I32P], C.c_int),
     "read_manifold": ([C.c_void_p, C.c_int, I32P, I32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
+    "set_option": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "profile_read": ([C.c_void_p, C.c_int, F64P, I64P, F64P], C.c_int),
 }
@@ -302,6 +304,9 @@ class NativeHandle:
         _, pc, _ = self.read_counters()
         k = min(pc, mp)
         return {key: val[:k] for key, val in out.items()}
+
+    def set_option(self, option: int, value: int) -> None:
+        self._check(self._fn("set_option")(self.h, int(option), int(value)))
 
     def profile_enable(self, on: bool) -> None:
         self._check(self._fn("profile_enable")(self.h, int(bool(on))))

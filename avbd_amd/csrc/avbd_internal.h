@@ -62,7 +62,8 @@ void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose, fl
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* csr_start, const int* csr_idx,
-                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass);
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
+                         int variant);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,

@@ -145,6 +145,9 @@ struct avbd_solver {
   int n_active_rows = 0, pair_count = 0, pool_count = 0, n_contacts = 0;
   int last_post = 0;
 
+  // options
+  int primal_variant = 0;
+
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -446,7 +449,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       CK(prof_begin(s, kPrimal));
       launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
                           s->bstat.p, s->csr_start.p, s->csr_idx.p, s->geo.p, s->gst.p, dt,
-                          post_pass);
+                          post_pass, s->primal_variant);
       CK(prof_end(s, pbytes));
     }
     if (it < I) {
@@ -1051,6 +1054,18 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* refa, float* n,
     }
   }
   return AVBD_OK;
+}
+
+int avbd_set_option(avbd_solver* s, int option, int value) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  switch (option) {
+    case AVBD_OPT_PRIMAL_VARIANT:
+      if (value < 0 || value > 2) return set_err(AVBD_E_ARG, "primal variant must be 0, 1 or 2");
+      s->primal_variant = value;
+      return AVBD_OK;
+    default:
+      return set_err(AVBD_E_ARG, "unknown option");
+  }
 }
 
 int avbd_profile_enable(avbd_solver* s, int on) {

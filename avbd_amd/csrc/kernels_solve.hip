@@ -25,6 +25,20 @@ __device__ __forceinline__ q4 f4q(float4 a) { return mk4(a.x, a.y, a.z, a.w); }
 __device__ __forceinline__ float4 p4(v3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 __device__ __forceinline__ float4 p4q(q4 a) { return make_float4(a.x, a.y, a.z, a.w); }
 
+// Component `k` of v (k = 0, 1, anything else -> z) without control flow.
+// Selecting with a ternary chain let the compiler sink the pose load into a
+// switch on k and leave its address undefined on one path; bit masks keep
+// the full vector live and select with v_and/v_or.
+__device__ __forceinline__ float pick3(v3 v, int k) {
+  int m0 = -(int)(k == 0), m1 = -(int)(k == 1), m2 = ~(m0 | m1);
+  int b = (__float_as_int(v.x) & m0) | (__float_as_int(v.y) & m1) | (__float_as_int(v.z) & m2);
+  return __int_as_float(b);
+}
+// Unit axis e_k (k = 0, 1, anything else -> z).
+__device__ __forceinline__ v3 axis3(int k) {
+  return mk3(k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, (k != 0 && k != 1) ? 1.0f : 0.0f);
+}
+
 __device__ __forceinline__ m33 load_m33(const float4* p) {
   float4 r0 = p[0], r1 = p[1], r2 = p[2];
   m33 m;
@@ -154,7 +168,7 @@ __device__ void prelude_group(int g, const float4* __restrict__ pose0, GroupGeo*
     q4 q0 = f4q(pose0[2 * a + 1]);
     v3 rw = add(x0, qrotate(q0, off_a));
     v3 anc = f3(G.g0);
-    float C0 = b == 0 ? rw.x - anc.x : (b == 1 ? rw.y - anc.y : rw.z - anc.z);
+    float C0 = pick3(rw, b) - pick3(anc, b);
     ac0[0] = (bits & kActN) ? main_alpha * C0 : 0.0f;
   }
   S.s0 = make_float4(lam_dec[0], nr > 1 ? lam_dec[1] : S.s0.y, nr > 2 ? lam_dec[2] : S.s0.z, ac0[0]);
@@ -197,17 +211,20 @@ void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose_cur
 // ---------------------------------------------------------------------------
 // primal
 // ---------------------------------------------------------------------------
-struct Acc {
-  m33 A, B, D;
-  v3 rl, ra;
+// Values one constraint row contributes to the body's 6x6 system
+// (primal_update_6dof loop body, kernels_6dof.py:507-564): Jacobian, the
+// Eq. 14 LHS stiffness, the clamped force and the geometric-stiffness
+// diagonal |f| G~ (used only when |f| > 0).
+struct RowVals {
+  v3 jl, ja;
+  float k, f;
+  v3 g;
 };
 
-// One constraint row's contribution (primal_update_6dof loop body,
-// kernels_6dof.py:507-564). `is_t` selects the friction-cone bounds.
-__device__ __forceinline__ void acc_row(Acc& acc, v3 jl, v3 ja, float C, v3 nG, v3 r_self,
-                                        bool hard, float kp, float lam, float ac0, bool is_t,
-                                        float lam_sib, bool was_sib, float mu_d, float mu_s,
-                                        float f_lo_n, float f_hi_n) {
+__device__ __forceinline__ void row_values(RowVals& o, bool& gflag, v3 jl, v3 ja, float C, v3 nG,
+                                           v3 r_self, bool hard, float kp, float lam, float ac0,
+                                           bool is_t, float lam_sib, bool was_sib, float mu_d,
+                                           float mu_s, float f_lo_n, float f_hi_n) {
   if (hard) C = C - ac0;
   float lam_eff = hard ? lam : 0.0f;
   float lam_plus = kp * C + lam_eff;
@@ -228,31 +245,20 @@ __device__ __forceinline__ void acc_row(Acc& acc, v3 jl, v3 ja, float C, v3 nG, 
     if (lam_plus < f_lo) k_lhs = fabs_(f_lo - lam_plus) / absC;
     else if (lam_plus > f_hi) k_lhs = fabs_(f_hi - lam_plus) / absC;
   }
-  float l[3] = {jl.x, jl.y, jl.z};
-  float an[3] = {ja.x, ja.y, ja.z};
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      acc.A.m[r][c] = acc.A.m[r][c] + (l[r] * l[c]) * k_lhs;
-      acc.B.m[r][c] = acc.B.m[r][c] + (an[r] * l[c]) * k_lhs;
-      acc.D.m[r][c] = acc.D.m[r][c] + (an[r] * an[c]) * k_lhs;
-    }
-  }
+  o.jl = jl;
+  o.ja = ja;
+  o.k = k_lhs;
+  o.f = f;
   float fmag = fabs_(f);
-  if (fmag > 0.0f) {
-    v3 gd = mul(geom_stiffness_diag(nG, r_self), fmag);
-    acc.D.m[0][0] = acc.D.m[0][0] + gd.x;
-    acc.D.m[1][1] = acc.D.m[1][1] + gd.y;
-    acc.D.m[2][2] = acc.D.m[2][2] + gd.z;
-  }
-  acc.rl = add(acc.rl, mul(jl, f));
-  acc.ra = add(acc.ra, mul(ja, f));
+  gflag = fmag > 0.0f;
+  o.g = gflag ? mul(geom_stiffness_diag(nG, r_self), fmag) : mk3(0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, const GroupGeo& G,
-                                             const GroupSt& S, const float4* __restrict__ pose,
-                                             int post_pass) {
+// Rows of group G seen from body i, in row order. Returns a bit mask of the
+// rows present and active (bit r) and sets gmask (bit r: |f| > 0).
+__device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, v3 xi, q4 qi,
+                                          const GroupGeo& G, const GroupSt& S,
+                                          const float4* __restrict__ pose, int post_pass) {
   const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
   const int bits = __float_as_int(S.s2.y);
   const float stiff = G.g0.w;
@@ -260,25 +266,34 @@ __device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, cons
   const bool wasN = (bits & kWasN) != 0;
   const float lam_n = S.s0.x;
   v3 off_a = f3(G.g1);
+  int mask = 0;
+  gmask = 0;
+  bool gf;
   if (kind == kGFloor) {
     v3 r_self = qrotate(qi, off_a);
     if (bits & kActN) {
       v3 n = mk3(0.0f, 1.0f, 0.0f);
       float C = (xi.y + r_self.y) - G.g0.y;
-      acc_row(acc, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
-              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+      row_values(out[0], gf, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
+                 post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+      mask |= 1;
+      gmask |= gf ? 1 : 0;
     }
     if (nr == 3) {
       v3 xr = add(xi, r_self);
       if (bits & kActT) {
         v3 t = mk3(1.0f, 0.0f, 0.0f);
-        acc_row(acc, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w,
-                true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        row_values(out[1], gf, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.y, S.s0.y,
+                   S.s1.w, true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        mask |= 2;
+        gmask |= gf ? 2 : 0;
       }
       if (bits & kActB) {
         v3 t = mk3(0.0f, 0.0f, 1.0f);
-        acc_row(acc, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.z, S.s0.z, S.s2.x,
-                true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        row_values(out[2], gf, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.z, S.s0.z,
+                   S.s2.x, true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        mask |= 4;
+        gmask |= gf ? 4 : 0;
       }
     }
   } else if (kind == kGBox) {
@@ -302,8 +317,10 @@ __device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, cons
     if (bits & kActN) {
       v3 jl = self_a ? n : neg(n);
       v3 ja = self_a ? cross(r_self, n) : neg(cross(r_self, n));
-      acc_row(acc, jl, ja, dot(n, d), n, r_self, hard, S.s1.x, S.s0.x,
-              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+      row_values(out[0], gf, jl, ja, dot(n, d), n, r_self, hard, S.s1.x, S.s0.x,
+                 post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
+      mask |= 1;
+      gmask |= gf ? 1 : 0;
     }
     if (nr == 3) {
       v3 t = orthonormal_basis(n);
@@ -311,14 +328,18 @@ __device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, cons
       if (bits & kActT) {
         v3 jl = self_a ? t : neg(t);
         v3 ja = self_a ? cross(r_self, t) : neg(cross(r_self, t));
-        acc_row(acc, jl, ja, dot(t, d), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w, true, lam_n,
-                wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        row_values(out[1], gf, jl, ja, dot(t, d), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w, true,
+                   lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        mask |= 2;
+        gmask |= gf ? 2 : 0;
       }
       if (bits & kActB) {
         v3 jl = self_a ? bb : neg(bb);
         v3 ja = self_a ? cross(r_self, bb) : neg(cross(r_self, bb));
-        acc_row(acc, jl, ja, dot(bb, d), bb, r_self, hard, S.s1.z, S.s0.z, S.s2.x, true, lam_n,
-                wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        row_values(out[2], gf, jl, ja, dot(bb, d), bb, r_self, hard, S.s1.z, S.s0.z, S.s2.x,
+                   true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
+        mask |= 4;
+        gmask |= gf ? 4 : 0;
       }
     }
   } else {  // PIN
@@ -327,17 +348,74 @@ __device__ __forceinline__ void primal_group(Acc& acc, int i, v3 xi, q4 qi, cons
       v3 r_self = qrotate(qi, off_a);
       v3 wp = add(xi, r_self);
       v3 anc = f3(G.g0);
-      v3 n;
-      float C;
-      if (axis == 0) { n = mk3(1.0f, 0.0f, 0.0f); C = wp.x - anc.x; }
-      else if (axis == 1) { n = mk3(0.0f, 1.0f, 0.0f); C = wp.y - anc.y; }
-      else { n = mk3(0.0f, 0.0f, 1.0f); C = wp.z - anc.z; }
-      acc_row(acc, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
-              post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, kInf);
+      v3 n = axis3(axis);
+      float C = pick3(wp, axis) - pick3(anc, axis);
+      row_values(out[0], gf, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
+                 post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, kInf);
+      mask |= 1;
+      gmask |= gf ? 1 : 0;
     }
   }
+  return mask;
 }
 
+struct Acc {
+  m33 A, B, D;
+  v3 rl, ra;
+};
+
+// Body-side terms of the system before any row (kernels_6dof.py:373-399).
+__device__ __forceinline__ void acc_init(Acc& acc, float m, const m33& Iw, v3 xi, q4 qi,
+                                         float4 xI, float4 qI, float inv_dt2) {
+  acc.A = mzero();
+  acc.A.m[0][0] = m * inv_dt2;
+  acc.A.m[1][1] = m * inv_dt2;
+  acc.A.m[2][2] = m * inv_dt2;
+  acc.D = mscale(Iw, inv_dt2);
+  acc.B = mzero();
+  acc.rl = mul(sub(xi, f3(xI)), m * inv_dt2);
+  q4 dq = qmul(qi, qconj(f4q(qI)));
+  acc.ra = mvmul(Iw, mul(quat_to_rotvec(dq), inv_dt2));
+}
+
+__device__ __forceinline__ void acc_add(Acc& acc, const RowVals& v, bool gflag) {
+  float l[3] = {v.jl.x, v.jl.y, v.jl.z};
+  float an[3] = {v.ja.x, v.ja.y, v.ja.z};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      acc.A.m[r][c] = acc.A.m[r][c] + (l[r] * l[c]) * v.k;
+      acc.B.m[r][c] = acc.B.m[r][c] + (an[r] * l[c]) * v.k;
+      acc.D.m[r][c] = acc.D.m[r][c] + (an[r] * an[c]) * v.k;
+    }
+  }
+  if (gflag) {
+    acc.D.m[0][0] = acc.D.m[0][0] + v.g.x;
+    acc.D.m[1][1] = acc.D.m[1][1] + v.g.y;
+    acc.D.m[2][2] = acc.D.m[2][2] + v.g.z;
+  }
+  acc.rl = add(acc.rl, mul(v.jl, v.f));
+  acc.ra = add(acc.ra, mul(v.ja, v.f));
+}
+
+// Schur-complement solve of [A B^T; B D] and the pose update
+// (kernels_6dof.py:566-582).
+__device__ __forceinline__ void solve_and_apply(const Acc& acc, int i, v3 xi, q4 qi,
+                                                float4* __restrict__ pose) {
+  m33 A_inv = minverse(acc.A);
+  m33 BAinv = mmul(acc.B, A_inv);
+  v3 rhs = sub(acc.ra, mvmul(BAinv, acc.rl));
+  m33 Sm = msub(acc.D, mmul(BAinv, mT(acc.B)));
+  m33 S_inv = minverse(Sm);
+  v3 d_th = mvmul(S_inv, rhs);
+  v3 d_x = mvmul(A_inv, sub(acc.rl, mvmul(mT(acc.B), d_th)));
+  pose[2 * i] = p4(sub(xi, d_x));
+  q4 dq2 = quat_from_rotvec(neg(d_th));
+  pose[2 * i + 1] = p4q(qnormalize(qmul(dq2, qi)));
+}
+
+// Lane-per-body variant: each lane walks its body's groups serially.
 __global__ void __launch_bounds__(64)
 k_primal_color(int count, const int* __restrict__ color_bodies, int base,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
@@ -351,46 +429,210 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
   float m = bstat[i].x;
   if (m <= 0.0f) return;
   float inv_dt2 = 1.0f / (dt * dt);
-  m33 Iw = load_m33(inert + 3 * i);
-  Acc acc;
-  acc.A = mzero();
-  acc.A.m[0][0] = m * inv_dt2;
-  acc.A.m[1][1] = m * inv_dt2;
-  acc.A.m[2][2] = m * inv_dt2;
-  acc.D = mscale(Iw, inv_dt2);
-  acc.B = mzero();
   v3 xi = f3(pose[2 * i]);
   q4 qi = f4q(pose[2 * i + 1]);
-  acc.rl = mul(sub(xi, f3(poseI[2 * i])), m * inv_dt2);
-  q4 dq = qmul(qi, qconj(f4q(poseI[2 * i + 1])));
-  acc.ra = mvmul(Iw, mul(quat_to_rotvec(dq), inv_dt2));
+  Acc acc;
+  acc_init(acc, m, load_m33(inert + 3 * i), xi, qi, poseI[2 * i], poseI[2 * i + 1], inv_dt2);
   const int k0 = csr_start[i], k1 = csr_start[i + 1];
   for (int k = k0; k < k1; ++k) {
     int g = csr_idx[k];
     GroupGeo G = geo[g];
     GroupSt S = gst[g];
-    primal_group(acc, i, xi, qi, G, S, pose, post_pass);
+    RowVals rv[3];
+    int gmask;
+    int mask = group_rows(rv, gmask, i, xi, qi, G, S, pose, post_pass);
+    for (int r = 0; r < 3; ++r)
+      if (mask & (1 << r)) acc_add(acc, rv[r], (gmask >> r) & 1);
   }
-  m33 A_inv = minverse(acc.A);
-  m33 BAinv = mmul(acc.B, A_inv);
-  v3 rhs = sub(acc.ra, mvmul(BAinv, acc.rl));
-  m33 Sm = msub(acc.D, mmul(BAinv, mT(acc.B)));
-  m33 S_inv = minverse(Sm);
-  v3 d_th = mvmul(S_inv, rhs);
-  v3 d_x = mvmul(A_inv, sub(acc.rl, mvmul(mT(acc.B), d_th)));
-  pose[2 * i] = p4(sub(xi, d_x));
-  q4 dq2 = quat_from_rotvec(neg(d_th));
-  pose[2 * i + 1] = p4q(qnormalize(qmul(dq2, qi)));
+  solve_and_apply(acc, i, xi, qi, pose);
+}
+
+// Tiled variant: one 64-lane wavefront per tile of BPW bodies of the color.
+//  A. lanes evaluate one (body, group) incidence each -> row values in LDS
+//  B. 30 lanes per body own one system entry each (A upper 6, B 9, D 9,
+//     r_lin 3, r_ang 3) and add the row contributions in CSR order
+//  C. one lane per body gathers its system and solves.
+// The additions per entry happen in the same order as the lane-per-body
+// variant, so both give identical bits.
+constexpr int kRowFields = 11;           // jl 3, ja 3, k, f, g 3
+constexpr int kSlotFields = 3 * kRowFields;
+constexpr int kComps = 30;
+
+template <int BPW>
+__global__ void __launch_bounds__(64)
+k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
+               float4* __restrict__ pose, const float4* __restrict__ poseI,
+               const float4* __restrict__ inert, const float4* __restrict__ bstat,
+               const int* __restrict__ csr_start, const int* __restrict__ csr_idx,
+               const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
+               int post_pass) {
+  __shared__ float rows[kSlotFields][65];
+  __shared__ int rmask[64];
+  __shared__ float sys[BPW][kComps];
+  __shared__ int b_i[BPW], b_deg[BPW], b_start[BPW], b_off[BPW + 1];
+  __shared__ float b_pose[BPW][7];
+  const int lane = threadIdx.x;
+  const int first = blockIdx.x * BPW;
+  const float inv_dt2 = 1.0f / (dt * dt);
+  if (lane < BPW) {
+    int slot = first + lane;
+    int i = -1, deg = 0, st = 0;
+    if (slot < count) {
+      i = color_bodies[base + slot];
+      if (bstat[i].x > 0.0f) {
+        st = csr_start[i];
+        deg = csr_start[i + 1] - st;
+      } else {
+        i = -1;
+      }
+    }
+    b_i[lane] = i;
+    b_deg[lane] = deg;
+    b_start[lane] = st;
+    if (i >= 0) {
+      float4 xp = pose[2 * i], qp = pose[2 * i + 1];
+      b_pose[lane][0] = xp.x; b_pose[lane][1] = xp.y; b_pose[lane][2] = xp.z;
+      b_pose[lane][3] = qp.x; b_pose[lane][4] = qp.y; b_pose[lane][5] = qp.z; b_pose[lane][6] = qp.w;
+      Acc acc;
+      acc_init(acc, bstat[i].x, load_m33(inert + 3 * i), mk3(xp.x, xp.y, xp.z),
+               mk4(qp.x, qp.y, qp.z, qp.w), poseI[2 * i], poseI[2 * i + 1], inv_dt2);
+      float* o = sys[lane];
+      o[0] = acc.A.m[0][0]; o[1] = acc.A.m[0][1]; o[2] = acc.A.m[0][2];
+      o[3] = acc.A.m[1][1]; o[4] = acc.A.m[1][2]; o[5] = acc.A.m[2][2];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          o[6 + 3 * r + c] = acc.B.m[r][c];
+          o[15 + 3 * r + c] = acc.D.m[r][c];
+        }
+      o[24] = acc.rl.x; o[25] = acc.rl.y; o[26] = acc.rl.z;
+      o[27] = acc.ra.x; o[28] = acc.ra.y; o[29] = acc.ra.z;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    b_off[0] = 0;
+    for (int b = 0; b < BPW; ++b) b_off[b + 1] = b_off[b] + b_deg[b];
+  }
+  __syncthreads();
+  const int total = b_off[BPW];
+  // accumulator lane role
+  const int ab = lane / kComps, ac = lane % kComps;
+  const bool is_acc = ab < BPW && b_i[ab < BPW ? ab : 0] >= 0 && lane < BPW * kComps;
+  // component -> (kind, r, c): kind 0 A, 1 B, 2 D, 3 rl, 4 ra
+  int ck, cr, cc;
+  if (ac < 6) {
+    const int rr[6] = {0, 0, 0, 1, 1, 2}, cc6[6] = {0, 1, 2, 1, 2, 2};
+    ck = 0; cr = rr[ac]; cc = cc6[ac];
+  } else if (ac < 15) {
+    ck = 1; cr = (ac - 6) / 3; cc = (ac - 6) % 3;
+  } else if (ac < 24) {
+    ck = 2; cr = (ac - 15) / 3; cc = (ac - 15) % 3;
+  } else if (ac < 27) {
+    ck = 3; cr = ac - 24; cc = 0;
+  } else {
+    ck = 4; cr = ac - 27; cc = 0;
+  }
+  float accv = is_acc ? sys[ab][ac] : 0.0f;
+  for (int p0 = 0; p0 < total; p0 += 64) {
+    // ---- A: row values of one incidence per lane ----
+    int j = p0 + lane;
+    int mask = 0, gmask = 0;
+    if (j < total) {
+      int b = 0;
+      while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
+      int i = b_i[b];
+      int g = csr_idx[b_start[b] + (j - b_off[b])];
+      GroupGeo G = geo[g];
+      GroupSt S = gst[g];
+      v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
+      q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
+      RowVals rv[3];
+      mask = group_rows(rv, gmask, i, xi, qi, G, S, pose, post_pass);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float* base_f = &rows[kRowFields * r][0];
+        (void)base_f;
+        rows[kRowFields * r + 0][lane] = rv[r].jl.x;
+        rows[kRowFields * r + 1][lane] = rv[r].jl.y;
+        rows[kRowFields * r + 2][lane] = rv[r].jl.z;
+        rows[kRowFields * r + 3][lane] = rv[r].ja.x;
+        rows[kRowFields * r + 4][lane] = rv[r].ja.y;
+        rows[kRowFields * r + 5][lane] = rv[r].ja.z;
+        rows[kRowFields * r + 6][lane] = rv[r].k;
+        rows[kRowFields * r + 7][lane] = rv[r].f;
+        rows[kRowFields * r + 8][lane] = rv[r].g.x;
+        rows[kRowFields * r + 9][lane] = rv[r].g.y;
+        rows[kRowFields * r + 10][lane] = rv[r].g.z;
+      }
+    }
+    rmask[lane] = mask | (gmask << 3);
+    __syncthreads();
+    // ---- B: ordered accumulation ----
+    if (is_acc) {
+      int s0 = b_off[ab] > p0 ? b_off[ab] - p0 : 0;
+      int s1 = b_off[ab + 1] - p0 < 64 ? b_off[ab + 1] - p0 : 64;
+      for (int sl = s0; sl < s1; ++sl) {
+        int mk = rmask[sl];
+        for (int r = 0; r < 3; ++r) {
+          if (!(mk & (1 << r))) continue;
+          const int fb = kRowFields * r;
+          if (ck == 0) {
+            accv = accv + (rows[fb + cr][sl] * rows[fb + cc][sl]) * rows[fb + 6][sl];
+          } else if (ck == 1) {
+            accv = accv + (rows[fb + 3 + cr][sl] * rows[fb + cc][sl]) * rows[fb + 6][sl];
+          } else if (ck == 2) {
+            accv = accv + (rows[fb + 3 + cr][sl] * rows[fb + 3 + cc][sl]) * rows[fb + 6][sl];
+            if (cr == cc && (mk & (8 << r))) accv = accv + rows[fb + 8 + cr][sl];
+          } else if (ck == 3) {
+            accv = accv + rows[fb + cr][sl] * rows[fb + 7][sl];
+          } else {
+            accv = accv + rows[fb + 3 + cr][sl] * rows[fb + 7][sl];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (is_acc) sys[ab][ac] = accv;
+  __syncthreads();
+  // ---- C: solve ----
+  if (lane < BPW && b_i[lane] >= 0) {
+    const float* o = sys[lane];
+    Acc acc;
+    acc.A.m[0][0] = o[0]; acc.A.m[0][1] = o[1]; acc.A.m[0][2] = o[2];
+    acc.A.m[1][0] = o[1]; acc.A.m[1][1] = o[3]; acc.A.m[1][2] = o[4];
+    acc.A.m[2][0] = o[2]; acc.A.m[2][1] = o[4]; acc.A.m[2][2] = o[5];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        acc.B.m[r][c] = o[6 + 3 * r + c];
+        acc.D.m[r][c] = o[15 + 3 * r + c];
+      }
+    acc.rl = mk3(o[24], o[25], o[26]);
+    acc.ra = mk3(o[27], o[28], o[29]);
+    const float* bp = b_pose[lane];
+    solve_and_apply(acc, b_i[lane], mk3(bp[0], bp[1], bp[2]), mk4(bp[3], bp[4], bp[5], bp[6]),
+                    pose);
+  }
 }
 
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* csr_start, const int* csr_idx,
-                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass) {
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
+                         int variant) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_primal_color, dim3((count + 63) / 64), dim3(64), 0, st, count,
-                     color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
-                     dt, post_pass);
+  if (variant == 0) {
+    hipLaunchKernelGGL(k_primal_color, dim3((count + 63) / 64), dim3(64), 0, st, count,
+                       color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
+                       dt, post_pass);
+  } else if (variant == 1) {
+    hipLaunchKernelGGL(k_primal_tiled<1>, dim3(count), dim3(64), 0, st, count, color_bodies, base,
+                       pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst, dt, post_pass);
+  } else {
+    hipLaunchKernelGGL(k_primal_tiled<2>, dim3((count + 1) / 2), dim3(64), 0, st, count,
+                       color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
+                       dt, post_pass);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -455,7 +697,7 @@ k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict
   } else {
     v3 rw = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
     v3 anc = f3(G.g0);
-    C[0] = b == 0 ? rw.x - anc.x : (b == 1 ? rw.y - anc.y : rw.z - anc.z);
+    C[0] = pick3(rw, b) - pick3(anc, b);
   }
   // normal / pin row
   if (bits & kActN) {

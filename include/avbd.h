@@ -180,6 +180,15 @@ int avbd_read_pairs(avbd_solver* s, int cap, int* a, int* b, int* overlap, int* 
 int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float* n_hat,
                        float* t_hat, float* b_hat, float* off_ref, float* off_inc);
 
+/* ---- options -------------------------------------------------------- */
+/* Kernel selection; no counterpart in the reference (its primal_group_size /
+ * primal_shuffle / primal_fused switches, solver_6dof.py:196-198, choose
+ * between its CUDA kernels). AVBD_OPT_PRIMAL_VARIANT: 0 = one lane per body,
+ * 1 = one wavefront per body, 2 = one wavefront per two bodies. All variants
+ * give identical results. */
+#define AVBD_OPT_PRIMAL_VARIANT 0
+int avbd_set_option(avbd_solver* s, int option, int value);
+
 /* ---- profiling ------------------------------------------------------- */
 /* When enabled, the primal, dual and prelude launches are bracketed by HIP
  * events on the solver stream; avbd_profile_read returns accumulated

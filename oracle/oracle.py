@@ -34,6 +34,7 @@ def load() -> C.CDLL:
         lib = C.CDLL(str(LIB))
         N.bind(lib, "orc_", names=[k for k in N.SIGNATURES if k not in ("create", "device_count",
                                                                         "pack_state",
+                                                                        "set_option",
                                                                         "profile_enable",
                                                                         "profile_read")])
         lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
@@ -81,6 +82,9 @@ class OracleHandle(N.NativeHandle):
         return bod, rows
 
     def profile_enable(self, on):
+        pass
+
+    def set_option(self, option, value):
         pass
 
 

@@ -68,3 +68,47 @@ def diff_report(g, o) -> dict:
 def assert_bit_exact(rep: dict, where: str = "") -> None:
     bad = [k for k, v in rep.items() if k.endswith(("_exact", "_equal")) and v is False]
     assert not bad, f"{where}: mismatch in {bad}: {rep}"
+
+
+def snapshot(g) -> dict:
+    """Everything that carries over from one substep to the next."""
+    nb = len(g._x)
+    keys, hstate = g._h.export_hash()
+    return {
+        "bodies": g._h.read_bodies(nb),
+        "rows": g._h.read_rows(g._gpu_pool_n_static),
+        "keys": keys,
+        "hstate": hstate,
+        "colors": g._h.read_colors(nb),
+        "n_active_colors": g._n_active_colors,
+        "color_dirty": g._color_dirty,
+        "caps": g._h.read_capacities(),
+        "bp_max_pairs": g._bp_max_pairs,
+    }
+
+
+def load_snapshot(o, snap: dict) -> None:
+    """Put a flushed checker solver into the snapshot's state."""
+    o._flush()
+    for field, key in enumerate(("x", "q", "v", "w", "prev_v", "prev_w")):
+        o._h.write_bodies(field, 0, snap["bodies"][key])
+    r = snap["rows"]
+    o._h.write_static_rows(r["lambda"], r["penalty"], r["active"], r["was_static"])
+    max_pairs, row_cap, pool_max, _ = snap["caps"]
+    o._h.set_capacities(max_pairs, row_cap, pool_max)
+    o._gpu_pool_n_capacity = row_cap
+    o._h.import_hash(snap["keys"], snap["hstate"])
+    o._h.write_colors(snap["colors"], snap["n_active_colors"])
+    o._color_dirty = snap["color_dirty"]
+    o._n_active_colors = snap["n_active_colors"]
+    o.num_active_colors = snap["n_active_colors"]
+    o._bp_max_pairs = snap["bp_max_pairs"]
+
+
+def within_tolerance(rep: dict, rel: float = 1e-5) -> list[str]:
+    """Float observables beyond `rel` (relative to the checker) after a substep."""
+    bad = []
+    for k in ("rows_lambda_maxrel",):
+        if rep.get(k, 0.0) > rel:
+            bad.append(k)
+    return bad
