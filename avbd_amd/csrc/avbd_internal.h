@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "avbd_fp.h"
 
 namespace avbd {
@@ -122,9 +124,13 @@ void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* ge
                       int* counts);
 void launch_adj_fill(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
                      const int* starts, int* cursor, int* nbr);
-void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* nbr_start,
-                         const int* nbr, const float* prio, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp, int n_blocks);
+void launch_adj_count_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                            const int* succ, int* counts);
+void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                           const int* succ, const int* starts, int* cursor, int* nbr);
+void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
+                         const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
+                         int* err, int* ticket, int mode_jp);
 void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
                           const float* prio, const int* color, int* color_next, int* uncolored);
 void launch_color_finish(hipStream_t st, int nb, int* color, int* counts);

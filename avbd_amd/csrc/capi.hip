@@ -102,7 +102,7 @@ struct avbd_solver {
   int cur = 0;
   DBuf<float4> poseI, inert, vel, pvel, bstat, lo, hi;
   DBuf<float> Ilocal, bfric, prio;
-  DBuf<int> order_id, order_jp;
+  DBuf<int> order_id, order_jp, pos_jp;
 
   // rows / groups
   int n_static_rows = 0, g_static = 0, row_cap = 0, n_dyn_cap = 0, g_cap = 0;
@@ -143,6 +143,7 @@ struct avbd_solver {
 
   // host mirrors of the last substep
   int n_active_rows = 0, pair_count = 0, pool_count = 0, n_contacts = 0;
+  int adj_pairs = 0;  // pairs whose emission this substep feeds the coloring adjacency
   int last_post = 0;
 
   // options
@@ -262,18 +263,22 @@ int recolor(avbd_solver* s, int mode) {
   hipStream_t st = s->st;
   const int nd = s->n_contacts;
   CK(prof_begin(s, kColoring));
+  // Neighbour sets of the dynamic contact groups, one entry per pair that
+  // emitted a group this substep.
+  const int np = nd > 0 ? s->adj_pairs : 0;
   launch_fill_int(st, s->nbr_count.p, nb, 0);
-  launch_adj_count(st, nd, s->g_static, s->geo.p, s->nbr_count.p);
+  launch_adj_count_pairs(st, np, s->pa.p, s->pb.p, s->succ.p, s->nbr_count.p);
   CK(scan(s, s->nbr_count.p, s->nbr_start.p, nb, s->nbr_start.p + nb));
   launch_fill_int(st, s->nbr_cursor.p, nb, 0);
-  CK(s->nbr.ensure(2 * (size_t)nd + 1));
-  launch_adj_fill(st, nd, s->g_static, s->geo.p, s->nbr_start.p, s->nbr_cursor.p, s->nbr.p);
+  CK(s->nbr.ensure(2 * (size_t)np + 1));
+  launch_adj_fill_pairs(st, np, s->pa.p, s->pb.p, s->succ.p, s->nbr_start.p, s->nbr_cursor.p,
+                        s->nbr.p);
   launch_fill_int(st, s->color.p, nb, -1);
   launch_fill_int(st, s->dcnt.p + 6, 1, 0);
   if (mode == AVBD_COLOR_JACOBI) {
     for (int round = 0; round < kMaxColors; ++round) {
-      launch_color_greedy(st, nb, s->order_id.p, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
-                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0, 0);
+      launch_color_greedy(st, nb, s->order_id.p, nullptr, s->nbr_start.p, s->nbr.p, s->color.p,
+                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0);
       launch_fill_int(st, s->dcnt.p + 5, 1, 0);
       launch_color_resolve(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
                            s->color_next.p, s->dcnt.p + 5);
@@ -282,8 +287,8 @@ int recolor(avbd_solver* s, int mode) {
       if (s->hcnt[5] == 0) break;
     }
   } else {
-    launch_color_greedy(st, nb, s->order_jp.p, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
-                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, 0);
+    launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
+                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1);
   }
   launch_color_finish(st, nb, s->color.p, s->ccounts.p);
   color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
@@ -347,6 +352,7 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   rep->pair_count = total;
   rep->pair_capacity = s->pair_cap;
   int kept = std::min(total, s->pair_cap);
+  s->adj_pairs = kept;
   if (fixed || total > 0) {
     CK(prof_begin(s, kContacts));
     bvh_write_pairs(st, nb, s->lo.p, s->hi.p, s->bstat.p, s->ws_bvh.p, s->bcount.p, s->boff.p,
@@ -388,6 +394,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   s->pool_count = 0;
   s->pair_count = 0;
   s->n_contacts = 0;
+  s->adj_pairs = 0;
   int n_active = s->n_static_rows;
   rep->row_capacity = s->row_cap;
   if (p->self_collide && nb >= 2) {
@@ -565,7 +572,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   CK(s->poseI.alloc(2 * nbb)); CK(s->inert.alloc(3 * nbb)); CK(s->vel.alloc(2 * nbb));
   CK(s->pvel.alloc(2 * nbb)); CK(s->bstat.alloc(nbb)); CK(s->lo.alloc(nbb)); CK(s->hi.alloc(nbb));
   CK(s->Ilocal.alloc(9 * nbb)); CK(s->bfric.alloc(nbb)); CK(s->prio.alloc(nbb));
-  CK(s->order_id.alloc(nbb)); CK(s->order_jp.alloc(nbb));
+  CK(s->order_id.alloc(nbb)); CK(s->order_jp.alloc(nbb)); CK(s->pos_jp.alloc(nbb));
   if (nb > 0) {
     HIPCK(hipMemcpy(s->pose[0].p, pose.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(s->pose[1].p, pose.data(), 2 * nb * sizeof(float4), hipMemcpyHostToDevice));
@@ -582,6 +589,9 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
     std::stable_sort(ord.begin(), ord.end(),
                      [&](int a, int b) { return sc->priority[a] > sc->priority[b]; });
     HIPCK(hipMemcpy(s->order_jp.p, ord.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> pos(nb);
+    for (int k = 0; k < nb; ++k) pos[ord[k]] = k;
+    HIPCK(hipMemcpy(s->pos_jp.p, pos.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   }
   // ---- static rows -> groups ----
   std::vector<GroupGeo> hg;
@@ -715,7 +725,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   s->n_active_colors = kMaxColors;
   for (int c = 0; c <= kMaxColors; ++c) s->cstart[c] = 0;
   s->n_active_rows = ns;
-  s->pair_count = s->pool_count = s->n_contacts = 0;
+  s->pair_count = s->pool_count = s->n_contacts = s->adj_pairs = 0;
   s->last_post = 0;
   HIPCK(hipStreamSynchronize(s->st));
   s->flushed = true;

@@ -366,88 +366,160 @@ __device__ __forceinline__ int lowest_free_color(unsigned used) {
   return fr ? (__ffs((int)fr) - 1) : (kMaxColors - 1);
 }
 
-__device__ __forceinline__ int ld_acq(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ int ld_relaxed(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_relaxed(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Body adjacency of the coloring: one entry per broadphase pair that emitted
+// at least one contact group (the same neighbour sets the dynamic groups
+// describe, without the per-contact-point duplicates).
+__global__ void __launch_bounds__(256)
+k_adj_count_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
+                  const int* __restrict__ succ, int* counts) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs || succ[p] <= 0) return;
+  int a = pa[p], b = pb[p];
+  if (a < 0 || b < 0 || a == b) return;
+  atomicAdd(&counts[a], 1);
+  atomicAdd(&counts[b], 1);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_fill_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
+                 const int* __restrict__ succ, const int* __restrict__ starts, int* cursor,
+                 int* nbr) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs || succ[p] <= 0) return;
+  int a = pa[p], b = pb[p];
+  if (a < 0 || b < 0 || a == b) return;
+  nbr[starts[a] + atomicAdd(&cursor[a], 1)] = b;
+  nbr[starts[b] + atomicAdd(&cursor[b], 1)] = a;
+}
+
+void launch_adj_count_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                            const int* succ, int* counts) {
+  if (n_pairs > 0)
+    hipLaunchKernelGGL(k_adj_count_pairs, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
+                       pa, pb, succ, counts);
+}
+
+void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                           const int* succ, const int* starts, int* cursor, int* nbr) {
+  if (n_pairs > 0)
+    hipLaunchKernelGGL(k_adj_fill_pairs, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
+                       pa, pb, succ, starts, cursor, nbr);
+}
+
+// Greedy coloring in a fixed processing order, computed as a dataflow: the
+// body at position k of `order` takes the lowest color not used by its
+// neighbours that come earlier in the order, which is what a serial loop over
+// the order produces (gpu_color_spec_assign executed serially,
+// kernels_6dof.py:3203-3243; Jones-Plassmann round numbers,
+// solver_6dof.py:770-806). One wavefront takes 64 consecutive positions per
+// ticket; tickets are handed out in order, so every position a wavefront
+// waits on belongs to a wavefront that already holds a ticket.
+//
 // pub[i]: -1 until body i is resolved. Jacobi: the chosen color. JP:
 // (min(round, 24) << 8) | (color + 1), color = -1 when round >= 24.
-__global__ void __launch_bounds__(256)
-k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ nbr_start,
-               const int* __restrict__ nbr, const float* __restrict__ prio,
-               const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp,
-               int* err) {
-  __shared__ int base_s;
+// pos: position of each body in `order` (JP); null means the identity order.
+// Each lane collects up to kPollBatch earlier neighbours in LDS and polls
+// all of them at once with relaxed loads (the published value is the only
+// data exchanged), so one dependency step costs one L2 round trip.
+constexpr int kPollBatch = 16;
+constexpr int kSpinCap = 1 << 20;
+
+__global__ void __launch_bounds__(64)
+k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ pos,
+               const int* __restrict__ nbr_start, const int* __restrict__ nbr,
+               const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err) {
+  __shared__ int nj_s[kPollBatch][64];
+  const int lane = threadIdx.x;
   for (;;) {
-    if (threadIdx.x == 0) base_s = atomicAdd(ticket, 256);
-    __syncthreads();
-    int base = base_s;
-    __syncthreads();
+    int base = 0;
+    if (lane == 0) base = atomicAdd(ticket, 64);
+    base = __shfl(base, 0, 64);
     if (base >= nb) return;
-    int k = base + threadIdx.x;
-    int i = k < nb ? order[k] : -1;
-    bool done = (i < 0) || (!mode_jp && color_in[i] != -1);
-    int s0 = 0, s1 = 0;
-    float pi = 0.0f;
+    const int k = base + lane;
+    const int i = k < nb ? order[k] : -1;
+    bool done = !(i >= 0 && (mode_jp || color_in[i] == -1));
+    int my_pos = 0, q = 0, q1 = 0;
     if (!done) {
-      s0 = nbr_start[i];
-      s1 = nbr_start[i + 1];
-      pi = prio[i];
+      my_pos = pos ? pos[i] : i;
+      q = nbr_start[i];
+      q1 = nbr_start[i + 1];
     }
-    int spins = 0;
-    while (!done) {
-      bool ready = true;
-      unsigned used = 0u;
-      int rmax = 0;
-      for (int q = s0; q < s1 && ready; ++q) {
-        int j = nbr[q];
-        if (!mode_jp) {
-          int cj = color_in[j];
-          if (cj >= 0) {
-            if (cj < kMaxColors) used |= 1u << cj;
-          } else if (j < i) {
-            int pj = ld_acq(&pub[j]);
-            if (pj < 0) ready = false;
-            else if (pj < kMaxColors) used |= 1u << pj;
-          }
-        } else {
-          float pj_ = prio[j];
-          if (pj_ > pi || (pj_ == pi && j < i)) {
-            int pj = ld_acq(&pub[j]);
-            if (pj < 0) {
-              ready = false;
+    unsigned used = 0u, pend = 0u, gt = 0u;
+    int rmax = 0, spins = 0;
+    // Per-lane state machine: collect a batch of earlier neighbours, poll it
+    // until resolved, repeat, publish. The loop exit is uniform over the
+    // wavefront (a ballot), so a lane's publishing store stays inside the
+    // loop and a lane waiting on an earlier lane of the same wavefront sees
+    // it on the next pass; with a per-lane exit the compiler may sink the
+    // store past the loop, where it runs only once every lane has left.
+    while (__builtin_amdgcn_ballot_w64(!done) != 0) {
+      if (!done) {
+        if (pend == 0u && q < q1) {
+          int m = 0;
+          gt = 0u;
+          for (; q < q1 && m < kPollBatch; ++q) {
+            const int j = nbr[q];
+            if (!mode_jp) {
+              const int cj = color_in[j];
+              if (cj >= 0) {
+                if (cj < kMaxColors) used |= 1u << cj;
+                continue;
+              }
+              if (j >= i) continue;
             } else {
-              int cj = (pj & 255) - 1;
-              int rj = (pj >> 8) + (j > i ? 1 : 0);
+              if ((pos ? pos[j] : j) >= my_pos) continue;
+              if (j > i) gt |= 1u << m;
+            }
+            nj_s[m][lane] = j;
+            pend |= 1u << m;
+            ++m;
+          }
+        }
+        if (pend != 0u) {
+          int v[kPollBatch];
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t)
+            v[t] = (pend >> t & 1u) ? ld_relaxed(&pub[nj_s[t][lane]]) : -1;
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t) {
+            if (v[t] < 0) continue;
+            pend &= ~(1u << t);
+            if (!mode_jp) {
+              if (v[t] < kMaxColors) used |= 1u << v[t];
+            } else {
+              const int cj = (v[t] & 255) - 1;
+              const int rj = (v[t] >> 8) + (int)(gt >> t & 1u);
               if (cj >= 0) used |= 1u << cj;
               rmax = rj > rmax ? rj : rmax;
             }
           }
+          if (pend != 0u && ++spins > kSpinCap) {
+            // A dependency never resolved: publish color 0 so waiters
+            // drain, and flag the failure for the host.
+            atomicExch(err, 1);
+            st_relaxed(&pub[i], mode_jp ? 1 : 0);
+            done = true;
+          }
+        } else if (q >= q1) {
+          if (!mode_jp) {
+            st_relaxed(&pub[i], lowest_free_color(used));
+          } else {
+            const int r = rmax < kJpRounds ? rmax : kJpRounds;
+            const int c = rmax < kJpRounds ? lowest_free_color(used) : -1;
+            st_relaxed(&pub[i], (r << 8) | (c + 1));
+          }
+          done = true;
         }
       }
-      if (ready) {
-        if (!mode_jp) {
-          st_rel(&pub[i], lowest_free_color(used));
-        } else {
-          int r = rmax < kJpRounds ? rmax : kJpRounds;
-          int c = rmax < kJpRounds ? lowest_free_color(used) : -1;
-          st_rel(&pub[i], (r << 8) | (c + 1));
-        }
-        done = true;
-      } else if (++spins > (1 << 20)) {
-        // A dependency never resolved: publish color 0 so waiters drain,
-        // and flag the failure for the host.
-        atomicExch(err, 1);
-        st_rel(&pub[i], mode_jp ? 1 : 0);
-        done = true;
-      } else {
-        __builtin_amdgcn_s_sleep(2);
-      }
+      if (__builtin_amdgcn_ballot_w64(!done && pend != 0u) != 0) __builtin_amdgcn_s_sleep(1);
     }
-    __syncthreads();
   }
 }
 
@@ -459,14 +531,16 @@ k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp) {
   else if (color[i] == -1) color[i] = pub[i];
 }
 
-void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* nbr_start,
-                         const int* nbr, const float* prio, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp, int /*n_blocks*/) {
+void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
+                         const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
+                         int* err, int* ticket, int mode_jp) {
   if (nb <= 0) return;
   launch_fill_int(st, pub, nb, -1);
   launch_fill_int(st, ticket, 1, 0);
-  hipLaunchKernelGGL(k_color_greedy, dim3((nb + 255) / 256), dim3(256), 0, st, nb, order,
-                     nbr_start, nbr, prio, color_in_out, pub, ticket, mode_jp, err);
+  // Enough wavefronts to cover the GPU; each loops over tickets.
+  const int waves = std::min((nb + 63) / 64, 4096);
+  hipLaunchKernelGGL(k_color_greedy, dim3(waves), dim3(64), 0, st, nb, order, pos, nbr_start, nbr,
+                     color_in_out, pub, ticket, mode_jp, err);
   hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
                      mode_jp);
 }

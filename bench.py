@@ -57,6 +57,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-substeps", type=int, default=1)
+    ap.add_argument("--primal-variant", type=int, default=None,
+                    help="0 lane per body, 1 wavefront per body, 2 wavefront per two bodies")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -68,6 +70,8 @@ def main() -> int:
     t_build = time.perf_counter()
     s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
+    if args.primal_variant is not None:
+        s._h.set_option(0, args.primal_variant)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
           f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
