@@ -147,7 +147,7 @@ struct avbd_solver {
   int last_post = 0;
 
   // options
-  int primal_variant = 0;
+  int primal_variant = 2;  // one wavefront per two bodies
 
   // profiling
   bool prof = false;

@@ -454,8 +454,18 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
 //  C. one lane per body gathers its system and solves.
 // The additions per entry happen in the same order as the lane-per-body
 // variant, so both give identical bits.
+//
+// Phase B has no per-row conditions. A row that is absent, inactive or a
+// padding slot is stored as jl = ja = +0, k = f = g = -0: each of its terms,
+// (+0 * +0) * -0 or (+0 * -0) * 1, is -0, and x + (-0) == x bit for bit for
+// every x (round to nearest), so adding it changes nothing. Likewise g = -0
+// stands for "no geometric-stiffness term" (|f| == 0), and lanes that own an
+// entry without such a term read a constant -0 field. Each body's slots are
+// padded to a multiple of four so that phase B reads four slots per float4.
 constexpr int kRowFields = 11;           // jl 3, ja 3, k, f, g 3
-constexpr int kSlotFields = 3 * kRowFields;
+constexpr int kFieldOne = 3 * kRowFields;          // constant 1.0
+constexpr int kFieldNegZero = kFieldOne + 1;       // constant -0.0
+constexpr int kSlotFields = kFieldNegZero + 1;
 constexpr int kComps = 30;
 
 template <int BPW>
@@ -466,14 +476,17 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
                const int* __restrict__ csr_start, const int* __restrict__ csr_idx,
                const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
                int post_pass) {
-  __shared__ float rows[kSlotFields][65];
-  __shared__ int rmask[64];
+  // Row fields of the 64 slots of a pass, one row of 68 floats per field
+  // (16-byte aligned so that four consecutive slots load as one float4).
+  __shared__ __align__(16) float rows[kSlotFields][68];
   __shared__ float sys[BPW][kComps];
   __shared__ int b_i[BPW], b_deg[BPW], b_start[BPW], b_off[BPW + 1];
   __shared__ float b_pose[BPW][7];
   const int lane = threadIdx.x;
   const int first = blockIdx.x * BPW;
   const float inv_dt2 = 1.0f / (dt * dt);
+  rows[kFieldOne][lane] = 1.0f;
+  rows[kFieldNegZero][lane] = -0.0f;
   if (lane < BPW) {
     int slot = first + lane;
     int i = -1, deg = 0, st = 0;
@@ -511,82 +524,95 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   __syncthreads();
   if (lane == 0) {
     b_off[0] = 0;
-    for (int b = 0; b < BPW; ++b) b_off[b + 1] = b_off[b] + b_deg[b];
+    for (int b = 0; b < BPW; ++b) b_off[b + 1] = b_off[b] + ((b_deg[b] + 3) & ~3);
   }
   __syncthreads();
   const int total = b_off[BPW];
   // accumulator lane role
   const int ab = lane / kComps, ac = lane % kComps;
   const bool is_acc = ab < BPW && b_i[ab < BPW ? ab : 0] >= 0 && lane < BPW * kComps;
-  // component -> (kind, r, c): kind 0 A, 1 B, 2 D, 3 rl, 4 ra
-  int ck, cr, cc;
+  // Entry owned by this lane and the fields it multiplies, per row:
+  //   A[r][c] += (jl_r * jl_c) * k      B[r][c] += (ja_r * jl_c) * k
+  //   D[r][c] += (ja_r * ja_c) * k, then += g_r on the diagonal
+  //   rl[r]   += (jl_r * f) * 1         ra[r]   += (ja_r * f) * 1
+  // (the factor 1.0 is exact); fg is a row field for diagonal D entries
+  // and the constant -0 field otherwise.
+  int fa, fb, fz = 6, fg = -1;
   if (ac < 6) {
     const int rr[6] = {0, 0, 0, 1, 1, 2}, cc6[6] = {0, 1, 2, 1, 2, 2};
-    ck = 0; cr = rr[ac]; cc = cc6[ac];
+    fa = rr[ac]; fb = cc6[ac];
   } else if (ac < 15) {
-    ck = 1; cr = (ac - 6) / 3; cc = (ac - 6) % 3;
+    fa = 3 + (ac - 6) / 3; fb = (ac - 6) % 3;
   } else if (ac < 24) {
-    ck = 2; cr = (ac - 15) / 3; cc = (ac - 15) % 3;
+    const int cr = (ac - 15) / 3, cc = (ac - 15) % 3;
+    fa = 3 + cr; fb = 3 + cc;
+    if (cr == cc) fg = 8 + cr;
   } else if (ac < 27) {
-    ck = 3; cr = ac - 24; cc = 0;
+    fa = ac - 24; fb = 7; fz = -1;
   } else {
-    ck = 4; cr = ac - 27; cc = 0;
+    fa = 3 + (ac - 27); fb = 7; fz = -1;
   }
   float accv = is_acc ? sys[ab][ac] : 0.0f;
   for (int p0 = 0; p0 < total; p0 += 64) {
     // ---- A: row values of one incidence per lane ----
-    int j = p0 + lane;
-    int mask = 0, gmask = 0;
+    const int j = p0 + lane;
     if (j < total) {
       int b = 0;
       while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
-      int i = b_i[b];
-      int g = csr_idx[b_start[b] + (j - b_off[b])];
-      GroupGeo G = geo[g];
-      GroupSt S = gst[g];
-      v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
-      q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
+      const int jj = j - b_off[b];
       RowVals rv[3];
-      mask = group_rows(rv, gmask, i, xi, qi, G, S, pose, post_pass);
+      int mask = 0, gmask = 0;
+      if (jj < b_deg[b]) {
+        const int g = csr_idx[b_start[b] + jj];
+        GroupGeo G = geo[g];
+        GroupSt S = gst[g];
+        v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
+        q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
+        mask = group_rows(rv, gmask, b_i[b], xi, qi, G, S, pose, post_pass);
+      }
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        float* base_f = &rows[kRowFields * r][0];
-        (void)base_f;
-        rows[kRowFields * r + 0][lane] = rv[r].jl.x;
-        rows[kRowFields * r + 1][lane] = rv[r].jl.y;
-        rows[kRowFields * r + 2][lane] = rv[r].jl.z;
-        rows[kRowFields * r + 3][lane] = rv[r].ja.x;
-        rows[kRowFields * r + 4][lane] = rv[r].ja.y;
-        rows[kRowFields * r + 5][lane] = rv[r].ja.z;
-        rows[kRowFields * r + 6][lane] = rv[r].k;
-        rows[kRowFields * r + 7][lane] = rv[r].f;
-        rows[kRowFields * r + 8][lane] = rv[r].g.x;
-        rows[kRowFields * r + 9][lane] = rv[r].g.y;
-        rows[kRowFields * r + 10][lane] = rv[r].g.z;
+        const bool on = (mask >> r) & 1;
+        const bool gon = on && ((gmask >> r) & 1);
+        float* o = &rows[kRowFields * r][0];
+        o[0 * 68 + lane] = on ? rv[r].jl.x : 0.0f;
+        o[1 * 68 + lane] = on ? rv[r].jl.y : 0.0f;
+        o[2 * 68 + lane] = on ? rv[r].jl.z : 0.0f;
+        o[3 * 68 + lane] = on ? rv[r].ja.x : 0.0f;
+        o[4 * 68 + lane] = on ? rv[r].ja.y : 0.0f;
+        o[5 * 68 + lane] = on ? rv[r].ja.z : 0.0f;
+        o[6 * 68 + lane] = on ? rv[r].k : -0.0f;
+        o[7 * 68 + lane] = on ? rv[r].f : -0.0f;
+        o[8 * 68 + lane] = gon ? rv[r].g.x : -0.0f;
+        o[9 * 68 + lane] = gon ? rv[r].g.y : -0.0f;
+        o[10 * 68 + lane] = gon ? rv[r].g.z : -0.0f;
       }
     }
-    rmask[lane] = mask | (gmask << 3);
     __syncthreads();
-    // ---- B: ordered accumulation ----
+    // ---- B: ordered accumulation, four slots per step ----
     if (is_acc) {
-      int s0 = b_off[ab] > p0 ? b_off[ab] - p0 : 0;
-      int s1 = b_off[ab + 1] - p0 < 64 ? b_off[ab + 1] - p0 : 64;
-      for (int sl = s0; sl < s1; ++sl) {
-        int mk = rmask[sl];
+      const int s0 = b_off[ab] > p0 ? b_off[ab] - p0 : 0;
+      const int s1 = b_off[ab + 1] - p0 < 64 ? b_off[ab + 1] - p0 : 64;
+      for (int sb = s0; sb < s1; sb += 4) {
+        __builtin_assume((sb & 3) == 0);  // body slot ranges start on multiples of 4
+        float4 X[3], Y[3], Z[3], Gd[3];
+#pragma unroll
         for (int r = 0; r < 3; ++r) {
-          if (!(mk & (1 << r))) continue;
-          const int fb = kRowFields * r;
-          if (ck == 0) {
-            accv = accv + (rows[fb + cr][sl] * rows[fb + cc][sl]) * rows[fb + 6][sl];
-          } else if (ck == 1) {
-            accv = accv + (rows[fb + 3 + cr][sl] * rows[fb + cc][sl]) * rows[fb + 6][sl];
-          } else if (ck == 2) {
-            accv = accv + (rows[fb + 3 + cr][sl] * rows[fb + 3 + cc][sl]) * rows[fb + 6][sl];
-            if (cr == cc && (mk & (8 << r))) accv = accv + rows[fb + 8 + cr][sl];
-          } else if (ck == 3) {
-            accv = accv + rows[fb + cr][sl] * rows[fb + 7][sl];
-          } else {
-            accv = accv + rows[fb + 3 + cr][sl] * rows[fb + 7][sl];
+          const int rb = kRowFields * r;
+          X[r] = *reinterpret_cast<const float4*>(&rows[rb + fa][sb]);
+          Y[r] = *reinterpret_cast<const float4*>(&rows[rb + fb][sb]);
+          Z[r] = *reinterpret_cast<const float4*>(&rows[fz < 0 ? kFieldOne : rb + fz][sb]);
+          Gd[r] = *reinterpret_cast<const float4*>(&rows[fg < 0 ? kFieldNegZero : rb + fg][sb]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float x = u == 0 ? X[r].x : u == 1 ? X[r].y : u == 2 ? X[r].z : X[r].w;
+            const float y = u == 0 ? Y[r].x : u == 1 ? Y[r].y : u == 2 ? Y[r].z : Y[r].w;
+            const float z = u == 0 ? Z[r].x : u == 1 ? Z[r].y : u == 2 ? Z[r].z : Z[r].w;
+            const float g = u == 0 ? Gd[r].x : u == 1 ? Gd[r].y : u == 2 ? Gd[r].z : Gd[r].w;
+            accv = (accv + (x * y) * z) + g;
           }
         }
       }

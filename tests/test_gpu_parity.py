@@ -38,24 +38,42 @@ def test_jones_plassmann_mode_matches():
         assert_bit_exact(diff_report(g, o), f"JP substep {k}")
 
 
+def _tiled_scene(name):
+    from avbd_amd.solver_6dof import Solver6DOF
+    if name == "pyramid5":
+        return scenes.pyramid(5, seed=8, solver_cls=Solver6DOF)
+    if name == "pile":
+        # boxes of mixed size landing on each other and on the floor
+        return scenes.random_pile(150, seed=9, lattice=(6, 6, 5), solver_cls=Solver6DOF)
+    # pinned boxes hanging beside a stack they lean on: pin, floor and
+    # box-box groups in the same bodies' constraint lists
+    s = Solver6DOF(dt=1.0 / 60.0, iterations=10, substeps=4)
+    s.enable_self_collision(True, default_friction=0.5)
+    h = 0.25
+    for k in range(4):
+        b = s.add_box((0.0, h + 2 * h * k, 0.0), (h, h, h), mass=10.0, friction=0.5)
+        s.add_floor_contact_box(b, friction=0.5)
+    for k in range(3):
+        p = s.add_box((0.52, 0.9 + 0.5 * k, 0.1 * k), (h, h, h), mass=5.0, friction=0.5)
+        s.add_pin_corner(p, body_local=(h, h, h), world_point=(0.52 + h, 0.9 + 0.5 * k + h, 0.1 * k + h))
+    return s
+
+
 @pytest.mark.parametrize("variant", [1, 2])
-def test_tiled_primal_bit_identical_to_lane_per_body(variant):
+@pytest.mark.parametrize("scene", ["pyramid5", "pile", "pins"])
+def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     """The tiled primal kernels sum every system entry in the same order as
     the lane-per-body kernel; all observables must agree bit for bit."""
-    from avbd_amd.solver_6dof import Solver6DOF
-
-    def make():
-        return scenes.pyramid(5, seed=8, solver_cls=Solver6DOF)
-
-    a, b = make(), make()
+    a, b = _tiled_scene(scene), _tiled_scene(scene)
     a._flush()
     b._flush()
     b._h.set_option(0, variant)
-    for k in range(24):
+    n_sub = {"pyramid5": 24, "pile": 200, "pins": 60}[scene]
+    for k in range(n_sub):
         one_substep(a)
         one_substep(b)
         ba, bb = a._h.read_bodies(len(a._x)), b._h.read_bodies(len(b._x))
         for key in ("x", "q", "v", "w"):
-            assert np.array_equal(ba[key], bb[key]), (variant, k, key)
+            assert np.array_equal(ba[key], bb[key]), (variant, scene, k, key)
         n = a._h.read_counters()[0]
         assert np.array_equal(a._h.read_rows(n)["lambda"], b._h.read_rows(n)["lambda"])
