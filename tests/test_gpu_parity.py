@@ -67,6 +67,7 @@ def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     a, b = _tiled_scene(scene), _tiled_scene(scene)
     a._flush()
     b._flush()
+    a._h.set_option(0, 0)          # lane per body
     b._h.set_option(0, variant)
     n_sub = {"pyramid5": 24, "pile": 200, "pins": 60}[scene]
     for k in range(n_sub):
