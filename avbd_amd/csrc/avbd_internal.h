@@ -63,9 +63,9 @@ void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose, fl
                           int post, int do_prelude);
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
-                         const float4* bstat, const int* csr_start, const int* csr_idx,
-                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int variant);
+                         const float4* bstat, const int* cpos, const int2* csr,
+                         const int* pos_map, const GroupGeo* geo, const GroupSt* gst, float dt,
+                         int post_pass, int variant);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
@@ -111,13 +111,18 @@ void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const Group
 // Exclusive scan of n ints; returns the total through *total_dev (device int).
 size_t scan_workspace_bytes(int n);
 void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws);
-void launch_csr_count(hipStream_t st, int nb, const int* st_count, int n_dyn, int g_first,
-                      const GroupGeo* geo, int* counts);
-void launch_csr_fill(hipStream_t st, int nb, const int* st_start, const int* st_idx,
-                     const int* starts, int* cursor, int n_dyn, int g_first,
-                     const GroupGeo* geo, int* csr_idx);
-void launch_csr_sort(hipStream_t st, int nb, const int* starts, const int* st_count,
-                     int* csr_idx);
+void launch_tcsr_pairs(hipStream_t st, int kept, const int* pa, const int* pb, const int* succ,
+                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a);
+void launch_tcsr_asb_fill(hipStream_t st, int kept, const int* pb, const int* succ,
+                          const int* asb_start, int* asb_cur, int* asb_list);
+void launch_tcsr_count(hipStream_t st, int nb, const int* cvals, const int* st_count,
+                       const int* asb_groups, const int* start_a, const int* end_a,
+                       const int* sscan, const int* succ, int* padc, int* pos_of);
+void launch_tcsr_posmap(hipStream_t st, int nb, const int* cvals, const int* pos_of, int* pos_map);
+void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_start,
+                      const int* st_idx, const int* asb_start, int* asb_list, const int* start_a,
+                      const int* end_a, const int* sscan, const int* succ, const int* pa,
+                      const int* pb, int g_static, const int* cpos, int2* csr);
 void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
                             const int* color, int* out);
 void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,

@@ -126,7 +126,12 @@ struct avbd_solver {
   DBuf<float4> hstate;
 
   // CSR
-  DBuf<int> csr_count, csr_start, csr_cursor, csr_idx;
+  // body -> group CSR in color-position order (see kernels_graph.hip)
+  DBuf<int> tc_work;        // 5 * nb: as-b pair count, as-b group count, as-a start, as-a end, cursor
+  DBuf<int> asb_start, asb_list, padc, cpos, pos_of, pos_map;
+  DBuf<int2> tcsr;
+  bool csr_built = false;       // a CSR has been built since the last flush
+  bool csr_order_stale = false; // recolored since that CSR was built
 
   // coloring
   DBuf<int> color, color_next, nbr_count, nbr_start, nbr_cursor, nbr, pub, ticket, ccounts;
@@ -318,15 +323,29 @@ int count_conflicts(avbd_solver* s, int* out) {
 
 int build_csr(avbd_solver* s) {
   const int nb = s->nb, nd = s->n_contacts;
+  const int kept = nd > 0 ? s->adj_pairs : 0;
   hipStream_t st = s->st;
   CK(prof_begin(s, kCsr));
-  launch_csr_count(st, nb, s->st_count.p, nd, s->g_static, s->geo.p, s->csr_count.p);
-  CK(scan(s, s->csr_count.p, s->csr_start.p, nb, s->csr_start.p + nb));
-  CK(s->csr_idx.ensure((size_t)s->st_total + 2 * (size_t)nd + 1));
-  launch_csr_fill(st, nb, s->st_start.p, s->st_idx.p, s->csr_start.p, s->csr_cursor.p, nd,
-                  s->g_static, s->geo.p, s->csr_idx.p);
-  launch_csr_sort(st, nb, s->csr_start.p, s->st_count.p, s->csr_idx.p);
+  int* asb_cnt = s->tc_work.p;
+  int* asb_groups = asb_cnt + nb;
+  int* start_a = asb_groups + nb;
+  int* end_a = start_a + nb;
+  int* asb_cur = end_a + nb;
+  launch_fill_int(st, s->tc_work.p, 5 * nb, 0);
+  CK(s->asb_list.ensure((size_t)std::max(kept, 1)));
+  launch_tcsr_pairs(st, kept, s->pa.p, s->pb.p, s->succ.p, asb_cnt, asb_groups, start_a, end_a);
+  CK(scan(s, asb_cnt, s->asb_start.p, nb, s->asb_start.p + nb));
+  launch_tcsr_asb_fill(st, kept, s->pb.p, s->succ.p, s->asb_start.p, asb_cur, s->asb_list.p);
+  launch_tcsr_count(st, nb, s->cvals.p, s->st_count.p, asb_groups, start_a, end_a, s->sscan.p,
+                    s->succ.p, s->padc.p, s->pos_of.p);
+  CK(scan(s, s->padc.p, s->cpos.p, nb, s->cpos.p + nb));
+  CK(s->tcsr.ensure((size_t)s->st_total + 2 * (size_t)nd + 3 * (size_t)nb + 1));
+  launch_tcsr_fill(st, nb, s->cvals.p, s->st_start.p, s->st_idx.p, s->asb_start.p, s->asb_list.p,
+                   start_a, end_a, s->sscan.p, s->succ.p, s->pa.p, s->pb.p, s->g_static, s->cpos.p,
+                   s->tcsr.p);
   CK(prof_end(s, 0.0));
+  s->csr_built = true;
+  s->csr_order_stale = false;
   return AVBD_OK;
 }
 
@@ -428,7 +447,17 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   }
   rep->n_active_colors = s->n_active_colors;
 
-  if (n_active > 0) CK(build_csr(s));
+  // Without live rows the reference keeps the previous body->constraint
+  // lists (solver_6dof.py:1366); if the color order changed since they were
+  // built, map each color position to the list's old position.
+  const int* pos_map = nullptr;
+  if (need) s->csr_order_stale = true;
+  if (n_active > 0) {
+    CK(build_csr(s));
+  } else if (s->csr_built && s->csr_order_stale) {
+    launch_tcsr_posmap(st, nb, s->cvals.p, s->pos_of.p, s->pos_map.p);
+    pos_map = s->pos_map.p;
+  }
 
   const int n_groups = s->g_static + s->n_contacts;
   float4* p0 = s->pose[s->cur].p;
@@ -455,7 +484,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       int cnt = s->cstart[c + 1] - s->cstart[c];
       CK(prof_begin(s, kPrimal));
       launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
-                          s->bstat.p, s->csr_start.p, s->csr_idx.p, s->geo.p, s->gst.p, dt,
+                          s->bstat.p, s->cpos.p, s->tcsr.p, pos_map, s->geo.p, s->gst.p, dt,
                           post_pass, s->primal_variant);
       CK(prof_end(s, pbytes));
     }
@@ -709,9 +738,13 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   launch_fill_int(s->st, s->hkeys.p, std::max(s->hash_cap, 1), kHashEmpty);
   HIPCK(hipMemsetAsync(s->hstate.p, 0, 2 * (size_t)std::max(s->hash_cap, 1) * sizeof(float4), s->st));
   // ---- CSR + coloring ----
-  CK(s->csr_count.alloc(nbb)); CK(s->csr_start.alloc(nbb + 1)); CK(s->csr_cursor.alloc(nbb));
-  CK(s->csr_idx.alloc(std::max(s->st_total, 1)));
-  HIPCK(hipMemsetAsync(s->csr_start.p, 0, (nbb + 1) * sizeof(int), s->st));
+  CK(s->tc_work.alloc(5 * nbb)); CK(s->asb_start.alloc(nbb + 1)); CK(s->asb_list.alloc(1));
+  CK(s->padc.alloc(nbb)); CK(s->cpos.alloc(nbb + 1)); CK(s->pos_of.alloc(nbb));
+  CK(s->pos_map.alloc(nbb));
+  s->csr_built = false;
+  s->csr_order_stale = false;
+  CK(s->tcsr.alloc(std::max(s->st_total, 1) + 3 * (size_t)nbb));
+  HIPCK(hipMemsetAsync(s->cpos.p, 0, (nbb + 1) * sizeof(int), s->st));
   CK(s->color.alloc(nbb)); CK(s->color_next.alloc(nbb)); CK(s->nbr_count.alloc(nbb));
   CK(s->nbr_start.alloc(nbb + 1)); CK(s->nbr_cursor.alloc(nbb)); CK(s->nbr.alloc(1));
   CK(s->pub.alloc(nbb)); CK(s->ticket.alloc(1)); CK(s->ccounts.alloc(kMaxColors));
@@ -894,6 +927,7 @@ int avbd_write_colors(avbd_solver* s, const int* c, int n_active_colors) {
   for (int i = 0; i < s->nb; ++i) cnt[c[i]]++;
   s->cstart[0] = 0;
   for (int k = 0; k < kMaxColors; ++k) s->cstart[k + 1] = s->cstart[k] + cnt[k];
+  s->csr_order_stale = true;
   color_bucket(s->st, s->nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
                s->ws_color.p);
   HIPCK(hipStreamSynchronize(s->st));
@@ -1070,7 +1104,7 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   switch (option) {
     case AVBD_OPT_PRIMAL_VARIANT:
-      if (value < 0 || value > 2) return set_err(AVBD_E_ARG, "primal variant must be 0, 1 or 2");
+      if (value < 0 || value > 4) return set_err(AVBD_E_ARG, "primal variant must be 0 to 4");
       s->primal_variant = value;
       return AVBD_OK;
     default:

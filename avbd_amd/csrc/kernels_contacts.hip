@@ -219,15 +219,10 @@ k_karras(int n, const unsigned* __restrict__ keys, int* __restrict__ left,
   if (i == 0) parent[0] = -1;
 }
 
-__device__ __forceinline__ float4 ld_coherent(const float4* p) {
-  const int* q = reinterpret_cast<const int*>(p);
-  float x = __int_as_float(__hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  float y = __int_as_float(__hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  float z = __int_as_float(__hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  return make_float4(x, y, z, 0.0f);
-}
-
 // Bottom-up union of child boxes; the second arrival at a node continues.
+// The arrival counter is an acquire-release atomic: the release publishes
+// this thread's child box, the acquire (which invalidates this CU's L1)
+// makes the sibling's box visible to the plain loads that follow.
 __global__ void __launch_bounds__(256)
 k_refit(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
         const int* __restrict__ ids, const int* __restrict__ left, const int* __restrict__ right,
@@ -236,22 +231,22 @@ k_refit(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
   if (k >= n) return;
   int node = n - 1 + k;
   int body = ids[k];
-  nlo[node] = lo[body];
-  nhi[node] = hi[body];
+  float4 blo = lo[body], bhi = hi[body];
+  nlo[node] = blo;
+  nhi[node] = bhi;
   if (n == 1) return;
-  __threadfence();
   int p = parent[node];
   for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
-    int prev = atomicAdd(&flags[p], 1);
+    int prev = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (prev == 0) return;
-    __threadfence();
-    int L = left[p], R = right[p];
-    // Children may have been written by other CUs: read through L2.
-    float4 a = ld_coherent(&nlo[L]), b = ld_coherent(&nlo[R]);
-    float4 c = ld_coherent(&nhi[L]), e = ld_coherent(&nhi[R]);
-    nlo[p] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
-    nhi[p] = make_float4(fmaxf(c.x, e.x), fmaxf(c.y, e.y), fmaxf(c.z, e.z), 0.0f);
-    __threadfence();
+    const int L = left[p], R = right[p];
+    const int other = (L == node) ? R : L;
+    const float4 olo = nlo[other], ohi = nhi[other];
+    blo = make_float4(fminf(blo.x, olo.x), fminf(blo.y, olo.y), fminf(blo.z, olo.z), 0.0f);
+    bhi = make_float4(fmaxf(bhi.x, ohi.x), fmaxf(bhi.y, ohi.y), fmaxf(bhi.z, ohi.z), 0.0f);
+    nlo[p] = blo;
+    nhi[p] = bhi;
+    node = p;
     p = parent[p];
   }
 }

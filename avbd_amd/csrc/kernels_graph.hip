@@ -3,10 +3,10 @@
 //  exclusive_scan       device-wide int scan (block scan + carry pass)
 //  radix_sort_pairs     LSD radix sort, 4-bit digits, stable ranks from
 //                       64-lane ballots (used for Morton codes and colors)
-//  csr_*                body -> contact-group CSR with each body's groups in
-//                       ascending order (gpu_csr_count/starts/scatter,
-//                       kernels_6dof.py:2626-2692, ascending as a serial
-//                       scatter leaves it)
+//  tcsr_*               body -> contact-group lists in color-position order,
+//                       each body's groups ascending as a serial scatter
+//                       leaves them (gpu_csr_count/starts/scatter,
+//                       kernels_6dof.py:2627-2692)
 //  count_conflicts      gpu_count_color_conflicts (kernels_6dof.py:3313-3336)
 //  adj_*                gpu_body_adj_count/scatter (kernels_6dof.py:3045-3096)
 //  k_color_greedy       the serial effect of gpu_color_spec_assign
@@ -222,83 +222,143 @@ void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_
 }
 
 // ---------------------------------------------------------------------------
-// body -> group CSR
+// body -> group CSR in color-position order
 // ---------------------------------------------------------------------------
+// Position k of the color bucketing (bodies sorted by color, ascending within
+// a color) owns entries [cpos[k], cpos[k+1]) of `csr`: its static groups in
+// row order, then its dynamic groups in ascending group index, then padding
+// {-1, -1} up to a multiple of four. Entry = {group, other body} (the body
+// itself for floor and pin groups). Dynamic groups come from the broadphase
+// pairs, which are in (a asc, b asc) order with their groups numbered in
+// pair order: body i's groups as the pair's b (pairs (x, i), x < i) all
+// precede its groups as the pair's a (a contiguous pair range), so the
+// ascending group order is "as-b pairs in pair order, then the as-a range".
+// This is the order the reference's CSR scatter produces when its row loop
+// runs serially (gpu_csr_count / gpu_csr_scatter, kernels_6dof.py:2627-2692).
+
+// Per kept pair: as-a range boundaries, as-b pair and group counts.
 __global__ void __launch_bounds__(256)
-k_csr_init(int nb, const int* __restrict__ st_count, int* __restrict__ counts) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nb) counts[i] = st_count[i];
-}
-
-__global__ void __launch_bounds__(256)
-k_csr_count_dyn(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, int* counts) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_dyn) return;
-  int4 hd = geo[g_first + e].hdr;
-  atomicAdd(&counts[hd.y], 1);
-  atomicAdd(&counts[hd.z], 1);
-}
-
-void launch_csr_count(hipStream_t st, int nb, const int* st_count, int n_dyn, int g_first,
-                      const GroupGeo* geo, int* counts) {
-  if (nb > 0)
-    hipLaunchKernelGGL(k_csr_init, dim3((nb + 255) / 256), dim3(256), 0, st, nb, st_count, counts);
-  if (n_dyn > 0)
-    hipLaunchKernelGGL(k_csr_count_dyn, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
-                       g_first, geo, counts);
-}
-
-__global__ void __launch_bounds__(256)
-k_csr_fill_static(int nb, const int* __restrict__ st_start, const int* __restrict__ st_idx,
-                  const int* __restrict__ starts, int* __restrict__ cursor, int* __restrict__ csr) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  int s0 = st_start[i], s1 = st_start[i + 1], o = starts[i];
-  for (int k = s0; k < s1; ++k) csr[o + (k - s0)] = st_idx[k];
-  cursor[i] = o + (s1 - s0);
-}
-
-__global__ void __launch_bounds__(256)
-k_csr_fill_dyn(int n_dyn, int g_first, const GroupGeo* __restrict__ geo, int* cursor, int* csr) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_dyn) return;
-  int g = g_first + e;
-  int4 hd = geo[g].hdr;
-  csr[atomicAdd(&cursor[hd.y], 1)] = g;
-  csr[atomicAdd(&cursor[hd.z], 1)] = g;
-}
-
-void launch_csr_fill(hipStream_t st, int nb, const int* st_start, const int* st_idx,
-                     const int* starts, int* cursor, int n_dyn, int g_first, const GroupGeo* geo,
-                     int* csr_idx) {
-  if (nb > 0)
-    hipLaunchKernelGGL(k_csr_fill_static, dim3((nb + 255) / 256), dim3(256), 0, st, nb, st_start,
-                       st_idx, starts, cursor, csr_idx);
-  if (n_dyn > 0)
-    hipLaunchKernelGGL(k_csr_fill_dyn, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn, g_first,
-                       geo, cursor, csr_idx);
-}
-
-// Sort each body's dynamic tail [starts[i] + st_count[i], starts[i+1]) ascending.
-__global__ void __launch_bounds__(256)
-k_csr_sort(int nb, const int* __restrict__ starts, const int* __restrict__ st_count,
-           int* __restrict__ csr) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  int b0 = starts[i] + st_count[i], b1 = starts[i + 1];
-  for (int a = b0 + 1; a < b1; ++a) {
-    int v = csr[a];
-    int b = a - 1;
-    while (b >= b0 && csr[b] > v) { csr[b + 1] = csr[b]; --b; }
-    csr[b + 1] = v;
+k_tcsr_pairs(int kept, const int* __restrict__ pa, const int* __restrict__ pb,
+             const int* __restrict__ succ, int* __restrict__ asb_cnt,
+             int* __restrict__ asb_groups, int* __restrict__ start_a, int* __restrict__ end_a) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kept) return;
+  const int a = pa[p], b = pb[p];
+  if (p == 0 || pa[p - 1] != a) start_a[a] = p;
+  if (p == kept - 1 || pa[p + 1] != a) end_a[a] = p + 1;
+  const int n = succ[p];
+  if (n > 0) {
+    atomicAdd(&asb_cnt[b], 1);
+    atomicAdd(&asb_groups[b], n);
   }
 }
 
-void launch_csr_sort(hipStream_t st, int nb, const int* starts, const int* st_count,
-                     int* csr_idx) {
+__global__ void __launch_bounds__(256)
+k_tcsr_asb_fill(int kept, const int* __restrict__ pb, const int* __restrict__ succ,
+                const int* __restrict__ asb_start, int* __restrict__ asb_cur,
+                int* __restrict__ asb_list) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kept || succ[p] <= 0) return;
+  const int b = pb[p];
+  asb_list[asb_start[b] + atomicAdd(&asb_cur[b], 1)] = p;
+}
+
+__device__ __forceinline__ int as_a_groups(int i, const int* start_a, const int* end_a,
+                                           const int* sscan, const int* succ) {
+  const int s = start_a[i], e = end_a[i];
+  return e > s ? (sscan[e - 1] + succ[e - 1]) - sscan[s] : 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_tcsr_count(int nb, const int* __restrict__ cvals, const int* __restrict__ st_count,
+             const int* __restrict__ asb_groups, const int* __restrict__ start_a,
+             const int* __restrict__ end_a, const int* __restrict__ sscan,
+             const int* __restrict__ succ, int* __restrict__ padc, int* __restrict__ pos_of) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int i = cvals[k];
+  pos_of[i] = k;
+  const int cnt = st_count[i] + asb_groups[i] + as_a_groups(i, start_a, end_a, sscan, succ);
+  padc[k] = (cnt + 3) & ~3;
+}
+
+__global__ void __launch_bounds__(256)
+k_tcsr_fill(int nb, const int* __restrict__ cvals, const int* __restrict__ st_start,
+            const int* __restrict__ st_idx, const int* __restrict__ asb_start,
+            int* __restrict__ asb_list, const int* __restrict__ start_a,
+            const int* __restrict__ end_a, const int* __restrict__ sscan,
+            const int* __restrict__ succ, const int* __restrict__ pa, const int* __restrict__ pb,
+            int g_static, const int* __restrict__ cpos, int2* __restrict__ csr) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int i = cvals[k];
+  int o = cpos[k];
+  const int end = cpos[k + 1];
+  for (int q = st_start[i]; q < st_start[i + 1]; ++q) csr[o++] = make_int2(st_idx[q], i);
+  // as-b pairs: a handful per body, sorted in place by pair index
+  const int l0 = asb_start[i], l1 = asb_start[i + 1];
+  for (int u = l0 + 1; u < l1; ++u) {
+    const int v = asb_list[u];
+    int w = u - 1;
+    while (w >= l0 && asb_list[w] > v) { asb_list[w + 1] = asb_list[w]; --w; }
+    asb_list[w + 1] = v;
+  }
+  for (int u = l0; u < l1; ++u) {
+    const int p = asb_list[u];
+    const int g0 = g_static + sscan[p], n = succ[p], other = pa[p];
+    for (int c = 0; c < n; ++c) csr[o++] = make_int2(g0 + c, other);
+  }
+  for (int p = start_a[i]; p < end_a[i]; ++p) {
+    const int g0 = g_static + sscan[p], n = succ[p], other = pb[p];
+    for (int c = 0; c < n; ++c) csr[o++] = make_int2(g0 + c, other);
+  }
+  while (o < end) csr[o++] = make_int2(-1, -1);
+}
+
+void launch_tcsr_pairs(hipStream_t st, int kept, const int* pa, const int* pb, const int* succ,
+                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a) {
+  if (kept > 0)
+    hipLaunchKernelGGL(k_tcsr_pairs, dim3((kept + 255) / 256), dim3(256), 0, st, kept, pa, pb, succ,
+                       asb_cnt, asb_groups, start_a, end_a);
+}
+
+void launch_tcsr_asb_fill(hipStream_t st, int kept, const int* pb, const int* succ,
+                          const int* asb_start, int* asb_cur, int* asb_list) {
+  if (kept > 0)
+    hipLaunchKernelGGL(k_tcsr_asb_fill, dim3((kept + 255) / 256), dim3(256), 0, st, kept, pb, succ,
+                       asb_start, asb_cur, asb_list);
+}
+
+void launch_tcsr_count(hipStream_t st, int nb, const int* cvals, const int* st_count,
+                       const int* asb_groups, const int* start_a, const int* end_a,
+                       const int* sscan, const int* succ, int* padc, int* pos_of) {
   if (nb > 0)
-    hipLaunchKernelGGL(k_csr_sort, dim3((nb + 255) / 256), dim3(256), 0, st, nb, starts, st_count,
-                       csr_idx);
+    hipLaunchKernelGGL(k_tcsr_count, dim3((nb + 255) / 256), dim3(256), 0, st, nb, cvals, st_count,
+                       asb_groups, start_a, end_a, sscan, succ, padc, pos_of);
+}
+
+// pos_map[k] = CSR position of the body now at color position k.
+__global__ void __launch_bounds__(256)
+k_tcsr_posmap(int nb, const int* __restrict__ cvals, const int* __restrict__ pos_of,
+              int* __restrict__ pos_map) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nb) pos_map[k] = pos_of[cvals[k]];
+}
+
+void launch_tcsr_posmap(hipStream_t st, int nb, const int* cvals, const int* pos_of, int* pos_map) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_tcsr_posmap, dim3((nb + 255) / 256), dim3(256), 0, st, nb, cvals, pos_of,
+                       pos_map);
+}
+
+void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_start,
+                      const int* st_idx, const int* asb_start, int* asb_list, const int* start_a,
+                      const int* end_a, const int* sscan, const int* succ, const int* pa,
+                      const int* pb, int g_static, const int* cpos, int2* csr) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_tcsr_fill, dim3((nb + 255) / 256), dim3(256), 0, st, nb, cvals, st_start,
+                       st_idx, asb_start, asb_list, start_a, end_a, sscan, succ, pa, pb, g_static,
+                       cpos, csr);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,12 +634,20 @@ void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const in
                        prio, color, color_next, uncolored);
 }
 
+// Uncolored bodies (-1) take color 0; per-color body counts go through a
+// block histogram in LDS, then one global atomic per color per block.
 __global__ void __launch_bounds__(256) k_color_finish(int nb, int* color, int* counts) {
+  __shared__ int h[kMaxColors];
+  if (threadIdx.x < kMaxColors) h[threadIdx.x] = 0;
+  __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  int c = color[i];
-  if (c == -1) { c = 0; color[i] = 0; }
-  if (c >= 0 && c < kMaxColors) atomicAdd(&counts[c], 1);
+  if (i < nb) {
+    int c = color[i];
+    if (c == -1) { c = 0; color[i] = 0; }
+    if (c >= 0 && c < kMaxColors) atomicAdd(&h[c], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxColors && h[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
 void launch_color_finish(hipStream_t st, int nb, int* color, int* counts) {

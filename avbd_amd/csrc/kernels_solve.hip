@@ -256,8 +256,9 @@ __device__ __forceinline__ void row_values(RowVals& o, bool& gflag, v3 jl, v3 ja
 
 // Rows of group G seen from body i, in row order. Returns a bit mask of the
 // rows present and active (bit r) and sets gmask (bit r: |f| > 0).
-__device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, v3 xi, q4 qi,
-                                          const GroupGeo& G, const GroupSt& S,
+// `other` is the other body of a box-box group (the body itself otherwise).
+__device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, int other, v3 xi,
+                                          q4 qi, const GroupGeo& G, const GroupSt& S,
                                           const float4* __restrict__ pose, int post_pass) {
   const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
   const int bits = __float_as_int(S.s2.y);
@@ -269,74 +270,52 @@ __device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, v3 
   int mask = 0;
   gmask = 0;
   bool gf;
-  if (kind == kGFloor) {
-    v3 r_self = qrotate(qi, off_a);
-    if (bits & kActN) {
-      v3 n = mk3(0.0f, 1.0f, 0.0f);
-      float C = (xi.y + r_self.y) - G.g0.y;
-      row_values(out[0], gf, n, cross(r_self, n), C, n, r_self, hard, S.s1.x, S.s0.x,
-                 post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
-      mask |= 1;
-      gmask |= gf ? 1 : 0;
-    }
-    if (nr == 3) {
-      v3 xr = add(xi, r_self);
-      if (bits & kActT) {
-        v3 t = mk3(1.0f, 0.0f, 0.0f);
-        row_values(out[1], gf, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.y, S.s0.y,
-                   S.s1.w, true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
-        mask |= 2;
-        gmask |= gf ? 2 : 0;
-      }
-      if (bits & kActB) {
-        v3 t = mk3(0.0f, 0.0f, 1.0f);
-        row_values(out[2], gf, t, cross(r_self, t), dot(t, xr), t, r_self, hard, S.s1.z, S.s0.z,
-                   S.s2.x, true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
-        mask |= 4;
-        gmask |= gf ? 4 : 0;
-      }
-    }
-  } else if (kind == kGBox) {
+  if (kind == kGFloor || kind == kGBox) {
+    // Floor corners and box-box contacts share one instruction stream: a
+    // floor row is a contact whose "other side" is the world, with the fixed
+    // directions y (normal), x and z (tangents). Each kind keeps its own
+    // arithmetic: the floor normal C is (x.y + r.y) - anchor.y and floor
+    // tangent C is t . (x + r), box rows use n, t, b . (w_a - w_b).
+    const bool is_floor = (kind == kGFloor);
     const int a = G.hdr.y, b = G.hdr.z;
-    const bool self_a = (a == i);
-    v3 off_b = f3(G.g2);
-    v3 r_self, wa, wb;
-    if (self_a) {
-      r_self = qrotate(qi, off_a);
-      v3 r_o = qrotate(f4q(pose[2 * b + 1]), off_b);
-      wa = add(xi, r_self);
-      wb = add(f3(pose[2 * b]), r_o);
-    } else {
-      v3 r_o = qrotate(f4q(pose[2 * a + 1]), off_a);
-      r_self = qrotate(qi, off_b);
-      wa = add(f3(pose[2 * a]), r_o);
-      wb = add(xi, r_self);
-    }
-    v3 d = sub(wa, wb);
-    v3 n = f3(G.g0);
+    const bool self_a = is_floor || (a == i);
+    (void)b;
+    const v3 off_b = f3(G.g2);
+    const v3 off_self = self_a ? off_a : off_b;
+    const v3 off_other = self_a ? off_b : off_a;
+    const v3 r_self = qrotate(qi, off_self);
+    const v3 r_o = qrotate(f4q(pose[2 * other + 1]), off_other);
+    const v3 ws = add(xi, r_self);
+    const v3 wo = add(f3(pose[2 * other]), r_o);
+    const v3 d = self_a ? sub(ws, wo) : sub(wo, ws);
+    const v3 D = is_floor ? ws : d;
+    const v3 gn = f3(G.g0);
+    const v3 n = is_floor ? mk3(0.0f, 1.0f, 0.0f) : gn;
     if (bits & kActN) {
-      v3 jl = self_a ? n : neg(n);
-      v3 ja = self_a ? cross(r_self, n) : neg(cross(r_self, n));
-      row_values(out[0], gf, jl, ja, dot(n, d), n, r_self, hard, S.s1.x, S.s0.x,
+      const float C = is_floor ? ws.y - G.g0.y : dot(n, d);
+      const v3 jl = self_a ? n : neg(n);
+      const v3 ja = self_a ? cross(r_self, n) : neg(cross(r_self, n));
+      row_values(out[0], gf, jl, ja, C, n, r_self, hard, S.s1.x, S.s0.x,
                  post_pass ? 0.0f : S.s0.w, false, 0.f, false, 0.f, 0.f, -kInf, 0.0f);
       mask |= 1;
       gmask |= gf ? 1 : 0;
     }
     if (nr == 3) {
-      v3 t = orthonormal_basis(n);
-      v3 bb = cross(n, t);
+      const v3 bt = orthonormal_basis(n);
+      const v3 t = is_floor ? mk3(1.0f, 0.0f, 0.0f) : bt;
+      const v3 bb = is_floor ? mk3(0.0f, 0.0f, 1.0f) : cross(n, bt);
       if (bits & kActT) {
-        v3 jl = self_a ? t : neg(t);
-        v3 ja = self_a ? cross(r_self, t) : neg(cross(r_self, t));
-        row_values(out[1], gf, jl, ja, dot(t, d), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w, true,
+        const v3 jl = self_a ? t : neg(t);
+        const v3 ja = self_a ? cross(r_self, t) : neg(cross(r_self, t));
+        row_values(out[1], gf, jl, ja, dot(t, D), t, r_self, hard, S.s1.y, S.s0.y, S.s1.w, true,
                    lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
         mask |= 2;
         gmask |= gf ? 2 : 0;
       }
       if (bits & kActB) {
-        v3 jl = self_a ? bb : neg(bb);
-        v3 ja = self_a ? cross(r_self, bb) : neg(cross(r_self, bb));
-        row_values(out[2], gf, jl, ja, dot(bb, d), bb, r_self, hard, S.s1.z, S.s0.z, S.s2.x,
+        const v3 jl = self_a ? bb : neg(bb);
+        const v3 ja = self_a ? cross(r_self, bb) : neg(cross(r_self, bb));
+        row_values(out[2], gf, jl, ja, dot(bb, D), bb, r_self, hard, S.s1.z, S.s0.z, S.s2.x,
                    true, lam_n, wasN, G.g1.w, G.g2.w, 0.f, 0.f);
         mask |= 4;
         gmask |= gf ? 4 : 0;
@@ -420,12 +399,14 @@ __global__ void __launch_bounds__(64)
 k_primal_color(int count, const int* __restrict__ color_bodies, int base,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
                const float4* __restrict__ inert, const float4* __restrict__ bstat,
-               const int* __restrict__ csr_start, const int* __restrict__ csr_idx,
-               const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
-               int post_pass) {
+               const int* __restrict__ cpos, const int2* __restrict__ csr,
+               const int* __restrict__ pos_map, const GroupGeo* __restrict__ geo,
+               const GroupSt* __restrict__ gst, float dt, int post_pass) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
-  int i = color_bodies[base + tid];
+  const int k = base + tid;
+  const int kc = pos_map ? pos_map[k] : k;  // CSR position of this body
+  int i = color_bodies[k];
   float m = bstat[i].x;
   if (m <= 0.0f) return;
   float inv_dt2 = 1.0f / (dt * dt);
@@ -433,14 +414,15 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
   q4 qi = f4q(pose[2 * i + 1]);
   Acc acc;
   acc_init(acc, m, load_m33(inert + 3 * i), xi, qi, poseI[2 * i], poseI[2 * i + 1], inv_dt2);
-  const int k0 = csr_start[i], k1 = csr_start[i + 1];
-  for (int k = k0; k < k1; ++k) {
-    int g = csr_idx[k];
-    GroupGeo G = geo[g];
-    GroupSt S = gst[g];
+  const int k0 = cpos[kc], k1 = cpos[kc + 1];
+  for (int e = k0; e < k1; ++e) {
+    const int2 ent = csr[e];
+    if (ent.x < 0) break;  // padding ends the list
+    GroupGeo G = geo[ent.x];
+    GroupSt S = gst[ent.x];
     RowVals rv[3];
     int gmask;
-    int mask = group_rows(rv, gmask, i, xi, qi, G, S, pose, post_pass);
+    int mask = group_rows(rv, gmask, i, ent.y, xi, qi, G, S, pose, post_pass);
     for (int r = 0; r < 3; ++r)
       if (mask & (1 << r)) acc_add(acc, rv[r], (gmask >> r) & 1);
   }
@@ -468,40 +450,67 @@ constexpr int kFieldNegZero = kFieldOne + 1;       // constant -0.0
 constexpr int kSlotFields = kFieldNegZero + 1;
 constexpr int kComps = 30;
 
+// Fields that system entry ac (A upper 0-5, B 6-14, D 15-23, rl 24-26,
+// ra 27-29) multiplies, per row:
+//   A[r][c] += (jl_r * jl_c) * k      B[r][c] += (ja_r * jl_c) * k
+//   D[r][c] += (ja_r * ja_c) * k, then += g_r on the diagonal
+//   rl[r]   += (jl_r * f) * 1         ra[r]   += (ja_r * f) * 1
+// (the factor 1.0 is exact). fz < 0 selects the constant 1.0 field and
+// fg < 0 the constant -0 field.
+__device__ __forceinline__ void entry_fields(int ac, int& fa, int& fb, int& fz, int& fg) {
+  fz = 6;
+  fg = -1;
+  if (ac < 6) {
+    const int rr = ac < 3 ? 0 : (ac < 5 ? 1 : 2);
+    const int cc = ac < 3 ? ac : (ac < 5 ? ac - 2 : 2);
+    fa = rr; fb = cc;
+  } else if (ac < 15) {
+    fa = 3 + (ac - 6) / 3; fb = (ac - 6) % 3;
+  } else if (ac < 24) {
+    const int cr = (ac - 15) / 3, cc = (ac - 15) % 3;
+    fa = 3 + cr; fb = 3 + cc;
+    if (cr == cc) fg = 8 + cr;
+  } else if (ac < 27) {
+    fa = ac - 24; fb = 7; fz = -1;
+  } else {
+    fa = 3 + (ac - 27); fb = 7; fz = -1;
+  }
+}
+
 template <int BPW>
 __global__ void __launch_bounds__(64)
 k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
                const float4* __restrict__ inert, const float4* __restrict__ bstat,
-               const int* __restrict__ csr_start, const int* __restrict__ csr_idx,
+               const int* __restrict__ cpos, const int2* __restrict__ csr,
                const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
                int post_pass) {
   // Row fields of the 64 slots of a pass, one row of 68 floats per field
   // (16-byte aligned so that four consecutive slots load as one float4).
   __shared__ __align__(16) float rows[kSlotFields][68];
   __shared__ float sys[BPW][kComps];
-  __shared__ int b_i[BPW], b_deg[BPW], b_start[BPW], b_off[BPW + 1];
+  __shared__ int b_i[BPW], b_off[BPW + 1];
   __shared__ float b_pose[BPW][7];
   const int lane = threadIdx.x;
   const int first = blockIdx.x * BPW;
   const float inv_dt2 = 1.0f / (dt * dt);
   rows[kFieldOne][lane] = 1.0f;
   rows[kFieldNegZero][lane] = -0.0f;
+  // Tile slots: the CSR entries of the tile's positions, contiguous and
+  // padded per body to a multiple of four.
+  const int e0 = cpos[base + first];
+  if (lane <= BPW) {
+    const int q = first + lane < count ? first + lane : count;
+    b_off[lane] = cpos[base + q] - e0;
+  }
   if (lane < BPW) {
     int slot = first + lane;
-    int i = -1, deg = 0, st = 0;
+    int i = -1;
     if (slot < count) {
       i = color_bodies[base + slot];
-      if (bstat[i].x > 0.0f) {
-        st = csr_start[i];
-        deg = csr_start[i + 1] - st;
-      } else {
-        i = -1;
-      }
+      if (!(bstat[i].x > 0.0f)) i = -1;
     }
     b_i[lane] = i;
-    b_deg[lane] = deg;
-    b_start[lane] = st;
     if (i >= 0) {
       float4 xp = pose[2 * i], qp = pose[2 * i + 1];
       b_pose[lane][0] = xp.x; b_pose[lane][1] = xp.y; b_pose[lane][2] = xp.z;
@@ -522,53 +531,39 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
     }
   }
   __syncthreads();
-  if (lane == 0) {
-    b_off[0] = 0;
-    for (int b = 0; b < BPW; ++b) b_off[b + 1] = b_off[b] + ((b_deg[b] + 3) & ~3);
-  }
-  __syncthreads();
   const int total = b_off[BPW];
-  // accumulator lane role
-  const int ab = lane / kComps, ac = lane % kComps;
-  const bool is_acc = ab < BPW && b_i[ab < BPW ? ab : 0] >= 0 && lane < BPW * kComps;
-  // Entry owned by this lane and the fields it multiplies, per row:
-  //   A[r][c] += (jl_r * jl_c) * k      B[r][c] += (ja_r * jl_c) * k
-  //   D[r][c] += (ja_r * ja_c) * k, then += g_r on the diagonal
-  //   rl[r]   += (jl_r * f) * 1         ra[r]   += (ja_r * f) * 1
-  // (the factor 1.0 is exact); fg is a row field for diagonal D entries
-  // and the constant -0 field otherwise.
-  int fa, fb, fz = 6, fg = -1;
-  if (ac < 6) {
-    const int rr[6] = {0, 0, 0, 1, 1, 2}, cc6[6] = {0, 1, 2, 1, 2, 2};
-    fa = rr[ac]; fb = cc6[ac];
-  } else if (ac < 15) {
-    fa = 3 + (ac - 6) / 3; fb = (ac - 6) % 3;
-  } else if (ac < 24) {
-    const int cr = (ac - 15) / 3, cc = (ac - 15) % 3;
-    fa = 3 + cr; fb = 3 + cc;
-    if (cr == cc) fg = 8 + cr;
-  } else if (ac < 27) {
-    fa = ac - 24; fb = 7; fz = -1;
-  } else {
-    fa = 3 + (ac - 27); fb = 7; fz = -1;
+  // Accumulator lanes: kLanes lanes per body, each owning kEpl entries
+  // (entry ac + k * kLanes, k < kEpl, when below 30).
+  constexpr int kLanes = 64 / BPW;
+  constexpr int kEpl = (kComps + kLanes - 1) / kLanes;
+  const int ab = lane / kLanes, al = lane % kLanes;
+  const bool body_ok = ab < BPW && b_i[ab < BPW ? ab : 0] >= 0;
+  int fa[kEpl], fb[kEpl], fz[kEpl], fg[kEpl];
+  bool own[kEpl];
+  float accv[kEpl];
+#pragma unroll
+  for (int k = 0; k < kEpl; ++k) {
+    const int ac = al + k * kLanes;
+    own[k] = body_ok && ac < kComps;
+    entry_fields(ac < kComps ? ac : 0, fa[k], fb[k], fz[k], fg[k]);
+    accv[k] = own[k] ? sys[ab][ac] : 0.0f;
   }
-  float accv = is_acc ? sys[ab][ac] : 0.0f;
+  const bool is_acc = own[0];
   for (int p0 = 0; p0 < total; p0 += 64) {
     // ---- A: row values of one incidence per lane ----
     const int j = p0 + lane;
     if (j < total) {
       int b = 0;
       while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
-      const int jj = j - b_off[b];
       RowVals rv[3];
       int mask = 0, gmask = 0;
-      if (jj < b_deg[b]) {
-        const int g = csr_idx[b_start[b] + jj];
-        GroupGeo G = geo[g];
-        GroupSt S = gst[g];
+      const int2 ent = csr[e0 + j];
+      if (ent.x >= 0 && b_i[b] >= 0) {
+        GroupGeo G = geo[ent.x];
+        GroupSt S = gst[ent.x];
         v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
         q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
-        mask = group_rows(rv, gmask, b_i[b], xi, qi, G, S, pose, post_pass);
+        mask = group_rows(rv, gmask, b_i[b], ent.y, xi, qi, G, S, pose, post_pass);
       }
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
@@ -595,31 +590,38 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
       const int s1 = b_off[ab + 1] - p0 < 64 ? b_off[ab + 1] - p0 : 64;
       for (int sb = s0; sb < s1; sb += 4) {
         __builtin_assume((sb & 3) == 0);  // body slot ranges start on multiples of 4
-        float4 X[3], Y[3], Z[3], Gd[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          const int rb = kRowFields * r;
-          X[r] = *reinterpret_cast<const float4*>(&rows[rb + fa][sb]);
-          Y[r] = *reinterpret_cast<const float4*>(&rows[rb + fb][sb]);
-          Z[r] = *reinterpret_cast<const float4*>(&rows[fz < 0 ? kFieldOne : rb + fz][sb]);
-          Gd[r] = *reinterpret_cast<const float4*>(&rows[fg < 0 ? kFieldNegZero : rb + fg][sb]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int k = 0; k < kEpl; ++k) {
+          float4 X[3], Y[3], Z[3], Gd[3];
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            const float x = u == 0 ? X[r].x : u == 1 ? X[r].y : u == 2 ? X[r].z : X[r].w;
-            const float y = u == 0 ? Y[r].x : u == 1 ? Y[r].y : u == 2 ? Y[r].z : Y[r].w;
-            const float z = u == 0 ? Z[r].x : u == 1 ? Z[r].y : u == 2 ? Z[r].z : Z[r].w;
-            const float g = u == 0 ? Gd[r].x : u == 1 ? Gd[r].y : u == 2 ? Gd[r].z : Gd[r].w;
-            accv = (accv + (x * y) * z) + g;
+            const int rb = kRowFields * r;
+            X[r] = *reinterpret_cast<const float4*>(&rows[rb + fa[k]][sb]);
+            Y[r] = *reinterpret_cast<const float4*>(&rows[rb + fb[k]][sb]);
+            Z[r] = *reinterpret_cast<const float4*>(&rows[fz[k] < 0 ? kFieldOne : rb + fz[k]][sb]);
+            Gd[r] = *reinterpret_cast<const float4*>(&rows[fg[k] < 0 ? kFieldNegZero : rb + fg[k]][sb]);
           }
+          float acc = accv[k];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              const float x = u == 0 ? X[r].x : u == 1 ? X[r].y : u == 2 ? X[r].z : X[r].w;
+              const float y = u == 0 ? Y[r].x : u == 1 ? Y[r].y : u == 2 ? Y[r].z : Y[r].w;
+              const float z = u == 0 ? Z[r].x : u == 1 ? Z[r].y : u == 2 ? Z[r].z : Z[r].w;
+              const float g = u == 0 ? Gd[r].x : u == 1 ? Gd[r].y : u == 2 ? Gd[r].z : Gd[r].w;
+              acc = (acc + (x * y) * z) + g;
+            }
+          }
+          accv[k] = acc;
         }
       }
     }
     __syncthreads();
   }
-  if (is_acc) sys[ab][ac] = accv;
+#pragma unroll
+  for (int k = 0; k < kEpl; ++k)
+    if (own[k]) sys[ab][al + k * kLanes] = accv[k];
   __syncthreads();
   // ---- C: solve ----
   if (lane < BPW && b_i[lane] >= 0) {
@@ -643,20 +645,30 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
 
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
-                         const float4* bstat, const int* csr_start, const int* csr_idx,
-                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int variant) {
+                         const float4* bstat, const int* cpos, const int2* csr,
+                         const int* pos_map, const GroupGeo* geo, const GroupSt* gst, float dt,
+                         int post_pass, int variant) {
   if (count <= 0) return;
-  if (variant == 0) {
+  // The tiled kernels read a tile's CSR entries as one contiguous range, so
+  // a CSR laid out for an earlier color order (pos_map) goes lane per body.
+  if (variant == 0 || pos_map) {
     hipLaunchKernelGGL(k_primal_color, dim3((count + 63) / 64), dim3(64), 0, st, count,
-                       color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
+                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, pos_map, geo, gst,
                        dt, post_pass);
   } else if (variant == 1) {
     hipLaunchKernelGGL(k_primal_tiled<1>, dim3(count), dim3(64), 0, st, count, color_bodies, base,
-                       pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst, dt, post_pass);
-  } else {
+                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+  } else if (variant == 2) {
     hipLaunchKernelGGL(k_primal_tiled<2>, dim3((count + 1) / 2), dim3(64), 0, st, count,
-                       color_bodies, base, pose, poseI, inert, bstat, csr_start, csr_idx, geo, gst,
+                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
+                       dt, post_pass);
+  } else if (variant == 3) {
+    hipLaunchKernelGGL(k_primal_tiled<4>, dim3((count + 3) / 4), dim3(64), 0, st, count,
+                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
+                       dt, post_pass);
+  } else {
+    hipLaunchKernelGGL(k_primal_tiled<8>, dim3((count + 7) / 8), dim3(64), 0, st, count,
+                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
                        dt, post_pass);
   }
 }

@@ -57,6 +57,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-substeps", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run 5 device-to-device copies of MB MiB (the reference point "
+                         "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body, 1 wavefront per body, 2 wavefront per two bodies")
     args = ap.parse_args()
@@ -64,6 +67,17 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the substep is not sharded across GPUs.", file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copied {n} bytes 5 times", file=sys.stderr, flush=True)
+        return 0
 
     from avbd_amd import scenes
 
