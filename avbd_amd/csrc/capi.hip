@@ -202,12 +202,14 @@ int prof_begin(avbd_solver* s, int cls) {
   s->ev_used += 2;
   return AVBD_OK;
 }
-int prof_end(avbd_solver* s, double bytes) {
+// Close the interval opened by prof_begin; it covered `launches` kernel
+// launches of its class (back to back on the solver stream).
+int prof_end(avbd_solver* s, double bytes, int launches = 1) {
   if (!s->prof) return AVBD_OK;
   auto& pe = s->ev_pending.back();
   HIPCK(hipEventRecord(s->ev_pool[pe.second + 1], s->st));
   s->prof_bytes[pe.first] += bytes;
-  s->prof_launch[pe.first] += 1;
+  s->prof_launch[pe.first] += launches;
   return AVBD_OK;
 }
 int prof_collect(avbd_solver* s) {
@@ -477,17 +479,24 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   }
   const int I = p->iterations;
   const int total = I + post;
-  const double pbytes = primal_sweep_bytes(s) / std::max(1, s->n_active_colors);
+  int nonempty = 0;
+  for (int c = 0; c < s->n_active_colors; ++c) nonempty += (s->cstart[c + 1] > s->cstart[c]);
+  const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
   for (int it = 0; it < total; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
+    // One timing interval per sweep (all colors back to back): per-launch
+    // events would add their own gaps between the color launches.
+    CK(prof_begin(s, kPrimal));
+    int launched = 0;
     for (int c = 0; c < s->n_active_colors; ++c) {
       int cnt = s->cstart[c + 1] - s->cstart[c];
-      CK(prof_begin(s, kPrimal));
+      if (cnt <= 0) continue;
       launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
                           s->bstat.p, s->cpos.p, s->tcsr.p, pos_map, s->geo.p, s->gst.p, dt,
                           post_pass, s->primal_variant);
-      CK(prof_end(s, pbytes));
+      ++launched;
     }
+    CK(prof_end(s, pbytes * launched, launched));
     if (it < I) {
       CK(prof_begin(s, kDual));
       launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta);

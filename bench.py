@@ -22,6 +22,21 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the primal sweep kernel measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration) on this
+# workload; the summary file records the commands and the raw counters.
+TRAFFIC_FILE = ROOT / "profiles" / "k_primal_tiled2_pyramid66_pmc.md"
+
+
+def measured_traffic(layers: int, variant: int) -> float | None:
+    """Bytes per primal launch from the committed PMC summary, when it was
+    taken on this workload and kernel variant; None otherwise."""
+    if layers != 66 or variant != 2 or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
@@ -84,8 +99,8 @@ def main() -> int:
     t_build = time.perf_counter()
     s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
-    if args.primal_variant is not None:
-        s._h.set_option(0, args.primal_variant)
+    variant = 2 if args.primal_variant is None else args.primal_variant
+    s._h.set_option(0, variant)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
           f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
@@ -134,12 +149,13 @@ def main() -> int:
                    "colors": s.num_active_colors, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_primal_color (per-color primal sweep)",
+            "kernel": ("k_primal_color" if variant == 0 else
+                       f"k_primal_tiled<{[1, 2, 4, 8][variant - 1]}>") + " (per-color primal sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.layers, variant),
             "avg_launch_us": 1e3 * prim_ms / prim_n if prim_n else None,
             "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
         },
