@@ -152,7 +152,7 @@ struct avbd_solver {
   int last_post = 0;
 
   // options
-  int primal_variant = 2;  // one wavefront per two bodies
+  int primal_variant = 5;  // one or two bodies per wavefront by color size
 
   // profiling
   bool prof = false;
@@ -1113,7 +1113,7 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   switch (option) {
     case AVBD_OPT_PRIMAL_VARIANT:
-      if (value < 0 || value > 4) return set_err(AVBD_E_ARG, "primal variant must be 0 to 4");
+      if (value < 0 || value > 5) return set_err(AVBD_E_ARG, "primal variant must be 0 to 5");
       s->primal_variant = value;
       return AVBD_OK;
     default:

@@ -256,10 +256,11 @@ __device__ __forceinline__ void row_values(RowVals& o, bool& gflag, v3 jl, v3 ja
 
 // Rows of group G seen from body i, in row order. Returns a bit mask of the
 // rows present and active (bit r) and sets gmask (bit r: |f| > 0).
-// `other` is the other body of a box-box group (the body itself otherwise).
-__device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, int other, v3 xi,
-                                          q4 qi, const GroupGeo& G, const GroupSt& S,
-                                          const float4* __restrict__ pose, int post_pass) {
+// (ox, oq) is the pose of the other body of a box-box group (of the body
+// itself for floor and pin groups, where it is not used).
+__device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, v3 xi, q4 qi,
+                                          const GroupGeo& G, const GroupSt& S, float4 ox,
+                                          float4 oq, int post_pass) {
   const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
   const int bits = __float_as_int(S.s2.y);
   const float stiff = G.g0.w;
@@ -284,9 +285,9 @@ __device__ __forceinline__ int group_rows(RowVals out[3], int& gmask, int i, int
     const v3 off_self = self_a ? off_a : off_b;
     const v3 off_other = self_a ? off_b : off_a;
     const v3 r_self = qrotate(qi, off_self);
-    const v3 r_o = qrotate(f4q(pose[2 * other + 1]), off_other);
+    const v3 r_o = qrotate(f4q(oq), off_other);
     const v3 ws = add(xi, r_self);
-    const v3 wo = add(f3(pose[2 * other]), r_o);
+    const v3 wo = add(f3(ox), r_o);
     const v3 d = self_a ? sub(ws, wo) : sub(wo, ws);
     const v3 D = is_floor ? ws : d;
     const v3 gn = f3(G.g0);
@@ -422,7 +423,8 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
     GroupSt S = gst[ent.x];
     RowVals rv[3];
     int gmask;
-    int mask = group_rows(rv, gmask, i, ent.y, xi, qi, G, S, pose, post_pass);
+    int mask = group_rows(rv, gmask, i, xi, qi, G, S, pose[2 * ent.y], pose[2 * ent.y + 1],
+                          post_pass);
     for (int r = 0; r < 3; ++r)
       if (mask & (1 << r)) acc_add(acc, rv[r], (gmask >> r) & 1);
   }
@@ -499,6 +501,19 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   // Tile slots: the CSR entries of the tile's positions, contiguous and
   // padded per body to a multiple of four.
   const int e0 = cpos[base + first];
+  const int total = cpos[base + (first + BPW < count ? first + BPW : count)] - e0;
+  // Loads of the first pass go out before the body set-up below, so that
+  // their latency overlaps it: list entry, group records, other body's pose.
+  int2 ent = lane < total ? csr[e0 + lane] : make_int2(-1, -1);
+  GroupGeo Gq;
+  GroupSt Sq;
+  float4 oxq, oqq;
+  if (ent.x >= 0) {
+    Gq = geo[ent.x];
+    Sq = gst[ent.x];
+    oxq = pose[2 * ent.y];
+    oqq = pose[2 * ent.y + 1];
+  }
   if (lane <= BPW) {
     const int q = first + lane < count ? first + lane : count;
     b_off[lane] = cpos[base + q] - e0;
@@ -531,7 +546,6 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
     }
   }
   __syncthreads();
-  const int total = b_off[BPW];
   // Accumulator lanes: kLanes lanes per body, each owning kEpl entries
   // (entry ac + k * kLanes, k < kEpl, when below 30).
   constexpr int kLanes = 64 / BPW;
@@ -552,18 +566,18 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   for (int p0 = 0; p0 < total; p0 += 64) {
     // ---- A: row values of one incidence per lane ----
     const int j = p0 + lane;
+    // list entry of this lane's slot in the next pass
+    const int jn = j + 64;
+    const int2 ent_next = jn < total ? csr[e0 + jn] : make_int2(-1, -1);
     if (j < total) {
       int b = 0;
       while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
       RowVals rv[3];
       int mask = 0, gmask = 0;
-      const int2 ent = csr[e0 + j];
       if (ent.x >= 0 && b_i[b] >= 0) {
-        GroupGeo G = geo[ent.x];
-        GroupSt S = gst[ent.x];
         v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
         q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
-        mask = group_rows(rv, gmask, b_i[b], ent.y, xi, qi, G, S, pose, post_pass);
+        mask = group_rows(rv, gmask, b_i[b], xi, qi, Gq, Sq, oxq, oqq, post_pass);
       }
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
@@ -582,6 +596,14 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
         o[9 * 68 + lane] = gon ? rv[r].g.y : -0.0f;
         o[10 * 68 + lane] = gon ? rv[r].g.z : -0.0f;
       }
+    }
+    // loads of the next pass, in flight during this pass's accumulation
+    ent = ent_next;
+    if (ent.x >= 0) {
+      Gq = geo[ent.x];
+      Sq = gst[ent.x];
+      oxq = pose[2 * ent.y];
+      oqq = pose[2 * ent.y + 1];
     }
     __syncthreads();
     // ---- B: ordered accumulation, four slots per step ----
@@ -655,21 +677,26 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
     hipLaunchKernelGGL(k_primal_color, dim3((count + 63) / 64), dim3(64), 0, st, count,
                        color_bodies, base, pose, poseI, inert, bstat, cpos, csr, pos_map, geo, gst,
                        dt, post_pass);
-  } else if (variant == 1) {
-    hipLaunchKernelGGL(k_primal_tiled<1>, dim3(count), dim3(64), 0, st, count, color_bodies, base,
-                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
-  } else if (variant == 2) {
-    hipLaunchKernelGGL(k_primal_tiled<2>, dim3((count + 1) / 2), dim3(64), 0, st, count,
-                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
-                       dt, post_pass);
-  } else if (variant == 3) {
-    hipLaunchKernelGGL(k_primal_tiled<4>, dim3((count + 3) / 4), dim3(64), 0, st, count,
-                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
-                       dt, post_pass);
   } else {
-    hipLaunchKernelGGL(k_primal_tiled<8>, dim3((count + 7) / 8), dim3(64), 0, st, count,
-                       color_bodies, base, pose, poseI, inert, bstat, cpos, csr, geo, gst,
-                       dt, post_pass);
+    // Bodies per wavefront: 1, 2, 4, 8 for variants 1-4; variant 5 takes one
+    // body per wavefront for colors of fewer than 4096 bodies (the launch
+    // time of a small color is the critical path of one wavefront, which is
+    // shorter with one body) and two otherwise.
+    const int bpw = variant == 1 ? 1 : variant == 2 ? 2 : variant == 3 ? 4 : variant == 4 ? 8
+                                     : (count < 4096 ? 1 : 2);
+    const dim3 grid((count + bpw - 1) / bpw), blk(64);
+    if (bpw == 1)
+      hipLaunchKernelGGL(k_primal_tiled<1>, grid, blk, 0, st, count, color_bodies, base, pose,
+                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+    else if (bpw == 2)
+      hipLaunchKernelGGL(k_primal_tiled<2>, grid, blk, 0, st, count, color_bodies, base, pose,
+                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+    else if (bpw == 4)
+      hipLaunchKernelGGL(k_primal_tiled<4>, grid, blk, 0, st, count, color_bodies, base, pose,
+                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+    else
+      hipLaunchKernelGGL(k_primal_tiled<8>, grid, blk, 0, st, count, color_bodies, base, pose,
+                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
   }
 }
 

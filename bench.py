@@ -31,7 +31,7 @@ TRAFFIC_FILE = ROOT / "profiles" / "k_primal_tiled2_pyramid66_pmc.md"
 def measured_traffic(layers: int, variant: int) -> float | None:
     """Bytes per primal launch from the committed PMC summary, when it was
     taken on this workload and kernel variant; None otherwise."""
-    if layers != 66 or variant != 2 or not TRAFFIC_FILE.exists():
+    if layers != 66 or variant not in (2, 5) or not TRAFFIC_FILE.exists():
         return None
     for line in TRAFFIC_FILE.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
@@ -76,7 +76,8 @@ def main() -> int:
                     help="only run 5 device-to-device copies of MB MiB (the reference point "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--primal-variant", type=int, default=None,
-                    help="0 lane per body, 1 wavefront per body, 2 wavefront per two bodies")
+                    help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
+                         "5 one or two bodies per wavefront by color size (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -99,7 +100,7 @@ def main() -> int:
     t_build = time.perf_counter()
     s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
-    variant = 2 if args.primal_variant is None else args.primal_variant
+    variant = 5 if args.primal_variant is None else args.primal_variant
     s._h.set_option(0, variant)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
@@ -150,6 +151,7 @@ def main() -> int:
         "roofline": {
             "bound": "hbm",
             "kernel": ("k_primal_color" if variant == 0 else
+                       "k_primal_tiled<1|2>" if variant == 5 else
                        f"k_primal_tiled<{[1, 2, 4, 8][variant - 1]}>") + " (per-color primal sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -184,8 +184,9 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
 /* Kernel selection; no counterpart in the reference (its primal_group_size /
  * primal_shuffle / primal_fused switches, solver_6dof.py:196-198, choose
  * between its CUDA kernels). AVBD_OPT_PRIMAL_VARIANT: 0 = one lane per body,
- * 1, 2, 3, 4 = one wavefront per 1, 2 (default), 4, 8 bodies. All variants
- * give identical results. */
+ * 1, 2, 3, 4 = one wavefront per 1, 2, 4, 8 bodies, 5 = one wavefront per
+ * body for colors under 4096 bodies and per two bodies otherwise (default).
+ * All variants give identical results. */
 #define AVBD_OPT_PRIMAL_VARIANT 0
 int avbd_set_option(avbd_solver* s, int option, int value);
 
