@@ -66,6 +66,11 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
                          const float4* bstat, const int* cpos, const int2* csr,
                          const int* pos_map, const GroupGeo* geo, const GroupSt* gst, float dt,
                          int post_pass, int variant);
+void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const int* color_bodies,
+                         const int* color, float4* pose, const float4* poseI, const float4* inert,
+                         const float4* bstat, const int* cpos, const int2* csr,
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
+                         int* done, int stamp, int* err);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,

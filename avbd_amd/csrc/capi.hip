@@ -130,6 +130,9 @@ struct avbd_solver {
   DBuf<int> tc_work;        // 5 * nb: as-b pair count, as-b group count, as-a start, as-a end, cursor
   DBuf<int> asb_start, asb_list, padc, cpos, pos_of, pos_map;
   DBuf<int2> tcsr;
+  // dataflow primal sweep: per-body stamp of the last sweep that moved it
+  DBuf<int> flow_done;
+  int flow_stamp = 0;
   bool csr_built = false;       // a CSR has been built since the last flush
   bool csr_order_stale = false; // recolored since that CSR was built
 
@@ -152,7 +155,7 @@ struct avbd_solver {
   int last_post = 0;
 
   // options
-  int primal_variant = 5;  // one or two bodies per wavefront by color size
+  int primal_variant = 6;  // one dataflow launch per sweep
 
   // profiling
   bool prof = false;
@@ -184,6 +187,9 @@ int readback(avbd_solver* s, int first, int count) {
   HIPCK(hipMemcpyAsync(s->hcnt + first, s->dcnt.p + first, count * sizeof(int),
                        hipMemcpyDeviceToHost, s->st));
   HIPCK(hipStreamSynchronize(s->st));
+  // dcnt[8]: set by a dataflow primal sweep whose wait hit its spin cap; it
+  // is copied at the end of every substep and checked at the next sync.
+  if (s->hcnt[8] != 0) return set_err(AVBD_E_STATE, "primal dataflow sweep did not complete");
   return AVBD_OK;
 }
 
@@ -482,21 +488,32 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   int nonempty = 0;
   for (int c = 0; c < s->n_active_colors; ++c) nonempty += (s->cstart[c + 1] > s->cstart[c]);
   const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
+  // variant 6: every color of a sweep in one dataflow launch
+  const bool flow = s->primal_variant == 6 && pos_map == nullptr;
   for (int it = 0; it < total; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
     // One timing interval per sweep (all colors back to back): per-launch
     // events would add their own gaps between the color launches.
     CK(prof_begin(s, kPrimal));
     int launched = 0;
-    for (int c = 0; c < s->n_active_colors; ++c) {
-      int cnt = s->cstart[c + 1] - s->cstart[c];
-      if (cnt <= 0) continue;
-      launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
-                          s->bstat.p, s->cpos.p, s->tcsr.p, pos_map, s->geo.p, s->gst.p, dt,
-                          post_pass, s->primal_variant);
-      ++launched;
+    if (flow) {
+      launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
+                          s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
+                          s->gst.p, dt, post_pass, s->flow_done.p, ++s->flow_stamp,
+                          s->dcnt.p + 8);
+      launched = nonempty;  // bodies of every color, in one launch
+      CK(prof_end(s, pbytes * launched, 1));
+    } else {
+      for (int c = 0; c < s->n_active_colors; ++c) {
+        int cnt = s->cstart[c + 1] - s->cstart[c];
+        if (cnt <= 0) continue;
+        launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
+                            s->bstat.p, s->cpos.p, s->tcsr.p, pos_map, s->geo.p, s->gst.p, dt,
+                            post_pass, s->primal_variant);
+        ++launched;
+      }
+      CK(prof_end(s, pbytes * launched, launched));
     }
-    CK(prof_end(s, pbytes * launched, launched));
     if (it < I) {
       CK(prof_begin(s, kDual));
       launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta);
@@ -517,6 +534,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
                         s->howner.p, s->slot_of.p, s->hstate.p);
     CK(prof_end(s, 0.0));
   }
+  HIPCK(hipMemcpyAsync(s->hcnt + 8, s->dcnt.p + 8, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipGetLastError());
   s->cur = 1 - s->cur;
   s->last_post = post;
@@ -568,6 +586,7 @@ int avbd_create(int device, avbd_solver** out) {
     delete s;
     return r;
   }
+  launch_fill_int(s->st, s->dcnt.p, 64, 0);
   *out = s;
   return AVBD_OK;
 }
@@ -750,6 +769,9 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   CK(s->tc_work.alloc(5 * nbb)); CK(s->asb_start.alloc(nbb + 1)); CK(s->asb_list.alloc(1));
   CK(s->padc.alloc(nbb)); CK(s->cpos.alloc(nbb + 1)); CK(s->pos_of.alloc(nbb));
   CK(s->pos_map.alloc(nbb));
+  CK(s->flow_done.alloc(nbb));
+  launch_fill_int(s->st, s->flow_done.p, nbb, 0);
+  s->flow_stamp = 0;
   s->csr_built = false;
   s->csr_order_stale = false;
   CK(s->tcsr.alloc(std::max(s->st_total, 1) + 3 * (size_t)nbb));
@@ -1113,7 +1135,7 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   switch (option) {
     case AVBD_OPT_PRIMAL_VARIANT:
-      if (value < 0 || value > 5) return set_err(AVBD_E_ARG, "primal variant must be 0 to 5");
+      if (value < 0 || value > 6) return set_err(AVBD_E_ARG, "primal variant must be 0 to 6");
       s->primal_variant = value;
       return AVBD_OK;
     default:

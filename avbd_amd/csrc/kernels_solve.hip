@@ -381,8 +381,7 @@ __device__ __forceinline__ void acc_add(Acc& acc, const RowVals& v, bool gflag) 
 
 // Schur-complement solve of [A B^T; B D] and the pose update
 // (kernels_6dof.py:566-582).
-__device__ __forceinline__ void solve_and_apply(const Acc& acc, int i, v3 xi, q4 qi,
-                                                float4* __restrict__ pose) {
+__device__ __forceinline__ void solve_pose(const Acc& acc, v3 xi, q4 qi, float4& xo, float4& qo) {
   m33 A_inv = minverse(acc.A);
   m33 BAinv = mmul(acc.B, A_inv);
   v3 rhs = sub(acc.ra, mvmul(BAinv, acc.rl));
@@ -390,9 +389,36 @@ __device__ __forceinline__ void solve_and_apply(const Acc& acc, int i, v3 xi, q4
   m33 S_inv = minverse(Sm);
   v3 d_th = mvmul(S_inv, rhs);
   v3 d_x = mvmul(A_inv, sub(acc.rl, mvmul(mT(acc.B), d_th)));
-  pose[2 * i] = p4(sub(xi, d_x));
+  xo = p4(sub(xi, d_x));
   q4 dq2 = quat_from_rotvec(neg(d_th));
-  pose[2 * i + 1] = p4q(qnormalize(qmul(dq2, qi)));
+  qo = p4q(qnormalize(qmul(dq2, qi)));
+}
+
+__device__ __forceinline__ void solve_and_apply(const Acc& acc, int i, v3 xi, q4 qi,
+                                                float4* __restrict__ pose) {
+  float4 xo, qo;
+  solve_pose(acc, xi, qi, xo, qo);
+  pose[2 * i] = xo;
+  pose[2 * i + 1] = qo;
+}
+
+// Agent-scope relaxed accesses (sc1): coherent across the GPU's L2 slices,
+// used for data exchanged between wavefronts of one launch.
+__device__ __forceinline__ float4 ld_agent(const float4* p) {
+  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+  unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float4(__uint_as_float((unsigned)a), __uint_as_float((unsigned)(a >> 32)),
+                     __uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
+}
+__device__ __forceinline__ void st_agent(float4* p, float4 v) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  __hip_atomic_store(q, (unsigned long long)__float_as_uint(v.x) |
+                            ((unsigned long long)__float_as_uint(v.y) << 32),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, (unsigned long long)__float_as_uint(v.z) |
+                                ((unsigned long long)__float_as_uint(v.w) << 32),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Lane-per-body variant: each lane walks its body's groups serially.
@@ -479,22 +505,55 @@ __device__ __forceinline__ void entry_fields(int ac, int& fa, int& fb, int& fz, 
   }
 }
 
-template <int BPW>
+// Dataflow sweep: every color of one sweep in one launch. Workgroup t is
+// tile t in color-major order (BPW consecutive positions of one color);
+// before using a neighbour of a lower color a tile waits until that
+// neighbour has published its pose of this sweep (done[j] == stamp), so each
+// body reads exactly the poses that color-by-color launches would give it.
+// A tile waits only on tiles with smaller indices. The dispatcher starts
+// workgroups in index order, so those are already running or done and every
+// wait ends; a spin cap with an error flag turns any violation of that into
+// a reported error instead of a hang. (A ticket counter would not depend on
+// dispatch order, but one device-wide atomic per tile serializes ~50k tiles
+// per sweep: measured 2x slower than the color-by-color launches.)
+struct SweepFlow {
+  int n_colors;
+  int n_tiles;
+  int tile_start[kMaxColors + 1];   // first tile of each color
+  int color_start[kMaxColors + 1];  // first position of each color
+  const int* color;                 // body -> color
+  int* done;                        // body -> stamp of the last sweep that moved it
+  int stamp;
+  int* err;
+};
+
+constexpr int kFlowSpinCap = 1 << 20;
+
+template <int BPW, bool kFlow>
 __global__ void __launch_bounds__(64)
-k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
+k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
                const float4* __restrict__ inert, const float4* __restrict__ bstat,
                const int* __restrict__ cpos, const int2* __restrict__ csr,
                const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst, float dt,
-               int post_pass) {
+               int post_pass, SweepFlow fl) {
   // Row fields of the 64 slots of a pass, one row of 68 floats per field
   // (16-byte aligned so that four consecutive slots load as one float4).
   __shared__ __align__(16) float rows[kSlotFields][68];
   __shared__ float sys[BPW][kComps];
-  __shared__ int b_i[BPW], b_off[BPW + 1];
+  __shared__ int b_i[BPW], b_id[BPW], b_off[BPW + 1];
   __shared__ float b_pose[BPW][7];
   const int lane = threadIdx.x;
-  const int first = blockIdx.x * BPW;
+  int count = count_arg, base = base_arg, first = blockIdx.x * BPW, my_color = 0;
+  if (kFlow) {
+    const int t = blockIdx.x;
+    int c = 0;
+    while (c + 1 < fl.n_colors && t >= fl.tile_start[c + 1]) ++c;
+    my_color = c;
+    base = fl.color_start[c];
+    count = fl.color_start[c + 1] - base;
+    first = (t - fl.tile_start[c]) * BPW;
+  }
   const float inv_dt2 = 1.0f / (dt * dt);
   rows[kFieldOne][lane] = 1.0f;
   rows[kFieldNegZero][lane] = -0.0f;
@@ -503,7 +562,8 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   const int e0 = cpos[base + first];
   const int total = cpos[base + (first + BPW < count ? first + BPW : count)] - e0;
   // Loads of the first pass go out before the body set-up below, so that
-  // their latency overlaps it: list entry, group records, other body's pose.
+  // their latency overlaps it: list entry, group records and (when the
+  // neighbour cannot move during this launch) the other body's pose.
   int2 ent = lane < total ? csr[e0 + lane] : make_int2(-1, -1);
   GroupGeo Gq;
   GroupSt Sq;
@@ -511,8 +571,10 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   if (ent.x >= 0) {
     Gq = geo[ent.x];
     Sq = gst[ent.x];
-    oxq = pose[2 * ent.y];
-    oqq = pose[2 * ent.y + 1];
+    if (!kFlow) {
+      oxq = pose[2 * ent.y];
+      oqq = pose[2 * ent.y + 1];
+    }
   }
   if (lane <= BPW) {
     const int q = first + lane < count ? first + lane : count;
@@ -520,12 +582,13 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
   }
   if (lane < BPW) {
     int slot = first + lane;
-    int i = -1;
+    int i = -1, id = -1;
     if (slot < count) {
-      i = color_bodies[base + slot];
+      id = i = color_bodies[base + slot];
       if (!(bstat[i].x > 0.0f)) i = -1;
     }
     b_i[lane] = i;
+    b_id[lane] = id;
     if (i >= 0) {
       float4 xp = pose[2 * i], qp = pose[2 * i + 1];
       b_pose[lane][0] = xp.x; b_pose[lane][1] = xp.y; b_pose[lane][2] = xp.z;
@@ -569,6 +632,32 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
     // list entry of this lane's slot in the next pass
     const int jn = j + 64;
     const int2 ent_next = jn < total ? csr[e0 + jn] : make_int2(-1, -1);
+    if (kFlow) {
+      // wait for this pass's lower-color neighbours, then read their poses
+      // through the coherent path
+      bool need = false;
+      if (ent.x >= 0 && ent.y >= 0) {
+        int bb = 0;
+        while (bb + 1 < BPW && j >= b_off[bb + 1]) ++bb;
+        need = ent.y != b_id[bb] && fl.color[ent.y] < my_color;
+      }
+      int spins = 0;
+      while (need) {
+        if (__hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            fl.stamp) {
+          need = false;
+        } else if (++spins > kFlowSpinCap) {
+          atomicExch(fl.err, 1);
+          need = false;
+        } else {
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      if (ent.x >= 0) {
+        oxq = ld_agent(pose + 2 * ent.y);
+        oqq = ld_agent(pose + 2 * ent.y + 1);
+      }
+    }
     if (j < total) {
       int b = 0;
       while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
@@ -602,8 +691,10 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
     if (ent.x >= 0) {
       Gq = geo[ent.x];
       Sq = gst[ent.x];
-      oxq = pose[2 * ent.y];
-      oqq = pose[2 * ent.y + 1];
+      if (!kFlow) {
+        oxq = pose[2 * ent.y];
+        oqq = pose[2 * ent.y + 1];
+      }
     }
     __syncthreads();
     // ---- B: ordered accumulation, four slots per step ----
@@ -660,9 +751,51 @@ k_primal_tiled(int count, const int* __restrict__ color_bodies, int base,
     acc.rl = mk3(o[24], o[25], o[26]);
     acc.ra = mk3(o[27], o[28], o[29]);
     const float* bp = b_pose[lane];
-    solve_and_apply(acc, b_i[lane], mk3(bp[0], bp[1], bp[2]), mk4(bp[3], bp[4], bp[5], bp[6]),
-                    pose);
+    float4 xo, qo;
+    solve_pose(acc, mk3(bp[0], bp[1], bp[2]), mk4(bp[3], bp[4], bp[5], bp[6]), xo, qo);
+    if (kFlow) {
+      st_agent(pose + 2 * b_i[lane], xo);
+      st_agent(pose + 2 * b_i[lane] + 1, qo);
+    } else {
+      pose[2 * b_i[lane]] = xo;
+      pose[2 * b_i[lane] + 1] = qo;
+    }
   }
+  if (kFlow && lane < BPW && b_id[lane] >= 0) {
+    // publish after the pose stores have completed (static bodies publish
+    // an unchanged pose)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(fl.done + b_id[lane], fl.stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// One dataflow launch for a whole sweep (all colors); cstart holds the
+// n_colors + 1 color starts; stamp must differ from every earlier sweep's.
+void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const int* color_bodies,
+                         const int* color, float4* pose, const float4* poseI, const float4* inert,
+                         const float4* bstat, const int* cpos, const int2* csr,
+                         const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
+                         int* done, int stamp, int* err) {
+  SweepFlow fl{};
+  fl.n_colors = n_colors;
+  int t = 0;
+  for (int c = 0; c < n_colors; ++c) {
+    fl.tile_start[c] = t;
+    fl.color_start[c] = cstart[c];
+    t += (cstart[c + 1] - cstart[c] + 1) / 2;
+  }
+  fl.tile_start[n_colors] = t;
+  fl.color_start[n_colors] = cstart[n_colors];
+  fl.n_tiles = t;
+  fl.color = color;
+  fl.done = done;
+  fl.stamp = stamp;
+  fl.err = err;
+  if (t <= 0) return;
+  hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0, pose,
+                     poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
 }
 
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
@@ -685,18 +818,19 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
     const int bpw = variant == 1 ? 1 : variant == 2 ? 2 : variant == 3 ? 4 : variant == 4 ? 8
                                      : (count < 4096 ? 1 : 2);
     const dim3 grid((count + bpw - 1) / bpw), blk(64);
+    SweepFlow none{};
     if (bpw == 1)
-      hipLaunchKernelGGL(k_primal_tiled<1>, grid, blk, 0, st, count, color_bodies, base, pose,
-                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+      hipLaunchKernelGGL((k_primal_tiled<1, false>), grid, blk, 0, st, count, color_bodies, base,
+                         pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, none);
     else if (bpw == 2)
-      hipLaunchKernelGGL(k_primal_tiled<2>, grid, blk, 0, st, count, color_bodies, base, pose,
-                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+      hipLaunchKernelGGL((k_primal_tiled<2, false>), grid, blk, 0, st, count, color_bodies, base,
+                         pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, none);
     else if (bpw == 4)
-      hipLaunchKernelGGL(k_primal_tiled<4>, grid, blk, 0, st, count, color_bodies, base, pose,
-                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+      hipLaunchKernelGGL((k_primal_tiled<4, false>), grid, blk, 0, st, count, color_bodies, base,
+                         pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, none);
     else
-      hipLaunchKernelGGL(k_primal_tiled<8>, grid, blk, 0, st, count, color_bodies, base, pose,
-                         poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass);
+      hipLaunchKernelGGL((k_primal_tiled<8, false>), grid, blk, 0, st, count, color_bodies, base,
+                         pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, none);
   }
 }
 

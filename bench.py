@@ -25,18 +25,19 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the primal sweep kernel measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration) on this
 # workload; the summary file records the commands and the raw counters.
-TRAFFIC_FILE = ROOT / "profiles" / "k_primal_tiled2_pyramid66_pmc.md"
+TRAFFIC_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_pmc.md"
 
 
 def measured_traffic(layers: int, variant: int) -> float | None:
     """Bytes per primal launch from the committed PMC summary, when it was
     taken on this workload and kernel variant; None otherwise."""
-    if layers != 66 or variant not in (2, 5) or not TRAFFIC_FILE.exists():
+    if layers != 66 or not TRAFFIC_FILE.exists():
         return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+    fields = dict(line.split(":", 1) for line in TRAFFIC_FILE.read_text().splitlines()
+                  if line.startswith(("traffic_bytes_per_launch:", "primal_variant:")))
+    if int(fields.get("primal_variant", "-1")) != variant:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
@@ -77,7 +78,8 @@ def main() -> int:
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
-                         "5 one or two bodies per wavefront by color size (default)")
+                         "5 one or two bodies per wavefront by color size; "
+                         "6 one dataflow launch per sweep (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -100,7 +102,7 @@ def main() -> int:
     t_build = time.perf_counter()
     s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
-    variant = 5 if args.primal_variant is None else args.primal_variant
+    variant = 6 if args.primal_variant is None else args.primal_variant
     s._h.set_option(0, variant)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
@@ -152,6 +154,7 @@ def main() -> int:
             "bound": "hbm",
             "kernel": ("k_primal_color" if variant == 0 else
                        "k_primal_tiled<1|2>" if variant == 5 else
+                       "k_primal_tiled<2, flow>" if variant == 6 else
                        f"k_primal_tiled<{[1, 2, 4, 8][variant - 1]}>") + " (per-color primal sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

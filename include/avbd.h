@@ -185,8 +185,10 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * primal_shuffle / primal_fused switches, solver_6dof.py:196-198, choose
  * between its CUDA kernels). AVBD_OPT_PRIMAL_VARIANT: 0 = one lane per body,
  * 1, 2, 3, 4 = one wavefront per 1, 2, 4, 8 bodies, 5 = one wavefront per
- * body for colors under 4096 bodies and per two bodies otherwise (default).
- * All variants give identical results. */
+ * body for colors under 4096 bodies and per two bodies otherwise, 6 = every
+ * color of a sweep in one launch, each two-body tile starting as soon as its
+ * lower-color neighbours have moved (default). All variants give identical
+ * results. */
 #define AVBD_OPT_PRIMAL_VARIANT 0
 int avbd_set_option(avbd_solver* s, int option, int value);
 
