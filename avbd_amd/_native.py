@@ -25,6 +25,7 @@ I64P = C.POINTER(C.c_longlong)
 COLOR_JACOBI = 0
 COLOR_JONES_PLASSMANN = 1
 OPT_PRIMAL_VARIANT = 0
+OPT_FUSE_DUAL = 1
 
 
 class AvbdError(RuntimeError):

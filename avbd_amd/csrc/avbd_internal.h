@@ -70,7 +70,8 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const int* color, float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int* done, int stamp, int* err);
+                         int* done, int stamp, int* err, int bpw, int fuse_dual, float beta,
+                         GroupSt* gst_w);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
@@ -140,7 +141,7 @@ void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int
                            const int* succ, const int* starts, int* cursor, int* nbr);
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp);
+                         int* err, int* ticket, int mode_jp, int* clamped);
 void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
                           const float* prio, const int* color, int* color_next, int* uncolored);
 void launch_color_finish(hipStream_t st, int nb, int* color, int* counts);

@@ -133,6 +133,7 @@ struct avbd_solver {
   // dataflow primal sweep: per-body stamp of the last sweep that moved it
   DBuf<int> flow_done;
   int flow_stamp = 0;
+  bool colors_conflict_free = false;  // no two neighbours share a color
   bool csr_built = false;       // a CSR has been built since the last flush
   bool csr_order_stale = false; // recolored since that CSR was built
 
@@ -156,6 +157,7 @@ struct avbd_solver {
 
   // options
   int primal_variant = 6;  // one dataflow launch per sweep
+  bool fuse_dual_opt = false;  // measured slower: the sweep is VALU-bound
 
   // profiling
   bool prof = false;
@@ -288,10 +290,11 @@ int recolor(avbd_solver* s, int mode) {
                         s->nbr.p);
   launch_fill_int(st, s->color.p, nb, -1);
   launch_fill_int(st, s->dcnt.p + 6, 1, 0);
+  launch_fill_int(st, s->dcnt.p + 9, 1, 0);
   if (mode == AVBD_COLOR_JACOBI) {
     for (int round = 0; round < kMaxColors; ++round) {
       launch_color_greedy(st, nb, s->order_id.p, nullptr, s->nbr_start.p, s->nbr.p, s->color.p,
-                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0);
+                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0, s->dcnt.p + 9);
       launch_fill_int(st, s->dcnt.p + 5, 1, 0);
       launch_color_resolve(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
                            s->color_next.p, s->dcnt.p + 5);
@@ -301,7 +304,7 @@ int recolor(avbd_solver* s, int mode) {
     }
   } else {
     launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
-                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1);
+                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, s->dcnt.p + 9);
   }
   launch_color_finish(st, nb, s->color.p, s->ccounts.p);
   color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
@@ -309,8 +312,10 @@ int recolor(avbd_solver* s, int mode) {
   CK(prof_end(s, 0.0));
   HIPCK(hipMemcpyAsync(s->hcnt + 16, s->ccounts.p, kMaxColors * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipMemcpyAsync(s->hcnt + 6, s->dcnt.p + 6, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCK(hipMemcpyAsync(s->hcnt + 9, s->dcnt.p + 9, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipStreamSynchronize(st));
   if (s->hcnt[6] != 0) return set_err(AVBD_E_STATE, "coloring dataflow did not converge");
+  s->colors_conflict_free = s->hcnt[9] == 0;
   int mx = 0;
   s->cstart[0] = 0;
   for (int c = 0; c < kMaxColors; ++c) {
@@ -452,6 +457,8 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   if (need) {
     CK(recolor(s, p->coloring_mode));
     rep->recolored = 1;
+  } else {
+    s->colors_conflict_free = true;  // the conflict count just read is 0
   }
   rep->n_active_colors = s->n_active_colors;
 
@@ -488,8 +495,12 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   int nonempty = 0;
   for (int c = 0; c < s->n_active_colors; ++c) nonempty += (s->cstart[c + 1] > s->cstart[c]);
   const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
-  // variant 6: every color of a sweep in one dataflow launch
-  const bool flow = s->primal_variant == 6 && pos_map == nullptr;
+  // variants 6-8: every color of a sweep in one dataflow launch with 2, 1
+  // or 4 bodies per wavefront
+  const bool flow = s->primal_variant >= 6 && pos_map == nullptr;
+  const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4 : 2;
+  // the dual rides along in the sweep when no two neighbours share a color
+  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_opt;
   for (int it = 0; it < total; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
     // One timing interval per sweep (all colors back to back): per-launch
@@ -500,7 +511,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
                           s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
                           s->gst.p, dt, post_pass, s->flow_done.p, ++s->flow_stamp,
-                          s->dcnt.p + 8);
+                          s->dcnt.p + 8, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p);
       launched = nonempty;  // bodies of every color, in one launch
       CK(prof_end(s, pbytes * launched, 1));
     } else {
@@ -514,7 +525,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       }
       CK(prof_end(s, pbytes * launched, launched));
     }
-    if (it < I) {
+    if (it < I && !fuse) {
       CK(prof_begin(s, kDual));
       launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta);
       CK(prof_end(s, dual_sweep_bytes(s)));
@@ -1135,8 +1146,11 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   switch (option) {
     case AVBD_OPT_PRIMAL_VARIANT:
-      if (value < 0 || value > 6) return set_err(AVBD_E_ARG, "primal variant must be 0 to 6");
+      if (value < 0 || value > 8) return set_err(AVBD_E_ARG, "primal variant must be 0 to 8");
       s->primal_variant = value;
+      return AVBD_OK;
+    case AVBD_OPT_FUSE_DUAL:
+      s->fuse_dual_opt = value != 0;
       return AVBD_OK;
     default:
       return set_err(AVBD_E_ARG, "unknown option");

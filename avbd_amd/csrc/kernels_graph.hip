@@ -494,7 +494,8 @@ constexpr int kSpinCap = 1 << 20;
 __global__ void __launch_bounds__(64)
 k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ pos,
                const int* __restrict__ nbr_start, const int* __restrict__ nbr,
-               const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err) {
+               const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err,
+               int* clamped) {
   __shared__ int nj_s[kPollBatch][64];
   const int lane = threadIdx.x;
   for (;;) {
@@ -568,6 +569,9 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
             done = true;
           }
         } else if (q >= q1) {
+          // all 32 colors taken (jacobi) or past the round limit (JP): the
+          // result may leave same-color neighbours
+          if (used == 0xffffffffu || (mode_jp && rmax >= kJpRounds)) atomicOr(clamped, 1);
           if (!mode_jp) {
             st_relaxed(&pub[i], lowest_free_color(used));
           } else {
@@ -593,14 +597,14 @@ k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp) {
 
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp) {
+                         int* err, int* ticket, int mode_jp, int* clamped) {
   if (nb <= 0) return;
   launch_fill_int(st, pub, nb, -1);
   launch_fill_int(st, ticket, 1, 0);
   // Enough wavefronts to cover the GPU; each loops over tickets.
   const int waves = std::min((nb + 63) / 64, 4096);
   hipLaunchKernelGGL(k_color_greedy, dim3(waves), dim3(64), 0, st, nb, order, pos, nbr_start, nbr,
-                     color_in_out, pub, ticket, mode_jp, err);
+                     color_in_out, pub, ticket, mode_jp, err, clamped);
   hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
                      mode_jp);
 }

@@ -457,6 +457,104 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
   solve_and_apply(acc, i, xi, qi, pose);
 }
 
+// ---------------------------------------------------------------------------
+// dual: one lane per group, rows n -> t -> b in order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float& lam, float& kp,
+                                         float ac0, float lo, float hi, float fracture,
+                                         bool is_t, bool used_static, int act_bit, int& bits,
+                                         float beta) {
+  if (hard) C = C - ac0;
+  float lam_eff = hard ? lam : 0.0f;
+  float nl = clampf_(kp * C + lam_eff, lo, hi);
+  lam = nl;
+  if (fabs_(nl) >= fracture) {
+    bits &= ~act_bit;
+    lam = 0.0f;
+    kp = 0.0f;
+    return;
+  }
+  if (nl > lo && nl < hi) {
+    float upper = fmin_(kPenaltyMax, stiff);
+    kp = fmin_(kp + beta * fabs_(C), upper);
+  } else if (is_t && used_static) {
+    bits &= ~kWasN;
+  }
+}
+
+// Dual update of one contact group (dual_update_6dof, kernels_6dof.py:1451-1548)
+// from the final poses of its bodies a (xa, qa) and b (xb, qb; unused for
+// floor and pin groups). Writes back only the state words that changed.
+__device__ __forceinline__ void dual_group(const GroupGeo& G, GroupSt S, GroupSt* __restrict__ out,
+                                           float4 xa, float4 qa, float4 xb, float4 qb,
+                                           float beta) {
+  const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
+  const int b = G.hdr.z;
+  int bits = __float_as_int(S.s2.y);
+  if (!(bits & (kActN | kActT | kActB))) return;
+  const float stiff = G.g0.w;
+  const bool hard = stiff >= kInf;
+  float lam[3] = {S.s0.x, S.s0.y, S.s0.z};
+  float kk[3] = {S.s1.x, S.s1.y, S.s1.z};
+  const float ac0[3] = {S.s0.w, S.s1.w, S.s2.x};
+  float C[3] = {0.f, 0.f, 0.f};
+  v3 off_a = f3(G.g1);
+  if (kind == kGFloor) {
+    v3 rw = add(f3(xa), qrotate(f4q(qa), off_a));
+    C[0] = rw.y - G.g0.y;
+    C[1] = dot(mk3(1.0f, 0.0f, 0.0f), rw);
+    C[2] = dot(mk3(0.0f, 0.0f, 1.0f), rw);
+  } else if (kind == kGBox) {
+    v3 ra = add(f3(xa), qrotate(f4q(qa), off_a));
+    v3 rb = add(f3(xb), qrotate(f4q(qb), f3(G.g2)));
+    v3 d = sub(ra, rb);
+    v3 n = f3(G.g0);
+    C[0] = dot(n, d);
+    if (nr == 3) {
+      v3 t = orthonormal_basis(n);
+      C[1] = dot(t, d);
+      C[2] = dot(cross(n, t), d);
+    }
+  } else {
+    v3 rw = add(f3(xa), qrotate(f4q(qa), off_a));
+    v3 anc = f3(G.g0);
+    C[0] = pick3(rw, b) - pick3(anc, b);
+  }
+  const int bits0 = bits;
+  // normal / pin row
+  if (bits & kActN) {
+    float hi0 = kind == kGPin ? kInf : 0.0f;
+    float frac0 = kind == kGPin ? G.g2.w : kInf;
+    dual_row(C[0], hard, stiff, lam[0], kk[0], ac0[0], -kInf, hi0, frac0, false, false, kActN,
+             bits, beta);
+  }
+  // tangent rows: bounds from this launch's normal lambda and was_static
+  for (int r = 1; r < nr; ++r) {
+    int act = r == 1 ? kActT : kActB;
+    if (!(bits & act)) continue;
+    bool used_static = (bits & kWasN) != 0;
+    float mu = used_static ? G.g2.w : G.g1.w;
+    float bound = mu * fabs_(lam[0]);
+    dual_row(C[r], hard, stiff, lam[r], kk[r], ac0[r], -bound, bound, kInf, true, used_static,
+             act, bits, beta);
+  }
+  // write back what changed, compared bit for bit (a separated floor corner
+  // keeps lambda = 0 and k, so most floor groups write nothing)
+  const float4 n0 = make_float4(lam[0], lam[1], lam[2], S.s0.w);
+  const float4 n1 = make_float4(kk[0], kk[1], kk[2], S.s1.w);
+  if (__float_as_int(n0.x) != __float_as_int(S.s0.x) || __float_as_int(n0.y) != __float_as_int(S.s0.y) ||
+      __float_as_int(n0.z) != __float_as_int(S.s0.z))
+    out->s0 = n0;
+  if (__float_as_int(n1.x) != __float_as_int(S.s1.x) || __float_as_int(n1.y) != __float_as_int(S.s1.y) ||
+      __float_as_int(n1.z) != __float_as_int(S.s1.z))
+    out->s1 = n1;
+  if (bits != bits0) {
+    float4 n2 = S.s2;
+    n2.y = __int_as_float(bits);
+    out->s2 = n2;
+  }
+}
+
 // Tiled variant: one 64-lane wavefront per tile of BPW bodies of the color.
 //  A. lanes evaluate one (body, group) incidence each -> row values in LDS
 //  B. 30 lanes per body own one system entry each (A upper 6, B 9, D 9,
@@ -525,6 +623,14 @@ struct SweepFlow {
   int* done;                        // body -> stamp of the last sweep that moved it
   int stamp;
   int* err;
+  // Dual update fused into the sweep: each group is updated by the tile of
+  // its higher-color body (the body itself for floor and pin groups), after
+  // that tile's solve, when both bodies' poses of this sweep are final and
+  // no other tile reads the group again in this sweep. Valid only for a
+  // coloring without same-color neighbours.
+  int fuse_dual;
+  float beta;
+  GroupSt* gst_w;
 };
 
 constexpr int kFlowSpinCap = 1 << 20;
@@ -589,6 +695,12 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     }
     b_i[lane] = i;
     b_id[lane] = id;
+    if (kFlow && i < 0 && id >= 0) {
+      // static body: its (unchanged) pose for the fused dual
+      const float4 xp = pose[2 * id], qp = pose[2 * id + 1];
+      b_pose[lane][0] = xp.x; b_pose[lane][1] = xp.y; b_pose[lane][2] = xp.z;
+      b_pose[lane][3] = qp.x; b_pose[lane][4] = qp.y; b_pose[lane][5] = qp.z; b_pose[lane][6] = qp.w;
+    }
     if (i >= 0) {
       float4 xp = pose[2 * i], qp = pose[2 * i + 1];
       b_pose[lane][0] = xp.x; b_pose[lane][1] = xp.y; b_pose[lane][2] = xp.z;
@@ -756,6 +868,8 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     if (kFlow) {
       st_agent(pose + 2 * b_i[lane], xo);
       st_agent(pose + 2 * b_i[lane] + 1, qo);
+      b_pose[lane][0] = xo.x; b_pose[lane][1] = xo.y; b_pose[lane][2] = xo.z;
+      b_pose[lane][3] = qo.x; b_pose[lane][4] = qo.y; b_pose[lane][5] = qo.z; b_pose[lane][6] = qo.w;
     } else {
       pose[2 * b_i[lane]] = xo;
       pose[2 * b_i[lane] + 1] = qo;
@@ -769,6 +883,34 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __hip_atomic_store(fl.done + b_id[lane], fl.stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  if (kFlow && fl.fuse_dual) {
+    __syncthreads();  // final poses of the tile's bodies in b_pose
+    for (int p0 = 0; p0 < total; p0 += 64) {
+      const int j = p0 + lane;
+      if (j >= total) continue;
+      const int2 e = csr[e0 + j];
+      if (e.x < 0) continue;
+      int b = 0;
+      while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
+      const int self = b_id[b];
+      if (self < 0) continue;
+      const bool owner = e.y == self || fl.color[e.y] < my_color;
+      if (!owner) continue;
+      const GroupGeo G = geo[e.x];
+      const GroupSt S = fl.gst_w[e.x];
+      const float4 xs = make_float4(b_pose[b][0], b_pose[b][1], b_pose[b][2], 0.0f);
+      const float4 qs = make_float4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
+      if (e.y == self) {
+        dual_group(G, S, fl.gst_w + e.x, xs, qs, xs, qs, fl.beta);
+      } else {
+        const float4 xo = ld_agent(pose + 2 * e.y), qo = ld_agent(pose + 2 * e.y + 1);
+        if (G.hdr.y == self)
+          dual_group(G, S, fl.gst_w + e.x, xs, qs, xo, qo, fl.beta);
+        else
+          dual_group(G, S, fl.gst_w + e.x, xo, qo, xs, qs, fl.beta);
+      }
+    }
+  }
 }
 
 // One dataflow launch for a whole sweep (all colors); cstart holds the
@@ -777,14 +919,18 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const int* color, float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int* done, int stamp, int* err) {
+                         int* done, int stamp, int* err, int bpw, int fuse_dual,
+                         float beta, GroupSt* gst_w) {
   SweepFlow fl{};
+  fl.fuse_dual = fuse_dual;
+  fl.beta = beta;
+  fl.gst_w = gst_w;
   fl.n_colors = n_colors;
   int t = 0;
   for (int c = 0; c < n_colors; ++c) {
     fl.tile_start[c] = t;
     fl.color_start[c] = cstart[c];
-    t += (cstart[c + 1] - cstart[c] + 1) / 2;
+    t += (cstart[c + 1] - cstart[c] + bpw - 1) / bpw;
   }
   fl.tile_start[n_colors] = t;
   fl.color_start[n_colors] = cstart[n_colors];
@@ -794,8 +940,15 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   fl.stamp = stamp;
   fl.err = err;
   if (t <= 0) return;
-  hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0, pose,
-                     poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
+  if (bpw == 1)
+    hipLaunchKernelGGL((k_primal_tiled<1, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
+  else if (bpw == 4)
+    hipLaunchKernelGGL((k_primal_tiled<4, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
+  else
+    hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
 }
 
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
@@ -834,91 +987,21 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
   }
 }
 
-// ---------------------------------------------------------------------------
-// dual: one lane per group, rows n -> t -> b in order
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float& lam, float& kp,
-                                         float ac0, float lo, float hi, float fracture,
-                                         bool is_t, bool used_static, int act_bit, int& bits,
-                                         float beta) {
-  if (hard) C = C - ac0;
-  float lam_eff = hard ? lam : 0.0f;
-  float nl = clampf_(kp * C + lam_eff, lo, hi);
-  lam = nl;
-  if (fabs_(nl) >= fracture) {
-    bits &= ~act_bit;
-    lam = 0.0f;
-    kp = 0.0f;
-    return;
-  }
-  if (nl > lo && nl < hi) {
-    float upper = fmin_(kPenaltyMax, stiff);
-    kp = fmin_(kp + beta * fabs_(C), upper);
-  } else if (is_t && used_static) {
-    bits &= ~kWasN;
-  }
-}
-
 __global__ void __launch_bounds__(256)
 k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict__ geo,
        GroupSt* __restrict__ gst, float beta) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_groups) return;
-  GroupGeo G = geo[g];
-  GroupSt S = gst[g];
-  const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
+  const GroupGeo G = geo[g];
+  const int kind = g_kind(G.hdr.x);
   const int a = G.hdr.y, b = G.hdr.z;
-  int bits = __float_as_int(S.s2.y);
-  if (!(bits & (kActN | kActT | kActB))) return;
-  const float stiff = G.g0.w;
-  const bool hard = stiff >= kInf;
-  float lam[3] = {S.s0.x, S.s0.y, S.s0.z};
-  float kk[3] = {S.s1.x, S.s1.y, S.s1.z};
-  const float ac0[3] = {S.s0.w, S.s1.w, S.s2.x};
-  float C[3] = {0.f, 0.f, 0.f};
-  v3 off_a = f3(G.g1);
-  if (kind == kGFloor) {
-    v3 rw = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
-    C[0] = rw.y - G.g0.y;
-    C[1] = dot(mk3(1.0f, 0.0f, 0.0f), rw);
-    C[2] = dot(mk3(0.0f, 0.0f, 1.0f), rw);
-  } else if (kind == kGBox) {
-    v3 ra = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
-    v3 rb = add(f3(pose[2 * b]), qrotate(f4q(pose[2 * b + 1]), f3(G.g2)));
-    v3 d = sub(ra, rb);
-    v3 n = f3(G.g0);
-    C[0] = dot(n, d);
-    if (nr == 3) {
-      v3 t = orthonormal_basis(n);
-      C[1] = dot(t, d);
-      C[2] = dot(cross(n, t), d);
-    }
-  } else {
-    v3 rw = add(f3(pose[2 * a]), qrotate(f4q(pose[2 * a + 1]), off_a));
-    v3 anc = f3(G.g0);
-    C[0] = pick3(rw, b) - pick3(anc, b);
+  const float4 xa = pose[2 * a], qa = pose[2 * a + 1];
+  float4 xb = xa, qb = qa;
+  if (kind == kGBox) {
+    xb = pose[2 * b];
+    qb = pose[2 * b + 1];
   }
-  // normal / pin row
-  if (bits & kActN) {
-    float hi0 = kind == kGPin ? kInf : 0.0f;
-    float frac0 = kind == kGPin ? G.g2.w : kInf;
-    dual_row(C[0], hard, stiff, lam[0], kk[0], ac0[0], -kInf, hi0, frac0, false, false, kActN,
-             bits, beta);
-  }
-  // tangent rows: bounds from this launch's normal lambda and was_static
-  for (int r = 1; r < nr; ++r) {
-    int act = r == 1 ? kActT : kActB;
-    if (!(bits & act)) continue;
-    bool used_static = (bits & kWasN) != 0;
-    float mu = used_static ? G.g2.w : G.g1.w;
-    float bound = mu * fabs_(lam[0]);
-    dual_row(C[r], hard, stiff, lam[r], kk[r], ac0[r], -bound, bound, kInf, true, used_static,
-             act, bits, beta);
-  }
-  S.s0.x = lam[0]; S.s0.y = lam[1]; S.s0.z = lam[2];
-  S.s1.x = kk[0]; S.s1.y = kk[1]; S.s1.z = kk[2];
-  S.s2.y = __int_as_float(bits);
-  gst[g] = S;
+  dual_group(G, gst[g], gst + g, xa, qa, xb, qb, beta);
 }
 
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,

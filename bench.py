@@ -25,6 +25,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the primal sweep kernel measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration) on this
 # workload; the summary file records the commands and the raw counters.
+PRIMAL_KERNELS = ["k_primal_color (one launch per color)", "k_primal_tiled<1> per color",
+                  "k_primal_tiled<2> per color", "k_primal_tiled<4> per color",
+                  "k_primal_tiled<8> per color", "k_primal_tiled<1|2> per color",
+                  "k_primal_tiled<2, flow> one launch", "k_primal_tiled<1, flow> one launch",
+                  "k_primal_tiled<4, flow> one launch"]
 TRAFFIC_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_pmc.md"
 
 
@@ -76,6 +81,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB MiB (the reference point "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--fuse-dual", type=int, default=0,
+                    help="1: dual update inside the dataflow sweep; 0: separate launch (default)")
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
                          "5 one or two bodies per wavefront by color size; "
@@ -104,6 +111,7 @@ def main() -> int:
     s._flush()
     variant = 6 if args.primal_variant is None else args.primal_variant
     s._h.set_option(0, variant)
+    s._h.set_option(1, args.fuse_dual)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
           f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
@@ -152,10 +160,7 @@ def main() -> int:
                    "colors": s.num_active_colors, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_primal_color" if variant == 0 else
-                       "k_primal_tiled<1|2>" if variant == 5 else
-                       "k_primal_tiled<2, flow>" if variant == 6 else
-                       f"k_primal_tiled<{[1, 2, 4, 8][variant - 1]}>") + " (per-color primal sweep)",
+            "kernel": PRIMAL_KERNELS[variant] + " (primal sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -190,6 +190,11 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * lower-color neighbours have moved (default). All variants give identical
  * results. */
 #define AVBD_OPT_PRIMAL_VARIANT 0
+/* AVBD_OPT_FUSE_DUAL: 1 = with the dataflow sweep (variants 6-8) and a
+ * coloring without same-color neighbours, each contact group's dual update
+ * runs inside the sweep, in the tile of its higher-color body; 0 (default) =
+ * a separate dual launch after every sweep. Both give identical results. */
+#define AVBD_OPT_FUSE_DUAL 1
 int avbd_set_option(avbd_solver* s, int option, int value);
 
 /* ---- profiling ------------------------------------------------------- */

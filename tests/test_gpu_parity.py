@@ -59,7 +59,7 @@ def _tiled_scene(name):
     return s
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, (6, "fused dual")])
 @pytest.mark.parametrize("scene", ["pyramid5", "pile", "pins"])
 def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     """The tiled primal kernels sum every system entry in the same order as
@@ -68,7 +68,11 @@ def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     a._flush()
     b._flush()
     a._h.set_option(0, 0)          # lane per body
-    b._h.set_option(0, variant)
+    if isinstance(variant, tuple):  # dataflow sweep with the dual inside it
+        b._h.set_option(0, variant[0])
+        b._h.set_option(1, 1)
+    else:
+        b._h.set_option(0, variant)
     n_sub = {"pyramid5": 24, "pile": 200, "pins": 60}[scene]
     for k in range(n_sub):
         one_substep(a)
