@@ -67,11 +67,15 @@ struct BvhWs {
   int* right;           // n-1
   int* parent;          // 2n-1
   int* escape;          // 2n-1
-  int* flags;           // n-1
+  int2* range;          // n-1: first and last sorted leaf under each internal node
   float4* nlo;          // 2n-1
   float4* nhi;          // 2n-1
+  float4* b1;           // 2 * ceil(n/64): boxes of 64-leaf blocks (lo, hi)
+  float4* b2;           // 2 * ceil(n/4096): boxes of 64-block superblocks
   void* radix_ws;
 };
+
+constexpr int kRefitB = 64;  // leaves per block; blocks per superblock
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -91,20 +95,26 @@ static BvhWs bvh_carve(void* base, int n) {
   w.right = (int*)take(ni * sizeof(int));
   w.parent = (int*)take(nn * sizeof(int));
   w.escape = (int*)take(nn * sizeof(int));
-  w.flags = (int*)take(ni * sizeof(int));
+  w.range = (int2*)take(ni * sizeof(int2));
   w.nlo = (float4*)take(nn * sizeof(float4));
   w.nhi = (float4*)take(nn * sizeof(float4));
+  const int nb1 = (n + kRefitB - 1) / kRefitB + 1, nb2 = (nb1 + kRefitB - 1) / kRefitB + 1;
+  w.b1 = (float4*)take(2 * (size_t)nb1 * sizeof(float4));
+  w.b2 = (float4*)take(2 * (size_t)nb2 * sizeof(float4));
   w.radix_ws = (void*)p;
   return w;
 }
 
 size_t bvh_workspace_bytes(int n) {
   size_t nn = 2 * (size_t)n, b = 0;
+  const size_t nb1 = ((size_t)n + kRefitB - 1) / kRefitB + 1, nb2 = (nb1 + kRefitB - 1) / kRefitB + 1;
   b += align256(8 * sizeof(int));
   b += 4 * align256(n * sizeof(int));
-  b += 3 * align256(n * sizeof(int));
+  b += 2 * align256(n * sizeof(int));
+  b += align256(n * sizeof(int2));
   b += 2 * align256(nn * sizeof(int));
   b += 2 * align256(nn * sizeof(float4));
+  b += align256(2 * nb1 * sizeof(float4)) + align256(2 * nb2 * sizeof(float4));
   b += radix_workspace_bytes(n);
   return b + 4096;
 }
@@ -189,10 +199,9 @@ __device__ __forceinline__ int bvh_delta(const unsigned* keys, int n, int i, int
 // Node ids: internal 0..n-2 (root 0), leaf k -> n-1+k.
 __global__ void __launch_bounds__(256)
 k_karras(int n, const unsigned* __restrict__ keys, int* __restrict__ left,
-         int* __restrict__ right, int* __restrict__ parent, int* __restrict__ flags) {
+         int* __restrict__ right, int* __restrict__ parent, int2* __restrict__ range) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  flags[i] = 0;
   int d = bvh_delta(keys, n, i, i + 1) - bvh_delta(keys, n, i, i - 1) > 0 ? 1 : -1;
   int dmin = bvh_delta(keys, n, i, i - d);
   int lmax = 2;
@@ -210,6 +219,7 @@ k_karras(int n, const unsigned* __restrict__ keys, int* __restrict__ left,
   } while (t > 1);
   int gamma = i + s * d + (d < 0 ? -1 : 0);
   int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
+  range[i] = make_int2(lo_i, hi_i);
   int L = (lo_i == gamma) ? (n - 1 + gamma) : gamma;
   int R = (hi_i == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
   left[i] = L;
@@ -219,36 +229,84 @@ k_karras(int n, const unsigned* __restrict__ keys, int* __restrict__ left,
   if (i == 0) parent[0] = -1;
 }
 
-// Bottom-up union of child boxes; the second arrival at a node continues.
-// The arrival counter is an acquire-release atomic: the release publishes
-// this thread's child box, the acquire (which invalidates this CU's L1)
-// makes the sibling's box visible to the plain loads that follow.
+// Node boxes without a bottom-up walk: leaf boxes in sorted order, then the
+// union of every 64 leaves (block) and of every 64 blocks (superblock); an
+// internal node's box is the union over its sorted leaf range [first, last],
+// read as whole superblocks and blocks in the middle and single leaves or
+// blocks at the ends. min/max are exact, so the boxes equal a bottom-up
+// union except possibly for the sign of a zero, which no overlap test sees.
 __global__ void __launch_bounds__(256)
-k_refit(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
-        const int* __restrict__ ids, const int* __restrict__ left, const int* __restrict__ right,
-        const int* __restrict__ parent, int* flags, float4* nlo, float4* nhi) {
+k_refit_leaves(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+               const int* __restrict__ ids, float4* __restrict__ nlo, float4* __restrict__ nhi) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  int node = n - 1 + k;
-  int body = ids[k];
-  float4 blo = lo[body], bhi = hi[body];
-  nlo[node] = blo;
-  nhi[node] = bhi;
-  if (n == 1) return;
-  int p = parent[node];
-  for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
-    int prev = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == 0) return;
-    const int L = left[p], R = right[p];
-    const int other = (L == node) ? R : L;
-    const float4 olo = nlo[other], ohi = nhi[other];
-    blo = make_float4(fminf(blo.x, olo.x), fminf(blo.y, olo.y), fminf(blo.z, olo.z), 0.0f);
-    bhi = make_float4(fmaxf(bhi.x, ohi.x), fmaxf(bhi.y, ohi.y), fmaxf(bhi.z, ohi.z), 0.0f);
-    nlo[p] = blo;
-    nhi[p] = bhi;
-    node = p;
-    p = parent[p];
+  const int body = ids[k];
+  nlo[n - 1 + k] = lo[body];
+  nhi[n - 1 + k] = hi[body];
+}
+
+__device__ __forceinline__ void box_union(float4& a_lo, float4& a_hi, float4 b_lo, float4 b_hi) {
+  a_lo = make_float4(fminf(a_lo.x, b_lo.x), fminf(a_lo.y, b_lo.y), fminf(a_lo.z, b_lo.z), 0.0f);
+  a_hi = make_float4(fmaxf(a_hi.x, b_hi.x), fmaxf(a_hi.y, b_hi.y), fmaxf(a_hi.z, b_hi.z), 0.0f);
+}
+
+// One 64-lane wavefront per block: union of 64 consecutive boxes (src lo at
+// src[k], hi at src_hi[k]) into dst[2 * blk] (lo), dst[2 * blk + 1] (hi).
+__global__ void __launch_bounds__(64)
+k_refit_block(int count, const float4* __restrict__ src_lo, const float4* __restrict__ src_hi,
+              int stride, float4* __restrict__ dst) {
+  const int blk = blockIdx.x, k = blk * kRefitB + threadIdx.x;
+  float4 l = make_float4(kInf, kInf, kInf, 0.0f), h = make_float4(-kInf, -kInf, -kInf, 0.0f);
+  if (k < count) {
+    l = src_lo[(size_t)k * stride];
+    h = src_hi[(size_t)k * stride];
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    float4 ol = make_float4(__shfl_down(l.x, o, 64), __shfl_down(l.y, o, 64), __shfl_down(l.z, o, 64), 0.0f);
+    float4 oh = make_float4(__shfl_down(h.x, o, 64), __shfl_down(h.y, o, 64), __shfl_down(h.z, o, 64), 0.0f);
+    box_union(l, h, ol, oh);
+  }
+  if (threadIdx.x == 0) {
+    dst[2 * blk] = l;
+    dst[2 * blk + 1] = h;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_refit_internal(int n, const int2* __restrict__ range, const float4* __restrict__ b1,
+                 const float4* __restrict__ b2, float4* __restrict__ nlo, float4* __restrict__ nhi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int2 r = range[i];
+  int a = r.x;
+  const int e = r.y + 1;  // leaves [a, e)
+  const float4* llo = nlo + (n - 1);
+  const float4* lhi = nhi + (n - 1);
+  float4 l = make_float4(kInf, kInf, kInf, 0.0f), h = make_float4(-kInf, -kInf, -kInf, 0.0f);
+  constexpr int S = kRefitB * kRefitB;
+  // leaves up to the next block boundary
+  while (a < e && (a % kRefitB) != 0) { box_union(l, h, llo[a], lhi[a]); ++a; }
+  // blocks up to the next superblock boundary
+  while (a + kRefitB <= e && (a % S) != 0) {
+    const int bk = a / kRefitB;
+    box_union(l, h, b1[2 * bk], b1[2 * bk + 1]);
+    a += kRefitB;
+  }
+  // whole superblocks
+  while (a + S <= e) {
+    const int sk = a / S;
+    box_union(l, h, b2[2 * sk], b2[2 * sk + 1]);
+    a += S;
+  }
+  // remaining whole blocks, then leaves
+  while (a + kRefitB <= e) {
+    const int bk = a / kRefitB;
+    box_union(l, h, b1[2 * bk], b1[2 * bk + 1]);
+    a += kRefitB;
+  }
+  while (a < e) { box_union(l, h, llo[a], lhi[a]); ++a; }
+  nlo[i] = l;
+  nhi[i] = h;
 }
 
 __global__ void __launch_bounds__(256)
@@ -276,9 +334,17 @@ int bvh_build(hipStream_t st, int n, const float4* lo, const float4* hi, void* w
   radix_sort_pairs(st, w.keys, w.ids, w.keys_tmp, w.ids_tmp, n, w.radix_ws);
   if (n > 1)
     hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, w.keys, w.left,
-                       w.right, w.parent, w.flags);
-  hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.ids, w.left,
-                     w.right, w.parent, w.flags, w.nlo, w.nhi);
+                       w.right, w.parent, w.range);
+  hipLaunchKernelGGL(k_refit_leaves, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.ids,
+                     w.nlo, w.nhi);
+  if (n > 1) {
+    const int nb1 = (n + kRefitB - 1) / kRefitB, nb2 = (nb1 + kRefitB - 1) / kRefitB;
+    hipLaunchKernelGGL(k_refit_block, dim3(nb1), dim3(64), 0, st, n, w.nlo + (n - 1),
+                       w.nhi + (n - 1), 1, w.b1);
+    hipLaunchKernelGGL(k_refit_block, dim3(nb2), dim3(64), 0, st, nb1, w.b1, w.b1 + 1, 2, w.b2);
+    hipLaunchKernelGGL(k_refit_internal, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, w.range,
+                       w.b1, w.b2, w.nlo, w.nhi);
+  }
   if (n > 1)
     hipLaunchKernelGGL(k_escape, dim3((2 * n - 1 + 255) / 256), dim3(256), 0, st, n, w.left,
                        w.right, w.parent, w.escape);
