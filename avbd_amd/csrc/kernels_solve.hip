@@ -570,10 +570,13 @@ __device__ __forceinline__ void dual_group(const GroupGeo& G, GroupSt S, GroupSt
 // stands for "no geometric-stiffness term" (|f| == 0), and lanes that own an
 // entry without such a term read a constant -0 field. Each body's slots are
 // padded to a multiple of four so that phase B reads four slots per float4.
-constexpr int kRowFields = 11;           // jl 3, ja 3, k, f, g 3
-constexpr int kFieldOne = 3 * kRowFields;          // constant 1.0
-constexpr int kFieldNegZero = kFieldOne + 1;       // constant -0.0
-constexpr int kSlotFields = kFieldNegZero + 1;
+// Per row: jl 3, ja 3, k, f, g 3, then two constant fields (1.0 and -0.0),
+// repeated in every row so that each operand's LDS address is the lane's
+// field offset plus the row offset.
+constexpr int kRowFields = 13;
+constexpr int kFieldOne = 11;
+constexpr int kFieldNegZero = 12;
+constexpr int kSlotFields = 3 * kRowFields;
 constexpr int kComps = 30;
 
 // Fields that system entry ac (A upper 0-5, B 6-14, D 15-23, rl 24-26,
@@ -581,11 +584,11 @@ constexpr int kComps = 30;
 //   A[r][c] += (jl_r * jl_c) * k      B[r][c] += (ja_r * jl_c) * k
 //   D[r][c] += (ja_r * ja_c) * k, then += g_r on the diagonal
 //   rl[r]   += (jl_r * f) * 1         ra[r]   += (ja_r * f) * 1
-// (the factor 1.0 is exact). fz < 0 selects the constant 1.0 field and
-// fg < 0 the constant -0 field.
+// (the factor 1.0 is exact). fz is k or the constant 1.0 field, fg is g_r
+// or the constant -0 field.
 __device__ __forceinline__ void entry_fields(int ac, int& fa, int& fb, int& fz, int& fg) {
   fz = 6;
-  fg = -1;
+  fg = kFieldNegZero;
   if (ac < 6) {
     const int rr = ac < 3 ? 0 : (ac < 5 ? 1 : 2);
     const int cc = ac < 3 ? ac : (ac < 5 ? ac - 2 : 2);
@@ -597,9 +600,9 @@ __device__ __forceinline__ void entry_fields(int ac, int& fa, int& fb, int& fz, 
     fa = 3 + cr; fb = 3 + cc;
     if (cr == cc) fg = 8 + cr;
   } else if (ac < 27) {
-    fa = ac - 24; fb = 7; fz = -1;
+    fa = ac - 24; fb = 7; fz = kFieldOne;
   } else {
-    fa = 3 + (ac - 27); fb = 7; fz = -1;
+    fa = 3 + (ac - 27); fb = 7; fz = kFieldOne;
   }
 }
 
@@ -661,8 +664,11 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     first = (t - fl.tile_start[c]) * BPW;
   }
   const float inv_dt2 = 1.0f / (dt * dt);
-  rows[kFieldOne][lane] = 1.0f;
-  rows[kFieldNegZero][lane] = -0.0f;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    rows[kRowFields * r + kFieldOne][lane] = 1.0f;
+    rows[kRowFields * r + kFieldNegZero][lane] = -0.0f;
+  }
   // Tile slots: the CSR entries of the tile's positions, contiguous and
   // padded per body to a multiple of four.
   const int e0 = cpos[base + first];
@@ -674,12 +680,16 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
   GroupGeo Gq;
   GroupSt Sq;
   float4 oxq, oqq;
+  int colq = 0, flagq = 0;  // dataflow: the other body's color and sweep stamp
   if (ent.x >= 0) {
     Gq = geo[ent.x];
     Sq = gst[ent.x];
     if (!kFlow) {
       oxq = pose[2 * ent.y];
       oqq = pose[2 * ent.y + 1];
+    } else {
+      colq = fl.color[ent.y];
+      flagq = __hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if (lane <= BPW) {
@@ -745,13 +755,14 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     const int jn = j + 64;
     const int2 ent_next = jn < total ? csr[e0 + jn] : make_int2(-1, -1);
     if (kFlow) {
-      // wait for this pass's lower-color neighbours, then read their poses
-      // through the coherent path
+      // wait for this pass's lower-color neighbours (their color and stamp
+      // were loaded with the group records), then read their poses through
+      // the coherent path, after the stamp was seen
       bool need = false;
       if (ent.x >= 0 && ent.y >= 0) {
         int bb = 0;
         while (bb + 1 < BPW && j >= b_off[bb + 1]) ++bb;
-        need = ent.y != b_id[bb] && fl.color[ent.y] < my_color;
+        need = ent.y != b_id[bb] && colq < my_color && flagq != fl.stamp;
       }
       int spins = 0;
       while (need) {
@@ -806,6 +817,9 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       if (!kFlow) {
         oxq = pose[2 * ent.y];
         oqq = pose[2 * ent.y + 1];
+      } else {
+        colq = fl.color[ent.y];
+        flagq = __hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     __syncthreads();
@@ -823,8 +837,8 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
             const int rb = kRowFields * r;
             X[r] = *reinterpret_cast<const float4*>(&rows[rb + fa[k]][sb]);
             Y[r] = *reinterpret_cast<const float4*>(&rows[rb + fb[k]][sb]);
-            Z[r] = *reinterpret_cast<const float4*>(&rows[fz[k] < 0 ? kFieldOne : rb + fz[k]][sb]);
-            Gd[r] = *reinterpret_cast<const float4*>(&rows[fg[k] < 0 ? kFieldNegZero : rb + fg[k]][sb]);
+            Z[r] = *reinterpret_cast<const float4*>(&rows[rb + fz[k]][sb]);
+            Gd[r] = *reinterpret_cast<const float4*>(&rows[rb + fg[k]][sb]);
           }
           float acc = accv[k];
 #pragma unroll
