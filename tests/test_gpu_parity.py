@@ -102,7 +102,7 @@ def test_mid_simulation_add_box_lockstep():
         one_substep(o)
     for s in (g, o):
         for j in range(2):
-            b = s.add_box((0.6 + 0.02 * j, 1.5 + 0.6 * j, 0.1), (0.2, 0.2, 0.2), mass=2.0,
+            b = s.add_box((0.75 + 0.02 * j, 1.5 + 0.6 * j, 0.1), (0.2, 0.2, 0.2), mass=2.0,
                           friction=0.5)
             s.add_floor_contact_box(b, friction=0.5)
         s._flush()
