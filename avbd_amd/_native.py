@@ -105,6 +105,8 @@ SIGNATURES = {
     "destroy": ([C.c_void_p], C.c_int),
     "flush": ([C.c_void_p, C.POINTER(Scene)], C.c_int),
     "substep": ([C.c_void_p, C.POINTER(Params), C.c_int, C.POINTER(Report)], C.c_int),
+    "substep_to_residual": ([C.c_void_p, C.POINTER(Params), C.c_int, C.c_float,
+                             C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(Report)], C.c_int),
     "read_bodies": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
     "write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
     "read_rows": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P, I32P, I32P, I32P, F32P], C.c_int),
@@ -197,6 +199,17 @@ class NativeHandle:
         rep = Report()
         self._check(self._fn("substep")(self.h, C.byref(params), int(bool(color_dirty)), C.byref(rep)))
         return rep
+
+    def substep_to_residual(self, params: Params, color_dirty: bool, tol: float):
+        """One substep iterated until the dual residual is <= tol (params.iterations
+        is the cap). Returns (report, iterations used, final residual)."""
+        rep = Report()
+        it = C.c_int(0)
+        r = C.c_float(0.0)
+        self._check(self._fn("substep_to_residual")(self.h, C.byref(params), int(bool(color_dirty)),
+                                                   float(tol), C.byref(it), C.byref(r),
+                                                   C.byref(rep)))
+        return rep, int(it.value), float(r.value)
 
     def read_bodies(self, n: int) -> dict[str, np.ndarray]:
         out = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "w", "prev_v", "prev_w")}

@@ -73,7 +73,7 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          int* done, int stamp, int* err, int bpw, int fuse_dual, float beta,
                          GroupSt* gst_w);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
-                 GroupSt* gst, float beta);
+                 GroupSt* gst, float beta, float* resid);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
                      float4* vel, float4* pvel, const float4* bstat, float dt, float max_lin,
                      float max_ang);

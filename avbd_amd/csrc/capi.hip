@@ -415,7 +415,11 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   return AVBD_OK;
 }
 
-int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep) {
+// tol >= 0: iterate until the dual's residual max |lambda_new - lambda| / k
+// is <= tol, with p->iterations as the cap (one readback per iteration); the
+// result equals a fixed-iteration substep with that many iterations.
+int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep,
+               float tol = -1.0f, int* iters_used = nullptr, float* resid_out = nullptr) {
   std::memset(rep, 0, sizeof(*rep));
   rep->n_active_colors = s->n_active_colors;
   if (s->nb == 0) return AVBD_OK;
@@ -490,8 +494,9 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
     max_lin = kInf;
     max_ang = kInf;
   }
-  const int I = p->iterations;
-  const int total = I + post;
+  int I = p->iterations;
+  const bool to_resid = tol >= 0.0f;
+  float resid = 0.0f;
   int nonempty = 0;
   for (int c = 0; c < s->n_active_colors; ++c) nonempty += (s->cstart[c + 1] > s->cstart[c]);
   const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
@@ -500,8 +505,8 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   const bool flow = s->primal_variant >= 6 && pos_map == nullptr;
   const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4 : 2;
   // the dual rides along in the sweep when no two neighbours share a color
-  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_opt;
-  for (int it = 0; it < total; ++it) {
+  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_opt && !to_resid;
+  for (int it = 0; it < I + post; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
     // One timing interval per sweep (all colors back to back): per-launch
     // events would add their own gaps between the color launches.
@@ -526,9 +531,16 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       CK(prof_end(s, pbytes * launched, launched));
     }
     if (it < I && !fuse) {
+      float* rd = to_resid ? reinterpret_cast<float*>(s->dcnt.p + 10) : nullptr;
+      if (rd) launch_fill_int(st, s->dcnt.p + 10, 1, 0);
       CK(prof_begin(s, kDual));
-      launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta);
+      launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta, rd);
       CK(prof_end(s, dual_sweep_bytes(s)));
+      if (rd) {
+        CK(readback(s, 10, 1));
+        std::memcpy(&resid, s->hcnt + 10, sizeof(float));
+        if (resid <= tol) I = it + 1;  // this was the last dual
+      }
     }
     if (it == I - 1) {
       CK(prof_begin(s, kOther));
@@ -545,6 +557,8 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
                         s->howner.p, s->slot_of.p, s->hstate.p);
     CK(prof_end(s, 0.0));
   }
+  if (iters_used) *iters_used = I;
+  if (resid_out) *resid_out = resid;
   HIPCK(hipMemcpyAsync(s->hcnt + 8, s->dcnt.p + 8, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipGetLastError());
   s->cur = 1 - s->cur;
@@ -812,6 +826,16 @@ int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_rep
   if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
   HIPCK(hipSetDevice(s->device));
   return do_substep(s, p, color_dirty, out);
+}
+
+int avbd_substep_to_residual(avbd_solver* s, const avbd_params* p, int color_dirty, float tol,
+                             int* iterations_used, float* residual, avbd_report* out) {
+  if (!s || !p || !out || !iterations_used || !residual)
+    return set_err(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
+  if (!(tol >= 0.0f)) return set_err(AVBD_E_ARG, "tolerance must be >= 0");
+  HIPCK(hipSetDevice(s->device));
+  return do_substep(s, p, color_dirty, out, tol, iterations_used, residual);
 }
 
 int avbd_read_bodies(avbd_solver* s, float* x, float* q, float* v, float* w, float* pv,

@@ -463,10 +463,13 @@ k_primal_color(int count, const int* __restrict__ color_bodies, int base,
 __device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float& lam, float& kp,
                                          float ac0, float lo, float hi, float fracture,
                                          bool is_t, bool used_static, int act_bit, int& bits,
-                                         float beta) {
+                                         float beta, float& rmax) {
   if (hard) C = C - ac0;
   float lam_eff = hard ? lam : 0.0f;
   float nl = clampf_(kp * C + lam_eff, lo, hi);
+  // residual of this row: |lambda_new - lambda| / k with lambda, k before the update
+  const float r = fabs_(nl - lam) / kp;
+  rmax = r > rmax ? r : rmax;
   lam = nl;
   if (fabs_(nl) >= fracture) {
     bits &= ~act_bit;
@@ -485,9 +488,10 @@ __device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float&
 // Dual update of one contact group (dual_update_6dof, kernels_6dof.py:1451-1548)
 // from the final poses of its bodies a (xa, qa) and b (xb, qb; unused for
 // floor and pin groups). Writes back only the state words that changed.
+// resid (optional): atomically raised to the group's largest row residual.
 __device__ __forceinline__ void dual_group(const GroupGeo& G, GroupSt S, GroupSt* __restrict__ out,
                                            float4 xa, float4 qa, float4 xb, float4 qb,
-                                           float beta) {
+                                           float beta, float* resid = nullptr) {
   const int h = G.hdr.x, kind = g_kind(h), nr = g_nrows(h);
   const int b = G.hdr.z;
   int bits = __float_as_int(S.s2.y);
@@ -521,12 +525,13 @@ __device__ __forceinline__ void dual_group(const GroupGeo& G, GroupSt S, GroupSt
     C[0] = pick3(rw, b) - pick3(anc, b);
   }
   const int bits0 = bits;
+  float rmax = 0.0f;
   // normal / pin row
   if (bits & kActN) {
     float hi0 = kind == kGPin ? kInf : 0.0f;
     float frac0 = kind == kGPin ? G.g2.w : kInf;
     dual_row(C[0], hard, stiff, lam[0], kk[0], ac0[0], -kInf, hi0, frac0, false, false, kActN,
-             bits, beta);
+             bits, beta, rmax);
   }
   // tangent rows: bounds from this launch's normal lambda and was_static
   for (int r = 1; r < nr; ++r) {
@@ -536,8 +541,9 @@ __device__ __forceinline__ void dual_group(const GroupGeo& G, GroupSt S, GroupSt
     float mu = used_static ? G.g2.w : G.g1.w;
     float bound = mu * fabs_(lam[0]);
     dual_row(C[r], hard, stiff, lam[r], kk[r], ac0[r], -bound, bound, kInf, true, used_static,
-             act, bits, beta);
+             act, bits, beta, rmax);
   }
+  if (resid && rmax > 0.0f) atomicMax(reinterpret_cast<int*>(resid), __float_as_int(rmax));
   // write back what changed, compared bit for bit (a separated floor corner
   // keeps lambda = 0 and k, so most floor groups write nothing)
   const float4 n0 = make_float4(lam[0], lam[1], lam[2], S.s0.w);
@@ -1003,7 +1009,7 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
 
 __global__ void __launch_bounds__(256)
 k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict__ geo,
-       GroupSt* __restrict__ gst, float beta) {
+       GroupSt* __restrict__ gst, float beta, float* resid) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_groups) return;
   const GroupGeo G = geo[g];
@@ -1015,14 +1021,14 @@ k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict
     xb = pose[2 * b];
     qb = pose[2 * b + 1];
   }
-  dual_group(G, gst[g], gst + g, xa, qa, xb, qb, beta);
+  dual_group(G, gst[g], gst + g, xa, qa, xb, qb, beta, resid);
 }
 
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
-                 GroupSt* gst, float beta) {
+                 GroupSt* gst, float beta, float* resid) {
   if (n_groups <= 0) return;
   hipLaunchKernelGGL(k_dual, dim3((n_groups + 255) / 256), dim3(256), 0, st, n_groups, pose, geo,
-                     gst, beta);
+                     gst, beta, resid);
 }
 
 // ---------------------------------------------------------------------------

@@ -624,6 +624,28 @@ class Solver6DOF:
         if fixed_cap:
             self._check_fixed_cap_overflow()
 
+    def step_to_residual(self, tol: float = 1e-4, max_iters: int = 200) -> list[tuple[int, float]]:
+        """One frame like step(), each substep iterated until the dual
+        residual r = max over active rows of |lambda_new - lambda| / k (metres)
+        is <= tol, at most `max_iters` iterations. Not in the reference: the
+        "iterations to residual" measurement. Returns (iterations, r) per
+        substep; a substep equals a fixed-iteration one with that count."""
+        if self._dirty:
+            self._flush()
+        n_b = len(self._x)
+        if n_b == 0:
+            return []
+        out = []
+        full_dt = self.dt
+        sub_dt = full_dt / self.substeps if self.substeps > 1 else full_dt
+        for _ in range(max(1, self.substeps)):
+            params = self._params(sub_dt)
+            params.iterations = int(max_iters)
+            rep, iters, r = self._h.substep_to_residual(params, self._color_dirty, tol)
+            self._apply_report(rep, n_b)
+            out.append((iters, r))
+        return out
+
     def _check_fixed_cap_overflow(self) -> None:
         """Once-per-frame overflow warnings of the fixed-capacity path
         (solver_6dof.py:1199-1230)."""

@@ -131,6 +131,15 @@ int avbd_flush(avbd_solver* s, const avbd_scene* scene);
  * (_run_iter_loop :1507-1679) and warm-start cache refresh (:1450-1464).
  * color_dirty: the host's _color_dirty flag. */
 int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out);
+/* The substep run to a residual instead of a fixed iteration count; no
+ * counterpart in the reference (the residual measurement of the learned
+ * warm-start study). After each dual sweep the residual
+ * r = max over active rows of |lambda_new - lambda| / k (metres; lambda and
+ * k before the update) is read back; the substep stops iterating at the
+ * first sweep with r <= tol, or at p->iterations. The result equals
+ * avbd_substep with iterations = *iterations_used. */
+int avbd_substep_to_residual(avbd_solver* s, const avbd_params* p, int color_dirty, float tol,
+                             int* iterations_used, float* residual, avbd_report* out);
 
 /* ---- readback / write-back ------------------------------------------ */
 /* positions()/orientations()/velocities()/angular_velocities()

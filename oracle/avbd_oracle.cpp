@@ -421,8 +421,11 @@ void primal_body(Orc& S, int i, float dt) {
 // ---------------------------------------------------------------------------
 // dual_update_6dof (kernels_6dof.py:1451-1548), serial over rows.
 // ---------------------------------------------------------------------------
-void dual(Orc& S, int n_rows) {
+// Returns the largest |lambda_new - lambda| / k over the active rows, the
+// residual r of the north-star measurement (lambda and k before the update).
+float dual(Orc& S, int n_rows) {
   const float beta = S.P.beta;
+  float rmax = 0.0f;
   for (int j = 0; j < n_rows; ++j) {
     if (S.cact[j] == 0) continue;
     int t = S.ctype[j];
@@ -464,6 +467,8 @@ void dual(Orc& S, int n_rows) {
       hi = bound;
     }
     float nl = clampf_(S.cpen[j] * C + lam_eff, lo, hi);
+    const float r = fabs_(nl - S.clam[j]) / S.cpen[j];
+    rmax = r > rmax ? r : rmax;
     S.clam[j] = nl;
     if (fabs_(nl) >= S.cfrac[j]) {
       S.cact[j] = 0;
@@ -478,6 +483,7 @@ void dual(Orc& S, int n_rows) {
       S.cwas[sib] = 0;
     }
   }
+  return rmax;
 }
 
 // finalize_and_cap_6dof (kernels_6dof.py:1871-1901)
@@ -1067,7 +1073,10 @@ void build_csr(Orc& S) {
 }
 
 // Solver6DOF._step_one (solver_6dof.py:1232-1464), readback (non-resident) path.
-int substep(Orc& S, const avbd_params* p, int color_dirty, avbd_report* rep) {
+// tol >= 0: iterate until the dual's residual is <= tol (p->iterations is
+// then the cap); the substep equals a fixed one with that many iterations.
+int substep(Orc& S, const avbd_params* p, int color_dirty, avbd_report* rep, float tol = -1.0f,
+            int* iters_used = nullptr, float* resid = nullptr) {
   std::memset(rep, 0, sizeof(*rep));
   S.P = *p;
   const float dt = p->dt;
@@ -1119,16 +1128,21 @@ int substep(Orc& S, const avbd_params* p, int color_dirty, avbd_report* rep) {
     max_lin = kInf;
     max_ang = kInf;
   }
-  const int I = p->iterations;
-  const int total = I + (p->post_stabilize ? 1 : 0);
-  for (int it = 0; it < total; ++it) {
+  int I = p->iterations;
+  float r = 0.0f;
+  for (int it = 0; it < I + (p->post_stabilize ? 1 : 0); ++it) {
     if (p->post_stabilize && it == I) cache_alpha_c0_post(S, rows);
     for (int c = 0; c < S.n_active_colors; ++c) {
       for (int k = S.color_starts[c]; k < S.color_starts[c + 1]; ++k) primal_body(S, S.color_bodies[k], dt);
     }
-    if (it < I) dual(S, rows);
+    if (it < I) {
+      r = dual(S, rows);
+      if (tol >= 0.0f && r <= tol) I = it + 1;
+    }
     if (it == I - 1) finalize(S, dt, max_lin, max_ang);
   }
+  if (iters_used) *iters_used = I;
+  if (resid) *resid = r;
   if (p->self_collide && S.hash_cap > 0) hash_refresh(S);
   return AVBD_OK;
 }
@@ -1220,6 +1234,13 @@ int orc_substep(void* h, const avbd_params* p, int color_dirty, avbd_report* out
   Orc& S = *static_cast<Orc*>(h);
   if (!S.flushed) return fail(AVBD_E_STATE, "substep before flush");
   return substep(S, p, color_dirty, out);
+}
+
+int orc_substep_to_residual(void* h, const avbd_params* p, int color_dirty, float tol,
+                            int* iters_used, float* resid, avbd_report* out) {
+  Orc& S = *static_cast<Orc*>(h);
+  if (!S.flushed) return fail(AVBD_E_STATE, "substep before flush");
+  return substep(S, p, color_dirty, out, tol < 0.0f ? 0.0f : tol, iters_used, resid);
 }
 
 int orc_read_bodies(void* h, float* x, float* q, float* v, float* w, float* pv, float* pw) {

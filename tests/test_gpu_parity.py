@@ -111,3 +111,17 @@ def test_mid_simulation_add_box_lockstep():
         one_substep(g)
         one_substep(o)
         assert_bit_exact(diff_report(g, o), f"mid-sim substep {k}")
+
+
+def test_residual_mode_lockstep():
+    """Substeps iterated to a dual residual of 1e-4: the same iteration
+    counts, bit-identical residuals and states on both implementations."""
+    from tests.test_oracle_residual import _resid_substep
+    g, o = pair(lambda **kw: scenes.pyramid(5, seed=23, **kw))
+    g._flush()
+    o._flush()
+    for k in range(12):
+        gi, gr = _resid_substep(g, 1e-4, 80)
+        oi, orr = _resid_substep(o, 1e-4, 80)
+        assert (gi, np.float32(gr).view(np.int32)) == (oi, np.float32(orr).view(np.int32)), k
+        assert_bit_exact(diff_report(g, o), f"residual substep {k}")
