@@ -124,7 +124,20 @@ SIGNATURES = {
     "set_option": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "profile_read": ([C.c_void_p, C.c_int, F64P, I64P, F64P], C.c_int),
+    # learned warm start (no counterpart in the reference)
+    "mlp_num_params": ([], C.c_int),
+    "ml_set_mode": ([C.c_void_p, C.c_int], C.c_int),
+    "ml_set_params": ([C.c_void_p, F32P, C.c_int], C.c_int),
+    "ml_read_samples": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P], C.c_int),
+    "mlp_forward": ([C.c_int, C.c_int, F32P, F32P, F32P], C.c_int),
+    "mlp_grad": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, F32P, F32P], C.c_int),
+    "mlp_train": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_int, C.c_int, C.c_float,
+                   C.c_uint, F32P], C.c_int),
 }
+
+# entry points of the learned warm start (the checker has none)
+ML_NAMES = ("mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
+            "mlp_grad", "mlp_train")
 
 _LIB = None
 
@@ -324,6 +337,26 @@ class NativeHandle:
 
     def profile_enable(self, on: bool) -> None:
         self._check(self._fn("profile_enable")(self.h, int(bool(on))))
+
+    # ---- learned warm start ----
+    def ml_set_mode(self, mode: int) -> None:
+        self._check(self._fn("ml_set_mode")(self.h, int(mode)))
+
+    def ml_set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self._fn("ml_set_params")(self.h, fptr(p), int(p.size)))
+
+    def ml_read_samples(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        n = C.c_int(0)
+        self._check(self._fn("ml_read_samples")(self.h, 0, None, None, None, C.byref(n)))
+        m = int(n.value)
+        feat = np.zeros((m, 24), np.float32)
+        targ = np.zeros((m, 4), np.float32)
+        mask = np.zeros(m, np.int32)
+        if m:
+            self._check(self._fn("ml_read_samples")(self.h, m, fptr(feat), fptr(targ), iptr(mask),
+                                                    C.byref(n)))
+        return feat, targ, mask
 
     def profile_read(self, n: int = 7):
         ms = np.zeros(n, np.float64)

@@ -107,11 +107,11 @@ void launch_emit_write(hipStream_t st, int n_pairs, const int* pair_a, const int
                        const float* bfric, float default_mu, float mult, const int* rpc,
                        const int* succ, const int* demand_scan, const int* succ_scan,
                        int n_static_rows, int g_static, GroupGeo* geo, GroupSt* gst,
-                       int hash_cap, const int* hkeys, const float4* hstate);
+                       int hash_cap, const int* hkeys, const float4* hstate, const int* hage);
 void launch_fill_int(hipStream_t st, int* p, int n, int v);
 void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner,
-                         int* slot_of, float4* hstate);
+                         int* slot_of, float4* hstate, int* hage);
 
 // ---- graph kernels (kernels_graph.hip) ----
 // Exclusive scan of n ints; returns the total through *total_dev (device int).
@@ -149,5 +149,23 @@ void launch_color_scatter(hipStream_t st, int nb, const int* color, const int* s
                           int* cursor, int* bodies);
 void launch_sort_segments(hipStream_t st, int nb, const int* starts, int* idx);
 int color_greedy_blocks(int device);
+
+// learned warm start (kernels_learn.hip)
+int mlp_num_params();
+int mlp_num_weights();
+void launch_contact_features(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                             const GroupSt* gst, const float4* pose, const float4* vel,
+                             const float4* bstat, float* feat);
+void launch_contact_targets(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                            const GroupSt* gst, const float* feat, float ag, float gamma,
+                            float* targ, int* ymask);
+void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* params, int g_first,
+                      GroupSt* gst, const GroupGeo* geo, float ag, float gamma, int mode,
+                      float* yout);
+void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
+                           const int* ymask, const float* params, float inv_total, float* grad,
+                           float* loss);
+void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,
+                 float b1, float b2, float c1, float c2, float eps);
 
 }  // namespace avbd

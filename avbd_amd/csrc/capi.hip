@@ -155,6 +155,14 @@ struct avbd_solver {
   int adj_pairs = 0;  // pairs whose emission this substep feeds the coloring adjacency
   int last_post = 0;
 
+  // learned warm start: 0 off, 1 log (features at emission, targets at the
+  // end of the substep), 2 apply the network to the initial dual state
+  int ml_mode = 0;
+  bool ml_have_params = false;
+  int ml_n = 0;              // groups featurised by the last substep
+  DBuf<float> ml_params, ml_feat, ml_targ;
+  DBuf<int> ml_mask, hage;   // hage: substeps each cached contact has lived
+
   // options
   int primal_variant = 6;  // one dataflow launch per sweep
   bool fuse_dual_opt = false;  // measured slower: the sweep is VALU-bound
@@ -401,7 +409,7 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
     launch_emit_write(st, kept, s->pa.p, s->pb.p, s->mfr.p, s->mfn.p, s->mfoff.p, s->bfric.p,
                       p->self_friction, p->friction_static_mult, s->rpc.p, s->succ.p, s->dscan.p,
                       s->sscan.p, s->n_static_rows, s->g_static, s->geo.p, s->gst.p, s->hash_cap,
-                      s->hkeys.p, s->hstate.p);
+                      s->hkeys.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
     CK(prof_end(s, 0.0));
     if (kept > 0) {
       CK(readback(s, 2, 2));
@@ -412,6 +420,32 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   }
   auto t1 = std::chrono::steady_clock::now();
   rep->broadphase_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return AVBD_OK;
+}
+
+int ensure_hage(avbd_solver* s) {
+  if (s->ml_mode == 0 || !s->flushed) return AVBD_OK;
+  const int hc = std::max(s->hash_cap, 1);
+  if (s->hage.p && s->hage.n >= (size_t)hc) return AVBD_OK;
+  CK(s->hage.alloc(hc));
+  launch_fill_int(s->st, s->hage.p, hc, 0);
+  return AVBD_OK;
+}
+
+// Features of the n emitted groups (bodies at the start of the substep) and,
+// in mode 2, the network's initial dual state for them.
+int ml_begin(avbd_solver* s, const avbd_params* p, int n) {
+  CK(s->ml_feat.ensure((size_t)n * 24));
+  CK(s->ml_targ.ensure((size_t)n * 4));
+  CK(s->ml_mask.ensure(n));
+  CK(prof_begin(s, kOther));
+  launch_contact_features(s->st, n, s->g_static, s->geo.p, s->gst.p, s->pose[s->cur].p, s->vel.p,
+                          s->bstat.p, s->ml_feat.p);
+  if (s->ml_mode == 2)
+    launch_mlp_infer(s->st, n, s->ml_feat.p, s->ml_params.p, s->g_static, s->gst.p, s->geo.p,
+                     p->alpha * p->gamma, p->gamma, 0, nullptr);
+  CK(prof_end(s, (double)n * (96.0 + 48.0 + 96.0 + (s->ml_mode == 2 ? 96.0 + 48.0 : 0.0)),
+              s->ml_mode == 2 ? 2 : 1));
   return AVBD_OK;
 }
 
@@ -447,6 +481,12 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       n_active = std::min(raw, cap);
     }
   }
+  // learned warm start: features of the emitted contact groups, then either
+  // the network's initial dual state (mode 2) or nothing more until the
+  // targets at the end of the substep (mode 1)
+  const int n_ml = p->self_collide && s->ml_mode != 0 ? std::min(s->n_contacts, s->pool_max) : 0;
+  s->ml_n = 0;
+  if (n_ml > 0) CK(ml_begin(s, p, n_ml));
   rep->n_active_rows = s->n_active_rows;
   rep->new_row_capacity = s->row_cap;
   rep->pool_count = s->pool_count;
@@ -548,13 +588,20 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       CK(prof_end(s, (double)nb * 132.0));
     }
   }
+  if (n_ml > 0 && s->ml_mode == 1) {
+    CK(prof_begin(s, kOther));
+    launch_contact_targets(st, n_ml, s->g_static, s->geo.p, s->gst.p, s->ml_feat.p,
+                           p->alpha * p->gamma, p->gamma, s->ml_targ.p, s->ml_mask.p);
+    CK(prof_end(s, (double)n_ml * (96.0 + 48.0 + 16.0 + 20.0)));
+    s->ml_n = n_ml;
+  }
   if (p->self_collide && s->hash_cap > 0) {
     CK(prof_begin(s, kOther));
     launch_fill_int(st, s->hkeys.p, s->hash_cap, kHashEmpty);
     launch_fill_int(st, s->howner.p, s->hash_cap, -1);
     int entries = std::min(s->n_contacts, s->pool_max);
     launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_cap, s->hkeys.p,
-                        s->howner.p, s->slot_of.p, s->hstate.p);
+                        s->howner.p, s->slot_of.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
     CK(prof_end(s, 0.0));
   }
   if (iters_used) *iters_used = I;
@@ -790,6 +837,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   CK(s->slot_of.alloc(std::max(s->pool_max, 1)));
   launch_fill_int(s->st, s->hkeys.p, std::max(s->hash_cap, 1), kHashEmpty);
   HIPCK(hipMemsetAsync(s->hstate.p, 0, 2 * (size_t)std::max(s->hash_cap, 1) * sizeof(float4), s->st));
+  s->hage.release();
   // ---- CSR + coloring ----
   CK(s->tc_work.alloc(5 * nbb)); CK(s->asb_start.alloc(nbb + 1)); CK(s->asb_list.alloc(1));
   CK(s->padc.alloc(nbb)); CK(s->cpos.alloc(nbb + 1)); CK(s->pos_of.alloc(nbb));
@@ -816,8 +864,9 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   s->n_active_rows = ns;
   s->pair_count = s->pool_count = s->n_contacts = s->adj_pairs = 0;
   s->last_post = 0;
-  HIPCK(hipStreamSynchronize(s->st));
   s->flushed = true;
+  CK(ensure_hage(s));
+  HIPCK(hipStreamSynchronize(s->st));
   return AVBD_OK;
 }
 
@@ -1093,6 +1142,10 @@ int avbd_import_hash(avbd_solver* s, int n, const int* keys, const float* st8) {
   }
   HIPCK(hipMemcpy(s->hkeys.p, k.data(), hc * sizeof(int), hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(s->hstate.p, v.data(), 2 * (size_t)hc * sizeof(float4), hipMemcpyHostToDevice));
+  if (s->hage.p) {
+    launch_fill_int(s->st, s->hage.p, (int)s->hage.n, 0);
+    HIPCK(hipStreamSynchronize(s->st));
+  }
   return AVBD_OK;
 }
 
@@ -1199,6 +1252,177 @@ int avbd_profile_read(avbd_solver* s, int nc, double* ms, long long* launches, d
     if (launches) launches[k] = s->prof_launch[k];
     if (bytes) bytes[k] = s->prof_bytes[k];
   }
+  return AVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned warm start
+// ---------------------------------------------------------------------------
+int avbd_mlp_num_params(void) { return mlp_num_params(); }
+
+int avbd_ml_set_mode(avbd_solver* s, int mode) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  if (mode < 0 || mode > 2) return set_err(AVBD_E_ARG, "learned mode must be 0, 1 or 2");
+  if (mode == 2 && !s->ml_have_params)
+    return set_err(AVBD_E_STATE, "learned mode 2 needs parameters (avbd_ml_set_params)");
+  HIPCK(hipSetDevice(s->device));
+  s->ml_mode = mode;
+  s->ml_n = 0;
+  CK(ensure_hage(s));
+  HIPCK(hipStreamSynchronize(s->st));
+  return AVBD_OK;
+}
+
+int avbd_ml_set_params(avbd_solver* s, const float* params, int n) {
+  if (!s || !params) return set_err(AVBD_E_ARG, "null argument");
+  if (n != mlp_num_params())
+    return set_err(AVBD_E_ARG, "expected " + std::to_string(mlp_num_params()) + " parameters");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  CK(s->ml_params.ensure(n));
+  HIPCK(hipMemcpy(s->ml_params.p, params, n * sizeof(float), hipMemcpyHostToDevice));
+  s->ml_have_params = true;
+  return AVBD_OK;
+}
+
+int avbd_ml_read_samples(avbd_solver* s, int cap, float* feat, float* targ, int* mask, int* n) {
+  if (!s || !n) return set_err(AVBD_E_ARG, "null argument");
+  if (s->ml_mode != 1) return set_err(AVBD_E_STATE, "samples are logged in learned mode 1 only");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  *n = s->ml_n;
+  const int m = std::min(cap, s->ml_n);
+  if (m <= 0) return AVBD_OK;
+  if (feat) HIPCK(hipMemcpy(feat, s->ml_feat.p, (size_t)m * 24 * sizeof(float), hipMemcpyDeviceToHost));
+  if (targ) HIPCK(hipMemcpy(targ, s->ml_targ.p, (size_t)m * 4 * sizeof(float), hipMemcpyDeviceToHost));
+  if (mask) HIPCK(hipMemcpy(mask, s->ml_mask.p, (size_t)m * sizeof(int), hipMemcpyDeviceToHost));
+  return AVBD_OK;
+}
+
+namespace {
+
+// device, stream and the sample arrays of one standalone network call
+struct MlpJob {
+  hipStream_t st = nullptr;
+  DBuf<float> X, Y, P, G, L;
+  DBuf<int> M, I;
+  ~MlpJob() {
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  }
+  int open(int device, int n, const float* x, const float* y, const int* mask,
+           const float* params) {
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
+      return set_err(AVBD_E_HIP, "no HIP device available");
+    if (device < 0 || device >= nd) return set_err(AVBD_E_ARG, "device ordinal out of range");
+    if (n <= 0 || !x || !params) return set_err(AVBD_E_ARG, "empty sample set or null argument");
+    HIPCK(hipSetDevice(device));
+    HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int np = mlp_num_params();
+    CK(X.alloc((size_t)n * 24));
+    CK(P.alloc(np));
+    HIPCK(hipMemcpy(X.p, x, (size_t)n * 24 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(P.p, params, np * sizeof(float), hipMemcpyHostToDevice));
+    if (y) {
+      CK(Y.alloc((size_t)n * 4));
+      CK(M.alloc(n));
+      HIPCK(hipMemcpy(Y.p, y, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice));
+      if (mask) {
+        HIPCK(hipMemcpy(M.p, mask, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+      } else {
+        launch_fill_int(st, M.p, n, 15);
+      }
+    }
+    return AVBD_OK;
+  }
+};
+
+}  // namespace
+
+int avbd_mlp_forward(int device, int n, const float* x, const float* params, float* y) {
+  if (!y) return set_err(AVBD_E_ARG, "null output");
+  MlpJob j;
+  CK(j.open(device, n, x, nullptr, nullptr, params));
+  CK(j.Y.alloc((size_t)n * 4));
+  launch_mlp_infer(j.st, n, j.X.p, j.P.p, 0, nullptr, nullptr, 1.0f, 1.0f, 1, j.Y.p);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(y, j.Y.p, (size_t)n * 4 * sizeof(float), hipMemcpyDeviceToHost, j.st));
+  HIPCK(hipStreamSynchronize(j.st));
+  return AVBD_OK;
+}
+
+int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* mask,
+                  const float* params, float* grad, float* loss) {
+  if (!y || !grad || !loss) return set_err(AVBD_E_ARG, "null argument");
+  MlpJob j;
+  CK(j.open(device, n, x, y, mask, params));
+  const int nw = mlp_num_weights();
+  CK(j.G.alloc(nw));
+  CK(j.L.alloc(1));
+  launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
+  launch_fill_int(j.st, reinterpret_cast<int*>(j.L.p), 1, 0);
+  launch_mlp_train_tile(j.st, n, nullptr, j.X.p, j.Y.p, j.M.p, j.P.p, 1.0f / (float)n, j.G.p, j.L.p);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(grad, j.G.p, nw * sizeof(float), hipMemcpyDeviceToHost, j.st));
+  HIPCK(hipMemcpyAsync(loss, j.L.p, sizeof(float), hipMemcpyDeviceToHost, j.st));
+  HIPCK(hipStreamSynchronize(j.st));
+  return AVBD_OK;
+}
+
+int avbd_mlp_train(int device, int n, const float* x, const float* y, const int* mask,
+                   float* params, int steps, int batch, float lr, unsigned seed, float* loss_hist) {
+  if (!y || steps < 0 || batch <= 0) return set_err(AVBD_E_ARG, "bad training arguments");
+  MlpJob j;
+  CK(j.open(device, n, x, y, mask, params));
+  const int nw = mlp_num_weights();
+  const int b = std::min(batch, n);
+  // minibatches: consecutive slices of seeded shuffles of 0..n-1
+  std::vector<int> order(n), idx((size_t)std::max(steps, 1) * b);
+  std::iota(order.begin(), order.end(), 0);
+  unsigned long long rng = 0x9E3779B97F4A7C15ull ^ seed;
+  auto next = [&rng]() {  // splitmix64
+    unsigned long long z = (rng += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int used = n;
+  for (size_t k = 0; k < idx.size(); ++k) {
+    if (used == n) {
+      for (int i = n - 1; i > 0; --i) std::swap(order[i], order[next() % (unsigned)(i + 1)]);
+      used = 0;
+    }
+    idx[k] = order[used++];
+  }
+  CK(j.I.alloc(idx.size()));
+  HIPCK(hipMemcpy(j.I.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  DBuf<float> m, v;
+  CK(j.G.alloc(nw));
+  CK(m.alloc(nw));
+  CK(v.alloc(nw));
+  CK(j.L.alloc(std::max(steps, 1)));
+  launch_fill_int(j.st, reinterpret_cast<int*>(m.p), nw, 0);
+  launch_fill_int(j.st, reinterpret_cast<int*>(v.p), nw, 0);
+  launch_fill_int(j.st, reinterpret_cast<int*>(j.L.p), std::max(steps, 1), 0);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double p1 = 1.0, p2 = 1.0;
+  for (int t = 0; t < steps; ++t) {
+    p1 *= b1;
+    p2 *= b2;
+    launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
+    launch_mlp_train_tile(j.st, b, j.I.p + (size_t)t * b, j.X.p, j.Y.p, j.M.p, j.P.p,
+                          1.0f / (float)b, j.G.p, j.L.p + t);
+    launch_adam(j.st, nw, j.P.p, j.G.p, m.p, v.p, lr, b1, b2, (float)(1.0 - p1), (float)(1.0 - p2),
+                eps);
+  }
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(params, j.P.p, mlp_num_params() * sizeof(float), hipMemcpyDeviceToHost, j.st));
+  if (loss_hist && steps > 0)
+    HIPCK(hipMemcpyAsync(loss_hist, j.L.p, steps * sizeof(float), hipMemcpyDeviceToHost, j.st));
+  HIPCK(hipStreamSynchronize(j.st));
   return AVBD_OK;
 }
 

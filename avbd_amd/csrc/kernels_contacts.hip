@@ -776,7 +776,8 @@ k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb
              float mult, const int* __restrict__ rpc, const int* __restrict__ succ,
              const int* __restrict__ dscan, const int* __restrict__ sscan, int n_static_rows,
              int g_static, GroupGeo* __restrict__ geo, GroupSt* __restrict__ gst, int hash_cap,
-             const int* __restrict__ hkeys, const float4* __restrict__ hstate) {
+             const int* __restrict__ hkeys, const float4* __restrict__ hstate,
+             const int* __restrict__ hage) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pairs) return;
   int ns = succ[p];
@@ -800,7 +801,7 @@ k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb
     G.g2 = make_float4(ob.x, ob.y, ob.z, mus);
     float ln = 0.f, lt = 0.f, lb = 0.f, kn = kPenaltyMin, kt = kPenaltyMinTangent,
           kbb = kPenaltyMinTangent;
-    int wt = 0, wb = 0;
+    int wt = 0, wb = 0, age = 0;
     if (hash_cap > 0) {
       int key = encode_pair_key(ka, kb, c);
       int h = hash_pair_slot(key, hash_cap);
@@ -811,6 +812,7 @@ k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb
           float4 s0 = hstate[2 * (size_t)slot], s1 = hstate[2 * (size_t)slot + 1];
           ln = s0.x; lt = s0.y; lb = s0.z; kn = s0.w; kt = s1.x; kbb = s1.y;
           wt = (int)s1.z; wb = (int)s1.w;
+          if (hage) age = hage[slot] + 1;
           break;
         }
         if (sk == kHashEmpty) break;
@@ -825,7 +827,8 @@ k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb
     GroupSt S;
     S.s0 = make_float4(ln, r == 3 ? lt : 0.0f, r == 3 ? lb : 0.0f, 0.0f);
     S.s1 = make_float4(kn, r == 3 ? kt : 0.0f, r == 3 ? kbb : 0.0f, 0.0f);
-    S.s2 = make_float4(0.0f, __int_as_float(bits), 0.0f, 0.0f);
+    // s2.z: substeps this contact has been cached (learned warm start only)
+    S.s2 = make_float4(0.0f, __int_as_float(bits), __int_as_float(age), 0.0f);
     geo[gbase + c] = G;
     gst[gbase + c] = S;
   }
@@ -836,11 +839,11 @@ void launch_emit_write(hipStream_t st, int n_pairs, const int* pa, const int* pb
                        const float* bfric, float default_mu, float mult, const int* rpc,
                        const int* succ, const int* dscan, const int* sscan, int n_static_rows,
                        int g_static, GroupGeo* geo, GroupSt* gst, int hash_cap, const int* hkeys,
-                       const float4* hstate) {
+                       const float4* hstate, const int* hage) {
   if (n_pairs <= 0) return;
   hipLaunchKernelGGL(k_emit_write, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, pa, pb,
                      mf_refa, mf_n, mf_off, bfric, default_mu, mult, rpc, succ, dscan, sscan,
-                     n_static_rows, g_static, geo, gst, hash_cap, hkeys, hstate);
+                     n_static_rows, g_static, geo, gst, hash_cap, hkeys, hstate, hage);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,7 +888,8 @@ k_hash_insert(int n, int g_first, const GroupGeo* __restrict__ geo,
 __global__ void __launch_bounds__(256)
 k_hash_store(int n, int g_first, const GroupGeo* __restrict__ geo,
              const GroupSt* __restrict__ gst, const int* __restrict__ howner,
-             const int* __restrict__ slot_of, float4* __restrict__ hstate) {
+             const int* __restrict__ slot_of, float4* __restrict__ hstate,
+             int* __restrict__ hage) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   int slot = slot_of[e];
@@ -902,16 +906,17 @@ k_hash_store(int n, int g_first, const GroupGeo* __restrict__ geo,
   }
   hstate[2 * (size_t)slot] = make_float4(S.s0.x, lt, lb, S.s1.x);
   hstate[2 * (size_t)slot + 1] = make_float4(kt, kb, wt, wb);
+  if (hage) hage[slot] = __float_as_int(S.s2.z);
 }
 
 void launch_hash_collect(hipStream_t st, int n, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner, int* slot_of,
-                         float4* hstate) {
+                         float4* hstate, int* hage) {
   if (n <= 0 || hash_cap <= 0) return;
   hipLaunchKernelGGL(k_hash_insert, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
                      hash_cap, hkeys, howner, slot_of);
   hipLaunchKernelGGL(k_hash_store, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
-                     howner, slot_of, hstate);
+                     howner, slot_of, hstate, hage);
 }
 
 }  // namespace avbd

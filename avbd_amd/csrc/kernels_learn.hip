@@ -1,0 +1,451 @@
+// kernels_learn.hip — learned warm start of contact lambdas on gfx950.
+//
+// A small MLP predicts the starting dual state (lambda_n, lambda_t,
+// lambda_b, log10 k_n) of each box-box contact group from 24 local
+// features. It only replaces the initial dual state written at emission;
+// the iterations that follow are the plain solver's, so the converged answer
+// is unchanged. No counterpart in the reference (its warm start is the
+// persistent-lambda cache, kernels_6dof.py:2695-2912, which stays the
+// fallback).
+//
+//  k_contact_features   24 features per dynamic group (below)
+//  k_mlp_infer          one wavefront per 16 contacts: three layers as
+//                       v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32
+//                       accumulation), weights staged in LDS, SiLU in fp32
+//  k_mlp_train_tile     forward + backward of 64 samples per workgroup in
+//                       fp32 on the same bf16-rounded weights and layer
+//                       inputs the inference kernel uses; weight gradients
+//                       summed per workgroup, then one atomic add each
+//  k_adam               Adam step on the fp32 master weights
+//
+// Network: x (24) -> 64 SiLU -> 64 SiLU -> 4. Parameters (fp32, in order):
+// W1 [24][64], b1 [64], W2 [64][64], b2 [64], W3 [64][4], b3 [4], then the
+// normalisation x_mean [24], x_std [24], y_mean [4], y_std [4].
+//
+// Features of group g (bodies a = reference, b = incident; r_a = R_a off_a):
+//   0-2  contact normal n (world)      3-5  off_a (body a frame)
+//   6-8  off_b (body b frame)          9    C0 = n . (w_a - w_b)
+//   10   v_rel . n   11 v_rel . t   12 v_rel . b   (v_rel = velocity of the
+//        contact point on a minus on b; t, b the Duff basis of n)
+//   13   1 / m_a     14  1 / m_b     15-17 slog(cached lambda n, t, b)
+//   18   contact age (substeps it has been cached; 0 = new)
+//   19-21 log10 cached k n, t, b     22 friction mu     23 1
+// Outputs are corrections to the plain warm start, so that a zero output
+// reproduces it exactly: with ag = alpha * gamma (the decay the prelude
+// applies),
+//   y_r = slog(lambda_r at the end of the substep) - slog(ag * cached lambda_r)
+//   y_3 = log10(k_n at the end of the substep) - log10(clamp(gamma * cached k_n))
+// slog(x) = sign(x) log(1 + |x|). Tangent outputs count only on contacts with
+// friction rows (mask 0b1111, else 0b1001).
+#include "avbd_internal.h"
+
+namespace avbd {
+
+__device__ __forceinline__ v3 f3(float4 a) { return mk3(a.x, a.y, a.z); }
+__device__ __forceinline__ q4 f4q(float4 a) { return mk4(a.x, a.y, a.z, a.w); }
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ float slog(float x) {
+  return x < 0.0f ? -log1pf(-x) : log1pf(x);
+}
+__device__ __forceinline__ float islog(float y) { return y < 0.0f ? -expm1f(-y) : expm1f(y); }
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+// round to bfloat16 (nearest even), kept in fp32
+__device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
+
+// ---------------------------------------------------------------------------
+// features
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_contact_features(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+                   const GroupSt* __restrict__ gst, const float4* __restrict__ pose,
+                   const float4* __restrict__ vel, const float4* __restrict__ bstat,
+                   float* __restrict__ feat) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  const GroupGeo G = geo[g_first + e];
+  const GroupSt S = gst[g_first + e];
+  const int a = G.hdr.y, b = G.hdr.z;
+  const v3 n = f3(G.g0), offa = f3(G.g1), offb = f3(G.g2);
+  const v3 xa = f3(pose[2 * a]), xb = f3(pose[2 * b]);
+  const q4 qa = f4q(pose[2 * a + 1]), qb = f4q(pose[2 * b + 1]);
+  const v3 ra = qrotate(qa, offa), rb = qrotate(qb, offb);
+  const v3 d = sub(add(xa, ra), add(xb, rb));
+  const v3 va = add(f3(vel[2 * a]), cross(f3(vel[2 * a + 1]), ra));
+  const v3 vb = add(f3(vel[2 * b]), cross(f3(vel[2 * b + 1]), rb));
+  const v3 vr = sub(va, vb);
+  const v3 t = orthonormal_basis(n), bt = cross(n, t);
+  const float ma = bstat[a].x, mb = bstat[b].x;
+  float* f = feat + (size_t)e * 24;
+  f[0] = n.x; f[1] = n.y; f[2] = n.z;
+  f[3] = offa.x; f[4] = offa.y; f[5] = offa.z;
+  f[6] = offb.x; f[7] = offb.y; f[8] = offb.z;
+  f[9] = dot(n, d);
+  f[10] = dot(n, vr); f[11] = dot(t, vr); f[12] = dot(bt, vr);
+  f[13] = ma > 0.0f ? 1.0f / ma : 0.0f;
+  f[14] = mb > 0.0f ? 1.0f / mb : 0.0f;
+  f[15] = slog(S.s0.x); f[16] = slog(S.s0.y); f[17] = slog(S.s0.z);
+  f[18] = (float)__float_as_int(S.s2.z);
+  f[19] = log10f(fmaxf(S.s1.x, 1.0f)); f[20] = log10f(fmaxf(S.s1.y, 1.0f));
+  f[21] = log10f(fmaxf(S.s1.z, 1.0f));
+  f[22] = G.g1.w;
+  f[23] = 1.0f;
+}
+
+void launch_contact_features(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                             const GroupSt* gst, const float4* pose, const float4* vel,
+                             const float4* bstat, float* feat) {
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_contact_features, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
+                       g_first, geo, gst, pose, vel, bstat, feat);
+}
+
+// The plain warm start of a group, from its features (cached state):
+// base[0..2] = slog(ag * lambda), base[3] = log10 of the decayed, clamped k_n.
+__device__ __forceinline__ void plain_start(const float* f, float ag, float gamma, float base[4]) {
+  for (int r = 0; r < 3; ++r) base[r] = slog(islog(f[15 + r]) * ag);
+  const float k = fminf(fmaxf(exp10f(f[19]) * gamma, kPenaltyMin), kPenaltyMax);
+  base[3] = log10f(k);
+}
+
+// Targets of the logged samples: the groups' dual state at the end of the
+// substep, as corrections to the plain warm start.
+__global__ void __launch_bounds__(256)
+k_contact_targets(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+                  const GroupSt* __restrict__ gst, const float* __restrict__ feat, float ag,
+                  float gamma, float* __restrict__ targ, int* __restrict__ ymask) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_dyn) return;
+  const GroupSt S = gst[g_first + e];
+  float base[4];
+  plain_start(feat + (size_t)e * 24, ag, gamma, base);
+  float* y = targ + (size_t)e * 4;
+  y[0] = slog(S.s0.x) - base[0];
+  y[1] = slog(S.s0.y) - base[1];
+  y[2] = slog(S.s0.z) - base[2];
+  y[3] = log10f(fmaxf(S.s1.x, 1.0f)) - base[3];
+  ymask[e] = g_nrows(geo[g_first + e].hdr.x) == 3 ? 15 : 9;
+}
+
+void launch_contact_targets(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+                            const GroupSt* gst, const float* feat, float ag, float gamma,
+                            float* targ, int* ymask) {
+  if (n_dyn > 0)
+    hipLaunchKernelGGL(k_contact_targets, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
+                       g_first, geo, gst, feat, ag, gamma, targ, ymask);
+}
+
+// ---------------------------------------------------------------------------
+// inference: MFMA bf16
+// ---------------------------------------------------------------------------
+constexpr int kIn = 24, kHid = 64, kOut = 4;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kIn * kHid, kOffW2 = kOffB1 + kHid,
+              kOffB2 = kOffW2 + kHid * kHid, kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kOut,
+              kNumWeights = kOffB3 + kOut, kOffXm = kNumWeights, kOffXs = kOffXm + kIn,
+              kOffYm = kOffXs + kIn, kOffYs = kOffYm + kOut, kNumParams = kOffYs + kOut;
+
+int mlp_num_weights() { return kNumWeights; }
+int mlp_num_params() { return kNumParams; }
+
+// 4 wavefronts x 16 contacts per tile of 64; a workgroup stages the weights
+// in LDS once (bf16, laid out as the MFMA B operands read them) and walks
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+//   mode 0: apply the correction to the groups' initial dual state (warm
+//           start); a non-finite output leaves the plain warm start
+//   mode 1: write the 4 network outputs (de-normalised) to `yout`
+__global__ void __launch_bounds__(256)
+k_mlp_infer(int n, const float* __restrict__ feat, const float* __restrict__ params,
+            int g_first, GroupSt* __restrict__ gst, const GroupGeo* __restrict__ geo, float ag,
+            float gamma, int mode, float* __restrict__ yout) {
+  __shared__ __bf16 w1[32][64];   // rows 24-31 zero (K padding)
+  __shared__ __bf16 w2[64][64];
+  __shared__ __bf16 w3[64][16];   // columns 4-15 zero
+  __shared__ float bias[64 + 64 + 16];
+  __shared__ float norm[kIn * 2 + kOut * 2];
+  __shared__ __bf16 act[4][16][64 + 8];  // per wavefront: layer input (bf16)
+  const int tid = threadIdx.x, wave = tid / 64, l = tid % 64;
+  for (int k = tid; k < 32 * 64; k += 256) {
+    const int r = k / 64, c = k % 64;
+    w1[r][c] = (__bf16)(r < kIn ? params[kOffW1 + r * kHid + c] : 0.0f);
+  }
+  for (int k = tid; k < 64 * 64; k += 256) w2[k / 64][k % 64] = (__bf16)params[kOffW2 + k];
+  for (int k = tid; k < 64 * 16; k += 256) {
+    const int r = k / 16, c = k % 16;
+    w3[r][c] = (__bf16)(c < kOut ? params[kOffW3 + r * kOut + c] : 0.0f);
+  }
+  for (int k = tid; k < 144; k += 256)
+    bias[k] = k < 64 ? params[kOffB1 + k] : k < 128 ? params[kOffB2 + k - 64]
+                                                    : (k - 128 < kOut ? params[kOffB3 + k - 128] : 0.f);
+  for (int k = tid; k < kIn * 2 + kOut * 2; k += 256) norm[k] = params[kOffXm + k];
+  const int n_tiles = (n + 63) / 64;
+  const int row = l % 16, kq = l / 16;  // A row / B column within a tile; K quarter
+  // The trip count is the same for every wavefront of the workgroup, so all
+  // reach every barrier; rows past the last sample compute on zeros and
+  // store nothing.
+  for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    __syncthreads();  // weights staged / previous tile's reads of act done
+    const int s0 = (tile * 4 + wave) * 16;
+    // layer input: normalised features, bf16, K padded to 32
+    for (int k = l; k < 16 * 32; k += 64) {
+      const int s = k / 32, c = k % 32;
+      float v = 0.0f;
+      if (c < kIn && s0 + s < n) v = (feat[(size_t)(s0 + s) * kIn + c] - norm[c]) / norm[kIn + c];
+      act[wave][s][c] = (__bf16)v;
+    }
+    __syncthreads();
+    floatx4 h[4];
+    // ---- layer 1: [16 x 32] x [32 x 64] ----
+    {
+      bf16x8 a;
+      for (int j = 0; j < 8; ++j) a[j] = act[wave][row][8 * kq + j];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 b;
+        for (int j = 0; j < 8; ++j) b[j] = w1[8 * kq + j][t * 16 + row];
+        floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    // SiLU, back to LDS as the next layer's input: D[4*kq + r][t*16 + row]
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      for (int r = 0; r < 4; ++r) {
+        const int c = t * 16 + row;
+        act[wave][4 * kq + r][c] = (__bf16)silu(h[t][r] + bias[c]);
+      }
+    __syncthreads();
+    // ---- layer 2: [16 x 64] x [64 x 64] ----
+    floatx4 h2[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f},
+                     {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a;
+      for (int j = 0; j < 8; ++j) a[j] = act[wave][row][32 * ks + 8 * kq + j];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 b;
+        for (int j = 0; j < 8; ++j) b[j] = w2[32 * ks + 8 * kq + j][t * 16 + row];
+        h2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, h2[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      for (int r = 0; r < 4; ++r) {
+        const int c = t * 16 + row;
+        act[wave][4 * kq + r][c] = (__bf16)silu(h2[t][r] + bias[64 + c]);
+      }
+    __syncthreads();
+    // ---- layer 3: [16 x 64] x [64 x 16] (4 used) ----
+    floatx4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = act[wave][row][32 * ks + 8 * kq + j];
+        b[j] = w3[32 * ks + 8 * kq + j][row];
+      }
+      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, y, 0, 0, 0);
+    }
+    // lane holds outputs (sample 4*kq + r, column row)
+    if (row < kOut) {
+      for (int r = 0; r < 4; ++r) {
+        const int s = s0 + 4 * kq + r;
+        if (s >= n) continue;
+        const float v = (y[r] + bias[128 + row]) * norm[2 * kIn + kOut + row] + norm[2 * kIn + row];
+        if (mode == 1) {
+          yout[(size_t)s * kOut + row] = v;
+          continue;
+        }
+        // zero or non-finite: keep the plain warm start as it is
+        if (v == 0.0f || !(fabsf(v) < 1.0e30f)) continue;
+        const float* f = feat + (size_t)s * kIn;
+        GroupSt* S = gst + g_first + s;
+        const int nr = g_nrows(geo[g_first + s].hdr.x);
+        if (row == 3) {
+          // start at 10^(base + v) after the prelude multiplies k by gamma
+          const float kp = fminf(fmaxf(exp10f(f[19]) * gamma, kPenaltyMin), kPenaltyMax);
+          const float lk = fminf(fmaxf(log10f(kp) + v, 6.0f), 9.0f);
+          S->s1.x = exp10f(lk) / gamma;
+        } else if (row == 0 || nr == 3) {
+          // start at islog(base + v) after the prelude multiplies lambda by ag
+          float lam = islog(slog(islog(f[15 + row]) * ag) + v) / ag;
+          if (row == 0) lam = fminf(lam, 0.0f);  // the normal force is compressive (<= 0)
+          if (row == 0) S->s0.x = lam;
+          else if (row == 1) S->s0.y = lam;
+          else S->s0.z = lam;
+        }
+      }
+    }
+  }
+}
+
+// tiles per workgroup: enough workgroups to fill 256 CUs, each staging the
+// 14 KB of weights once for several tiles
+void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* params, int g_first,
+                      GroupSt* gst, const GroupGeo* geo, float ag, float gamma, int mode,
+                      float* yout) {
+  if (n <= 0) return;
+  const int tiles = (n + 63) / 64;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(std::min(tiles, 1024)), dim3(256), 0, st, n, feat, params,
+                     g_first, gst, geo, ag, gamma, mode, yout);
+}
+
+// ---------------------------------------------------------------------------
+// training: fp32 forward/backward on bf16-rounded weights and layer inputs
+// ---------------------------------------------------------------------------
+// One workgroup (256 lanes) per tile of 64 samples; sample s of the batch is
+// row idx[s] of X, Y, ymask (row s when idx is null). Loss = sum over samples
+// and used outputs of (yhat - y)^2 * inv_total, y normalised; mask bit o of
+// ymask says whether output o is used.
+__global__ void __launch_bounds__(256)
+k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X,
+                 const float* __restrict__ Y, const int* __restrict__ ymask,
+                 const float* __restrict__ params, float inv_total, float* __restrict__ grad,
+                 float* __restrict__ loss) {
+  __shared__ float W1[kIn * kHid], W2[kHid * kHid], W3[kHid * kOut];
+  __shared__ float B1[kHid], B2[kHid], B3[kOut];
+  __shared__ float xin[64][kIn];
+  __shared__ float z1[64][kHid], a1[64][kHid], z2[64][kHid], a2[64][kHid];
+  __shared__ float dy[64][kOut];
+  __shared__ float lsum;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < kIn * kHid; k += 256) W1[k] = bf16r(params[kOffW1 + k]);
+  for (int k = tid; k < kHid * kHid; k += 256) W2[k] = bf16r(params[kOffW2 + k]);
+  for (int k = tid; k < kHid * kOut; k += 256) W3[k] = bf16r(params[kOffW3 + k]);
+  for (int k = tid; k < kHid; k += 256) { B1[k] = params[kOffB1 + k]; B2[k] = params[kOffB2 + k]; }
+  if (tid < kOut) B3[tid] = params[kOffB3 + tid];
+  if (tid == 0) lsum = 0.0f;
+  const int s0 = blockIdx.x * 64;
+  __shared__ int rows[64];
+  if (tid < 64) rows[tid] = s0 + tid < n ? (idx ? idx[s0 + tid] : s0 + tid) : -1;
+  __syncthreads();
+  for (int k = tid; k < 64 * kIn; k += 256) {
+    const int s = k / kIn, c = k % kIn;
+    float v = 0.0f;
+    if (rows[s] >= 0) v = (X[(size_t)rows[s] * kIn + c] - params[kOffXm + c]) / params[kOffXs + c];
+    xin[s][c] = bf16r(v);
+  }
+  __syncthreads();
+  // forward
+  for (int k = tid; k < 64 * kHid; k += 256) {
+    const int s = k / kHid, u = k % kHid;
+    float z = 0.0f;
+    for (int i = 0; i < kIn; ++i) z += xin[s][i] * W1[i * kHid + u];
+    z += B1[u];
+    z1[s][u] = z;
+    a1[s][u] = bf16r(silu(z));
+  }
+  __syncthreads();
+  for (int k = tid; k < 64 * kHid; k += 256) {
+    const int s = k / kHid, u = k % kHid;
+    float z = 0.0f;
+    for (int i = 0; i < kHid; ++i) z += a1[s][i] * W2[i * kHid + u];
+    z += B2[u];
+    z2[s][u] = z;
+    a2[s][u] = bf16r(silu(z));
+  }
+  __syncthreads();
+  for (int k = tid; k < 64 * kOut; k += 256) {
+    const int s = k / kOut, o = k % kOut;
+    float d = 0.0f;
+    if (rows[s] >= 0 && (ymask[rows[s]] >> o & 1)) {
+      float yh = B3[o];
+      for (int i = 0; i < kHid; ++i) yh += a2[s][i] * W3[i * kOut + o];
+      const float yt = (Y[(size_t)rows[s] * kOut + o] - params[kOffYm + o]) / params[kOffYs + o];
+      d = yh - yt;
+      atomicAdd(&lsum, d * d * inv_total);
+    }
+    dy[s][o] = 2.0f * d * inv_total;
+  }
+  __syncthreads();
+  // backward: grads of layer 3
+  for (int k = tid; k < kHid * kOut; k += 256) {
+    const int i = k / kOut, o = k % kOut;
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += a2[s][i] * dy[s][o];
+    atomicAdd(&grad[kOffW3 + k], g);
+  }
+  if (tid < kOut) {
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += dy[s][tid];
+    atomicAdd(&grad[kOffB3 + tid], g);
+  }
+  // dz2 (reuse z2)
+  for (int k = tid; k < 64 * kHid; k += 256) {
+    const int s = k / kHid, u = k % kHid;
+    float g = 0.0f;
+    for (int o = 0; o < kOut; ++o) g += dy[s][o] * W3[u * kOut + o];
+    z2[s][u] = g * silu_grad(z2[s][u]);
+  }
+  __syncthreads();
+  for (int k = tid; k < kHid * kHid; k += 256) {
+    const int i = k / kHid, u = k % kHid;
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += a1[s][i] * z2[s][u];
+    atomicAdd(&grad[kOffW2 + k], g);
+  }
+  for (int k = tid; k < kHid; k += 256) {
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += z2[s][k];
+    atomicAdd(&grad[kOffB2 + k], g);
+  }
+  // dz1 (reuse z1)
+  for (int k = tid; k < 64 * kHid; k += 256) {
+    const int s = k / kHid, u = k % kHid;
+    float g = 0.0f;
+    for (int v = 0; v < kHid; ++v) g += z2[s][v] * W2[u * kHid + v];
+    z1[s][u] = g * silu_grad(z1[s][u]);
+  }
+  __syncthreads();
+  for (int k = tid; k < kIn * kHid; k += 256) {
+    const int i = k / kHid, u = k % kHid;
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += xin[s][i] * z1[s][u];
+    atomicAdd(&grad[kOffW1 + k], g);
+  }
+  for (int k = tid; k < kHid; k += 256) {
+    float g = 0.0f;
+    for (int s = 0; s < 64; ++s) g += z1[s][k];
+    atomicAdd(&grad[kOffB1 + k], g);
+  }
+  __syncthreads();
+  if (tid == 0) atomicAdd(loss, lsum);
+}
+
+__global__ void __launch_bounds__(256)
+k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+       float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
+                           const int* ymask, const float* params, float inv_total, float* grad,
+                           float* loss) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_mlp_train_tile, dim3((n + 63) / 64), dim3(256), 0, st, n, idx, X, Y,
+                       ymask, params, inv_total, grad, loss);
+}
+
+void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,
+                 float b1, float b2, float c1, float c2, float eps) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, n, w, g, m, v, lr, b1, b2,
+                       c1, c2, eps);
+}
+
+}  // namespace avbd

@@ -272,6 +272,8 @@ class Solver6DOF:
         self._graph_signature: tuple | None = None
         self._graph_supported: bool | None = None
         self._flush_snapshot: dict | None = None
+        self._warm_start_mode = 0          # learned warm start: 0 plain, 1 log, 2 learned
+        self._warm_start_params: np.ndarray | None = None
 
     # ---- scene building (solver_6dof.py:446-578) ----------------------------
     def add_box(self, position, half_extents, mass: float, orientation=(0.0, 0.0, 0.0, 1.0),
@@ -545,6 +547,8 @@ class Solver6DOF:
         sc, keep = self.make_scene_struct(arrs)
         self._h.flush(sc)
         del keep
+        if self._warm_start_mode or self._warm_start_params is not None:
+            self._apply_warm_start()
         self._flush_snapshot = arrs
         n_b = arrs["n_bodies"]
         n_static = arrs["n_static_rows"]
@@ -624,12 +628,14 @@ class Solver6DOF:
         if fixed_cap:
             self._check_fixed_cap_overflow()
 
-    def step_to_residual(self, tol: float = 1e-4, max_iters: int = 200) -> list[tuple[int, float]]:
+    def step_to_residual(self, tol: float = 1e-4, max_iters: int = 200,
+                         on_substep=None) -> list[tuple[int, float]]:
         """One frame like step(), each substep iterated until the dual
         residual r = max over active rows of |lambda_new - lambda| / k (metres)
         is <= tol, at most `max_iters` iterations. Not in the reference: the
         "iterations to residual" measurement. Returns (iterations, r) per
-        substep; a substep equals a fixed-iteration one with that count."""
+        substep; a substep equals a fixed-iteration one with that count.
+        `on_substep(k)`, if given, runs after substep k."""
         if self._dirty:
             self._flush()
         n_b = len(self._x)
@@ -644,7 +650,39 @@ class Solver6DOF:
             rep, iters, r = self._h.substep_to_residual(params, self._color_dirty, tol)
             self._apply_report(rep, n_b)
             out.append((iters, r))
+            if on_substep is not None:
+                on_substep(len(out) - 1)
         return out
+
+    # ---- learned warm start (not in the reference) ---------------------------
+    def set_learned_warm_start(self, mode: str = "plain", params=None) -> None:
+        """Initial dual state of box-box contacts: "plain" = the persistent-
+        lambda cache (the reference's warm start), "log" = plain, and each
+        substep's contact features and converged dual state are kept for
+        learned_samples(), "learned" = the network with `params` (layout of
+        avbd_amd.learned) corrects the plain warm start."""
+        from .learned import MODES
+        if mode not in MODES:
+            raise ValueError(f"unknown warm-start mode {mode!r}; expected one of {tuple(MODES)}")
+        if params is not None:
+            self._warm_start_params = np.ascontiguousarray(params, np.float32)
+        if mode == "learned" and self._warm_start_params is None:
+            raise ValueError("the learned warm start needs network parameters")
+        self._warm_start_mode = MODES[mode]
+        if self._h is not None and not self._dirty:
+            self._apply_warm_start()
+
+    def _apply_warm_start(self) -> None:
+        if self._warm_start_params is not None:
+            self._h.ml_set_params(self._warm_start_params)
+        self._h.ml_set_mode(self._warm_start_mode)
+
+    def learned_samples(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """The last substep's logged samples (mode "log"): features [n, 24],
+        targets [n, 4] and output masks [n]."""
+        if self._h is None:
+            raise RuntimeError("no substep has run")
+        return self._h.ml_read_samples()
 
     def _check_fixed_cap_overflow(self) -> None:
         """Once-per-frame overflow warnings of the fixed-capacity path

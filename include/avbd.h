@@ -216,6 +216,42 @@ int avbd_profile_enable(avbd_solver* s, int on);
 int avbd_profile_read(avbd_solver* s, int n_classes, double* ms, long long* launches,
                       double* bytes);
 
+/* ---- learned warm start ----------------------------------------------
+ * No counterpart in the reference: its warm start is the persistent-lambda
+ * cache alone (kernels_6dof.py:2695-2912), which stays the fallback. A
+ * 24 -> 64 -> 64 -> 4 MLP (SiLU, bf16 weights on MFMA, fp32 accumulation)
+ * predicts, per box-box contact group, corrections to the plain warm start
+ * of lambda (n, t, b) and k_n; it changes only the initial dual state of a
+ * substep, never the iterations after it.
+ * Parameters (avbd_mlp_num_params() floats): W1[24][64], b1[64], W2[64][64],
+ * b2[64], W3[64][4], b3[4], x_mean[24], x_std[24], y_mean[4], y_std[4].
+ * Sample layout: 24 features, 4 targets and one output mask per contact
+ * group (bit o set = output o is used; tangents only with friction rows). */
+int avbd_mlp_num_params(void);
+/* mode 0 = off (plain warm start), 1 = log samples (features at emission,
+ * targets at the end of each substep), 2 = apply the network (needs
+ * avbd_ml_set_params first). Modes 1 and 2 also track contact ages. */
+int avbd_ml_set_mode(avbd_solver* s, int mode);
+int avbd_ml_set_params(avbd_solver* s, const float* params, int n);
+/* The last substep's logged samples (mode 1): copies min(cap, *n) of them;
+ * *n = how many there are. feat [*n][24], targ [*n][4], mask [*n]. */
+int avbd_ml_read_samples(avbd_solver* s, int cap, float* feat, float* targ, int* mask, int* n);
+/* Standalone network calls on host arrays (device `device`):
+ * forward: y [n][4] = de-normalised outputs for x [n][24];
+ * grad: gradient of the masked mean squared error (normalised targets,
+ *   summed over used outputs, averaged over samples) w.r.t. the first
+ *   avbd_mlp_num_params() - 56 parameters (the weights; the normalisation
+ *   is fixed), and the loss;
+ * train: `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of
+ *   `batch` samples drawn from seeded shuffles; params updated in place,
+ *   loss_hist [steps] = each minibatch's loss (may be NULL). */
+int avbd_mlp_forward(int device, int n, const float* x, const float* params, float* y);
+int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* mask,
+                  const float* params, float* grad, float* loss);
+int avbd_mlp_train(int device, int n, const float* x, const float* y, const int* mask,
+                   float* params, int steps, int batch, float lr, unsigned seed,
+                   float* loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

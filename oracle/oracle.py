@@ -36,7 +36,8 @@ def load() -> C.CDLL:
                                                                         "pack_state",
                                                                         "set_option",
                                                                         "profile_enable",
-                                                                        "profile_read")])
+                                                                        "profile_read")
+                                   and k not in N.ML_NAMES])
         lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.orc_create.restype = C.c_int
         lib.orc_set_capacities.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]

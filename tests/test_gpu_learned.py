@@ -1,0 +1,132 @@
+"""Learned warm start on the GPU: the MFMA inference and the training
+kernel against the numpy reference of the network (avbd_amd.learned), and
+the solver-side guarantees (a zero network is the plain warm start bit for
+bit; logging does not perturb the trajectory)."""
+import numpy as np
+import pytest
+
+from avbd_amd import learned as L
+from avbd_amd import scenes
+from avbd_amd.solver_6dof import Solver6DOF
+from tests.parity import assert_bit_exact, diff_report, one_substep
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed=0):
+    rng = np.random.default_rng(seed)
+    X = (rng.normal(0.0, 1.0, (n, L.N_IN)) * rng.uniform(0.1, 5.0, L.N_IN)
+         + rng.normal(0.0, 2.0, L.N_IN)).astype(np.float32)
+    Y = rng.normal(0.0, 0.5, (n, L.N_OUT)).astype(np.float32)
+    M = rng.choice([9, 15], size=n).astype(np.int32)
+    return X, Y, M
+
+
+def _params(X, Y, M, seed=1):
+    p = L.init_params(*L.normalisation(X, Y, M), seed=seed)
+    P = L.unpack(p)
+    rng = np.random.default_rng(seed + 7)
+    P["W3"][:] = rng.normal(0.0, 0.3, P["W3"].shape)
+    for k in ("b1", "b2", "b3"):
+        P[k][:] = rng.normal(0.0, 0.1, P[k].shape)
+    return p
+
+
+@pytest.mark.parametrize("n", [1, 63, 1037, 200_003])
+def test_mfma_forward_matches_numpy(n):
+    """bf16 operands, fp32 accumulation: the kernel and the reference round
+    the same values to bf16; they may differ where an fp32 sum lands next to
+    a bf16 rounding boundary. Tolerance (outputs in units of y_std): 99.9 %
+    within 1e-4, all within 2e-2. n = 200,003 walks every workgroup over
+    several tiles and ends on a ragged one."""
+    X, Y, M = _data(n, seed=n)
+    p = _params(X, Y, M)
+    ys = L.unpack(p)["y_std"]
+    got = L.forward_gpu(p, X)
+    ref, _ = L.forward_np(p, X)
+    err = np.abs(got - ref) / ys
+    print(n, "max", err.max(), "q999", np.quantile(err, 0.999), flush=True)
+    assert np.quantile(err, 0.999) <= 1e-4
+    assert err.max() <= 2e-2
+
+
+@pytest.mark.parametrize("n", [64, 777])
+def test_training_gradient_matches_numpy(n):
+    """Loss and weight gradient of the training kernel against the numpy
+    backward on the same bf16-rounded weights and layer inputs: loss within
+    1e-4 relative, gradient within 1e-3 relative L2 and 1e-2 per block."""
+    X, Y, M = _data(n, seed=n + 1)
+    p = _params(X, Y, M, seed=3)
+    loss, g = L.grad_gpu(p, X, Y, M)
+    loss_ref, g_ref = L.loss_grad_np(p, X, Y, M)
+    print(n, loss, loss_ref, np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref), flush=True)
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    assert np.linalg.norm(g - g_ref) <= 1e-3 * np.linalg.norm(g_ref)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        o, s = L._OFF[k]
+        a, b = g[o:o + int(np.prod(s))], g_ref[o:o + int(np.prod(s))]
+        assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-9, k
+
+
+def test_training_fits_a_smooth_target():
+    """Adam on the GPU fits a smooth function of the features: the held-out
+    loss falls below a tenth of the starting one."""
+    rng = np.random.default_rng(11)
+    X = rng.normal(0.0, 1.0, (20_000, L.N_IN)).astype(np.float32)
+    Y = np.stack([np.tanh(X[:, 0] + 0.5 * X[:, 1]), np.sin(X[:, 2]) * X[:, 3],
+                  0.3 * X[:, 4] ** 2, X[:, 5] - X[:, 6]], axis=1).astype(np.float32)
+    M = np.full(len(X), 15, np.int32)
+    tr, te = slice(0, 16_000), slice(16_000, None)
+    p0 = L.init_params(*L.normalisation(X[tr], Y[tr], M[tr]), seed=2)
+    l0 = L.loss_grad_np(p0, X[te], Y[te], M[te])[0]
+    p1, hist = L.train_gpu(p0, X[tr], Y[tr], M[tr], steps=1500, batch=512, lr=2e-3, seed=4)
+    l1 = L.loss_grad_np(p1, X[te], Y[te], M[te])[0]
+    print("held-out loss", l0, "->", l1, "minibatch", hist[:3], hist[-3:], flush=True)
+    assert np.all(np.isfinite(hist)) and l1 < 0.1 * l0
+    # the normalisation is not trained
+    assert np.array_equal(p1[L.N_WEIGHTS:], p0[L.N_WEIGHTS:])
+
+
+def _zero_params():
+    X, Y, M = _data(256)
+    p = L.init_params(*L.normalisation(X, Y, M))
+    p[:L.N_WEIGHTS] = 0.0
+    return p
+
+
+@pytest.mark.parametrize("mode", ["learned", "log"])
+def test_zero_network_and_logging_keep_the_plain_trajectory(mode):
+    """A network that outputs zero leaves the plain warm start untouched,
+    and logging only reads: both trajectories equal the plain one bit for
+    bit (pyramid of 55 cubes, 24 substeps)."""
+    a = scenes.pyramid(5, seed=13, solver_cls=Solver6DOF)
+    b = scenes.pyramid(5, seed=13, solver_cls=Solver6DOF)
+    a._flush()
+    b.set_learned_warm_start(mode, _zero_params())
+    b._flush()
+    for k in range(24):
+        one_substep(a)
+        one_substep(b)
+        assert_bit_exact(diff_report(a, b), f"{mode} substep {k}")
+
+
+def test_logged_samples_of_a_resting_stack():
+    """Four 1 m cubes at rest: every contact group is a face-face point with
+    normal +-y, body offsets on the cubes' horizontal faces, inverse masses 1/1000, the
+    friction coefficient 0.5, friction rows (mask 15), and an age that counts
+    the substeps the contact has persisted."""
+    s = scenes.cube_stack(4, seed=0, solver_cls=Solver6DOF)
+    s.set_learned_warm_start("log")
+    ages = []
+    for k in range(12):
+        one_substep(s)
+        X, Y, M = s.learned_samples()
+        assert len(X) >= 12 and np.all(M == 15)
+        assert np.allclose(np.abs(X[:, 1]), 1.0, atol=1e-2)
+        # face-face: both offsets on a horizontal face of their cube
+        assert np.allclose(np.abs(X[:, [4, 7]]), 0.5, atol=2e-2)
+        assert np.all(np.abs(X[:, 3:9]) <= 0.52)
+        assert np.allclose(X[:, 13:15], 1e-3) and np.allclose(X[:, 22], 0.5)
+        assert np.all(X[:, 23] == 1.0) and np.all(np.isfinite(Y))
+        ages.append(X[:, 18].max())
+    assert ages[-1] >= 8 and ages == sorted(ages)
