@@ -130,9 +130,9 @@ SIGNATURES = {
     "ml_set_params": ([C.c_void_p, F32P, C.c_int], C.c_int),
     "ml_read_samples": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P], C.c_int),
     "mlp_forward": ([C.c_int, C.c_int, F32P, F32P, F32P], C.c_int),
-    "mlp_grad": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, F32P, F32P], C.c_int),
+    "mlp_grad": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_float, F32P, F32P], C.c_int),
     "mlp_train": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_int, C.c_int, C.c_float,
-                   C.c_uint, F32P], C.c_int),
+                   C.c_float, C.c_uint, F32P], C.c_int),
 }
 
 # entry points of the learned warm start (the checker has none)

@@ -163,8 +163,8 @@ void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* par
                       GroupSt* gst, const GroupGeo* geo, float ag, float gamma, int mode,
                       float* yout);
 void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
-                           const int* ymask, const float* params, float inv_total, float* grad,
-                           float* loss);
+                           const int* ymask, const float* params, float inv_total, float huber,
+                           float* grad, float* loss);
 void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,
                  float b1, float b2, float c1, float c2, float eps);
 

@@ -432,17 +432,17 @@ int ensure_hage(avbd_solver* s) {
   return AVBD_OK;
 }
 
-// Features of the n emitted groups (bodies at the start of the substep) and,
-// in mode 2, the network's initial dual state for them.
+// Features of the first n groups (static, then emitted; bodies at the start
+// of the substep) and, in mode 2, the network's initial dual state for them.
 int ml_begin(avbd_solver* s, const avbd_params* p, int n) {
   CK(s->ml_feat.ensure((size_t)n * 24));
   CK(s->ml_targ.ensure((size_t)n * 4));
   CK(s->ml_mask.ensure(n));
   CK(prof_begin(s, kOther));
-  launch_contact_features(s->st, n, s->g_static, s->geo.p, s->gst.p, s->pose[s->cur].p, s->vel.p,
+  launch_contact_features(s->st, n, 0, s->geo.p, s->gst.p, s->pose[s->cur].p, s->vel.p,
                           s->bstat.p, s->ml_feat.p);
   if (s->ml_mode == 2)
-    launch_mlp_infer(s->st, n, s->ml_feat.p, s->ml_params.p, s->g_static, s->gst.p, s->geo.p,
+    launch_mlp_infer(s->st, n, s->ml_feat.p, s->ml_params.p, 0, s->gst.p, s->geo.p,
                      p->alpha * p->gamma, p->gamma, 0, nullptr);
   CK(prof_end(s, (double)n * (96.0 + 48.0 + 96.0 + (s->ml_mode == 2 ? 96.0 + 48.0 : 0.0)),
               s->ml_mode == 2 ? 2 : 1));
@@ -484,7 +484,9 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   // learned warm start: features of the emitted contact groups, then either
   // the network's initial dual state (mode 2) or nothing more until the
   // targets at the end of the substep (mode 1)
-  const int n_ml = p->self_collide && s->ml_mode != 0 ? std::min(s->n_contacts, s->pool_max) : 0;
+  // groups: static (floor, pins) first, then this substep's box-box contacts
+  const int n_ml = s->ml_mode == 0 ? 0
+                   : s->g_static + (p->self_collide ? std::min(s->n_contacts, s->pool_max) : 0);
   s->ml_n = 0;
   if (n_ml > 0) CK(ml_begin(s, p, n_ml));
   rep->n_active_rows = s->n_active_rows;
@@ -590,7 +592,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   }
   if (n_ml > 0 && s->ml_mode == 1) {
     CK(prof_begin(s, kOther));
-    launch_contact_targets(st, n_ml, s->g_static, s->geo.p, s->gst.p, s->ml_feat.p,
+    launch_contact_targets(st, n_ml, 0, s->geo.p, s->gst.p, s->ml_feat.p,
                            p->alpha * p->gamma, p->gamma, s->ml_targ.p, s->ml_mask.p);
     CK(prof_end(s, (double)n_ml * (96.0 + 48.0 + 16.0 + 20.0)));
     s->ml_n = n_ml;
@@ -1355,7 +1357,7 @@ int avbd_mlp_forward(int device, int n, const float* x, const float* params, flo
 }
 
 int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* mask,
-                  const float* params, float* grad, float* loss) {
+                  const float* params, float huber, float* grad, float* loss) {
   if (!y || !grad || !loss) return set_err(AVBD_E_ARG, "null argument");
   MlpJob j;
   CK(j.open(device, n, x, y, mask, params));
@@ -1364,7 +1366,8 @@ int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* 
   CK(j.L.alloc(1));
   launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
   launch_fill_int(j.st, reinterpret_cast<int*>(j.L.p), 1, 0);
-  launch_mlp_train_tile(j.st, n, nullptr, j.X.p, j.Y.p, j.M.p, j.P.p, 1.0f / (float)n, j.G.p, j.L.p);
+  launch_mlp_train_tile(j.st, n, nullptr, j.X.p, j.Y.p, j.M.p, j.P.p, 1.0f / (float)n, huber, j.G.p,
+                        j.L.p);
   HIPCK(hipGetLastError());
   HIPCK(hipMemcpyAsync(grad, j.G.p, nw * sizeof(float), hipMemcpyDeviceToHost, j.st));
   HIPCK(hipMemcpyAsync(loss, j.L.p, sizeof(float), hipMemcpyDeviceToHost, j.st));
@@ -1373,7 +1376,8 @@ int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* 
 }
 
 int avbd_mlp_train(int device, int n, const float* x, const float* y, const int* mask,
-                   float* params, int steps, int batch, float lr, unsigned seed, float* loss_hist) {
+                   float* params, int steps, int batch, float lr, float huber, unsigned seed,
+                   float* loss_hist) {
   if (!y || steps < 0 || batch <= 0) return set_err(AVBD_E_ARG, "bad training arguments");
   MlpJob j;
   CK(j.open(device, n, x, y, mask, params));
@@ -1414,7 +1418,7 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
     p2 *= b2;
     launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
     launch_mlp_train_tile(j.st, b, j.I.p + (size_t)t * b, j.X.p, j.Y.p, j.M.p, j.P.p,
-                          1.0f / (float)b, j.G.p, j.L.p + t);
+                          1.0f / (float)b, huber, j.G.p, j.L.p + t);
     launch_adam(j.st, nw, j.P.p, j.G.p, m.p, v.p, lr, b1, b2, (float)(1.0 - p1), (float)(1.0 - p2),
                 eps);
   }

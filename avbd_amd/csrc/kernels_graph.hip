@@ -206,8 +206,9 @@ static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned*
     int* tv = va; va = vb; vb = tv;
   }
   if (ka != keys) {
-    hipMemcpyAsync(keys, ka, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st);
-    hipMemcpyAsync(vals, va, n * sizeof(int), hipMemcpyDeviceToDevice, st);
+    // errors surface at the caller's hipGetLastError / synchronisation
+    (void)hipMemcpyAsync(keys, ka, n * sizeof(unsigned), hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(vals, va, n * sizeof(int), hipMemcpyDeviceToDevice, st);
   }
 }
 

@@ -1,8 +1,8 @@
 // kernels_learn.hip — learned warm start of contact lambdas on gfx950.
 //
 // A small MLP predicts the starting dual state (lambda_n, lambda_t,
-// lambda_b, log10 k_n) of each box-box contact group from 24 local
-// features. It only replaces the initial dual state written at emission;
+// lambda_b, log10 k_n) of each box-box and floor contact group from 24
+// local features. It only replaces the initial dual state written at emission;
 // the iterations that follow are the plain solver's, so the converged answer
 // is unchanged. No counterpart in the reference (its warm start is the
 // persistent-lambda cache, kernels_6dof.py:2695-2912, which stays the
@@ -22,17 +22,22 @@
 // W1 [24][64], b1 [64], W2 [64][64], b2 [64], W3 [64][4], b3 [4], then the
 // normalisation x_mean [24], x_std [24], y_mean [4], y_std [4].
 //
-// Features of group g (bodies a = reference, b = incident; r_a = R_a off_a):
-//   0-2  contact normal n (world)      3-5  off_a (body a frame)
-//   6-8  off_b (body b frame)          9    C0 = n . (w_a - w_b)
-//   10   v_rel . n   11 v_rel . t   12 v_rel . b   (v_rel = velocity of the
-//        contact point on a minus on b; t, b the Duff basis of n)
-//   13   1 / m_a     14  1 / m_b     15-17 slog(cached lambda n, t, b)
-//   18   contact age (substeps it has been cached; 0 = new)
-//   19-21 log10 cached k n, t, b     22 friction mu     23 1
-// Outputs are corrections to the plain warm start, so that a zero output
-// reproduces it exactly: with ag = alpha * gamma (the decay the prelude
-// applies),
+// Features of a box-box group (bodies a = reference, b = incident;
+// r_a = R_a off_a) or a floor group (body a, the floor as b):
+//   0-2  contact normal n (world; +y on the floor)
+//   3-5  off_a (body a frame)          6-8  off_b (body b frame; 0 on the floor)
+//   9    C0 = n . (w_a - w_b) (on the floor: corner height above it)
+//   10-12 v_rel along the rows' directions (n, t, b: the Duff basis of n for
+//        box-box, world x and z on the floor); v_rel = velocity of the
+//        contact point on a minus on b
+//   13   1 / m_a     14  1 / m_b (0 on the floor)
+//   15-17 slog(lambda n, t, b at the start of the substep)
+//   18   contact age (substeps it has been cached; 0 = new or floor)
+//   19-21 log10 k n, t, b            22 friction mu
+//   23   kind: 0 box-box, 1 floor, -1 pin (pins are left to the plain start)
+// Outputs are corrections to the plain warm start (the cached or persistent
+// state the prelude decays), so that a zero output reproduces it exactly:
+// with ag = alpha * gamma (the decay the prelude applies),
 //   y_r = slog(lambda_r at the end of the substep) - slog(ag * cached lambda_r)
 //   y_3 = log10(k_n at the end of the substep) - log10(clamp(gamma * cached k_n))
 // slog(x) = sign(x) log(1 + |x|). Tangent outputs count only on contacts with
@@ -63,47 +68,74 @@ __device__ __forceinline__ float bf16r(float x) { return (float)(__bf16)x; }
 // features
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_contact_features(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+k_contact_features(int n_groups, int g_first, const GroupGeo* __restrict__ geo,
                    const GroupSt* __restrict__ gst, const float4* __restrict__ pose,
                    const float4* __restrict__ vel, const float4* __restrict__ bstat,
                    float* __restrict__ feat) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_dyn) return;
+  if (e >= n_groups) return;
   const GroupGeo G = geo[g_first + e];
   const GroupSt S = gst[g_first + e];
-  const int a = G.hdr.y, b = G.hdr.z;
-  const v3 n = f3(G.g0), offa = f3(G.g1), offb = f3(G.g2);
-  const v3 xa = f3(pose[2 * a]), xb = f3(pose[2 * b]);
-  const q4 qa = f4q(pose[2 * a + 1]), qb = f4q(pose[2 * b + 1]);
-  const v3 ra = qrotate(qa, offa), rb = qrotate(qb, offb);
-  const v3 d = sub(add(xa, ra), add(xb, rb));
-  const v3 va = add(f3(vel[2 * a]), cross(f3(vel[2 * a + 1]), ra));
-  const v3 vb = add(f3(vel[2 * b]), cross(f3(vel[2 * b + 1]), rb));
-  const v3 vr = sub(va, vb);
-  const v3 t = orthonormal_basis(n), bt = cross(n, t);
-  const float ma = bstat[a].x, mb = bstat[b].x;
+  const int kind = g_kind(G.hdr.x);
   float* f = feat + (size_t)e * 24;
+  if (kind == kGPin) {  // not predicted: all features zero, kind -1
+    for (int k = 0; k < 23; ++k) f[k] = 0.0f;
+    f[23] = -1.0f;
+    return;
+  }
+  const int a = G.hdr.y;
+  const v3 offa = f3(G.g1);
+  const v3 xa = f3(pose[2 * a]);
+  const q4 qa = f4q(pose[2 * a + 1]);
+  const v3 ra = qrotate(qa, offa);
+  const v3 va = add(f3(vel[2 * a]), cross(f3(vel[2 * a + 1]), ra));
+  const float ma = bstat[a].x;
+  v3 n, offb, vb_n;  // vb_n: the relative velocity in the rows' (n, t, b) basis
+  float C0, inv_mb;
+  if (kind == kGFloor) {
+    // rows: +y normal, world x and z tangents; the floor point is G.g0
+    n = mk3(0.0f, 1.0f, 0.0f);
+    offb = mk3(0.0f, 0.0f, 0.0f);
+    C0 = xa.y + ra.y - G.g0.y;
+    vb_n = mk3(va.y, va.x, va.z);
+    inv_mb = 0.0f;
+  } else {
+    const int b = G.hdr.z;
+    n = f3(G.g0);
+    offb = f3(G.g2);
+    const v3 xb = f3(pose[2 * b]);
+    const q4 qb = f4q(pose[2 * b + 1]);
+    const v3 rb = qrotate(qb, offb);
+    const v3 d = sub(add(xa, ra), add(xb, rb));
+    const v3 vb = add(f3(vel[2 * b]), cross(f3(vel[2 * b + 1]), rb));
+    const v3 vr = sub(va, vb);
+    const v3 t = orthonormal_basis(n), bt = cross(n, t);
+    C0 = dot(n, d);
+    vb_n = mk3(dot(n, vr), dot(t, vr), dot(bt, vr));
+    const float mb = bstat[b].x;
+    inv_mb = mb > 0.0f ? 1.0f / mb : 0.0f;
+  }
   f[0] = n.x; f[1] = n.y; f[2] = n.z;
   f[3] = offa.x; f[4] = offa.y; f[5] = offa.z;
   f[6] = offb.x; f[7] = offb.y; f[8] = offb.z;
-  f[9] = dot(n, d);
-  f[10] = dot(n, vr); f[11] = dot(t, vr); f[12] = dot(bt, vr);
+  f[9] = C0;
+  f[10] = vb_n.x; f[11] = vb_n.y; f[12] = vb_n.z;
   f[13] = ma > 0.0f ? 1.0f / ma : 0.0f;
-  f[14] = mb > 0.0f ? 1.0f / mb : 0.0f;
+  f[14] = inv_mb;
   f[15] = slog(S.s0.x); f[16] = slog(S.s0.y); f[17] = slog(S.s0.z);
   f[18] = (float)__float_as_int(S.s2.z);
   f[19] = log10f(fmaxf(S.s1.x, 1.0f)); f[20] = log10f(fmaxf(S.s1.y, 1.0f));
   f[21] = log10f(fmaxf(S.s1.z, 1.0f));
   f[22] = G.g1.w;
-  f[23] = 1.0f;
+  f[23] = kind == kGFloor ? 1.0f : 0.0f;
 }
 
-void launch_contact_features(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+void launch_contact_features(hipStream_t st, int n, int g_first, const GroupGeo* geo,
                              const GroupSt* gst, const float4* pose, const float4* vel,
                              const float4* bstat, float* feat) {
-  if (n_dyn > 0)
-    hipLaunchKernelGGL(k_contact_features, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
-                       g_first, geo, gst, pose, vel, bstat, feat);
+  if (n > 0)
+    hipLaunchKernelGGL(k_contact_features, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first,
+                       geo, gst, pose, vel, bstat, feat);
 }
 
 // The plain warm start of a group, from its features (cached state):
@@ -117,11 +149,11 @@ __device__ __forceinline__ void plain_start(const float* f, float ag, float gamm
 // Targets of the logged samples: the groups' dual state at the end of the
 // substep, as corrections to the plain warm start.
 __global__ void __launch_bounds__(256)
-k_contact_targets(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+k_contact_targets(int n, int g_first, const GroupGeo* __restrict__ geo,
                   const GroupSt* __restrict__ gst, const float* __restrict__ feat, float ag,
                   float gamma, float* __restrict__ targ, int* __restrict__ ymask) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_dyn) return;
+  if (e >= n) return;
   const GroupSt S = gst[g_first + e];
   float base[4];
   plain_start(feat + (size_t)e * 24, ag, gamma, base);
@@ -130,14 +162,15 @@ k_contact_targets(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
   y[1] = slog(S.s0.y) - base[1];
   y[2] = slog(S.s0.z) - base[2];
   y[3] = log10f(fmaxf(S.s1.x, 1.0f)) - base[3];
-  ymask[e] = g_nrows(geo[g_first + e].hdr.x) == 3 ? 15 : 9;
+  const int h = geo[g_first + e].hdr.x;
+  ymask[e] = g_kind(h) == kGPin ? 0 : (g_nrows(h) == 3 ? 15 : 9);
 }
 
-void launch_contact_targets(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
+void launch_contact_targets(hipStream_t st, int n, int g_first, const GroupGeo* geo,
                             const GroupSt* gst, const float* feat, float ag, float gamma,
                             float* targ, int* ymask) {
-  if (n_dyn > 0)
-    hipLaunchKernelGGL(k_contact_targets, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
+  if (n > 0)
+    hipLaunchKernelGGL(k_contact_targets, dim3((n + 255) / 256), dim3(256), 0, st, n,
                        g_first, geo, gst, feat, ag, gamma, targ, ymask);
 }
 
@@ -268,7 +301,9 @@ k_mlp_infer(int n, const float* __restrict__ feat, const float* __restrict__ par
         if (v == 0.0f || !(fabsf(v) < 1.0e30f)) continue;
         const float* f = feat + (size_t)s * kIn;
         GroupSt* S = gst + g_first + s;
-        const int nr = g_nrows(geo[g_first + s].hdr.x);
+        const int h = geo[g_first + s].hdr.x;
+        if (g_kind(h) == kGPin) continue;
+        const int nr = g_nrows(h);
         if (row == 3) {
           // start at 10^(base + v) after the prelude multiplies k by gamma
           const float kp = fminf(fmaxf(exp10f(f[19]) * gamma, kPenaltyMin), kPenaltyMax);
@@ -303,12 +338,15 @@ void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* par
 // ---------------------------------------------------------------------------
 // One workgroup (256 lanes) per tile of 64 samples; sample s of the batch is
 // row idx[s] of X, Y, ymask (row s when idx is null). Loss = sum over samples
-// and used outputs of (yhat - y)^2 * inv_total, y normalised; mask bit o of
-// ymask says whether output o is used.
+// and used outputs of h(yhat - y) * inv_total, y normalised, with
+// h(d) = d^2 (huber <= 0) or the Huber loss d^2 for |d| <= huber and
+// huber (2|d| - huber) beyond; mask bit o of ymask says whether output o is
+// used.
 __global__ void __launch_bounds__(256)
 k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X,
                  const float* __restrict__ Y, const int* __restrict__ ymask,
-                 const float* __restrict__ params, float inv_total, float* __restrict__ grad,
+                 const float* __restrict__ params, float inv_total, float huber,
+                 float* __restrict__ grad,
                  float* __restrict__ loss) {
   __shared__ float W1[kIn * kHid], W2[kHid * kHid], W3[kHid * kOut];
   __shared__ float B1[kHid], B2[kHid], B3[kOut];
@@ -361,7 +399,12 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
       for (int i = 0; i < kHid; ++i) yh += a2[s][i] * W3[i * kOut + o];
       const float yt = (Y[(size_t)rows[s] * kOut + o] - params[kOffYm + o]) / params[kOffYs + o];
       d = yh - yt;
-      atomicAdd(&lsum, d * d * inv_total);
+      if (huber > 0.0f && fabsf(d) > huber) {
+        atomicAdd(&lsum, huber * (2.0f * fabsf(d) - huber) * inv_total);
+        d = copysignf(huber, d);
+      } else {
+        atomicAdd(&lsum, d * d * inv_total);
+      }
     }
     dy[s][o] = 2.0f * d * inv_total;
   }
@@ -434,11 +477,11 @@ k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restr
 }
 
 void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
-                           const int* ymask, const float* params, float inv_total, float* grad,
-                           float* loss) {
+                           const int* ymask, const float* params, float inv_total, float huber,
+                           float* grad, float* loss) {
   if (n > 0)
     hipLaunchKernelGGL(k_mlp_train_tile, dim3((n + 63) / 64), dim3(256), 0, st, n, idx, X, Y,
-                       ymask, params, inv_total, grad, loss);
+                       ymask, params, inv_total, huber, grad, loss);
 }
 
 void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,

@@ -1,9 +1,10 @@
 """Learned warm start of contact lambdas on the MI355X solver.
 
 A 24 -> 64 -> 64 -> 4 MLP (SiLU; bf16 weights and layer inputs on MFMA,
-fp32 accumulation) predicts, for every box-box contact group at emission,
-corrections to the plain warm start (the persistent-lambda cache decayed by
-alpha * gamma, kernels_6dof.py:2695-2912 and solver_6dof.py:1318-1330):
+fp32 accumulation) predicts, for every box-box contact group at emission and
+every floor contact group, corrections to the plain warm start (the cached
+or persistent state decayed by alpha * gamma, kernels_6dof.py:2695-2912 and
+the substep prelude, kernels_6dof.py:3140-3260):
 
     y_r = slog(lambda_r converged) - slog(alpha gamma lambda_r cached), r = n, t, b
     y_3 = log10(k_n converged)     - log10(clamp(gamma k_n cached))
@@ -52,7 +53,7 @@ del _o, _name, _shape
 FEATURE_NAMES = (
     "n_x", "n_y", "n_z", "offa_x", "offa_y", "offa_z", "offb_x", "offb_y", "offb_z", "C0",
     "vrel_n", "vrel_t", "vrel_b", "inv_mass_a", "inv_mass_b", "slog_lam_n", "slog_lam_t",
-    "slog_lam_b", "age", "log10_k_n", "log10_k_t", "log10_k_b", "mu", "one")
+    "slog_lam_b", "age", "log10_k_n", "log10_k_t", "log10_k_b", "mu", "kind")
 MODES = {"plain": 0, "log": 1, "learned": 2}
 WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "warm_start_mlp.npz"
 
@@ -164,11 +165,13 @@ def forward_np(params: np.ndarray, X: np.ndarray, bf16: bool = True):
 
 
 def loss_grad_np(params: np.ndarray, X: np.ndarray, Y: np.ndarray, mask: np.ndarray,
-                 bf16: bool = True):
-    """Masked mean squared error on normalised targets (summed over the used
-    outputs, averaged over samples) and its gradient w.r.t. the weights
-    (N_WEIGHTS values, parameter order). With bf16 the roundings are treated
-    as identities in the backward pass, as the training kernel does."""
+                 bf16: bool = True, huber: float = 0.0):
+    """Masked loss on normalised targets (summed over the used outputs,
+    averaged over samples): squared error d^2, or with huber > 0 the Huber
+    loss (d^2 for |d| <= huber, huber (2|d| - huber) beyond), and its
+    gradient w.r.t. the weights (N_WEIGHTS values, parameter order). With
+    bf16 the roundings are treated as identities in the backward pass, as the
+    training kernel does."""
     P = unpack(params)
     n = len(X)
     _, c = forward_np(params, X, bf16)
@@ -178,7 +181,12 @@ def loss_grad_np(params: np.ndarray, X: np.ndarray, Y: np.ndarray, mask: np.ndar
         yt = (np.asarray(Y, np.float64) - P["y_mean"]) / P["y_std"]
     m = np.stack([(np.asarray(mask) >> o & 1) for o in range(N_OUT)], axis=1).astype(np.float64)
     d = (c["yn"] - yt) * m
-    loss = float(np.sum(d * d) / n)
+    if huber > 0:
+        big = np.abs(d) > huber
+        loss = float((np.sum(np.where(big, huber * (2 * np.abs(d) - huber), d * d))) / n)
+        d = np.clip(d, -huber, huber)
+    else:
+        loss = float(np.sum(d * d) / n)
     dy = 2.0 * d / n
     g = {}
     g["W3"] = c["a2"].T @ dy
@@ -234,26 +242,26 @@ def forward_gpu(params: np.ndarray, X: np.ndarray, device: int = 0) -> np.ndarra
     return y
 
 
-def grad_gpu(params, X, Y, mask=None, device: int = 0):
+def grad_gpu(params, X, Y, mask=None, huber: float = 0.0, device: int = 0):
     X, Y, p = _f32(X, N_IN), _f32(Y, N_OUT), _f32(params)
     m = _mask(mask, len(X))
     g = np.zeros(N_WEIGHTS, np.float32)
     loss = np.zeros(1, np.float32)
     _check(_lib().avbd_mlp_grad(int(device), len(X), N.fptr(X), N.fptr(Y), N.iptr(m), N.fptr(p),
-                                N.fptr(g), N.fptr(loss)))
+                                C.c_float(huber), N.fptr(g), N.fptr(loss)))
     return float(loss[0]), g
 
 
 def train_gpu(params, X, Y, mask=None, steps: int = 1000, batch: int = 4096, lr: float = 1e-3,
-              seed: int = 0, device: int = 0):
+              seed: int = 0, huber: float = 0.0, device: int = 0):
     """`steps` Adam steps on the GPU; returns (new params, per-step loss)."""
     X, Y = _f32(X, N_IN), _f32(Y, N_OUT)
     p = np.array(params, np.float32, copy=True)
     m = _mask(mask, len(X))
     hist = np.zeros(max(int(steps), 1), np.float32)
     _check(_lib().avbd_mlp_train(int(device), len(X), N.fptr(X), N.fptr(Y), N.iptr(m), N.fptr(p),
-                                 int(steps), int(batch), C.c_float(lr), C.c_uint(seed & 0xFFFFFFFF),
-                                 N.fptr(hist)))
+                                 int(steps), int(batch), C.c_float(lr), C.c_float(huber),
+                                 C.c_uint(seed & 0xFFFFFFFF), N.fptr(hist)))
     return p, hist[:int(steps)]
 
 
@@ -314,6 +322,18 @@ def iterations_to_residual(solver, frames: int, mode: str = "plain", params=None
     for _ in range(frames):
         its.extend(i for i, _r in solver.step_to_residual(tol, max_iters))
     return np.asarray(its, np.int64)
+
+
+def undo_decay_params(like: np.ndarray, alpha: float = 0.99, gamma: float = 0.99) -> np.ndarray:
+    """A constant network: every normal-row correction is log(alpha gamma),
+    i.e. the start is the cached lambda without the prelude's decay (to
+    first order in 1/|lambda|). The trivial rule a learned start has to
+    beat."""
+    p = np.array(like, np.float32, copy=True)
+    p[:N_WEIGHTS] = 0.0
+    P = unpack(p)
+    P["b3"][0] = np.log(alpha * gamma) / P["y_std"][0] - P["y_mean"][0] / P["y_std"][0]
+    return p
 
 
 def compare(builder, frames: int, params, tol: float = 1e-4, max_iters: int = 200) -> dict:

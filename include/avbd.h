@@ -238,8 +238,9 @@ int avbd_ml_set_params(avbd_solver* s, const float* params, int n);
 int avbd_ml_read_samples(avbd_solver* s, int cap, float* feat, float* targ, int* mask, int* n);
 /* Standalone network calls on host arrays (device `device`):
  * forward: y [n][4] = de-normalised outputs for x [n][24];
- * grad: gradient of the masked mean squared error (normalised targets,
- *   summed over used outputs, averaged over samples) w.r.t. the first
+ * grad: gradient of the masked loss (normalised targets, summed over used
+ *   outputs, averaged over samples; squared error, or the Huber loss with
+ *   threshold `huber` when huber > 0) w.r.t. the first
  *   avbd_mlp_num_params() - 56 parameters (the weights; the normalisation
  *   is fixed), and the loss;
  * train: `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of
@@ -247,9 +248,9 @@ int avbd_ml_read_samples(avbd_solver* s, int cap, float* feat, float* targ, int*
  *   loss_hist [steps] = each minibatch's loss (may be NULL). */
 int avbd_mlp_forward(int device, int n, const float* x, const float* params, float* y);
 int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* mask,
-                  const float* params, float* grad, float* loss);
+                  const float* params, float huber, float* grad, float* loss);
 int avbd_mlp_train(int device, int n, const float* x, const float* y, const int* mask,
-                   float* params, int steps, int batch, float lr, unsigned seed,
+                   float* params, int steps, int batch, float lr, float huber, unsigned seed,
                    float* loss_hist);
 
 #ifdef __cplusplus

@@ -53,12 +53,14 @@ def test_mfma_forward_matches_numpy(n):
 @pytest.mark.parametrize("n", [64, 777])
 def test_training_gradient_matches_numpy(n):
     """Loss and weight gradient of the training kernel against the numpy
-    backward on the same bf16-rounded weights and layer inputs: loss within
+    backward on the same bf16-rounded weights and layer inputs (squared
+    error, and the Huber loss on the ragged case): loss within
     1e-4 relative, gradient within 1e-3 relative L2 and 1e-2 per block."""
     X, Y, M = _data(n, seed=n + 1)
     p = _params(X, Y, M, seed=3)
-    loss, g = L.grad_gpu(p, X, Y, M)
-    loss_ref, g_ref = L.loss_grad_np(p, X, Y, M)
+    huber = 0.0 if n == 64 else 0.5   # the ragged case with the Huber loss
+    loss, g = L.grad_gpu(p, X, Y, M, huber=huber)
+    loss_ref, g_ref = L.loss_grad_np(p, X, Y, M, huber=huber)
     print(n, loss, loss_ref, np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref), flush=True)
     assert abs(loss - loss_ref) <= 1e-4 * loss_ref
     assert np.linalg.norm(g - g_ref) <= 1e-3 * np.linalg.norm(g_ref)
@@ -111,22 +113,34 @@ def test_zero_network_and_logging_keep_the_plain_trajectory(mode):
 
 
 def test_logged_samples_of_a_resting_stack():
-    """Four 1 m cubes at rest: every contact group is a face-face point with
-    normal +-y, body offsets on the cubes' horizontal faces, inverse masses 1/1000, the
-    friction coefficient 0.5, friction rows (mask 15), and an age that counts
-    the substeps the contact has persisted."""
+    """Four 1 m cubes at rest: the 32 floor corner groups (normal +y, offsets
+    at the corners, the four bottom ones on the floor), then one sample per
+    box-box contact group (3 rows each with friction, all with mask 15);
+    normals +-y, body offsets on the
+    cubes' horizontal faces, inverse masses 1/1000, friction coefficient 0.5;
+    the age counts the substeps a contact has persisted."""
     s = scenes.cube_stack(4, seed=0, solver_cls=Solver6DOF)
     s.set_learned_warm_start("log")
-    ages = []
+    ages, counts = [], []
     for k in range(12):
         one_substep(s)
         X, Y, M = s.learned_samples()
-        assert len(X) >= 12 and np.all(M == 15)
+        # 8 floor corner groups per cube first, then the box-box groups
+        assert len(X) > 32 and np.all(X[:32, 23] == 1.0) and np.all(X[32:, 23] == 0.0)
+        floor = X[:32]
+        assert np.array_equal(floor[:, :3], np.tile([0.0, 1.0, 0.0], (32, 1)).astype(np.float32))
+        assert np.allclose(np.abs(floor[:, 3:6]), 0.5) and np.all(floor[:, 6:9] == 0)
+        assert np.allclose(np.sort(floor[:, 9])[:4], 0.0, atol=2e-3)  # bottom corners on the floor
+        X, Y, M = X[32:], Y[32:], M[32:]
+        live = min(s._h.read_counters()[0], s._gpu_pool_n_capacity)
+        dyn_rows = live - s._gpu_pool_n_static
+        counts.append(len(X))
+        print(k, len(X), dyn_rows, X[:, 18].tolist(), flush=True)
+        assert dyn_rows == 3 * len(X) and len(X) > 0 and np.all(M == 15)
         assert np.allclose(np.abs(X[:, 1]), 1.0, atol=1e-2)
-        # face-face: both offsets on a horizontal face of their cube
         assert np.allclose(np.abs(X[:, [4, 7]]), 0.5, atol=2e-2)
         assert np.all(np.abs(X[:, 3:9]) <= 0.52)
         assert np.allclose(X[:, 13:15], 1e-3) and np.allclose(X[:, 22], 0.5)
-        assert np.all(X[:, 23] == 1.0) and np.all(np.isfinite(Y))
+        assert np.all(np.isfinite(Y))
         ages.append(X[:, 18].max())
-    assert ages[-1] >= 8 and ages == sorted(ages)
+    assert ages == sorted(ages) and ages[-1] >= 8, (ages, counts)

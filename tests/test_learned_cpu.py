@@ -117,3 +117,30 @@ def test_params_round_trip(tmp_path):
     assert np.array_equal(L.load_params(f), p)
     with pytest.raises(ValueError):
         L.unpack(p[:-1])
+
+
+def test_huber_backward_matches_finite_differences():
+    """Huber loss (threshold 0.3 on normalised errors): gradient against
+    central differences, and the loss equals the squared error where every
+    error is under the threshold."""
+    X, Y, M = _data(48, seed=4)
+    p = _params(X, Y, M, seed=2).astype(np.float64)
+    loss, g = L.loss_grad_np(p, X, Y, M, bf16=False, huber=0.3)
+    rng = np.random.default_rng(6)
+    d = np.zeros_like(p)
+    d[:L.N_WEIGHTS] = rng.normal(0, 1, L.N_WEIGHTS)
+    eps = 1e-6
+    fd = (L.loss_grad_np(p + eps * d, X, Y, M, bf16=False, huber=0.3)[0]
+          - L.loss_grad_np(p - eps * d, X, Y, M, bf16=False, huber=0.3)[0]) / (2 * eps)
+    assert abs(fd - g @ d[:L.N_WEIGHTS]) <= 1e-5 * max(1.0, abs(fd))
+    big = L.loss_grad_np(p, X, Y, M, bf16=False, huber=1e6)
+    mse = L.loss_grad_np(p, X, Y, M, bf16=False)
+    assert big[0] == pytest.approx(mse[0], rel=1e-12) and np.allclose(big[1], mse[1], rtol=1e-12)
+    assert loss < mse[0]
+
+
+def test_undo_decay_network_outputs_log_alpha_gamma():
+    X, Y, M = _data()
+    p = L.undo_decay_params(_params(X, Y, M))
+    y, _ = L.forward_np(p, X, bf16=False)
+    assert np.allclose(y[:, 0], np.log(0.99 * 0.99), rtol=1e-6) and np.all(y[:, 1:] == 0)
