@@ -183,6 +183,11 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kIn * kHid, kOffW2 = kOffB1 + kHid,
               kNumWeights = kOffB3 + kOut, kOffXm = kNumWeights, kOffXs = kOffXm + kIn,
               kOffYm = kOffXs + kIn, kOffYs = kOffYm + kOut, kNumParams = kOffYs + kOut;
 
+// Largest correction applied (slog units for lambda: a factor e^4 ~ 55;
+// decades for k); a larger output is taken as out of the training
+// distribution and the plain warm start is kept.
+constexpr float kMaxCorrection = 4.0f;
+
 int mlp_num_weights() { return kNumWeights; }
 int mlp_num_params() { return kNumParams; }
 
@@ -297,8 +302,9 @@ k_mlp_infer(int n, const float* __restrict__ feat, const float* __restrict__ par
           yout[(size_t)s * kOut + row] = v;
           continue;
         }
-        // zero or non-finite: keep the plain warm start as it is
-        if (v == 0.0f || !(fabsf(v) < 1.0e30f)) continue;
+        // zero, non-finite or beyond the trusted range: keep the plain warm
+        // start as it is
+        if (v == 0.0f || !(fabsf(v) <= kMaxCorrection)) continue;
         const float* f = feat + (size_t)s * kIn;
         GroupSt* S = gst + g_first + s;
         const int h = geo[g_first + s].hdr.x;

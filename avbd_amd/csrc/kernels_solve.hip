@@ -467,8 +467,10 @@ __device__ __forceinline__ void dual_row(float C, bool hard, float stiff, float&
   if (hard) C = C - ac0;
   float lam_eff = hard ? lam : 0.0f;
   float nl = clampf_(kp * C + lam_eff, lo, hi);
-  // residual of this row: |lambda_new - lambda| / k with lambda, k before the update
-  const float r = fabs_(nl - lam) / kp;
+  // residual of this row: |lambda_new - lambda| / k with lambda, k before the
+  // update; a non-finite state counts as an infinite residual
+  float r = fabs_(nl - lam) / kp;
+  r = r != r ? kInf : r;
   rmax = r > rmax ? r : rmax;
   lam = nl;
   if (fabs_(nl) >= fracture) {

@@ -33,13 +33,16 @@ def _log(msg: str) -> None:
 
 
 def gather(args, rng) -> L.Trace:
-    """Traces of piles (n log-uniform in [min, max]) and of resting pyramids."""
+    """Traces of piles (n log-uniform in [min, max]), resting pyramids and
+    cube stacks."""
     xs, ys, ms = [], [], []
     sizes = np.exp(rng.uniform(np.log(args.min_boxes), np.log(args.max_boxes), args.train_piles))
     jobs = [(f"pile {n}", lambda n=int(n), k=k: L.pile(n, seed=1000 + k), args.frames)
             for k, n in enumerate(np.round(sizes).astype(int))]
     jobs += [(f"pyramid {lay}", lambda lay=lay, k=k: scenes.pyramid(lay, seed=2000 + k),
               args.pyramid_frames) for k, lay in enumerate(args.train_pyramids)]
+    jobs += [(f"stack {h}", lambda h=h, k=k: scenes.cube_stack(h, seed=3000 + k),
+              args.pyramid_frames) for k, h in enumerate(args.train_stacks)]
     for k, (name, build, frames) in enumerate(jobs):
         t0 = time.perf_counter()
         tr = _log_capped(build(), args, frames, seed=k)
@@ -148,6 +151,7 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--frames", type=int, default=120)
     p.add_argument("--keep-cap", type=int, default=500)
     p.add_argument("--train-pyramids", type=int, nargs="*", default=[6, 8, 10, 12, 14])
+    p.add_argument("--train-stacks", type=int, nargs="*", default=[3, 5, 8])
     p.add_argument("--pyramid-frames", type=int, default=60)
     p.add_argument("--huber", type=float, default=0.0,
                    help="Huber threshold on normalised errors (0 = squared error)")

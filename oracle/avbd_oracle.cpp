@@ -467,7 +467,8 @@ float dual(Orc& S, int n_rows) {
       hi = bound;
     }
     float nl = clampf_(S.cpen[j] * C + lam_eff, lo, hi);
-    const float r = fabs_(nl - S.clam[j]) / S.cpen[j];
+    float r = fabs_(nl - S.clam[j]) / S.cpen[j];
+    r = r != r ? kInf : r;  // a non-finite state counts as an infinite residual
     rmax = r > rmax ? r : rmax;
     S.clam[j] = nl;
     if (fabs_(nl) >= S.cfrac[j]) {

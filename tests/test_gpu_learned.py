@@ -96,15 +96,26 @@ def _zero_params():
     return p
 
 
-@pytest.mark.parametrize("mode", ["learned", "log"])
+def _out_of_range_params():
+    """Every output far beyond the trusted correction range."""
+    p = _zero_params()
+    L.unpack(p)["b3"][:] = 1e4
+    return p
+
+
+@pytest.mark.parametrize("mode", ["learned", "log", "out-of-range"])
 def test_zero_network_and_logging_keep_the_plain_trajectory(mode):
     """A network that outputs zero leaves the plain warm start untouched,
-    and logging only reads: both trajectories equal the plain one bit for
-    bit (pyramid of 55 cubes, 24 substeps)."""
+    one whose outputs are all out of range falls back to it, and logging only
+    reads: each trajectory equals the plain one bit for bit (pyramid of 55
+    cubes, 24 substeps)."""
     a = scenes.pyramid(5, seed=13, solver_cls=Solver6DOF)
     b = scenes.pyramid(5, seed=13, solver_cls=Solver6DOF)
     a._flush()
-    b.set_learned_warm_start(mode, _zero_params())
+    if mode == "out-of-range":
+        b.set_learned_warm_start("learned", _out_of_range_params())
+    else:
+        b.set_learned_warm_start(mode, _zero_params())
     b._flush()
     for k in range(24):
         one_substep(a)
@@ -144,3 +155,25 @@ def test_logged_samples_of_a_resting_stack():
         assert np.all(np.isfinite(Y))
         ages.append(X[:, 18].max())
     assert ages == sorted(ages) and ages[-1] >= 8, (ages, counts)
+
+
+def test_shipped_network_keeps_the_converged_answer():
+    """The committed network on a resting pyramid of 91 cubes, every substep
+    iterated to the residual 1e-4 (cap 200): states stay finite, every
+    substep that stops early meets the tolerance, and after 30 frames the
+    cubes sit within 5 mm of where the plain warm start puts them."""
+    p = L.load_params()
+    a = scenes.pyramid(6, seed=21, solver_cls=Solver6DOF)
+    b = scenes.pyramid(6, seed=21, solver_cls=Solver6DOF)
+    b.set_learned_warm_start("learned", p)
+    ia, ib = [], []
+    for _ in range(30):
+        ia += a.step_to_residual(1e-4, 200)
+        rb = b.step_to_residual(1e-4, 200)
+        ib += rb
+        assert all(np.isfinite(r) and (r <= 1e-4 or it == 200) for it, r in rb)
+    xa, xb = a.positions(), b.positions()
+    assert np.all(np.isfinite(xb))
+    print("iterations plain", np.mean([i for i, _ in ia]), "learned", np.mean([i for i, _ in ib]),
+          "max position difference", np.abs(xa - xb).max(), flush=True)
+    assert np.abs(xa - xb).max() < 5e-3
