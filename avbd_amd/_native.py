@@ -26,6 +26,7 @@ COLOR_JACOBI = 0
 COLOR_JONES_PLASSMANN = 1
 OPT_PRIMAL_VARIANT = 0
 OPT_FUSE_DUAL = 1
+OPT_LEARNED_SCOPE = 2
 
 
 class AvbdError(RuntimeError):

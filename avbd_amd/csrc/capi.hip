@@ -158,6 +158,7 @@ struct avbd_solver {
   // learned warm start: 0 off, 1 log (features at emission, targets at the
   // end of the substep), 2 apply the network to the initial dual state
   int ml_mode = 0;
+  int ml_scope = 0;          // 1: only groups with an empty plain start
   bool ml_have_params = false;
   int ml_n = 0;              // groups featurised by the last substep
   DBuf<float> ml_params, ml_feat, ml_targ;
@@ -443,7 +444,7 @@ int ml_begin(avbd_solver* s, const avbd_params* p, int n) {
                           s->bstat.p, s->ml_feat.p);
   if (s->ml_mode == 2)
     launch_mlp_infer(s->st, n, s->ml_feat.p, s->ml_params.p, 0, s->gst.p, s->geo.p,
-                     p->alpha * p->gamma, p->gamma, 0, nullptr);
+                     p->alpha * p->gamma, p->gamma, s->ml_scope == 1 ? 2 : 0, nullptr);
   CK(prof_end(s, (double)n * (96.0 + 48.0 + 96.0 + (s->ml_mode == 2 ? 96.0 + 48.0 : 0.0)),
               s->ml_mode == 2 ? 2 : 1));
   return AVBD_OK;
@@ -1230,6 +1231,10 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
       return AVBD_OK;
     case AVBD_OPT_FUSE_DUAL:
       s->fuse_dual_opt = value != 0;
+      return AVBD_OK;
+    case AVBD_OPT_LEARNED_SCOPE:
+      if (value < 0 || value > 1) return set_err(AVBD_E_ARG, "learned scope must be 0 or 1");
+      s->ml_scope = value;
       return AVBD_OK;
     default:
       return set_err(AVBD_E_ARG, "unknown option");

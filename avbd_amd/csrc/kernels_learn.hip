@@ -196,6 +196,8 @@ int mlp_num_params() { return kNumParams; }
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ...
 //   mode 0: apply the correction to the groups' initial dual state (warm
 //           start); a non-finite output leaves the plain warm start
+//   mode 2: as mode 0, only on groups whose plain start is empty (normal
+//           lambda 0)
 //   mode 1: write the 4 network outputs (de-normalised) to `yout`
 __global__ void __launch_bounds__(256)
 k_mlp_infer(int n, const float* __restrict__ feat, const float* __restrict__ params,
@@ -308,7 +310,7 @@ k_mlp_infer(int n, const float* __restrict__ feat, const float* __restrict__ par
         const float* f = feat + (size_t)s * kIn;
         GroupSt* S = gst + g_first + s;
         const int h = geo[g_first + s].hdr.x;
-        if (g_kind(h) == kGPin) continue;
+        if (g_kind(h) == kGPin || (mode == 2 && f[15] != 0.0f)) continue;
         const int nr = g_nrows(h);
         if (row == 3) {
           // start at 10^(base + v) after the prelude multiplies k by gamma

@@ -314,10 +314,11 @@ def log_trace(solver, frames: int, tol: float = 1e-4, max_iters: int = 200,
 
 
 def iterations_to_residual(solver, frames: int, mode: str = "plain", params=None,
-                           tol: float = 1e-4, max_iters: int = 200) -> np.ndarray:
+                           tol: float = 1e-4, max_iters: int = 200,
+                           scope: str = "all") -> np.ndarray:
     """Iterations each substep needed to reach the residual tolerance
     (capped at max_iters), under the plain or the learned warm start."""
-    solver.set_learned_warm_start(mode, params)
+    solver.set_learned_warm_start(mode, params, scope)
     its = []
     for _ in range(frames):
         its.extend(i for i, _r in solver.step_to_residual(tol, max_iters))
@@ -336,10 +337,11 @@ def undo_decay_params(like: np.ndarray, alpha: float = 0.99, gamma: float = 0.99
     return p
 
 
-def compare(builder, frames: int, params, tol: float = 1e-4, max_iters: int = 200) -> dict:
+def compare(builder, frames: int, params, tol: float = 1e-4, max_iters: int = 200,
+            scope: str = "all") -> dict:
     """Iterations to residual from the same scene, plain vs learned warm start."""
     plain = iterations_to_residual(builder(), frames, "plain", None, tol, max_iters)
-    learned = iterations_to_residual(builder(), frames, "learned", params, tol, max_iters)
+    learned = iterations_to_residual(builder(), frames, "learned", params, tol, max_iters, scope)
     return {"plain_mean": float(plain.mean()), "learned_mean": float(learned.mean()),
             "plain_total": int(plain.sum()), "learned_total": int(learned.sum()),
             "plain_capped": int((plain >= max_iters).sum()),

@@ -274,6 +274,7 @@ class Solver6DOF:
         self._flush_snapshot: dict | None = None
         self._warm_start_mode = 0          # learned warm start: 0 plain, 1 log, 2 learned
         self._warm_start_params: np.ndarray | None = None
+        self._warm_start_scope = 0
 
     # ---- scene building (solver_6dof.py:446-578) ----------------------------
     def add_box(self, position, half_extents, mass: float, orientation=(0.0, 0.0, 0.0, 1.0),
@@ -655,15 +656,21 @@ class Solver6DOF:
         return out
 
     # ---- learned warm start (not in the reference) ---------------------------
-    def set_learned_warm_start(self, mode: str = "plain", params=None) -> None:
-        """Initial dual state of box-box contacts: "plain" = the persistent-
-        lambda cache (the reference's warm start), "log" = plain, and each
-        substep's contact features and converged dual state are kept for
-        learned_samples(), "learned" = the network with `params` (layout of
-        avbd_amd.learned) corrects the plain warm start."""
+    def set_learned_warm_start(self, mode: str = "plain", params=None,
+                               scope: str = "all") -> None:
+        """Initial dual state of box-box and floor contact groups: "plain" =
+        the persistent-lambda cache (the reference's warm start), "log" =
+        plain, and each substep's contact features and converged dual state
+        are kept for learned_samples(), "learned" = the network with `params`
+        (layout of avbd_amd.learned) corrects the plain warm start; with
+        scope="empty" only groups whose cache holds no normal lambda (new
+        contacts, floor corners not pressed) take the network's start."""
         from .learned import MODES
         if mode not in MODES:
             raise ValueError(f"unknown warm-start mode {mode!r}; expected one of {tuple(MODES)}")
+        if scope not in ("all", "empty"):
+            raise ValueError(f"unknown learned scope {scope!r}; expected 'all' or 'empty'")
+        self._warm_start_scope = 1 if scope == "empty" else 0
         if params is not None:
             self._warm_start_params = np.ascontiguousarray(params, np.float32)
         if mode == "learned" and self._warm_start_params is None:
@@ -675,6 +682,7 @@ class Solver6DOF:
     def _apply_warm_start(self) -> None:
         if self._warm_start_params is not None:
             self._h.ml_set_params(self._warm_start_params)
+        self._h.set_option(N.OPT_LEARNED_SCOPE, self._warm_start_scope)
         self._h.ml_set_mode(self._warm_start_mode)
 
     def learned_samples(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:

@@ -88,6 +88,9 @@ def train(tr: L.Trace, args, rng):
     perm = rng.permutation(n)
     n_val = max(1, n // 20)
     val, fit = perm[:n_val], perm[n_val:]
+    if args.scope == "empty":   # train on the groups the network will start
+        fit = fit[tr.X[fit, 15] == 0.0]
+        val = val[tr.X[val, 15] == 0.0]
     X, Y, M = tr.X[fit], tr.Y[fit], tr.mask[fit]
     if args.clip > 0:   # bound the corrections the network is asked to make
         Y = np.clip(Y, -args.clip, args.clip)
@@ -129,7 +132,7 @@ def evaluate(p, args) -> list[dict]:
                       lambda: scenes.pyramid(args.eval_pyramid, seed=9500)))
     for name, builder in cases:
         t0 = time.perf_counter()
-        r = L.compare(builder, args.eval_frames, p, args.tol, args.max_iters)
+        r = L.compare(builder, args.eval_frames, p, args.tol, args.max_iters, args.scope)
         r = {k: v for k, v in r.items() if not isinstance(v, np.ndarray)}
         r["scene"] = name
         u = L.iterations_to_residual(builder(), args.eval_frames, "learned",
@@ -153,6 +156,8 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--train-pyramids", type=int, nargs="*", default=[6, 8, 10, 12, 14])
     p.add_argument("--train-stacks", type=int, nargs="*", default=[3, 5, 8])
     p.add_argument("--pyramid-frames", type=int, default=60)
+    p.add_argument("--scope", choices=("all", "empty"), default="all",
+                   help="groups the network starts: all, or those with an empty cache")
     p.add_argument("--huber", type=float, default=0.0,
                    help="Huber threshold on normalised errors (0 = squared error)")
     p.add_argument("--clip", type=float, default=0.0,
@@ -195,7 +200,7 @@ def main(argv=None) -> int:
     meta = {"train_piles": args.train_piles, "boxes": [args.min_boxes, args.max_boxes],
             "frames": args.frames, "samples": int(len(tr.X)), "tol": args.tol,
             "max_iters": args.max_iters, "steps": args.steps, "batch": args.batch,
-            "lr": args.lr, "huber": args.huber, "clip": args.clip, "history": hist, "plain_val_loss": base, "eval": ev,
+            "lr": args.lr, "scope": args.scope, "huber": args.huber, "clip": args.clip, "history": hist, "plain_val_loss": base, "eval": ev,
             "seconds": time.perf_counter() - t0}
     L.save_params(p, args.out, meta=json.dumps(meta))
     if args.report is not None:

@@ -204,6 +204,11 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * runs inside the sweep, in the tile of its higher-color body; 0 (default) =
  * a separate dual launch after every sweep. Both give identical results. */
 #define AVBD_OPT_FUSE_DUAL 1
+/* AVBD_OPT_LEARNED_SCOPE (learned warm start, mode 2): 0 (default) = every
+ * box-box and floor group, 1 = only groups whose plain start is empty
+ * (normal lambda 0: new contacts, floor corners not pressed last substep);
+ * groups with a cached lambda keep the plain start. */
+#define AVBD_OPT_LEARNED_SCOPE 2
 int avbd_set_option(avbd_solver* s, int option, int value);
 
 /* ---- profiling ------------------------------------------------------- */
