@@ -116,6 +116,13 @@ def save_params(params: np.ndarray, path: Path = WEIGHTS_PATH, **meta) -> None:
              **{k: np.asarray(v) for k, v in meta.items()})
 
 
+def load_meta(path: Path = WEIGHTS_PATH) -> dict:
+    """The training record stored beside the parameters (JSON), or {}."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        return json.loads(str(z["meta"])) if "meta" in z.files else {}
+
+
 def load_params(path: Path = WEIGHTS_PATH) -> np.ndarray:
     with np.load(path, allow_pickle=False) as z:
         p = np.asarray(z["params"], np.float32)

@@ -165,7 +165,7 @@ def test_shipped_network_keeps_the_converged_answer():
     p = L.load_params()
     a = scenes.pyramid(6, seed=21, solver_cls=Solver6DOF)
     b = scenes.pyramid(6, seed=21, solver_cls=Solver6DOF)
-    b.set_learned_warm_start("learned", p)
+    b.set_learned_warm_start("learned", p, scope=L.load_meta().get("scope", "all"))
     ia, ib = [], []
     for _ in range(30):
         ia += a.step_to_residual(1e-4, 200)
