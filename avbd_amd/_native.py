@@ -125,6 +125,8 @@ SIGNATURES = {
     "set_option": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "profile_read": ([C.c_void_p, C.c_int, F64P, I64P, F64P], C.c_int),
+    # batched scenes (no counterpart in the reference)
+    "set_scenes": ([C.c_void_p, C.c_int, I32P, C.c_float], C.c_int),
     # learned warm start (no counterpart in the reference)
     "mlp_num_params": ([], C.c_int),
     "ml_set_mode": ([C.c_void_p, C.c_int], C.c_int),
@@ -136,8 +138,8 @@ SIGNATURES = {
                    C.c_float, C.c_uint, F32P], C.c_int),
 }
 
-# entry points of the learned warm start (the checker has none)
-ML_NAMES = ("mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
+# entry points of batched scenes and the learned warm start (the checker has none)
+ML_NAMES = ("set_scenes", "mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
             "mlp_grad", "mlp_train")
 
 _LIB = None
@@ -338,6 +340,10 @@ class NativeHandle:
 
     def profile_enable(self, on: bool) -> None:
         self._check(self._fn("profile_enable")(self.h, int(bool(on))))
+
+    def set_scenes(self, first_body: np.ndarray, pitch: float) -> None:
+        f = np.ascontiguousarray(first_body, np.int32)
+        self._check(self._fn("set_scenes")(self.h, int(f.size - 1), iptr(f), float(pitch)))
 
     # ---- learned warm start ----
     def ml_set_mode(self, mode: int) -> None:

@@ -84,11 +84,17 @@ void launch_aabb(hipStream_t st, int nb, const float4* pose, const float4* bstat
 int bvh_build(hipStream_t st, int nb, const float4* lo, const float4* hi, void* ws,
               size_t ws_bytes);
 size_t bvh_workspace_bytes(int nb);
+// tlo/thi: the boxes the tree was built over (lo/hi, or per-scene shifted
+// copies); scene: per-body scene index or null (one scene)
 void bvh_count_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
-                     const float4* bstat, const void* ws, int* counts);
+                     const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
+                     const void* ws, int* counts);
 void bvh_write_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
-                     const float4* bstat, const void* ws, const int* counts, const int* offsets,
-                     int cap, int* pair_a, int* pair_b);
+                     const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
+                     const void* ws, const int* counts, const int* offsets, int cap, int* pa,
+                     int* pb);
+void launch_scene_shift(hipStream_t st, int nb, const float4* lo, const float4* hi,
+                        const int* scene, int g, float pitch, float4* slo, float4* shi);
 void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
                       int* vals_tmp, int n, void* ws);
 size_t radix_workspace_bytes(int n);
@@ -129,8 +135,11 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
                       const int* st_idx, const int* asb_start, int* asb_list, const int* start_a,
                       const int* end_a, const int* sscan, const int* succ, const int* pa,
                       const int* pb, int g_static, const int* cpos, int2* csr);
+// scene / scene_dirty (batched scenes, else null): a conflict marks its scene
 void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
-                            const int* color, int* out);
+                            const int* color, int* out, const int* scene, int* scene_dirty);
+void launch_color_reset(hipStream_t st, int nb, int* color, const int* scene,
+                        const int* scene_dirty);
 void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
                       int* counts);
 void launch_adj_fill(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
@@ -139,9 +148,11 @@ void launch_adj_count_pairs(hipStream_t st, int n_pairs, const int* pa, const in
                             const int* succ, int* counts);
 void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
                            const int* succ, const int* starts, int* cursor, int* nbr);
+// scene_dirty (batched scenes, else null): bodies of clean scenes keep their color
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp, int* clamped);
+                         int* err, int* ticket, int mode_jp, int* clamped, const int* scene,
+                         const int* scene_dirty);
 void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
                           const float* prio, const int* color, int* color_next, int* uncolored);
 void launch_color_finish(hipStream_t st, int nb, int* color, int* counts);

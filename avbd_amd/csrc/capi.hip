@@ -155,6 +155,14 @@ struct avbd_solver {
   int adj_pairs = 0;  // pairs whose emission this substep feeds the coloring adjacency
   int last_post = 0;
 
+  // batched scenes (avbd_set_scenes): per-body scene index, per-scene recolor
+  // flags, and the broad phase's per-scene shifted boxes
+  int n_scenes = 0;
+  int scene_grid = 1;
+  float scene_pitch = 0.0f;
+  DBuf<int> scene_of, scene_dirty;
+  DBuf<float4> slo, shi;
+
   // learned warm start: 0 off, 1 log (features at emission, targets at the
   // end of the substep), 2 apply the network to the initial dual state
   int ml_mode = 0;
@@ -297,13 +305,17 @@ int recolor(avbd_solver* s, int mode) {
   CK(s->nbr.ensure(2 * (size_t)np + 1));
   launch_adj_fill_pairs(st, np, s->pa.p, s->pb.p, s->succ.p, s->nbr_start.p, s->nbr_cursor.p,
                         s->nbr.p);
-  launch_fill_int(st, s->color.p, nb, -1);
+  // batched scenes: only the scenes marked dirty are recolored; the others
+  // keep their colors (no edge crosses scenes)
+  const int* scene = s->n_scenes > 0 ? s->scene_of.p : nullptr;
+  const int* dirty = s->n_scenes > 0 ? s->scene_dirty.p : nullptr;
+  launch_color_reset(st, nb, s->color.p, scene, dirty);
   launch_fill_int(st, s->dcnt.p + 6, 1, 0);
   launch_fill_int(st, s->dcnt.p + 9, 1, 0);
   if (mode == AVBD_COLOR_JACOBI) {
     for (int round = 0; round < kMaxColors; ++round) {
       launch_color_greedy(st, nb, s->order_id.p, nullptr, s->nbr_start.p, s->nbr.p, s->color.p,
-                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0, s->dcnt.p + 9);
+                          s->pub.p, s->dcnt.p + 6, s->ticket.p, 0, s->dcnt.p + 9, scene, dirty);
       launch_fill_int(st, s->dcnt.p + 5, 1, 0);
       launch_color_resolve(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
                            s->color_next.p, s->dcnt.p + 5);
@@ -313,7 +325,7 @@ int recolor(avbd_solver* s, int mode) {
     }
   } else {
     launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
-                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, s->dcnt.p + 9);
+                        s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, s->dcnt.p + 9, scene, dirty);
   }
   launch_color_finish(st, nb, s->color.p, s->ccounts.p);
   color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
@@ -335,9 +347,13 @@ int recolor(avbd_solver* s, int mode) {
   return AVBD_OK;
 }
 
+// Batched scenes: also marks each scene with a conflict in scene_dirty.
 int count_conflicts(avbd_solver* s, int* out) {
   launch_fill_int(s->st, s->dcnt.p + 4, 1, 0);
-  launch_count_conflicts(s->st, s->n_contacts, s->g_static, s->geo.p, s->color.p, s->dcnt.p + 4);
+  if (s->n_scenes > 0) launch_fill_int(s->st, s->scene_dirty.p, s->n_scenes, 0);
+  launch_count_conflicts(s->st, s->n_contacts, s->g_static, s->geo.p, s->color.p, s->dcnt.p + 4,
+                         s->n_scenes > 0 ? s->scene_of.p : nullptr,
+                         s->n_scenes > 0 ? s->scene_dirty.p : nullptr);
   CK(readback(s, 4, 1));
   *out = s->hcnt[4];
   return AVBD_OK;
@@ -379,8 +395,20 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   CK(ensure_pairs(s, std::max(256, 16 * nb)));
   float4* pc = s->pose[s->cur].p;
   launch_aabb(st, nb, pc, s->bstat.p, p->margin, s->lo.p, s->hi.p);
-  bvh_build(st, nb, s->lo.p, s->hi.p, s->ws_bvh.p, s->ws_bvh.n);
-  bvh_count_pairs(st, nb, s->lo.p, s->hi.p, s->bstat.p, s->ws_bvh.p, s->bcount.p);
+  // batched scenes: the tree is built over per-scene shifted boxes
+  const bool batched = s->n_scenes > 0;
+  const float4* tlo = s->lo.p;
+  const float4* thi = s->hi.p;
+  const int* scene = nullptr;
+  if (batched) {
+    launch_scene_shift(st, nb, s->lo.p, s->hi.p, s->scene_of.p, s->scene_grid, s->scene_pitch,
+                       s->slo.p, s->shi.p);
+    tlo = s->slo.p;
+    thi = s->shi.p;
+    scene = s->scene_of.p;
+  }
+  bvh_build(st, nb, tlo, thi, s->ws_bvh.p, s->ws_bvh.n);
+  bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p, s->bcount.p);
   CK(scan(s, s->bcount.p, s->boff.p, nb, s->dcnt.p + 0));
   CK(prof_end(s, 0.0));
   CK(readback(s, 0, 1));
@@ -396,8 +424,8 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   s->adj_pairs = kept;
   if (fixed || total > 0) {
     CK(prof_begin(s, kContacts));
-    bvh_write_pairs(st, nb, s->lo.p, s->hi.p, s->bstat.p, s->ws_bvh.p, s->bcount.p, s->boff.p,
-                    s->pair_cap, s->pa.p, s->pb.p);
+    bvh_write_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
+                    s->bcount.p, s->boff.p, s->pair_cap, s->pa.p, s->pb.p);
     launch_narrowphase(st, kept, pc, s->bstat.p, s->pa.p, s->pb.p, p->margin, s->pov.p, s->psat.p,
                        s->pn.p, s->pdepth.p, s->mfc.p, s->mfr.p, s->mfoff.p, s->mfn.p);
     launch_emit_demand(st, kept, s->pa.p, s->pb.p, s->mfc.p, s->mfr.p, s->bfric.p,
@@ -500,6 +528,8 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   if (!need) {
     CK(count_conflicts(s, &rep->color_conflicts));
     need = rep->color_conflicts > 0;
+  } else if (s->n_scenes > 0) {
+    launch_fill_int(st, s->scene_dirty.p, s->n_scenes, 1);  // every scene recolors
   }
   if (need) {
     CK(recolor(s, p->coloring_mode));
@@ -867,6 +897,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   s->n_active_rows = ns;
   s->pair_count = s->pool_count = s->n_contacts = s->adj_pairs = 0;
   s->last_post = 0;
+  s->n_scenes = 0;  // a flush replaces the scene; avbd_set_scenes splits it again
   s->flushed = true;
   CK(ensure_hage(s));
   HIPCK(hipStreamSynchronize(s->st));
@@ -1259,6 +1290,43 @@ int avbd_profile_read(avbd_solver* s, int nc, double* ms, long long* launches, d
     if (launches) launches[k] = s->prof_launch[k];
     if (bytes) bytes[k] = s->prof_bytes[k];
   }
+  return AVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched scenes
+// ---------------------------------------------------------------------------
+int avbd_set_scenes(avbd_solver* s, int n_scenes, const int* first_body, float pitch) {
+  if (!s || !s->flushed) return set_err(AVBD_E_STATE, "set_scenes before flush");
+  if (n_scenes < 1 || !first_body) return set_err(AVBD_E_ARG, "need at least one scene");
+  if (first_body[0] != 0 || first_body[n_scenes] != s->nb)
+    return set_err(AVBD_E_ARG, "scene ranges must cover bodies 0 .. n_bodies");
+  for (int k = 0; k < n_scenes; ++k)
+    if (first_body[k + 1] < first_body[k])
+      return set_err(AVBD_E_ARG, "scene ranges must be non-decreasing");
+  if (!(pitch > 0.0f) || !(pitch < 1.0e30f)) return set_err(AVBD_E_ARG, "pitch must be positive");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  if (n_scenes == 1) {
+    s->n_scenes = 0;
+    return AVBD_OK;
+  }
+  const int nbb = std::max(s->nb, 1);
+  std::vector<int> sc(nbb, 0);
+  for (int k = 0; k < n_scenes; ++k)
+    for (int i = first_body[k]; i < first_body[k + 1]; ++i) sc[i] = k;
+  CK(s->scene_of.alloc(nbb));
+  CK(s->scene_dirty.alloc(n_scenes));
+  CK(s->slo.alloc(nbb));
+  CK(s->shi.alloc(nbb));
+  HIPCK(hipMemcpy(s->scene_of.p, sc.data(), nbb * sizeof(int), hipMemcpyHostToDevice));
+  launch_fill_int(s->st, s->scene_dirty.p, n_scenes, 1);
+  int g = 1;
+  while ((long long)g * g * g < n_scenes) ++g;
+  s->scene_grid = g;
+  s->scene_pitch = pitch;
+  s->n_scenes = n_scenes;
+  HIPCK(hipStreamSynchronize(s->st));
   return AVBD_OK;
 }
 

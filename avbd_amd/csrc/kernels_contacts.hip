@@ -53,6 +53,32 @@ void launch_aabb(hipStream_t st, int nb, const float4* pose, const float4* bstat
                      lo, hi);
 }
 
+// Batched scenes: each scene's boxes moved to its own cell of a g x g x g grid
+// of pitch `pitch` for the tree. Rounding is monotonic (x <= y implies
+// fl(x + d) <= fl(y + d)), so two boxes of one scene that overlap still
+// overlap after the shift: the tree prunes conservatively, and the leaf test
+// runs on the unshifted boxes.
+__global__ void __launch_bounds__(256)
+k_scene_shift(int nb, const float4* __restrict__ lo, const float4* __restrict__ hi,
+              const int* __restrict__ scene, int g, float pitch, float4* __restrict__ slo,
+              float4* __restrict__ shi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int sc = scene[i];
+  const float ox = (float)(sc % g) * pitch, oy = (float)((sc / g) % g) * pitch,
+              oz = (float)(sc / (g * g)) * pitch;
+  const float4 a = lo[i], b = hi[i];
+  slo[i] = make_float4(a.x + ox, a.y + oy, a.z + oz, 0.0f);
+  shi[i] = make_float4(b.x + ox, b.y + oy, b.z + oz, 0.0f);
+}
+
+void launch_scene_shift(hipStream_t st, int nb, const float4* lo, const float4* hi,
+                        const int* scene, int g, float pitch, float4* slo, float4* shi) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_scene_shift, dim3((nb + 255) / 256), dim3(256), 0, st, nb, lo, hi, scene,
+                       g, pitch, slo, shi);
+}
+
 // ---------------------------------------------------------------------------
 // LBVH
 // ---------------------------------------------------------------------------
@@ -356,12 +382,29 @@ __device__ __forceinline__ bool box_overlap(float4 alo, float4 ahi, float4 blo, 
          alo.z <= bhi.z && blo.z <= ahi.z;
 }
 
+// Boxes of the broad phase: lo/hi are the bodies' AABBs; the tree was built
+// over tlo/thi (the same arrays, or per-scene shifted copies), and scene (or
+// null) keeps pairs inside one scene.
+struct BroadBoxes {
+  const float4* lo;
+  const float4* hi;
+  const float4* tlo;
+  const float4* thi;
+  const int* scene;
+};
+
+__device__ __forceinline__ bool leaf_pair(const BroadBoxes& B, int i, int j, float4 qlo,
+                                          float4 qhi, bool st_i, const float4* bstat) {
+  return box_overlap(qlo, qhi, B.lo[j], B.hi[j]) && !(st_i && bstat[j].x <= 0.0f) &&
+         (B.scene == nullptr || B.scene[j] == B.scene[i]);
+}
+
 // Traverse for body i; visit(j) for every body j > i with overlapping AABB that
 // is not static-static. Returns the number visited.
 template <bool kWrite>
-__device__ int bvh_query(int n, int i, const float4* __restrict__ lo, const float4* __restrict__ hi,
-                         const float4* __restrict__ bstat, const BvhWs& w, int* out, int cap_left) {
-  float4 qlo = lo[i], qhi = hi[i];
+__device__ int bvh_query(int n, int i, const BroadBoxes& B, const float4* __restrict__ bstat,
+                         const BvhWs& w, int* out, int cap_left) {
+  const float4 qlo = B.lo[i], qhi = B.hi[i], tqlo = B.tlo[i], tqhi = B.thi[i];
   bool st_i = bstat[i].x <= 0.0f;
   int cnt = 0;
   if (n == 1) return 0;
@@ -370,12 +413,12 @@ __device__ int bvh_query(int n, int i, const float4* __restrict__ lo, const floa
   while (node != -1 && ++steps <= max_steps) {
     if (node >= n - 1) {
       int j = w.ids[node - (n - 1)];
-      if (j > i && box_overlap(qlo, qhi, lo[j], hi[j]) && !(st_i && bstat[j].x <= 0.0f)) {
+      if (j > i && leaf_pair(B, i, j, qlo, qhi, st_i, bstat)) {
         if (kWrite && cnt < cap_left) out[cnt] = j;
         ++cnt;
       }
       node = w.escape[node];
-    } else if (box_overlap(qlo, qhi, w.nlo[node], w.nhi[node])) {
+    } else if (box_overlap(tqlo, tqhi, w.nlo[node], w.nhi[node])) {
       node = w.left[node];
     } else {
       node = w.escape[node];
@@ -385,18 +428,17 @@ __device__ int bvh_query(int n, int i, const float4* __restrict__ lo, const floa
 }
 
 __global__ void __launch_bounds__(256)
-k_bvh_count(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
-            const float4* __restrict__ bstat, BvhWs w, int* __restrict__ counts) {
+k_bvh_count(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
+            int* __restrict__ counts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  counts[i] = bvh_query<false>(n, i, lo, hi, bstat, w, nullptr, 0);
+  counts[i] = bvh_query<false>(n, i, B, bstat, w, nullptr, 0);
 }
 
 // Smallest overlapping partner of body i that is greater than `after`.
-__device__ int bvh_next_partner(int n, int i, int after, const float4* __restrict__ lo,
-                                const float4* __restrict__ hi, const float4* __restrict__ bstat,
-                                const BvhWs& w) {
-  float4 qlo = lo[i], qhi = hi[i];
+__device__ int bvh_next_partner(int n, int i, int after, const BroadBoxes& B,
+                                const float4* __restrict__ bstat, const BvhWs& w) {
+  const float4 qlo = B.lo[i], qhi = B.hi[i], tqlo = B.tlo[i], tqhi = B.thi[i];
   bool st_i = bstat[i].x <= 0.0f;
   int best = 0x7fffffff;
   int node = 0;
@@ -404,11 +446,9 @@ __device__ int bvh_next_partner(int n, int i, int after, const float4* __restric
   while (node != -1 && ++steps <= max_steps) {
     if (node >= n - 1) {
       int j = w.ids[node - (n - 1)];
-      if (j > i && j > after && j < best && box_overlap(qlo, qhi, lo[j], hi[j]) &&
-          !(st_i && bstat[j].x <= 0.0f))
-        best = j;
+      if (j > i && j > after && j < best && leaf_pair(B, i, j, qlo, qhi, st_i, bstat)) best = j;
       node = w.escape[node];
-    } else if (box_overlap(qlo, qhi, w.nlo[node], w.nhi[node])) {
+    } else if (box_overlap(tqlo, tqhi, w.nlo[node], w.nhi[node])) {
       node = w.left[node];
     } else {
       node = w.escape[node];
@@ -418,10 +458,9 @@ __device__ int bvh_next_partner(int n, int i, int after, const float4* __restric
 }
 
 __global__ void __launch_bounds__(256)
-k_bvh_write(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
-            const float4* __restrict__ bstat, BvhWs w, const int* __restrict__ counts,
-            const int* __restrict__ offsets, int cap, int* __restrict__ pa,
-            int* __restrict__ pb) {
+k_bvh_write(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
+            const int* __restrict__ counts, const int* __restrict__ offsets, int cap,
+            int* __restrict__ pa, int* __restrict__ pb) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int off = offsets[i];
@@ -430,7 +469,7 @@ k_bvh_write(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
   int cnt = counts[i];
   int* seg = pb + off;
   if (cnt <= room) {
-    bvh_query<true>(n, i, lo, hi, bstat, w, seg, cnt);
+    bvh_query<true>(n, i, B, bstat, w, seg, cnt);
     // Canonical order: this body's partners ascending (insertion sort).
     for (int a = 1; a < cnt; ++a) {
       int v = seg[a];
@@ -444,7 +483,7 @@ k_bvh_write(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
     // partners, the prefix of the canonical list.
     int last = -1;
     for (int a = 0; a < room; ++a) {
-      last = bvh_next_partner(n, i, last, lo, hi, bstat, w);
+      last = bvh_next_partner(n, i, last, B, bstat, w);
       seg[a] = last;
       pa[off + a] = i;
     }
@@ -452,18 +491,21 @@ k_bvh_write(int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
 }
 
 void bvh_count_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
-                     const float4* bstat, const void* ws, int* counts) {
+                     const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
+                     const void* ws, int* counts) {
   BvhWs w = bvh_carve(const_cast<void*>(ws), n);
-  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, bstat, w,
-                     counts);
+  const BroadBoxes B{lo, hi, tlo, thi, scene};
+  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts);
 }
 
 void bvh_write_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
-                     const float4* bstat, const void* ws, const int* counts, const int* offsets,
-                     int cap, int* pa, int* pb) {
+                     const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
+                     const void* ws, const int* counts, const int* offsets, int cap, int* pa,
+                     int* pb) {
   BvhWs w = bvh_carve(const_cast<void*>(ws), n);
-  hipLaunchKernelGGL(k_bvh_write, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, bstat, w,
-                     counts, offsets, cap, pa, pb);
+  const BroadBoxes B{lo, hi, tlo, thi, scene};
+  hipLaunchKernelGGL(k_bvh_write, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts,
+                     offsets, cap, pa, pb);
 }
 
 // ---------------------------------------------------------------------------

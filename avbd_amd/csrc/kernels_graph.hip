@@ -367,13 +367,16 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_count_conflicts(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
-                  const int* __restrict__ color, int* out) {
+                  const int* __restrict__ color, int* out, const int* __restrict__ scene,
+                  int* __restrict__ scene_dirty) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   int c = 0;
   if (e < n_dyn) {
     int4 hd = geo[g_first + e].hdr;
-    if (hd.y >= 0 && hd.z >= 0 && hd.y != hd.z && color[hd.y] == color[hd.z])
+    if (hd.y >= 0 && hd.z >= 0 && hd.y != hd.z && color[hd.y] == color[hd.z]) {
       c = g_nrows(hd.x);
+      if (scene_dirty) scene_dirty[scene[hd.y]] = 1;  // batched: the scene recolors
+    }
   }
   // wave-level sum, one atomic per wave
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -381,10 +384,27 @@ k_count_conflicts(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
 }
 
 void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
-                            const int* color, int* out) {
+                            const int* color, int* out, const int* scene, int* scene_dirty) {
   if (n_dyn > 0)
     hipLaunchKernelGGL(k_count_conflicts, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
-                       g_first, geo, color, out);
+                       g_first, geo, color, out, scene, scene_dirty);
+}
+
+// Start of a recolor: every body uncolored, or (batched scenes) only the
+// bodies of scenes marked dirty; the others keep their colors.
+__global__ void __launch_bounds__(256)
+k_color_reset(int nb, int* __restrict__ color, const int* __restrict__ scene,
+              const int* __restrict__ scene_dirty) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  if (scene_dirty == nullptr || scene_dirty[scene[i]]) color[i] = -1;
+}
+
+void launch_color_reset(hipStream_t st, int nb, int* color, const int* scene,
+                        const int* scene_dirty) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_color_reset, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, scene,
+                       scene_dirty);
 }
 
 __global__ void __launch_bounds__(256)
@@ -496,7 +516,7 @@ __global__ void __launch_bounds__(64)
 k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ pos,
                const int* __restrict__ nbr_start, const int* __restrict__ nbr,
                const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err,
-               int* clamped) {
+               int* clamped, const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
   __shared__ int nj_s[kPollBatch][64];
   const int lane = threadIdx.x;
   for (;;) {
@@ -506,7 +526,10 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
     if (base >= nb) return;
     const int k = base + lane;
     const int i = k < nb ? order[k] : -1;
-    bool done = !(i >= 0 && (mode_jp || color_in[i] == -1));
+    // a body of a clean scene (batched) keeps its color and has no neighbour
+    // outside its scene, so nobody waits on it
+    bool done = !(i >= 0 && (mode_jp || color_in[i] == -1)) ||
+                (i >= 0 && scene_dirty != nullptr && !scene_dirty[scene[i]]);
     int my_pos = 0, q = 0, q1 = 0;
     if (!done) {
       my_pos = pos ? pos[i] : i;
@@ -589,25 +612,28 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
 }
 
 __global__ void __launch_bounds__(256)
-k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp) {
+k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp,
+              const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
+  if (scene_dirty != nullptr && !scene_dirty[scene[i]]) return;
   if (mode_jp) color[i] = (pub[i] & 255) - 1;
   else if (color[i] == -1) color[i] = pub[i];
 }
 
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
-                         int* err, int* ticket, int mode_jp, int* clamped) {
+                         int* err, int* ticket, int mode_jp, int* clamped, const int* scene,
+                         const int* scene_dirty) {
   if (nb <= 0) return;
   launch_fill_int(st, pub, nb, -1);
   launch_fill_int(st, ticket, 1, 0);
   // Enough wavefronts to cover the GPU; each loops over tickets.
   const int waves = std::min((nb + 63) / 64, 4096);
   hipLaunchKernelGGL(k_color_greedy, dim3(waves), dim3(64), 0, st, nb, order, pos, nbr_start, nbr,
-                     color_in_out, pub, ticket, mode_jp, err, clamped);
+                     color_in_out, pub, ticket, mode_jp, err, clamped, scene, scene_dirty);
   hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
-                     mode_jp);
+                     mode_jp, scene, scene_dirty);
 }
 
 __global__ void __launch_bounds__(256)

@@ -221,6 +221,17 @@ int avbd_profile_enable(avbd_solver* s, int on);
 int avbd_profile_read(avbd_solver* s, int n_classes, double* ms, long long* launches,
                       double* bytes);
 
+/* ---- batched scenes --------------------------------------------------
+ * No counterpart in the reference (one Solver6DOF per scene there). After a
+ * flush of the concatenation of independent scenes (bodies of scene k are
+ * [first_body[k], first_body[k+1]); static rows refer to their own bodies),
+ * this splits it: the broad phase pairs only bodies of one scene (its tree
+ * is built over each scene's boxes moved to its own cell of a grid of pitch
+ * `pitch` metres; the pitch only affects speed), and the recolor gate
+ * recolors only the scenes with a color conflict. Each scene then steps
+ * exactly as it would alone. A flush undoes the split. */
+int avbd_set_scenes(avbd_solver* s, int n_scenes, const int* first_body, float pitch);
+
 /* ---- learned warm start ----------------------------------------------
  * No counterpart in the reference: its warm start is the persistent-lambda
  * cache alone (kernels_6dof.py:2695-2912), which stays the fallback. A
