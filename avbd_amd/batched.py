@@ -22,6 +22,7 @@ Not in the reference, which steps one scene per Solver6DOF.
 from __future__ import annotations
 
 import dataclasses
+from collections.abc import Sequence
 
 import numpy as np
 
@@ -59,19 +60,15 @@ class BatchedSolver6DOF(Solver6DOF):
                          unsafe_fixed_capacity=s0._unsafe_fixed_capacity)
         self._scenes = list(scenes)
         self.enable_self_collision(s0._self_collide, s0._self_friction)
-        first = [0]
+        first, rfirst = [0], [0]
         for s in scenes:
-            b0, r0 = len(self._x), len(self._rows)
             for lst in ("_x", "_q", "_v", "_omega", "_mass", "_inv_I_local", "_I_local",
                         "_half_extents", "_friction"):
                 getattr(self, lst).extend(getattr(s, lst))
-            for r in s._rows:
-                self._rows.append(dataclasses.replace(
-                    r, body_a=r.body_a + b0,
-                    sibling=r.sibling + r0 if r.sibling >= 0 else r.sibling,
-                    partner=r.partner + r0 if r.partner >= 0 else r.partner))
             first.append(len(self._x))
+            rfirst.append(rfirst[-1] + len(s._rows))
         self._first = np.asarray(first, np.int32)
+        self._rows = _RowsView(self._scenes, self._first, np.asarray(rfirst, np.int64))
         self._pitch = float(pitch) if pitch is not None else self._auto_pitch()
 
     @property
@@ -103,12 +100,30 @@ class BatchedSolver6DOF(Solver6DOF):
         return super().add_box(*a, **kw)
 
     def _scene_arrays(self) -> dict:
-        arrs = super()._scene_arrays()
+        """The scenes' own flush arrays end to end: body indices of rows
+        shifted by the scene's first body, sibling/partner row indices by
+        its first row; priorities and capacities as each scene has them
+        alone (capacities summed)."""
         parts = [s._scene_arrays() for s in self._scenes]
-        arrs["priority"] = np.concatenate([p["priority"] for p in parts]).astype(np.float32)
+        nb = np.array([p["n_bodies"] for p in parts], np.int64)
+        nr = np.array([p["n_static_rows"] for p in parts], np.int64)
+        b0 = np.repeat(np.concatenate([[0], np.cumsum(nb)[:-1]]), nr).astype(np.int32)
+        r0 = np.repeat(np.concatenate([[0], np.cumsum(nr)[:-1]]), nr).astype(np.int32)
+        out = {"n_bodies": int(nb.sum()), "n_static_rows": int(nr.sum())}
+        for key, val in parts[0].items():
+            if isinstance(val, np.ndarray):
+                out[key] = np.concatenate([p[key] for p in parts])
+        out["body_a"] = out["body_a"] + b0
+        for key in ("sibling", "partner"):
+            out[key] = np.where(out[key] >= 0, out[key] + r0, out[key]).astype(np.int32)
         for key in ("n_dyn_capacity", "hash_cap", "pool_max", "flush_max_pairs"):
-            arrs[key] = int(sum(p[key] for p in parts))
-        return arrs
+            out[key] = int(sum(p[key] for p in parts))
+        out["max_pairs"] = int(self._bp_max_pairs)
+        if self._h is not None and self.x is not None and self._gpu_pool_ready:
+            prev = self._h.read_bodies(self._n_b_flushed)   # state of a re-flush
+            for key in ("x", "q", "v", "w", "prev_v", "prev_w"):
+                out[key] = prev[key]
+        return out
 
     def _flush(self) -> None:
         was = self._dirty
@@ -132,3 +147,29 @@ class BatchedSolver6DOF(Solver6DOF):
     def angular_velocities(self, scene: int | None = None) -> np.ndarray:
         w = super().angular_velocities()
         return w if scene is None else w[self.scene_slice(scene)]
+
+
+class _RowsView(Sequence):
+    """The batch's static rows, re-indexed on access (no copy of the
+    scenes' row lists)."""
+
+    def __init__(self, scenes, first_body, first_row):
+        self._scenes, self._fb, self._fr = scenes, first_body, first_row
+
+    def __len__(self) -> int:
+        return int(self._fr[-1])
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        k = int(np.searchsorted(self._fr, i, side="right")) - 1
+        r = self._scenes[k]._rows[i - int(self._fr[k])]
+        r0 = int(self._fr[k])
+        return dataclasses.replace(
+            r, body_a=r.body_a + int(self._fb[k]),
+            sibling=r.sibling + r0 if r.sibling >= 0 else r.sibling,
+            partner=r.partner + r0 if r.partner >= 0 else r.partner)

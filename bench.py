@@ -1,11 +1,17 @@
 """Throughput of the Solver6DOF substep on one MI355X.
 
-Workload: the square pyramid of 98,021 cubes (66 layers, half-extent 0.25 m,
-125 kg, mu_d 0.5 / mu_s 0.6, seeded 1 mm jitter), dt = 1/60, 4 substeps,
-20 iterations, FP32. One step = one frame (4 substeps: contacts, coloring,
-CSR, prelude, 21 primal sweeps, 20 dual sweeps, finalize, cache refresh).
-Metric: body-substeps/s = bodies x substeps x frames / wall time, with the
-scene resident in HBM before the timed region starts.
+Workload (default): the square pyramid of 98,021 cubes (66 layers,
+half-extent 0.25 m, 125 kg, mu_d 0.5 / mu_s 0.6, seeded 1 mm jitter),
+dt = 1/60, 4 substeps, 20 iterations, FP32. One step = one frame
+(4 substeps: contacts, coloring, CSR, prelude, 21 primal sweeps, 20 dual
+sweeps, finalize, cache refresh). Metric: body-substeps/s = bodies x
+substeps x frames / wall time, with the scene resident in HBM before the
+timed region starts. The line also carries iterations to the dual residual
+1e-4 (cap 200), plain vs learned warm start, on a 285-cube pyramid.
+
+--workload batched: 2048 independent piles of n = round(exp(U(ln 8,
+ln 1024))) boxes each (the 10k-box pile's generator per scene), 10
+iterations, stepped as one batch; same metric over all scenes.
 
 Prints one JSON line.
 """
@@ -69,9 +75,46 @@ def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
     }
 
 
+def batched_scene(n_scenes: int, seed: int):
+    """2048 piles of log-uniform size in [8, 1024] boxes as one batch."""
+    import numpy as np
+    from avbd_amd import learned
+    from avbd_amd.batched import BatchedSolver6DOF
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(8), np.log(1024), n_scenes))).astype(int)
+    parts = []
+    t0 = time.perf_counter()
+    for k, n in enumerate(sizes):
+        parts.append(learned.pile(int(n), seed=seed * 100003 + k))
+        if (k + 1) % 256 == 0:
+            print(f"# built {k + 1}/{n_scenes} scenes ({time.perf_counter() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+    return BatchedSolver6DOF(parts), sizes
+
+
+def iters_to_residual(frames: int = 30) -> dict:
+    """Iterations per substep to the dual residual 1e-4 (cap 200) on a
+    9-layer pyramid, plain warm start vs the shipped learned one."""
+    from avbd_amd import learned, scenes
+
+    p = learned.load_params()
+    scope = learned.load_meta().get("scope", "all")
+    build = lambda: scenes.pyramid(9, seed=9500)
+    r = learned.compare(build, frames, p, 1e-4, 200, scope)
+    return {"scene": "pyramid of 285 cubes (9 layers), 30 frames x 4 substeps",
+            "tol": 1e-4, "cap": 200, "plain": r["plain_mean"], "learned": r["learned_mean"],
+            "plain_capped": r["plain_capped"], "learned_capped": r["learned_capped"],
+            "scope": scope}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("pyramid", "batched"), default="pyramid")
+    ap.add_argument("--scenes", type=int, default=2048, help="batched: number of scenes")
+    ap.add_argument("--residual", type=int, default=1,
+                    help="1: add iterations to residual, plain vs learned warm start")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--layers", type=int, default=66)
@@ -107,7 +150,10 @@ def main() -> int:
     from avbd_amd import scenes
 
     t_build = time.perf_counter()
-    s = scenes.pyramid(args.layers, seed=args.seed)
+    if args.workload == "batched":
+        s, sizes = batched_scene(args.scenes, args.seed)
+    else:
+        s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
     variant = 6 if args.primal_variant is None else args.primal_variant
     s._h.set_option(0, variant)
@@ -140,8 +186,17 @@ def main() -> int:
                  for i, c in enumerate(classes)}
     print(f"# kernel time by class over the timed region: {breakdown}", file=sys.stderr,
           flush=True)
+    if args.workload == "batched":
+        workload = (f"{args.scenes} independent piles of 8-1024 boxes (log-uniform, "
+                    f"{n_b} boxes in all) stepped as one batch, dt 1/60, {s.substeps} substeps, "
+                    f"{s.iterations} iterations")
+        metric = f"body-substeps/s, {args.scenes} batched scenes"
+    else:
+        workload = (f"pyramid of {n_b} cubes ({args.layers} layers), dt 1/60, "
+                    f"{s.substeps} substeps, {s.iterations} iterations")
+        metric = "body-substeps/s @100k boxes"
     out = {
-        "metric": "body-substeps/s @100k boxes",
+        "metric": metric,
         "value": value,
         "unit": "body-substeps/s",
         "n_gpus": 1,
@@ -153,8 +208,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"pyramid of {n_b} cubes ({args.layers} layers), dt 1/60, "
-                               f"{s.substeps} substeps, {s.iterations} iterations",
+        "config": {"workload": workload,
                    "bodies": n_b, "substeps_per_step": s.substeps, "iterations": s.iterations,
                    "contacts": n_contacts, "pairs": n_pairs, "rows": n_rows,
                    "colors": s.num_active_colors, "parallelism": "single GPU"},
@@ -165,13 +219,22 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.layers, variant),
+            "traffic": (measured_traffic(args.layers, variant) if args.workload == "pyramid"
+                    else None),
             "avg_launch_us": 1e3 * prim_ms / prim_n if prim_n else None,
             "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
         },
         "kernel_ms": breakdown,
     }
-    if args.cpu_baseline:
+    if args.workload == "batched":
+        out["config"]["scenes"] = args.scenes
+        out["config"]["scene_bodies_min_max"] = [int(sizes.min()), int(sizes.max())]
+    if args.residual and args.workload == "pyramid":
+        t0 = time.perf_counter()
+        out["iters_to_residual"] = iters_to_residual()
+        print(f"# iterations to residual: {out['iters_to_residual']} "
+              f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    if args.cpu_baseline and args.workload == "pyramid":
         out["cpu_baseline"] = cpu_baseline(args.layers, args.seed, args.cpu_substeps)
     print(json.dumps(out), flush=True)
     return 0
