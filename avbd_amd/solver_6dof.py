@@ -275,6 +275,7 @@ class Solver6DOF:
         self._warm_start_mode = 0          # learned warm start: 0 plain, 1 log, 2 learned
         self._warm_start_params: np.ndarray | None = None
         self._warm_start_scope = 0
+        self.primal_variant = 6            # the library's default primal sweep kernel
 
     # ---- scene building (solver_6dof.py:446-578) ----------------------------
     def add_box(self, position, half_extents, mass: float, orientation=(0.0, 0.0, 0.0, 1.0),
@@ -654,6 +655,31 @@ class Solver6DOF:
             if on_substep is not None:
                 on_substep(len(out) - 1)
         return out
+
+    # ---- primal kernel autotuner (not in the reference) -----------------------
+    PRIMAL_VARIANTS = (1, 2, 5, 6, 7, 8)
+
+    def autotune_primal(self, candidates=None, frames: int = 1) -> dict:
+        """Pick the fastest primal sweep kernel for this scene by timing real
+        frames. Every variant gives bit-identical results, so the frames
+        spent timing are ordinary simulation frames. Returns {variant: ms of
+        primal sweeps per frame} and keeps the fastest."""
+        if self._dirty:
+            self._flush()
+        cand = tuple(candidates) if candidates is not None else self.PRIMAL_VARIANTS
+        times = {}
+        for v in cand:
+            self._h.set_option(N.OPT_PRIMAL_VARIANT, int(v))
+            self._h.profile_enable(True)
+            for _ in range(max(1, int(frames))):
+                self.step()
+            ms, _launches, _bytes = self._h.profile_read(1)
+            self._h.profile_enable(False)
+            times[int(v)] = float(ms[0]) / max(1, int(frames))
+        best = min(times, key=times.get)
+        self._h.set_option(N.OPT_PRIMAL_VARIANT, best)
+        self.primal_variant = best
+        return times
 
     # ---- learned warm start (not in the reference) ---------------------------
     def set_learned_warm_start(self, mode: str = "plain", params=None,

@@ -125,3 +125,23 @@ def test_residual_mode_lockstep():
         oi, orr = _resid_substep(o, 1e-4, 80)
         assert (gi, np.float32(gr).view(np.int32)) == (oi, np.float32(orr).view(np.int32)), k
         assert_bit_exact(diff_report(g, o), f"residual substep {k}")
+
+
+def test_autotuned_solver_keeps_the_trajectory():
+    """The autotuner times real frames with each primal kernel; since all of
+    them give identical results the tuned solver stays bit-identical to an
+    untuned one."""
+    from avbd_amd.solver_6dof import Solver6DOF
+    a = scenes.random_pile(150, seed=9, lattice=(6, 6, 5), solver_cls=Solver6DOF)
+    b = scenes.random_pile(150, seed=9, lattice=(6, 6, 5), solver_cls=Solver6DOF)
+    times = b.autotune_primal(frames=1)
+    print("primal ms per frame by variant", times, "picked", b.primal_variant, flush=True)
+    assert b.primal_variant in times and times[b.primal_variant] == min(times.values())
+    for _ in range(len(times)):
+        a.step()
+    for _ in range(5):
+        a.step()
+        b.step()
+        ba, bb = a._h.read_bodies(len(a._x)), b._h.read_bodies(len(b._x))
+        for key in ("x", "q", "v", "w"):
+            assert np.array_equal(ba[key], bb[key]), key
