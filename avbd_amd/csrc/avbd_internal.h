@@ -86,13 +86,16 @@ int bvh_build(hipStream_t st, int nb, const float4* lo, const float4* hi, void* 
 size_t bvh_workspace_bytes(int nb);
 // tlo/thi: the boxes the tree was built over (lo/hi, or per-scene shifted
 // copies); scene: per-body scene index or null (one scene)
+// slots: nb x kPartnerSlots ints, each body's first partners sorted by the
+// count pass and copied by the write pass
+constexpr int kPartnerSlots = 16;
 void bvh_count_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, int* counts);
+                     const void* ws, int* counts, int* slots);
 void bvh_write_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, const int* counts, const int* offsets, int cap, int* pa,
-                     int* pb);
+                     const void* ws, const int* counts, const int* slots, const int* offsets,
+                     int cap, int* pa, int* pb);
 void launch_scene_shift(hipStream_t st, int nb, const float4* lo, const float4* hi,
                         const int* scene, int g, float pitch, float4* slo, float4* shi);
 void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
@@ -135,9 +138,12 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
                       const int* st_idx, const int* asb_start, int* asb_list, const int* start_a,
                       const int* end_a, const int* sscan, const int* succ, const int* pa,
                       const int* pb, int g_static, const int* cpos, int2* csr);
-// scene / scene_dirty (batched scenes, else null): a conflict marks its scene
-void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
-                            const int* color, int* out, const int* scene, int* scene_dirty);
+// Rows of the contact groups whose two bodies share a color, counted per
+// pair: pair p emitted succ[p] groups of rpc[p] rows. scene / scene_dirty
+// (batched scenes, else null): a conflict marks its scene.
+void launch_count_conflicts(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                            const int* succ, const int* rpc, const int* color, int* out,
+                            const int* scene, int* scene_dirty);
 void launch_color_reset(hipStream_t st, int nb, int* color, const int* scene,
                         const int* scene_dirty);
 void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,

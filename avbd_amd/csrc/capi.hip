@@ -119,6 +119,7 @@ struct avbd_solver {
   DBuf<float4> pn, mfoff, mfn;
   DBuf<float> pdepth;
   DBuf<int> bcount, boff;
+  DBuf<int> pslots;  // nb x kPartnerSlots: each body's first partners, sorted
 
   // hash
   int hash_cap = 0, pool_max = 0;
@@ -351,7 +352,8 @@ int recolor(avbd_solver* s, int mode) {
 int count_conflicts(avbd_solver* s, int* out) {
   launch_fill_int(s->st, s->dcnt.p + 4, 1, 0);
   if (s->n_scenes > 0) launch_fill_int(s->st, s->scene_dirty.p, s->n_scenes, 0);
-  launch_count_conflicts(s->st, s->n_contacts, s->g_static, s->geo.p, s->color.p, s->dcnt.p + 4,
+  launch_count_conflicts(s->st, s->n_contacts > 0 ? s->adj_pairs : 0, s->pa.p, s->pb.p, s->succ.p,
+                         s->rpc.p, s->color.p, s->dcnt.p + 4,
                          s->n_scenes > 0 ? s->scene_of.p : nullptr,
                          s->n_scenes > 0 ? s->scene_dirty.p : nullptr);
   CK(readback(s, 4, 1));
@@ -408,7 +410,9 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
     scene = s->scene_of.p;
   }
   bvh_build(st, nb, tlo, thi, s->ws_bvh.p, s->ws_bvh.n);
-  bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p, s->bcount.p);
+  CK(s->pslots.ensure((size_t)nb * kPartnerSlots));
+  bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p, s->bcount.p,
+                  s->pslots.p);
   CK(scan(s, s->bcount.p, s->boff.p, nb, s->dcnt.p + 0));
   CK(prof_end(s, 0.0));
   CK(readback(s, 0, 1));
@@ -425,7 +429,7 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   if (fixed || total > 0) {
     CK(prof_begin(s, kContacts));
     bvh_write_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
-                    s->bcount.p, s->boff.p, s->pair_cap, s->pa.p, s->pb.p);
+                    s->bcount.p, s->pslots.p, s->boff.p, s->pair_cap, s->pa.p, s->pb.p);
     launch_narrowphase(st, kept, pc, s->bstat.p, s->pa.p, s->pb.p, p->margin, s->pov.p, s->psat.p,
                        s->pn.p, s->pdepth.p, s->mfc.p, s->mfr.p, s->mfoff.p, s->mfn.p);
     launch_emit_demand(st, kept, s->pa.p, s->pb.p, s->mfc.p, s->mfr.p, s->bfric.p,

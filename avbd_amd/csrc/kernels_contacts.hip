@@ -156,16 +156,24 @@ __global__ void k_bounds_init(int* b) {
   else if (threadIdx.x < 6) b[threadIdx.x] = (int)0x80000000;
 }
 
+// Grid-stride over the bodies with a bounded number of workgroups: the
+// same-address atomics of the final step serialise, so there are at most
+// kBoundsBlocks x 6 of them.
+constexpr int kBoundsBlocks = 240;
+
 __global__ void __launch_bounds__(256)
 k_bounds(int n, const float4* __restrict__ lo, const float4* __restrict__ hi, int* b) {
   __shared__ int smin[3][256], smax[3][256];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
   int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
   int mx[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 l = lo[i], h = hi[i];
     float c[3] = {(l.x + h.x) * 0.5f, (l.y + h.y) * 0.5f, (l.z + h.z) * 0.5f};
-    for (int k = 0; k < 3; ++k) { mn[k] = ord_f(c[k]); mx[k] = mn[k]; }
+    for (int k = 0; k < 3; ++k) {
+      const int o = ord_f(c[k]);
+      mn[k] = min(mn[k], o);
+      mx[k] = max(mx[k], o);
+    }
   }
   for (int k = 0; k < 3; ++k) { smin[k][threadIdx.x] = mn[k]; smax[k][threadIdx.x] = mx[k]; }
   __syncthreads();
@@ -354,7 +362,8 @@ k_escape(int n, const int* __restrict__ left, const int* __restrict__ right,
 int bvh_build(hipStream_t st, int n, const float4* lo, const float4* hi, void* ws, size_t) {
   BvhWs w = bvh_carve(ws, n);
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, w.bounds);
-  hipLaunchKernelGGL(k_bounds, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.bounds);
+  hipLaunchKernelGGL(k_bounds, dim3(std::min((n + 255) / 256, kBoundsBlocks)), dim3(256), 0, st, n,
+                     lo, hi, w.bounds);
   hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, n, lo, hi, w.bounds, w.keys,
                      w.ids);
   radix_sort_pairs(st, w.keys, w.ids, w.keys_tmp, w.ids_tmp, n, w.radix_ws);
@@ -427,12 +436,27 @@ __device__ int bvh_query(int n, int i, const BroadBoxes& B, const float4* __rest
   return cnt;
 }
 
+// One traversal per body: counts every partner and keeps the first
+// kPartnerSlots of them, sorted, in `slots` (nb x kPartnerSlots), so the
+// write pass only copies them; a body with more partners is traversed again
+// there.
 __global__ void __launch_bounds__(256)
 k_bvh_count(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
-            int* __restrict__ counts) {
+            int* __restrict__ counts, int* __restrict__ slots) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  counts[i] = bvh_query<false>(n, i, B, bstat, w, nullptr, 0);
+  int* seg = slots + (size_t)i * kPartnerSlots;
+  const int cnt = bvh_query<true>(n, i, B, bstat, w, seg, kPartnerSlots);
+  if (cnt <= kPartnerSlots) {
+    // canonical order: partners ascending (insertion sort)
+    for (int a = 1; a < cnt; ++a) {
+      int v = seg[a];
+      int b = a - 1;
+      while (b >= 0 && seg[b] > v) { seg[b + 1] = seg[b]; --b; }
+      seg[b + 1] = v;
+    }
+  }
+  counts[i] = cnt;
 }
 
 // Smallest overlapping partner of body i that is greater than `after`.
@@ -459,8 +483,9 @@ __device__ int bvh_next_partner(int n, int i, int after, const BroadBoxes& B,
 
 __global__ void __launch_bounds__(256)
 k_bvh_write(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
-            const int* __restrict__ counts, const int* __restrict__ offsets, int cap,
-            int* __restrict__ pa, int* __restrict__ pb) {
+            const int* __restrict__ counts, const int* __restrict__ slots,
+            const int* __restrict__ offsets, int cap, int* __restrict__ pa,
+            int* __restrict__ pb) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int off = offsets[i];
@@ -468,7 +493,13 @@ k_bvh_write(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
   if (room <= 0 || n == 1) return;
   int cnt = counts[i];
   int* seg = pb + off;
-  if (cnt <= room) {
+  if (cnt <= room && cnt <= kPartnerSlots) {
+    const int* src = slots + (size_t)i * kPartnerSlots;
+    for (int a = 0; a < cnt; ++a) {
+      seg[a] = src[a];
+      pa[off + a] = i;
+    }
+  } else if (cnt <= room) {
     bvh_query<true>(n, i, B, bstat, w, seg, cnt);
     // Canonical order: this body's partners ascending (insertion sort).
     for (int a = 1; a < cnt; ++a) {
@@ -492,20 +523,21 @@ k_bvh_write(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
 
 void bvh_count_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, int* counts) {
+                     const void* ws, int* counts, int* slots) {
   BvhWs w = bvh_carve(const_cast<void*>(ws), n);
   const BroadBoxes B{lo, hi, tlo, thi, scene};
-  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts);
+  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts,
+                     slots);
 }
 
 void bvh_write_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, const int* counts, const int* offsets, int cap, int* pa,
-                     int* pb) {
+                     const void* ws, const int* counts, const int* slots, const int* offsets,
+                     int cap, int* pa, int* pb) {
   BvhWs w = bvh_carve(const_cast<void*>(ws), n);
   const BroadBoxes B{lo, hi, tlo, thi, scene};
   hipLaunchKernelGGL(k_bvh_write, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts,
-                     offsets, cap, pa, pb);
+                     slots, offsets, cap, pa, pb);
 }
 
 // ---------------------------------------------------------------------------

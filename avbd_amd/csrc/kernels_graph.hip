@@ -7,7 +7,8 @@
 //                       each body's groups ascending as a serial scatter
 //                       leaves them (gpu_csr_count/starts/scatter,
 //                       kernels_6dof.py:2627-2692)
-//  count_conflicts      gpu_count_color_conflicts (kernels_6dof.py:3313-3336)
+//  count_conflicts      gpu_count_color_conflicts (kernels_6dof.py:3313-3336),
+//                       counted per emitting pair instead of per row
 //  adj_*                gpu_body_adj_count/scatter (kernels_6dof.py:3045-3096)
 //  k_color_greedy       the serial effect of gpu_color_spec_assign
 //                       (kernels_6dof.py:3233-3262, greedy in body order) or
@@ -366,16 +367,20 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
 // coloring
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_count_conflicts(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
+k_count_conflicts(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
+                  const int* __restrict__ succ, const int* __restrict__ rpc,
                   const int* __restrict__ color, int* out, const int* __restrict__ scene,
                   int* __restrict__ scene_dirty) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
   int c = 0;
-  if (e < n_dyn) {
-    int4 hd = geo[g_first + e].hdr;
-    if (hd.y >= 0 && hd.z >= 0 && hd.y != hd.z && color[hd.y] == color[hd.z]) {
-      c = g_nrows(hd.x);
-      if (scene_dirty) scene_dirty[scene[hd.y]] = 1;  // batched: the scene recolors
+  if (p < n_pairs) {
+    const int ns = succ[p];
+    if (ns > 0) {
+      const int a = pa[p], b = pb[p];
+      if (color[a] == color[b]) {
+        c = ns * rpc[p];
+        if (scene_dirty) scene_dirty[scene[a]] = 1;  // batched: the scene recolors
+      }
     }
   }
   // wave-level sum, one atomic per wave
@@ -383,11 +388,12 @@ k_count_conflicts(int n_dyn, int g_first, const GroupGeo* __restrict__ geo,
   if (__lane_id() == 0 && c) atomicAdd(out, c);
 }
 
-void launch_count_conflicts(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
-                            const int* color, int* out, const int* scene, int* scene_dirty) {
-  if (n_dyn > 0)
-    hipLaunchKernelGGL(k_count_conflicts, dim3((n_dyn + 255) / 256), dim3(256), 0, st, n_dyn,
-                       g_first, geo, color, out, scene, scene_dirty);
+void launch_count_conflicts(hipStream_t st, int n_pairs, const int* pa, const int* pb,
+                            const int* succ, const int* rpc, const int* color, int* out,
+                            const int* scene, int* scene_dirty) {
+  if (n_pairs > 0)
+    hipLaunchKernelGGL(k_count_conflicts, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
+                       pa, pb, succ, rpc, color, out, scene, scene_dirty);
 }
 
 // Start of a recolor: every body uncolored, or (batched scenes) only the
