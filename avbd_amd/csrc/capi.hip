@@ -123,6 +123,11 @@ struct avbd_solver {
 
   // hash
   int hash_cap = 0, pool_max = 0;
+  // Slots of the table as last built: about twice its entries (at most
+  // hash_cap, the reference's capacity, which sets the allocation). Lookups
+  // find the same entries at any capacity above the entry count, and a table
+  // sized to the live contacts keeps the scattered probes in cache.
+  int hash_live = 0;
   DBuf<int> hkeys, howner, slot_of;
   DBuf<float4> hstate;
 
@@ -441,7 +446,7 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
     CK(scan(s, s->reserve.p, s->rscan.p, kept, s->dcnt.p + 3));
     launch_emit_write(st, kept, s->pa.p, s->pb.p, s->mfr.p, s->mfn.p, s->mfoff.p, s->bfric.p,
                       p->self_friction, p->friction_static_mult, s->rpc.p, s->succ.p, s->dscan.p,
-                      s->sscan.p, s->n_static_rows, s->g_static, s->geo.p, s->gst.p, s->hash_cap,
+                      s->sscan.p, s->n_static_rows, s->g_static, s->geo.p, s->gst.p, s->hash_live,
                       s->hkeys.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
     CK(prof_end(s, 0.0));
     if (kept > 0) {
@@ -454,6 +459,11 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   auto t1 = std::chrono::steady_clock::now();
   rep->broadphase_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
   return AVBD_OK;
+}
+
+int live_hash_slots(const avbd_solver* s, int entries) {
+  const long long want = std::max(4096LL, 2LL * entries + 1);
+  return (int)std::min<long long>(s->hash_cap, want);
 }
 
 int ensure_hage(avbd_solver* s) {
@@ -634,10 +644,11 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   }
   if (p->self_collide && s->hash_cap > 0) {
     CK(prof_begin(s, kOther));
-    launch_fill_int(st, s->hkeys.p, s->hash_cap, kHashEmpty);
-    launch_fill_int(st, s->howner.p, s->hash_cap, -1);
     int entries = std::min(s->n_contacts, s->pool_max);
-    launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_cap, s->hkeys.p,
+    s->hash_live = live_hash_slots(s, entries);
+    launch_fill_int(st, s->hkeys.p, s->hash_live, kHashEmpty);
+    launch_fill_int(st, s->howner.p, s->hash_live, -1);
+    launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_live, s->hkeys.p,
                         s->howner.p, s->slot_of.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
     CK(prof_end(s, 0.0));
   }
@@ -874,6 +885,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   CK(s->slot_of.alloc(std::max(s->pool_max, 1)));
   launch_fill_int(s->st, s->hkeys.p, std::max(s->hash_cap, 1), kHashEmpty);
   HIPCK(hipMemsetAsync(s->hstate.p, 0, 2 * (size_t)std::max(s->hash_cap, 1) * sizeof(float4), s->st));
+  s->hash_live = std::min(s->hash_cap, 4096);
   s->hage.release();
   // ---- CSR + coloring ----
   CK(s->tc_work.alloc(5 * nbb)); CK(s->asb_start.alloc(nbb + 1)); CK(s->asb_list.alloc(1));
@@ -1134,7 +1146,7 @@ int avbd_export_hash(avbd_solver* s, int cap, int* keys, float* st8, int* n) {
   if (!s || !s->flushed) return set_err(AVBD_E_STATE, "read before flush");
   HIPCK(hipSetDevice(s->device));
   HIPCK(hipStreamSynchronize(s->st));
-  int hc = s->hash_cap;
+  int hc = s->hash_cap > 0 ? s->hash_live : 0;  // the slots of the table as built
   *n = 0;
   if (hc <= 0) return AVBD_OK;
   std::vector<int> k(hc);
@@ -1161,8 +1173,10 @@ int avbd_import_hash(avbd_solver* s, int n, const int* keys, const float* st8) {
   if (!s || !s->flushed) return set_err(AVBD_E_STATE, "write before flush");
   HIPCK(hipSetDevice(s->device));
   HIPCK(hipStreamSynchronize(s->st));
-  int hc = s->hash_cap;
-  if (hc <= 0) return n == 0 ? AVBD_OK : set_err(AVBD_E_ARG, "no hash table");
+  if (s->hash_cap <= 0) return n == 0 ? AVBD_OK : set_err(AVBD_E_ARG, "no hash table");
+  if (n > s->pool_max) return set_err(AVBD_E_ARG, "more cache entries than the pool holds");
+  const int hc = live_hash_slots(s, n);
+  s->hash_live = hc;
   std::vector<int> k(hc, kHashEmpty);
   std::vector<float4> v(2 * (size_t)hc, make_float4(0.f, 0.f, 0.f, 0.f));
   for (int e = 0; e < n; ++e) {
