@@ -118,6 +118,8 @@ void launch_emit_write(hipStream_t st, int n_pairs, const int* pair_a, const int
                        int n_static_rows, int g_static, GroupGeo* geo, GroupSt* gst,
                        int hash_cap, const int* hkeys, const float4* hstate, const int* hage);
 void launch_fill_int(hipStream_t st, int* p, int n, int v);
+void launch_hash_row0(hipStream_t st, const GroupGeo* geo, const GroupSt* gst, int hash_cap,
+                      int* hkeys, float4* hstate, int* hage);
 void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner,
                          int* slot_of, float4* hstate, int* hage);

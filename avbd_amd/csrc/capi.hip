@@ -509,6 +509,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   s->n_contacts = 0;
   s->adj_pairs = 0;
   int n_active = s->n_static_rows;
+  bool pool_reset = false;  // the pool grew this substep (see the cache refresh)
   rep->row_capacity = s->row_cap;
   if (p->self_collide && nb >= 2) {
     CK(emit_contacts(s, p, fixed, rep));
@@ -518,8 +519,10 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       int raw = s->n_active_rows, cap = s->row_cap;
       if (raw > cap) {
         rep->row_overflow = 1;
+        const int pool_before = s->pool_max;
         CK(grow_row_pool(s, std::max(2 * s->n_dyn_cap, (raw - s->n_static_rows) * 2),
                          s->g_static + s->n_contacts));
+        pool_reset = s->pool_max > pool_before;
       }
       n_active = std::min(raw, cap);
     }
@@ -645,11 +648,18 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   if (p->self_collide && s->hash_cap > 0) {
     CK(prof_begin(s, kOther));
     int entries = std::min(s->n_contacts, s->pool_max);
-    s->hash_live = live_hash_slots(s, entries);
+    s->hash_live = live_hash_slots(s, pool_reset ? 1 : entries);
     launch_fill_int(st, s->hkeys.p, s->hash_live, kHashEmpty);
     launch_fill_int(st, s->howner.p, s->hash_live, -1);
-    launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_live, s->hkeys.p,
-                        s->howner.p, s->slot_of.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
+    if (!pool_reset) {
+      launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_live, s->hkeys.p,
+                          s->howner.p, s->slot_of.p, s->hstate.p,
+                          s->ml_mode ? s->hage.p : nullptr);
+    } else if (entries > 0) {
+      // the reference's refresh after a pool growth reads zeroed pool indices
+      launch_hash_row0(st, s->geo.p, s->gst.p, s->hash_live, s->hkeys.p, s->hstate.p,
+                       s->ml_mode ? s->hage.p : nullptr);
+    }
     CK(prof_end(s, 0.0));
   }
   if (iters_used) *iters_used = I;

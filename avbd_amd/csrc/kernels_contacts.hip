@@ -983,6 +983,36 @@ k_hash_store(int n, int g_first, const GroupGeo* __restrict__ geo,
   if (hage) hage[slot] = __float_as_int(S.s2.z);
 }
 
+// The cache refresh of a substep whose row-pool overflow grew the pool: the
+// reference re-creates its pool index arrays zeroed (solver_6dof.py:1762-1768)
+// before the refresh reads them (:1450-1464), so every pool entry names row
+// 0 for its normal and both tangent rows and contact index 0. The table then
+// holds one entry: row 0's bodies (key with contact 0) and row 0's state in
+// all three row slots, if row 0 is active. Row 0 is group 0's first row.
+__global__ void k_hash_row0(const GroupGeo* __restrict__ geo, const GroupSt* __restrict__ gst,
+                            int hash_cap, int* hkeys, float4* hstate, int* hage) {
+  if (threadIdx.x != 0) return;
+  const GroupGeo G = geo[0];
+  const GroupSt S = gst[0];
+  const int bits = __float_as_int(S.s2.y);
+  if (!(bits & kActN)) return;
+  const int a = G.hdr.y, b = G.hdr.z;
+  const int ka = a < b ? a : b, kb = a < b ? b : a;
+  const int key = encode_pair_key(ka, kb, 0);
+  const int slot = hash_pair_slot(key, hash_cap);
+  const float lam = S.s0.x, k = S.s1.x, was = (bits & kWasN) ? 1.0f : 0.0f;
+  hkeys[slot] = key;
+  hstate[2 * (size_t)slot] = make_float4(lam, lam, lam, k);
+  hstate[2 * (size_t)slot + 1] = make_float4(k, k, was, was);
+  if (hage) hage[slot] = 0;
+}
+
+void launch_hash_row0(hipStream_t st, const GroupGeo* geo, const GroupSt* gst, int hash_cap,
+                      int* hkeys, float4* hstate, int* hage) {
+  hipLaunchKernelGGL(k_hash_row0, dim3(1), dim3(64), 0, st, geo, gst, hash_cap, hkeys, hstate,
+                     hage);
+}
+
 void launch_hash_collect(hipStream_t st, int n, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner, int* slot_of,
                          float4* hstate, int* hage) {
