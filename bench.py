@@ -35,7 +35,7 @@ PRIMAL_KERNELS = ["k_primal_color (one launch per color)", "k_primal_tiled<1> pe
                   "k_primal_tiled<2> per color", "k_primal_tiled<4> per color",
                   "k_primal_tiled<8> per color", "k_primal_tiled<1|2> per color",
                   "k_primal_tiled<2, flow> one launch", "k_primal_tiled<1, flow> one launch",
-                  "k_primal_tiled<4, flow> one launch"]
+                  "k_primal_tiled<4, flow> one launch", "k_primal_tiled<8, flow> one launch"]
 TRAFFIC_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_pmc.md"
 
 
@@ -129,7 +129,8 @@ def main() -> int:
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
                          "5 one or two bodies per wavefront by color size; "
-                         "6 one dataflow launch per sweep (default)")
+                         "6-9 one dataflow launch per sweep with 2, 1, 4, 8 bodies per tile; "
+                         "default: picked by the autotuner among 6-9 after the warm-up")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -155,8 +156,9 @@ def main() -> int:
     else:
         s = scenes.pyramid(args.layers, seed=args.seed)
     s._flush()
-    variant = 6 if args.primal_variant is None else args.primal_variant
-    s._h.set_option(0, variant)
+    variant = args.primal_variant
+    if variant is not None:
+        s._h.set_option(0, variant)
     s._h.set_option(1, args.fuse_dual)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
@@ -166,6 +168,14 @@ def main() -> int:
         s.step()
         print(f"# warmup frame {w}: {1e3 * (time.perf_counter() - t0):.1f} ms", file=sys.stderr,
               flush=True)
+    tuned = None
+    if variant is None:
+        # the primal sweep kernel for this scene, by timing one frame with
+        # each (identical results, so these are ordinary untimed frames)
+        tuned = s.autotune_primal(frames=1)
+        variant = s.primal_variant
+        print(f"# primal autotune (ms of primal sweeps per frame): {tuned} -> variant {variant}",
+              file=sys.stderr, flush=True)
     s._h.read_counters()
     s.positions()   # synchronises the stream
     s._h.profile_enable(True)
@@ -211,7 +221,10 @@ def main() -> int:
         "config": {"workload": workload,
                    "bodies": n_b, "substeps_per_step": s.substeps, "iterations": s.iterations,
                    "contacts": n_contacts, "pairs": n_pairs, "rows": n_rows,
-                   "colors": s.num_active_colors, "parallelism": "single GPU"},
+                   "colors": s.num_active_colors, "parallelism": "single GPU",
+                   "primal_variant": variant,
+                   "primal_autotune_ms": ({str(k): round(v, 3) for k, v in tuned.items()}
+                                          if tuned else None)},
         "roofline": {
             "bound": "hbm",
             "kernel": PRIMAL_KERNELS[variant] + " (primal sweep)",
