@@ -594,7 +594,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   // or 4 bodies per wavefront
   const bool flow = s->primal_variant >= 6 && pos_map == nullptr;
   const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4
-                      : s->primal_variant == 9 ? 8 : 2;
+                      : s->primal_variant == 9 ? 8 : s->primal_variant == 10 ? 3 : 2;
   // the dual rides along in the sweep when no two neighbours share a color
   const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_opt && !to_resid;
   for (int it = 0; it < I + post; ++it) {
@@ -1286,7 +1286,7 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   switch (option) {
     case AVBD_OPT_PRIMAL_VARIANT:
-      if (value < 0 || value > 9) return set_err(AVBD_E_ARG, "primal variant must be 0 to 9");
+      if (value < 0 || value > 10) return set_err(AVBD_E_ARG, "primal variant must be 0 to 10");
       s->primal_variant = value;
       return AVBD_OK;
     case AVBD_OPT_FUSE_DUAL:

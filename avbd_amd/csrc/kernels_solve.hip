@@ -971,6 +971,9 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   else if (bpw == 8)
     hipLaunchKernelGGL((k_primal_tiled<8, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
+  else if (bpw == 3)
+    hipLaunchKernelGGL((k_primal_tiled<3, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+                       pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else
     hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);

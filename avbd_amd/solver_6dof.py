@@ -657,7 +657,7 @@ class Solver6DOF:
         return out
 
     # ---- primal kernel autotuner (not in the reference) -----------------------
-    PRIMAL_VARIANTS = (6, 7, 8, 9)   # the dataflow sweeps, one to eight bodies per tile
+    PRIMAL_VARIANTS = (6, 7, 8, 9, 10)   # the dataflow sweeps, one to eight bodies per tile
 
     def autotune_primal(self, candidates=None, frames: int = 1) -> dict:
         """Pick the fastest primal sweep kernel for this scene by timing real

@@ -35,7 +35,8 @@ PRIMAL_KERNELS = ["k_primal_color (one launch per color)", "k_primal_tiled<1> pe
                   "k_primal_tiled<2> per color", "k_primal_tiled<4> per color",
                   "k_primal_tiled<8> per color", "k_primal_tiled<1|2> per color",
                   "k_primal_tiled<2, flow> one launch", "k_primal_tiled<1, flow> one launch",
-                  "k_primal_tiled<4, flow> one launch", "k_primal_tiled<8, flow> one launch"]
+                  "k_primal_tiled<4, flow> one launch", "k_primal_tiled<8, flow> one launch",
+                  "k_primal_tiled<3, flow> one launch"]
 TRAFFIC_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_pmc.md"
 
 
@@ -129,8 +130,8 @@ def main() -> int:
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
                          "5 one or two bodies per wavefront by color size; "
-                         "6-9 one dataflow launch per sweep with 2, 1, 4, 8 bodies per tile; "
-                         "default: picked by the autotuner among 6-9 after the warm-up")
+                         "6-10 one dataflow launch per sweep with 2, 1, 4, 8, 3 bodies per tile; "
+                         "default: picked by the autotuner among 6-10 after the warm-up")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

@@ -196,12 +196,12 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * 1, 2, 3, 4 = one wavefront per 1, 2, 4, 8 bodies, 5 = one wavefront per
  * body for colors under 4096 bodies and per two bodies otherwise, 6 = every
  * color of a sweep in one launch, each two-body tile starting as soon as its
- * lower-color neighbours have moved (default), 7, 8, 9 = the same with one,
- * four, eight bodies per tile. All variants give identical results; few
+ * lower-color neighbours have moved (default), 7, 8, 9, 10 = the same with
+ * one, four, eight, three bodies per tile. All variants give identical results; few
  * contacts per body favour more bodies per tile (Solver6DOF.autotune_primal
  * picks by timing). */
 #define AVBD_OPT_PRIMAL_VARIANT 0
-/* AVBD_OPT_FUSE_DUAL: 1 = with the dataflow sweep (variants 6-9) and a
+/* AVBD_OPT_FUSE_DUAL: 1 = with the dataflow sweep (variants 6-10) and a
  * coloring without same-color neighbours, each contact group's dual update
  * runs inside the sweep, in the tile of its higher-color body; 0 (default) =
  * a separate dual launch after every sweep. Both give identical results. */

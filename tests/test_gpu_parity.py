@@ -67,8 +67,8 @@ def _tiled_scene(name):
     return s
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, (6, "fused dual"),
-                                     (9, "fused dual")])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, (6, "fused dual"),
+                                     (9, "fused dual"), (10, "fused dual")])
 @pytest.mark.parametrize("scene", ["pyramid5", "pile", "pins"])
 def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     """The tiled primal kernels sum every system entry in the same order as
