@@ -123,7 +123,7 @@ struct avbd_solver {
 
   // hash
   int hash_cap = 0, pool_max = 0;
-  // Slots of the table as last built: about twice its entries (at most
+  // Slots of the table as last built: about four times its entries (at most
   // hash_cap, the reference's capacity, which sets the allocation). Lookups
   // find the same entries at any capacity above the entry count, and a table
   // sized to the live contacts keeps the scattered probes in cache.
@@ -462,7 +462,8 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
 }
 
 int live_hash_slots(const avbd_solver* s, int entries) {
-  const long long want = std::max(4096LL, 2LL * entries + 1);
+  // load factor at most 1/4: short linear probes for the inserts' CAS
+  const long long want = std::max(4096LL, 4LL * entries + 1);
   return (int)std::min<long long>(s->hash_cap, want);
 }
 

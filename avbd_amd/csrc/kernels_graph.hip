@@ -366,34 +366,44 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
 // ---------------------------------------------------------------------------
 // coloring
 // ---------------------------------------------------------------------------
+// Grid-stride over the pairs with at most kConflictBlocks workgroups and one
+// atomic per workgroup: the count is a single address, and same-address
+// atomics serialise.
+constexpr int kConflictBlocks = 240;
+
 __global__ void __launch_bounds__(256)
 k_count_conflicts(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
                   const int* __restrict__ succ, const int* __restrict__ rpc,
                   const int* __restrict__ color, int* out, const int* __restrict__ scene,
                   int* __restrict__ scene_dirty) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int wsum[4];
   int c = 0;
-  if (p < n_pairs) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     const int ns = succ[p];
     if (ns > 0) {
       const int a = pa[p], b = pb[p];
       if (color[a] == color[b]) {
-        c = ns * rpc[p];
+        c += ns * rpc[p];
         if (scene_dirty) scene_dirty[scene[a]] = 1;  // batched: the scene recolors
       }
     }
   }
-  // wave-level sum, one atomic per wave
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-  if (__lane_id() == 0 && c) atomicAdd(out, c);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    if (t) atomicAdd(out, t);
+  }
 }
 
 void launch_count_conflicts(hipStream_t st, int n_pairs, const int* pa, const int* pb,
                             const int* succ, const int* rpc, const int* color, int* out,
                             const int* scene, int* scene_dirty) {
   if (n_pairs > 0)
-    hipLaunchKernelGGL(k_count_conflicts, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
-                       pa, pb, succ, rpc, color, out, scene, scene_dirty);
+    hipLaunchKernelGGL(k_count_conflicts,
+                       dim3(std::min((n_pairs + 255) / 256, kConflictBlocks)), dim3(256), 0, st,
+                       n_pairs, pa, pb, succ, rpc, color, out, scene, scene_dirty);
 }
 
 // Start of a recolor: every body uncolored, or (batched scenes) only the
