@@ -140,6 +140,7 @@ struct avbd_solver {
   DBuf<int> flow_done;
   int flow_stamp = 0;
   bool colors_conflict_free = false;  // no two neighbours share a color
+  bool conflicts_ready = false;  // this substep's conflict count is in hcnt[4]
   bool csr_built = false;       // a CSR has been built since the last flush
   bool csr_order_stale = false; // recolored since that CSR was built
 
@@ -353,14 +354,21 @@ int recolor(avbd_solver* s, int mode) {
   return AVBD_OK;
 }
 
-// Batched scenes: also marks each scene with a conflict in scene_dirty.
-int count_conflicts(avbd_solver* s, int* out) {
+// Rows of this substep's contact groups whose bodies share a color, into
+// dcnt[4]. Batched scenes: also marks each scene with a conflict in
+// scene_dirty.
+void launch_conflict_count(avbd_solver* s) {
   launch_fill_int(s->st, s->dcnt.p + 4, 1, 0);
   if (s->n_scenes > 0) launch_fill_int(s->st, s->scene_dirty.p, s->n_scenes, 0);
-  launch_count_conflicts(s->st, s->n_contacts > 0 ? s->adj_pairs : 0, s->pa.p, s->pb.p, s->succ.p,
-                         s->rpc.p, s->color.p, s->dcnt.p + 4,
+  // pairs that emitted no group (succ 0) count nothing
+  launch_count_conflicts(s->st, s->adj_pairs, s->pa.p, s->pb.p, s->succ.p, s->rpc.p, s->color.p,
+                         s->dcnt.p + 4,
                          s->n_scenes > 0 ? s->scene_of.p : nullptr,
                          s->n_scenes > 0 ? s->scene_dirty.p : nullptr);
+}
+
+int count_conflicts(avbd_solver* s, int* out) {
+  launch_conflict_count(s);
   CK(readback(s, 4, 1));
   *out = s->hcnt[4];
   return AVBD_OK;
@@ -450,7 +458,11 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
                       s->hkeys.p, s->hstate.p, s->ml_mode ? s->hage.p : nullptr);
     CK(prof_end(s, 0.0));
     if (kept > 0) {
-      CK(readback(s, 2, 2));
+      // the recolor gate's conflict count, on this substep's pairs and the
+      // previous colors, read back with the emission totals (one sync)
+      launch_conflict_count(s);
+      s->conflicts_ready = true;
+      CK(readback(s, 2, 3));
       s->n_contacts = s->hcnt[2];
       s->n_active_rows = s->n_static_rows + s->hcnt[3];
       s->pool_count = s->n_contacts;
@@ -509,6 +521,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   s->pair_count = 0;
   s->n_contacts = 0;
   s->adj_pairs = 0;
+  s->conflicts_ready = false;
   int n_active = s->n_static_rows;
   bool pool_reset = false;  // the pool grew this substep (see the cache refresh)
   rep->row_capacity = s->row_cap;
@@ -544,7 +557,14 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
 
   bool need = color_dirty != 0;
   if (!need) {
-    CK(count_conflicts(s, &rep->color_conflicts));
+    if (s->conflicts_ready) {
+      rep->color_conflicts = s->hcnt[4];  // counted with the emission
+    } else if (s->adj_pairs == 0) {
+      rep->color_conflicts = 0;
+      if (s->n_scenes > 0) launch_fill_int(st, s->scene_dirty.p, s->n_scenes, 0);
+    } else {
+      CK(count_conflicts(s, &rep->color_conflicts));
+    }
     need = rep->color_conflicts > 0;
   } else if (s->n_scenes > 0) {
     launch_fill_int(st, s->scene_dirty.p, s->n_scenes, 1);  // every scene recolors
