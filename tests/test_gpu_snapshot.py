@@ -18,6 +18,11 @@ CASES = [
     ("pile300_f45", lambda cls: scenes.random_pile(300, seed=22, lattice=(7, 7, 7), solver_cls=cls), 45),
     ("pyramid12_jp_f20", lambda cls: scenes.pyramid(12, seed=23, solver_cls=cls,
                                                     coloring_mode="jones_plassmann"), 20),
+    # the full-size scenes of the throughput measurement: the 98,021-cube
+    # pyramid (66 layers, 20 iterations) after 2 frames, and the 10,000-box
+    # pile after 60 frames (its lower layers landed)
+    ("pyramid66_f2", lambda cls: scenes.pyramid(66, seed=0, solver_cls=cls), 2),
+    ("pile10k_f60", lambda cls: scenes.random_pile(10000, seed=0, solver_cls=cls), 60),
 ]
 
 
