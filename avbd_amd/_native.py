@@ -90,6 +90,40 @@ class Report(C.Structure):
     ]
 
 
+class PParams(C.Structure):
+    """avbd_p_params: one particle Solver.step's settings."""
+    _fields_ = [
+        ("dt", C.c_float),
+        ("iterations", C.c_int),
+        ("gravity", C.c_float * 3),
+        ("alpha", C.c_float),
+        ("beta", C.c_float),
+        ("gamma", C.c_float),
+        ("post_stabilize", C.c_int),
+        ("max_speed", C.c_float),
+    ]
+
+
+class PScene(C.Structure):
+    """avbd_p_scene: the particle solver's flushed scene (pointers stay owned
+    by the caller for the duration of avbd_p_flush)."""
+    _fields_ = [
+        ("n_bodies", C.c_int),
+        ("x", F32P), ("v", F32P), ("prev_v", F32P), ("mass", F32P),
+        ("color", I32P), ("n_colors", C.c_int),
+        ("n_rows", C.c_int),
+        ("type", I32P), ("body_a", I32P), ("body_b", I32P), ("anchor", F32P),
+        ("rest", F32P), ("stiffness", F32P), ("fmin", F32P), ("fmax", F32P),
+        ("fracture", F32P), ("sibling", I32P), ("friction", F32P),
+        ("lambda_", F32P), ("penalty", F32P), ("active", I32P),
+        ("con_start", I32P), ("con_idx", I32P),
+        ("n_tets", C.c_int),
+        ("tet_v", I32P), ("tet_rest_vol", F32P), ("tet_stiffness", F32P),
+        ("tet_fracture", F32P), ("tet_lambda", F32P), ("tet_penalty", F32P),
+        ("tet_active", I32P), ("tet_start", I32P), ("tet_idx", I32P),
+    ]
+
+
 def fptr(a: np.ndarray):
     return a.ctypes.data_as(F32P)
 
@@ -136,11 +170,23 @@ SIGNATURES = {
     "mlp_grad": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_float, F32P, F32P], C.c_int),
     "mlp_train": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_int, C.c_int, C.c_float,
                    C.c_float, C.c_uint, F32P], C.c_int),
+    # 3-DOF particle solver (Solver, src/avbd3d/solver.py)
+    "p_last_error": ([], C.c_char_p),
+    "p_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "p_destroy": ([C.c_void_p], C.c_int),
+    "p_flush": ([C.c_void_p, C.POINTER(PScene)], C.c_int),
+    "p_step": ([C.c_void_p, C.POINTER(PParams)], C.c_int),
+    "p_read": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, I32P, F32P, F32P, I32P], C.c_int),
+    "p_write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
+    "p_write_rows": ([C.c_void_p, C.c_int, C.c_int, F32P, F32P], C.c_int),
 }
 
 # entry points of batched scenes and the learned warm start (the checker has none)
 ML_NAMES = ("set_scenes", "mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
             "mlp_grad", "mlp_train")
+
+# entry points of the particle solver (bound separately by the checker)
+P_NAMES = tuple(k for k in SIGNATURES if k.startswith("p_"))
 
 _LIB = None
 
@@ -373,3 +419,83 @@ class NativeHandle:
                                              launches.ctypes.data_as(I64P),
                                              nbytes.ctypes.data_as(F64P)))
         return ms, launches, nbytes
+
+
+class ParticleHandle:
+    """One particle-solver handle (avbd_p_*); all state lives on the GPU."""
+
+    prefix = "avbd_"
+
+    def __init__(self, device: int = 0, lib: C.CDLL | None = None):
+        self.lib = lib if lib is not None else load_library()
+        self.h = C.c_void_p()
+        self._check(self._fn("p_create")(int(device), C.byref(self.h)))
+        self.nb = self.nc = self.nt = 0
+
+    def _fn(self, name: str):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self._fn("p_last_error")()
+            raise AvbdError(f"{self.prefix}p: {msg.decode() if msg else 'error'} (code {rc})")
+
+    def close(self) -> None:
+        if self.h:
+            self._fn("p_destroy")(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def flush(self, arrays: dict) -> None:
+        """Upload a scene given as a dict of numpy arrays named like the
+        avbd_p_scene fields ("lambda" for lambda_)."""
+        keep = {}
+        sc = PScene()
+        for name, ctype in PScene._fields_:
+            key = "lambda" if name == "lambda_" else name
+            val = arrays[key]
+            if ctype is C.c_int:
+                setattr(sc, name, int(val))
+            elif ctype is F32P:
+                a = np.ascontiguousarray(val, np.float32)
+                keep[name] = a
+                setattr(sc, name, fptr(a))
+            else:
+                a = np.ascontiguousarray(val, np.int32)
+                keep[name] = a
+                setattr(sc, name, iptr(a))
+        self._check(self._fn("p_flush")(self.h, C.byref(sc)))
+        self.nb, self.nc, self.nt = sc.n_bodies, sc.n_rows, sc.n_tets
+
+    def step(self, params: PParams) -> None:
+        self._check(self._fn("p_step")(self.h, C.byref(params)))
+
+    def read(self) -> dict[str, np.ndarray]:
+        nb, nc, nt = self.nb, self.nc, self.nt
+        out = {"x": np.zeros((nb, 3), np.float32), "v": np.zeros((nb, 3), np.float32),
+               "prev_v": np.zeros((nb, 3), np.float32), "lambda": np.zeros(nc, np.float32),
+               "penalty": np.zeros(nc, np.float32), "active": np.zeros(nc, np.int32),
+               "tet_lambda": np.zeros(nt, np.float32), "tet_penalty": np.zeros(nt, np.float32),
+               "tet_active": np.zeros(nt, np.int32)}
+        self._check(self._fn("p_read")(
+            self.h, fptr(out["x"]), fptr(out["v"]), fptr(out["prev_v"]), fptr(out["lambda"]),
+            fptr(out["penalty"]), iptr(out["active"]), fptr(out["tet_lambda"]),
+            fptr(out["tet_penalty"]), iptr(out["tet_active"])))
+        return out
+
+    def write_bodies(self, field: int, first: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, np.float32).reshape(-1, 3)
+        self._check(self._fn("p_write_bodies")(self.h, int(field), int(first), int(v.shape[0]),
+                                               fptr(v)))
+
+    def write_rows(self, first: int, n: int, anchor=None, rest=None) -> None:
+        a = None if anchor is None else np.ascontiguousarray(anchor, np.float32).reshape(-1, 3)
+        r = None if rest is None else np.ascontiguousarray(rest, np.float32).reshape(-1)
+        self._check(self._fn("p_write_rows")(self.h, int(first), int(n),
+                                             None if a is None else fptr(a),
+                                             None if r is None else fptr(r)))

@@ -1,0 +1,608 @@
+// kernels_particles.hip — the 3-DOF particle solver step on gfx950 and its
+// C ABI (avbd_p_*, declared in include/avbd.h). Reference: Solver.step
+// (src/avbd3d/solver.py:1124-1423) and its kernels (src/avbd3d/kernels.py).
+//
+// The host (avbd_amd/solver.py) owns the scene, the body coloring and the
+// body -> row / body -> tet lists; the library holds the device state and
+// issues one step: predict, warm start, alpha C0, then per iteration one
+// primal launch per color (over that color's bodies only), the dual and the
+// tet dual, and the velocity finalize. Rows are one record per row, as in
+// the reference; a contact's tangent rows follow their normal row and are
+// updated in the normal row's lane after it, which is the order a serial
+// dual gives them (a tangent's friction bound reads its normal's new lambda).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd.h"
+#include "particles.h"
+
+using namespace avbd;
+using namespace avbd::particles;
+
+namespace {
+
+thread_local std::string p_err;
+int perr(int code, const std::string& m) {
+  p_err = m;
+  return code;
+}
+
+#define PHIP(expr)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return perr(AVBD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+template <class T>
+struct PBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PBuf() = default;
+  PBuf(const PBuf&) = delete;
+  PBuf& operator=(const PBuf&) = delete;
+  ~PBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int upload(const T* src, size_t count) {
+    release();
+    const size_t c = count ? count : 1;
+    if (hipMalloc(&p, c * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return perr(AVBD_E_NOMEM, "hipMalloc failed");
+    }
+    n = c;
+    if (count && src && hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return perr(AVBD_E_HIP, "hipMemcpy (upload) failed");
+    return AVBD_OK;
+  }
+};
+
+__device__ __forceinline__ v3 ld3(const float4* a, int i) {
+  const float4 t = a[i];
+  return mk3(t.x, t.y, t.z);
+}
+__device__ __forceinline__ float4 st3(v3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// predict_inertial (kernels.py:161-207)
+__global__ void __launch_bounds__(256)
+k_p_predict(int nb, float4* __restrict__ x, const float4* __restrict__ v,
+            const float4* __restrict__ prev_v, const float* __restrict__ mass, float dt, float gx,
+            float gy, float gz, float4* __restrict__ initial, float4* __restrict__ inertial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const v3 xi = ld3(x, i);
+  initial[i] = st3(xi);
+  if (mass[i] <= 0.0f) {
+    inertial[i] = st3(xi);
+    return;
+  }
+  const v3 g = mk3(gx, gy, gz);
+  const v3 g_dt2 = mul(g, dt * dt);
+  const v3 vi = ld3(v, i);
+  inertial[i] = st3(add(add(xi, mul(vi, dt)), g_dt2));
+  const v3 accel = divs(sub(vi, ld3(prev_v, i)), dt);
+  const float g_norm = length(g);
+  float w = 0.0f;
+  if (g_norm > 0.0f) {
+    const v3 g_hat = divs(g, g_norm);
+    const float accel_ext = dot(accel, g_hat);
+    w = clampf_(accel_ext / g_norm, 0.0f, 1.0f);
+  }
+  x[i] = st3(add(add(xi, mul(vi, dt)), mul(g_dt2, w)));
+}
+
+// warmstart_duals (kernels.py:210-236)
+__global__ void __launch_bounds__(256)
+k_p_warmstart(int nc, const int* __restrict__ type, const float* __restrict__ stiff, float alpha,
+              float gamma, int post, float* __restrict__ lam, float* __restrict__ pen) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nc) return;
+  float l = lam[j], p = pen[j];
+  warmstart_row(type[j], stiff[j], alpha, gamma, post, l, p);
+  lam[j] = l;
+  pen[j] = p;
+}
+
+// tet_warmstart_duals (kernels.py:815-836)
+__global__ void __launch_bounds__(256)
+k_p_tet_warmstart(int nt, const float* __restrict__ stiff, float alpha, float gamma, int post,
+                  float* __restrict__ lam, float* __restrict__ pen) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  float p = clampf_(pen[t] * gamma, kPenaltyMinTet, kPenaltyMax);
+  if (post == 0) lam[t] = lam[t] * alpha * gamma;
+  const float s = stiff[t];
+  if (!(s != s) && s < kInf) p = fmin_(p, s);
+  pen[t] = p;
+}
+
+// cache_alpha_C0 (kernels.py:740-802)
+__global__ void __launch_bounds__(256)
+k_p_cache_c0(int nc, const float4* __restrict__ x, const float4* __restrict__ initial,
+             const int* __restrict__ type, const int* __restrict__ ra, const int* __restrict__ rb,
+             const float4* __restrict__ anchor, const float* __restrict__ rest,
+             const int* __restrict__ active, float alpha, float* __restrict__ ac0) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nc) return;
+  if (active[j] == 0) {
+    ac0[j] = 0.0f;
+    return;
+  }
+  const int t = type[j], a = ra[j], b = rb[j];
+  const bool has_b = b >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
+  const v3 an = ld3(anchor, j);
+  if (t == kRowTangent) {  // frame-start positions, alpha 1 (kernels.py:785-800)
+    ac0[j] = row_C(t, ld3(initial, a), has_b ? ld3(initial, b) : mk3(0.f, 0.f, 0.f), has_b, an,
+                   rest[j]);
+    return;
+  }
+  const float C0 = row_C(t, ld3(x, a), has_b ? ld3(x, b) : mk3(0.f, 0.f, 0.f), has_b, an, rest[j]);
+  ac0[j] = alpha * C0;
+}
+
+// tet_cache_alpha_C0 (kernels.py:838-864)
+__global__ void __launch_bounds__(256)
+k_p_tet_cache_c0(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
+                 const float* __restrict__ rest_vol, const int* __restrict__ active, float alpha,
+                 float* __restrict__ ac0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  if (active[t] == 0) {
+    ac0[t] = 0.0f;
+    return;
+  }
+  const int4 q = tv[t];
+  const float V = tet_volume(ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w));
+  const float C0 = V / rest_vol[t] - 1.0f;
+  ac0[t] = alpha * C0;
+}
+
+// primal_update for the bodies of one color (kernels.py:335-608): one lane
+// per body, rows in the body's list order, then its tets, then the 3x3 solve.
+__global__ void __launch_bounds__(256)
+k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
+           const float4* __restrict__ inertial, const float* __restrict__ mass,
+           const int* __restrict__ type, const int* __restrict__ ra, const int* __restrict__ rb,
+           const float4* __restrict__ anchor, const float* __restrict__ rest,
+           const float* __restrict__ stiff, const float* __restrict__ lam,
+           const float* __restrict__ pen, const float* __restrict__ fmin,
+           const float* __restrict__ fmax, const float* __restrict__ ac0,
+           const int* __restrict__ active, const int* __restrict__ sibling,
+           const float* __restrict__ friction, const int* __restrict__ con_start,
+           const int* __restrict__ con_idx, const int4* __restrict__ tv,
+           const float* __restrict__ tet_rest, const float* __restrict__ tet_pen,
+           const float* __restrict__ tet_ac0, const int* __restrict__ tet_active,
+           const int* __restrict__ tet_start, const int* __restrict__ tet_idx, float dt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int i = bodies[k];
+  const float m = mass[i];
+  if (m <= 0.0f) return;
+  const float inv_dt2 = 1.0f / (dt * dt);
+  const float mdt2 = m * inv_dt2;
+  m33 lhs = mzero();
+  lhs.m[0][0] = mdt2;
+  lhs.m[1][1] = mdt2;
+  lhs.m[2][2] = mdt2;
+  const v3 xi = ld3(x, i);
+  v3 rhs = mul(sub(xi, ld3(inertial, i)), mdt2);
+  for (int e = con_start[i]; e < con_start[i + 1]; ++e) {
+    const int j = con_idx[e];
+    if (active[j] == 0) continue;
+    const int t = type[j], a = ra[j], b = rb[j];
+    const bool is_pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
+    const bool has_b = b >= 0 && !is_pin;
+    const bool i_is_a = a == i;
+    const int other = has_b ? (i_is_a ? b : a) : i;
+    float C;
+    v3 J;
+    row_CJ(t, i_is_a, xi, ld3(x, other), has_b, ld3(anchor, j), rest[j], C, J);
+    const bool is_t = t == kRowTangent;
+    primal_add(lhs, rhs, J, C, stiff[j], ac0[j], lam[j], pen[j], is_t, friction[j],
+               is_t ? lam[sibling[j]] : 0.0f, fmin[j], fmax[j]);
+  }
+  for (int e = tet_start[i]; e < tet_start[i + 1]; ++e) {
+    const int enc = tet_idx[e];
+    const int tid = enc >> 2, slot = enc & 3;
+    if (tet_active[tid] == 0) continue;
+    const int4 q = tv[tid];
+    tet_primal_add(lhs, rhs, ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w), slot,
+                   tet_rest[tid], tet_ac0[tid], tet_pen[tid]);
+  }
+  const m33 inv = minverse(lhs);
+  x[i] = st3(sub(xi, mvmul(inv, rhs)));
+}
+
+// dual_update (kernels.py:611-701): a non-tangent row's lane updates it and
+// then the tangent rows that follow it and name it as their sibling.
+__device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
+                                           const int* __restrict__ type,
+                                           const int* __restrict__ ra, const int* __restrict__ rb,
+                                           const float4* __restrict__ anchor,
+                                           const float* __restrict__ rest,
+                                           const float* __restrict__ stiff,
+                                           float* __restrict__ lam, float* __restrict__ pen,
+                                           const float* __restrict__ fmin,
+                                           const float* __restrict__ fmax,
+                                           const float* __restrict__ ac0,
+                                           int* __restrict__ active,
+                                           const float* __restrict__ fracture,
+                                           const int* __restrict__ sibling,
+                                           const float* __restrict__ friction, float beta) {
+  if (active[j] == 0) return;
+  const int t = type[j], a = ra[j], b = rb[j];
+  const bool has_b = b >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
+  const float C = row_C(t, ld3(x, a), has_b ? ld3(x, b) : mk3(0.f, 0.f, 0.f), has_b,
+                        ld3(anchor, j), rest[j]);
+  float l = lam[j], p = pen[j];
+  const bool alive = dual_row(t, C, stiff[j], ac0[j], l, p, fmin[j], fmax[j], fracture[j],
+                              friction[j], t == kRowTangent ? lam[sibling[j]] : 0.0f, beta);
+  lam[j] = l;
+  pen[j] = p;
+  if (!alive) active[j] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_p_dual(int nc, const float4* __restrict__ x, const int* __restrict__ type,
+         const int* __restrict__ ra, const int* __restrict__ rb, const float4* __restrict__ anchor,
+         const float* __restrict__ rest, const float* __restrict__ stiff, float* __restrict__ lam,
+         float* __restrict__ pen, const float* __restrict__ fmin, const float* __restrict__ fmax,
+         const float* __restrict__ ac0, int* __restrict__ active,
+         const float* __restrict__ fracture, const int* __restrict__ sibling,
+         const float* __restrict__ friction, float beta) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nc || type[j] == kRowTangent) return;
+  p_dual_one(j, x, type, ra, rb, anchor, rest, stiff, lam, pen, fmin, fmax, ac0, active, fracture,
+             sibling, friction, beta);
+  for (int k = j + 1; k < nc && type[k] == kRowTangent && sibling[k] == j; ++k)
+    p_dual_one(k, x, type, ra, rb, anchor, rest, stiff, lam, pen, fmin, fmax, ac0, active,
+               fracture, sibling, friction, beta);
+}
+
+// tet_dual_update (kernels.py:866-912)
+__global__ void __launch_bounds__(256)
+k_p_tet_dual(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
+             const float* __restrict__ rest_vol, const float* __restrict__ stiff,
+             float* __restrict__ lam, float* __restrict__ pen, const float* __restrict__ ac0,
+             int* __restrict__ active, const float* __restrict__ fracture, float beta) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt || active[t] == 0) return;
+  const int4 q = tv[t];
+  const float V = tet_volume(ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w));
+  const float C = V / rest_vol[t] - 1.0f - ac0[t];
+  const float nl = pen[t] * C;
+  lam[t] = nl;
+  if (fabs_(nl) >= fracture[t]) {
+    active[t] = 0;
+    lam[t] = 0.0f;
+    pen[t] = 0.0f;
+    return;
+  }
+  const float upper = fmin_(kPenaltyMax, stiff[t]);
+  pen[t] = fmin_(pen[t] + beta * fabs_(C), upper);
+}
+
+// finalize_velocity + cap_velocity (kernels.py:707-737)
+__global__ void __launch_bounds__(256)
+k_p_finalize(int nb, const float4* __restrict__ x, const float4* __restrict__ initial,
+             const float* __restrict__ mass, float dt, float4* __restrict__ v,
+             float4* __restrict__ prev_v, int cap, float max_speed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  v3 vi = ld3(v, i);
+  prev_v[i] = st3(vi);
+  if (mass[i] > 0.0f) vi = divs(sub(ld3(x, i), ld3(initial, i)), dt);
+  if (cap) {
+    const float speed = length(vi);
+    if (speed > max_speed) vi = mul(vi, max_speed / speed);
+  }
+  v[i] = st3(vi);
+}
+
+constexpr int kB = 256;
+inline dim3 grid(int n) { return dim3((n + kB - 1) / kB); }
+
+}  // namespace
+
+struct avbd_p_solver {
+  int device = 0;
+  hipStream_t st = nullptr;
+  bool flushed = false;
+  int nb = 0, nc = 0, nt = 0, n_colors = 0;
+  std::vector<int> cstart;  // per-color ranges of `bodies`
+  PBuf<float4> x, v, prev_v, initial, inertial, anchor;
+  PBuf<float> mass, rest, stiff, fmin, fmax, fracture, friction, lam, pen, ac0;
+  PBuf<int> bodies, type, ra, rb, active, sibling, con_start, con_idx;
+  PBuf<int4> tv;
+  PBuf<float> tet_rest, tet_stiff, tet_fracture, tet_lam, tet_pen, tet_ac0;
+  PBuf<int> tet_active, tet_start, tet_idx;
+};
+
+namespace {
+
+std::vector<float4> pack3(const float* a, int n) {
+  std::vector<float4> o((size_t)n);
+  for (int i = 0; i < n; ++i) o[i] = make_float4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0f);
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* avbd_p_last_error(void) { return p_err.c_str(); }
+
+int avbd_p_create(int device, avbd_p_solver** out) {
+  int n = 0;
+  if (!out) return perr(AVBD_E_ARG, "null output");
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return perr(AVBD_E_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return perr(AVBD_E_ARG, "device ordinal out of range");
+  PHIP(hipSetDevice(device));
+  avbd_p_solver* s = new avbd_p_solver();
+  s->device = device;
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return perr(AVBD_E_HIP, "hipStreamCreate failed");
+  }
+  *out = s;
+  return AVBD_OK;
+}
+
+int avbd_p_destroy(avbd_p_solver* s) {
+  if (!s) return AVBD_OK;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->st);
+  (void)hipStreamDestroy(s->st);
+  delete s;
+  return AVBD_OK;
+}
+
+#define PCK(expr)                 \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != AVBD_OK) return r_; \
+  } while (0)
+
+int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
+  if (!s || !sc) return perr(AVBD_E_ARG, "null argument");
+  if (sc->n_bodies < 0 || sc->n_rows < 0 || sc->n_tets < 0 || sc->n_colors < 0)
+    return perr(AVBD_E_ARG, "negative size");
+  const int nb = sc->n_bodies, nc = sc->n_rows, nt = sc->n_tets;
+  for (int j = 0; j < nc; ++j) {
+    const int t = sc->type[j], a = sc->body_a[j], b = sc->body_b[j];
+    const bool pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
+    if (a < 0 || a >= nb || (!pin && b >= nb) || (t == kRowDistance && b < 0))
+      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " names a body out of range");
+    if (t == kRowTangent) {
+      // a tangent row follows its normal row, possibly after other tangents
+      // of the same normal (add_floor_contact's layout)
+      const int sib = sc->sibling[j];
+      bool ok = sib >= 0 && sib < j && sc->type[sib] != kRowTangent;
+      for (int m = sib + 1; ok && m < j; ++m)
+        ok = sc->type[m] == kRowTangent && sc->sibling[m] == sib;
+      if (!ok)
+        return perr(AVBD_E_ARG, "tangent row " + std::to_string(j) +
+                                    " does not follow its normal row");
+    }
+    if (t != kRowPinX && t != kRowPinY && t != kRowPinZ && t != kRowDistance && t != kRowFloor &&
+        t != kRowTangent && t != kRowPlane)
+      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " has an unsupported type");
+  }
+  for (int i = 0; i < nb; ++i)
+    if (sc->color[i] < 0 || sc->color[i] >= sc->n_colors)
+      return perr(AVBD_E_ARG, "body color out of range");
+  // body -> row and body -> tet lists: every entry the primal reads must
+  // name a row / tet / vertex that exists
+  if (sc->con_start[0] != 0 || sc->tet_start[0] != 0)
+    return perr(AVBD_E_ARG, "body lists must start at 0");
+  for (int i = 0; i < nb; ++i)
+    if (sc->con_start[i + 1] < sc->con_start[i] || sc->tet_start[i + 1] < sc->tet_start[i])
+      return perr(AVBD_E_ARG, "body lists are not monotone");
+  for (int k = 0; k < sc->con_start[nb]; ++k)
+    if (sc->con_idx[k] < 0 || sc->con_idx[k] >= nc)
+      return perr(AVBD_E_ARG, "body -> row list names a row out of range");
+  for (int k = 0; k < sc->tet_start[nb]; ++k)
+    if (sc->tet_idx[k] < 0 || (sc->tet_idx[k] >> 2) >= nt)
+      return perr(AVBD_E_ARG, "body -> tet list names a tet out of range");
+  for (int k = 0; k < 4 * nt; ++k)
+    if (sc->tet_v[k] < 0 || sc->tet_v[k] >= nb)
+      return perr(AVBD_E_ARG, "tet vertex out of range");
+  PHIP(hipSetDevice(s->device));
+  PHIP(hipStreamSynchronize(s->st));
+  s->nb = nb;
+  s->nc = nc;
+  s->nt = nt;
+  s->n_colors = sc->n_colors;
+  // bodies bucketed by color (ascending body index inside a color)
+  s->cstart.assign(sc->n_colors + 1, 0);
+  for (int i = 0; i < nb; ++i) s->cstart[sc->color[i] + 1]++;
+  for (int c = 0; c < sc->n_colors; ++c) s->cstart[c + 1] += s->cstart[c];
+  std::vector<int> order((size_t)nb), cur(s->cstart.begin(), s->cstart.end() - 1);
+  for (int i = 0; i < nb; ++i) order[cur[sc->color[i]]++] = i;
+  PCK(s->bodies.upload(order.data(), nb));
+  const auto x = pack3(sc->x, nb), v = pack3(sc->v, nb), pv = pack3(sc->prev_v, nb);
+  PCK(s->x.upload(x.data(), nb));
+  PCK(s->v.upload(v.data(), nb));
+  PCK(s->prev_v.upload(pv.data(), nb));
+  PCK(s->initial.upload(x.data(), nb));
+  PCK(s->inertial.upload(x.data(), nb));
+  PCK(s->mass.upload(sc->mass, nb));
+  const auto an = pack3(sc->anchor, nc);
+  PCK(s->anchor.upload(an.data(), nc));
+  PCK(s->type.upload(sc->type, nc));
+  PCK(s->ra.upload(sc->body_a, nc));
+  PCK(s->rb.upload(sc->body_b, nc));
+  PCK(s->rest.upload(sc->rest, nc));
+  PCK(s->stiff.upload(sc->stiffness, nc));
+  PCK(s->fmin.upload(sc->fmin, nc));
+  PCK(s->fmax.upload(sc->fmax, nc));
+  PCK(s->fracture.upload(sc->fracture, nc));
+  PCK(s->sibling.upload(sc->sibling, nc));
+  PCK(s->friction.upload(sc->friction, nc));
+  PCK(s->lam.upload(sc->lambda, nc));
+  PCK(s->pen.upload(sc->penalty, nc));
+  PCK(s->active.upload(sc->active, nc));
+  std::vector<float> zeros((size_t)std::max(nc, nt), 0.0f);
+  PCK(s->ac0.upload(zeros.data(), nc));
+  PCK(s->con_start.upload(sc->con_start, nb + 1));
+  PCK(s->con_idx.upload(sc->con_idx, sc->con_start[nb]));
+  std::vector<int4> tv((size_t)nt);
+  for (int t = 0; t < nt; ++t)
+    tv[t] = make_int4(sc->tet_v[4 * t], sc->tet_v[4 * t + 1], sc->tet_v[4 * t + 2],
+                      sc->tet_v[4 * t + 3]);
+  PCK(s->tv.upload(tv.data(), nt));
+  PCK(s->tet_rest.upload(sc->tet_rest_vol, nt));
+  PCK(s->tet_stiff.upload(sc->tet_stiffness, nt));
+  PCK(s->tet_fracture.upload(sc->tet_fracture, nt));
+  PCK(s->tet_lam.upload(sc->tet_lambda, nt));
+  PCK(s->tet_pen.upload(sc->tet_penalty, nt));
+  PCK(s->tet_active.upload(sc->tet_active, nt));
+  PCK(s->tet_ac0.upload(zeros.data(), nt));
+  PCK(s->tet_start.upload(sc->tet_start, nb + 1));
+  PCK(s->tet_idx.upload(sc->tet_idx, sc->tet_start[nb]));
+  s->flushed = true;
+  return AVBD_OK;
+}
+
+// Solver.step without self-collision (solver.py:1124-1400).
+int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
+  if (!s || !p) return perr(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return perr(AVBD_E_STATE, "step before flush");
+  const int nb = s->nb, nc = s->nc, nt = s->nt;
+  if (nb == 0) return AVBD_OK;
+  PHIP(hipSetDevice(s->device));
+  hipStream_t st = s->st;
+  const float dt = p->dt;
+  hipLaunchKernelGGL(k_p_predict, grid(nb), dim3(kB), 0, st, nb, s->x.p, s->v.p, s->prev_v.p,
+                     s->mass.p, dt, p->gravity[0], p->gravity[1], p->gravity[2], s->initial.p,
+                     s->inertial.p);
+  const int post = p->post_stabilize ? 1 : 0;
+  auto cache = [&](float alpha) {
+    if (nc > 0)
+      hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
+                         s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->active.p, alpha,
+                         s->ac0.p);
+    if (nt > 0)
+      hipLaunchKernelGGL(k_p_tet_cache_c0, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
+                         s->tet_rest.p, s->tet_active.p, alpha, s->tet_ac0.p);
+  };
+  if (nc > 0)
+    hipLaunchKernelGGL(k_p_warmstart, grid(nc), dim3(kB), 0, st, nc, s->type.p, s->stiff.p,
+                       p->alpha, p->gamma, post, s->lam.p, s->pen.p);
+  if (nc > 0)
+    hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
+                       s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->active.p,
+                       p->alpha, s->ac0.p);
+  if (nt > 0) {
+    hipLaunchKernelGGL(k_p_tet_warmstart, grid(nt), dim3(kB), 0, st, nt, s->tet_stiff.p, p->alpha,
+                       p->gamma, post, s->tet_lam.p, s->tet_pen.p);
+    hipLaunchKernelGGL(k_p_tet_cache_c0, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
+                       s->tet_rest.p, s->tet_active.p, p->alpha, s->tet_ac0.p);
+  }
+  const int I = p->iterations;
+  const bool capv = p->max_speed > 0.0f && p->max_speed < kInf;
+  auto finalize = [&]() {
+    hipLaunchKernelGGL(k_p_finalize, grid(nb), dim3(kB), 0, st, nb, s->x.p, s->initial.p,
+                       s->mass.p, dt, s->v.p, s->prev_v.p, capv ? 1 : 0, p->max_speed);
+  };
+  for (int it = 0; it < I + post; ++it) {
+    if (post && it == I) cache(0.0f);
+    for (int c = 0; c < s->n_colors; ++c) {
+      const int cnt = s->cstart[c + 1] - s->cstart[c];
+      if (cnt <= 0) continue;
+      hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt, s->bodies.p + s->cstart[c],
+                         s->x.p, s->inertial.p, s->mass.p, s->type.p, s->ra.p, s->rb.p,
+                         s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p, s->fmin.p,
+                         s->fmax.p, s->ac0.p, s->active.p, s->sibling.p, s->friction.p,
+                         s->con_start.p, s->con_idx.p, s->tv.p, s->tet_rest.p, s->tet_pen.p,
+                         s->tet_ac0.p, s->tet_active.p, s->tet_start.p, s->tet_idx.p, dt);
+    }
+    if (nc > 0 && it < I)
+      hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->type.p, s->ra.p,
+                         s->rb.p, s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p,
+                         s->fmin.p, s->fmax.p, s->ac0.p, s->active.p, s->fracture.p,
+                         s->sibling.p, s->friction.p, p->beta);
+    if (nt > 0 && it < I)
+      hipLaunchKernelGGL(k_p_tet_dual, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
+                         s->tet_rest.p, s->tet_stiff.p, s->tet_lam.p, s->tet_pen.p, s->tet_ac0.p,
+                         s->tet_active.p, s->tet_fracture.p, p->beta);
+    if (it == I - 1 && nt == 0) finalize();
+  }
+  if (nt > 0) finalize();
+  PHIP(hipGetLastError());
+  return AVBD_OK;
+}
+
+int avbd_p_read(avbd_p_solver* s, float* x, float* v, float* prev_v, float* lambda,
+                float* penalty, int* active, float* tet_lambda, float* tet_penalty,
+                int* tet_active) {
+  if (!s || !s->flushed) return perr(AVBD_E_STATE, "read before flush");
+  PHIP(hipSetDevice(s->device));
+  PHIP(hipStreamSynchronize(s->st));
+  std::vector<float4> tmp((size_t)std::max(s->nb, 1));
+  auto unpack = [&](const PBuf<float4>& b, float* out) -> int {
+    if (!out || s->nb == 0) return AVBD_OK;
+    PHIP(hipMemcpy(tmp.data(), b.p, s->nb * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->nb; ++i) {
+      out[3 * i] = tmp[i].x;
+      out[3 * i + 1] = tmp[i].y;
+      out[3 * i + 2] = tmp[i].z;
+    }
+    return AVBD_OK;
+  };
+  PCK(unpack(s->x, x));
+  PCK(unpack(s->v, v));
+  PCK(unpack(s->prev_v, prev_v));
+  if (s->nc > 0) {
+    if (lambda) PHIP(hipMemcpy(lambda, s->lam.p, s->nc * sizeof(float), hipMemcpyDeviceToHost));
+    if (penalty) PHIP(hipMemcpy(penalty, s->pen.p, s->nc * sizeof(float), hipMemcpyDeviceToHost));
+    if (active) PHIP(hipMemcpy(active, s->active.p, s->nc * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  if (s->nt > 0) {
+    if (tet_lambda)
+      PHIP(hipMemcpy(tet_lambda, s->tet_lam.p, s->nt * sizeof(float), hipMemcpyDeviceToHost));
+    if (tet_penalty)
+      PHIP(hipMemcpy(tet_penalty, s->tet_pen.p, s->nt * sizeof(float), hipMemcpyDeviceToHost));
+    if (tet_active)
+      PHIP(hipMemcpy(tet_active, s->tet_active.p, s->nt * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  return AVBD_OK;
+}
+
+int avbd_p_write_bodies(avbd_p_solver* s, int field, int first, int n, const float* vals) {
+  if (!s || !s->flushed) return perr(AVBD_E_STATE, "write before flush");
+  if (first < 0 || n < 0 || first + n > s->nb || !vals) return perr(AVBD_E_ARG, "bad range");
+  PBuf<float4>* dst = field == 0 ? &s->x : field == 1 ? &s->v : nullptr;
+  if (!dst) return perr(AVBD_E_ARG, "field must be 0 (x) or 1 (v)");
+  PHIP(hipSetDevice(s->device));
+  PHIP(hipStreamSynchronize(s->st));
+  const auto packed = pack3(vals, n);
+  if (n > 0)
+    PHIP(hipMemcpy(dst->p + first, packed.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  return AVBD_OK;
+}
+
+int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, const float* rest) {
+  if (!s || !s->flushed) return perr(AVBD_E_STATE, "write before flush");
+  if (first < 0 || n < 0 || first + n > s->nc) return perr(AVBD_E_ARG, "bad range");
+  PHIP(hipSetDevice(s->device));
+  PHIP(hipStreamSynchronize(s->st));
+  if (anchor && n > 0) {
+    const auto packed = pack3(anchor, n);
+    PHIP(hipMemcpy(s->anchor.p + first, packed.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (rest && n > 0)
+    PHIP(hipMemcpy(s->rest.p + first, rest, n * sizeof(float), hipMemcpyHostToDevice));
+  return AVBD_OK;
+}
+
+}  // extern "C"

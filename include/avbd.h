@@ -271,6 +271,78 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
                    float* params, int steps, int batch, float lr, float huber, unsigned seed,
                    float* loss_hist);
 
+/* ---- 3-DOF particle solver ---------------------------------------------
+ * Solver.step (src/avbd3d/solver.py:1124-1400) without self-collision:
+ * particles with pin (x/y/z), distance, floor (with x/z friction rows),
+ * plane and tet-volume constraints. The host (avbd_amd/solver.py, after
+ * Solver._flush :964-1122) supplies the rows in the reference's per-row
+ * layout, the body coloring (coloring.py:41-55) and the body -> row and
+ * body -> tet lists (solver.py:922-962). Vectors are 3 floats. */
+typedef struct avbd_p_params {
+  float dt;
+  int iterations;
+  float gravity[3];
+  float alpha, beta, gamma;
+  int post_stabilize;
+  float max_speed;             /* <= 0 or +inf: no cap */
+} avbd_p_params;
+
+typedef struct avbd_p_scene {
+  int n_bodies;
+  const float* x;
+  const float* v;
+  const float* prev_v;
+  const float* mass;
+  const int* color;            /* per body, < n_colors */
+  int n_colors;
+  int n_rows;
+  const int* type;             /* 0-2 pin x/y/z, 3 distance, 4 floor, 6 tangent, 10 plane */
+  const int* body_a;
+  const int* body_b;
+  const float* anchor;         /* 3 per row */
+  const float* rest;
+  const float* stiffness;
+  const float* fmin;
+  const float* fmax;
+  const float* fracture;
+  const int* sibling;
+  const float* friction;
+  const float* lambda;
+  const float* penalty;
+  const int* active;
+  const int* con_start;        /* n_bodies + 1 */
+  const int* con_idx;
+  int n_tets;
+  const int* tet_v;            /* 4 per tet */
+  const float* tet_rest_vol;
+  const float* tet_stiffness;
+  const float* tet_fracture;
+  const float* tet_lambda;
+  const float* tet_penalty;
+  const int* tet_active;
+  const int* tet_start;        /* n_bodies + 1 */
+  const int* tet_idx;          /* (tet << 2) | vertex slot */
+} avbd_p_scene;
+
+typedef struct avbd_p_solver avbd_p_solver;
+const char* avbd_p_last_error(void);
+int avbd_p_create(int device, avbd_p_solver** out);
+int avbd_p_destroy(avbd_p_solver* s);
+/* Solver._flush: replaces the device state with the scene's. */
+int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* scene);
+/* One Solver.step: predict, warm start, alpha C0, iterations of per-color
+ * primal launches and the dual, velocity finalize. */
+int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p);
+/* positions()/velocities()/lambdas()/active(); any pointer may be NULL. */
+int avbd_p_read(avbd_p_solver* s, float* x, float* v, float* prev_v, float* lambda,
+                float* penalty, int* active, float* tet_lambda, float* tet_penalty,
+                int* tet_active);
+/* set_position / set_velocity (solver.py:899-920): field 0 = x, 1 = v. */
+int avbd_p_write_bodies(avbd_p_solver* s, int field, int first, int n, const float* values);
+/* set_plane_contact_bulk (solver.py:868-897): rows [first, first+n); anchor
+ * (3 per row) and rest may be NULL. */
+int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, const float* rest);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,11 @@ def load() -> C.CDLL:
                                        C.POINTER(C.c_int), F, F, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), F, F]
         lib.orc_probe_pair.restype = C.c_int
+        lib.orc_p_probe_row.argtypes = [C.c_int, C.c_int, F, F, C.c_int, F, C.c_float,
+                                        C.POINTER(C.c_float), F]
+        lib.orc_p_probe_row.restype = None
+        lib.orc_p_probe_tet.argtypes = [F, C.c_int, C.c_float, C.c_float, C.c_float, F, F]
+        lib.orc_p_probe_tet.restype = None
         _L = lib
     return _L
 
@@ -100,6 +105,27 @@ def oracle_solver_class():
     return OracleSolver6DOF
 
 
+class ParticleOracleHandle(N.ParticleHandle):
+    """The CPU restatement of the particle step behind the ParticleHandle
+    method set (orc_p_*)."""
+
+    prefix = "orc_"
+
+    def __init__(self, device: int = 0):
+        super().__init__(device, lib=load())
+
+
+def particle_solver_class():
+    """avbd_amd.solver.Solver whose handle is the CPU restatement."""
+    from avbd_amd.solver import Solver
+
+    class OracleSolver(Solver):
+        def _make_handle(self):
+            return ParticleOracleHandle()
+
+    return OracleSolver
+
+
 def probe_pair(xa, qa, ea, xb, qb, eb, margin=0.005):
     """SAT + manifold of one explicit box pair through the restatement."""
     lib = load()
@@ -113,3 +139,24 @@ def probe_pair(xa, qa, ea, xb, qb, eb, margin=0.005):
                        C.byref(sat), n, depth, C.byref(cnt), C.byref(refa), offr, offi)
     return {"overlap": ov.value, "sat_idx": sat.value, "n_hat": n, "depth": float(depth[0]),
             "count": cnt.value, "ref_is_a": refa.value, "off_ref": offr, "off_inc": offi}
+
+
+def probe_particle_row(t, i_is_a, xi, xo, has_b, anchor, rest):
+    """(C, dC/dx_i) of one particle-solver row through the restatement."""
+    lib = load()
+    f = lambda v: np.ascontiguousarray(v, np.float32)
+    c = C.c_float(0.0)
+    J = np.zeros(3, np.float32)
+    lib.orc_p_probe_row(int(t), int(bool(i_is_a)), f(xi), f(xo), int(bool(has_b)), f(anchor),
+                        float(rest), C.byref(c), J)
+    return float(c.value), J
+
+
+def probe_tet(p, slot, V0, ac0, pen):
+    """One tet's (lhs, rhs) contribution to vertex `slot` through the restatement."""
+    lib = load()
+    lhs = np.zeros(9, np.float32)
+    rhs = np.zeros(3, np.float32)
+    lib.orc_p_probe_tet(np.ascontiguousarray(p, np.float32).reshape(-1), int(slot), float(V0),
+                        float(ac0), float(pen), lhs, rhs)
+    return lhs.reshape(3, 3), rhs

@@ -1,0 +1,127 @@
+"""Particle solver on the GPU against the CPU checker, step by step: every
+array the step updates (positions, velocities, previous velocities, row
+lambda / penalty / active, tet lambda / penalty / active) must agree bit for
+bit after every step."""
+import math
+
+import numpy as np
+import pytest
+
+from avbd_amd import particle_scenes as PS
+from avbd_amd.solver import Solver
+from oracle.oracle import particle_solver_class
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("x", "v", "prev_v", "lambda", "penalty", "active", "tet_lambda", "tet_penalty",
+        "tet_active")
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.int32) if a.dtype == np.float32 else a
+
+
+def _compare(g, o, where):
+    for k in KEYS:
+        if not np.array_equal(_bits(g[k]), _bits(o[k])):
+            bad = np.argwhere(_bits(g[k]) != _bits(o[k]))
+            i = tuple(bad[0])
+            raise AssertionError(f"{where}: {k} differs at {i} (gpu {g[k][i]!r}, "
+                                 f"checker {o[k][i]!r}; {len(bad)} entries)")
+
+
+def _lockstep(build, steps, hook=None, **kw):
+    print(f"lockstep {build.__name__} steps={steps} {kw}", flush=True)
+    sg, so = Solver(**kw), particle_solver_class()(**kw)
+    build(sg)
+    build(so)
+    for f in range(steps):
+        if hook is not None:
+            hook(sg, f)
+            hook(so, f)
+        sg.step()
+        so.step()
+        _compare(sg.state(), so.state(), f"step {f}")
+    return sg, so
+
+
+def chain_fracture(s):
+    PS.chain(s, 6, 0.5, fracture=30.0)
+
+
+def test_chain_with_fracture_lockstep():
+    sg, _ = _lockstep(chain_fracture, 150, dt=1 / 60, iterations=10)
+    assert sg.active()[3:].min() == 0  # a link broke during the run
+
+
+def soft_block(s):
+    PS.soft_block(s, n=(3, 3, 3), size=(0.4, 0.4, 0.4), center=(0.0, 0.5, 0.0))
+
+
+def test_soft_block_on_floor_lockstep():
+    """tets, soft distance rows, floor rows with friction rows, post-stab and
+    the finalize after it (solver.py:1374-1390)."""
+    _lockstep(soft_block, 90, dt=1 / 60, iterations=12)
+
+
+def cloth(s):
+    PS.cloth(s, 20, 16, spacing=0.05)
+
+
+def test_cloth_without_post_stab_lockstep():
+    _lockstep(cloth, 60, dt=1 / 60, iterations=8, post_stabilize=False, max_speed=2.0)
+
+
+def plates(s):
+    for k in range(8):
+        b = s.add_particle((0.1 * k, 0.3, 0.05 * k), 1.0 + 0.1 * k, collide=False, friction=0.3)
+        s.add_floor_contact(b, floor_y=0.0)
+        s.add_plane_contact(b, (1.0, 0.2, 0.0), -0.5)
+    a, b = s.add_particle((2.0, 1.0, 0.0), 1.0), s.add_particle((2.0, 0.7, 0.0), 1.0)
+    s.add_pin_axes(a, (2.0, 1.0, 0.0), (False, True, True))
+    s.add_distance(a, b, 0.3, stiffness=2.0e4)
+
+
+def plates_hook(s, f):
+    if f == 10:
+        s.set_plane_contact_bulk([type("H", (), {"index": 1 + 4 * k})() for k in range(8)],
+                                 offset=0.2)
+    if f == 20:
+        s.set_velocity(type("B", (), {"index": 3})(), (1.0, 2.0, 0.0))
+        s.add_impulse(type("B", (), {"index": 8})(), (0.5, 0.0, 0.5))
+    if f == 30:  # a body added mid-run: re-flush with state kept
+        c = s.add_particle((3.0, 0.5, 0.0), 1.0, collide=False)
+        s.add_floor_contact(c, friction=0.6)
+
+
+def test_plates_perturbations_and_reflush_lockstep():
+    _lockstep(plates, 60, hook=plates_hook, dt=1 / 60, iterations=10)
+
+
+def test_large_cloth_one_step_bit_exact():
+    """256 x 256 cloth (65,536 particles, 261,122 rows), 3 steps."""
+    def big(s):
+        PS.cloth(s, 256, 256, spacing=0.02)
+    sg, _ = _lockstep(big, 3, dt=1 / 60, iterations=10)
+    assert sg.num_colors >= 4
+
+
+def test_empty_scene_and_static_only():
+    s = Solver()
+    s.step()
+    assert s.positions().shape == (0, 3)
+    s.add_particle((1.0, 2.0, 3.0), 0.0)
+    for _ in range(3):
+        s.step()
+    assert s.positions()[0].tolist() == [1.0, 2.0, 3.0]
+
+
+def test_flush_rejects_bad_rows():
+    from avbd_amd import _native as N
+    s = Solver()
+    a = s.add_particle((0, 0, 0), 1.0)
+    s.add_particle((0, 1, 0), 1.0)
+    s.add_distance(a, type("B", (), {"index": 7})(), 1.0)
+    with pytest.raises(N.AvbdError):
+        s.step()
