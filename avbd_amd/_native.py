@@ -179,14 +179,21 @@ SIGNATURES = {
     "p_read": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, I32P, F32P, F32P, I32P], C.c_int),
     "p_write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
     "p_write_rows": ([C.c_void_p, C.c_int, C.c_int, F32P, F32P], C.c_int),
+    "p_set_option": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "p_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
+    "p_profile_read": ([C.c_void_p, F64P, I64P, F64P], C.c_int),
 }
 
 # entry points of batched scenes and the learned warm start (the checker has none)
 ML_NAMES = ("set_scenes", "mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
             "mlp_grad", "mlp_train")
 
-# entry points of the particle solver (bound separately by the checker)
+# entry points of the particle solver; the checker has all but the last
+# three (graph replay and launch timing are properties of the GPU path)
 P_NAMES = tuple(k for k in SIGNATURES if k.startswith("p_"))
+P_GPU_ONLY = ("p_set_option", "p_profile_enable", "p_profile_read")
+P_OPT_GRAPH = 0
+P_OPT_PRIMAL_VARIANT = 1
 
 _LIB = None
 
@@ -499,3 +506,20 @@ class ParticleHandle:
         self._check(self._fn("p_write_rows")(self.h, int(first), int(n),
                                              None if a is None else fptr(a),
                                              None if r is None else fptr(r)))
+
+    def set_graph(self, on: bool) -> None:
+        self.set_option(P_OPT_GRAPH, int(bool(on)))
+
+    def set_option(self, option: int, value: int) -> None:
+        self._check(self._fn("p_set_option")(self.h, int(option), int(value)))
+
+    def profile_enable(self, on: bool) -> None:
+        self._check(self._fn("p_profile_enable")(self.h, int(bool(on))))
+
+    def profile_read(self):
+        ms = np.zeros(3, np.float64)
+        n = np.zeros(3, np.int64)
+        b = np.zeros(3, np.float64)
+        self._check(self._fn("p_profile_read")(self.h, ms.ctypes.data_as(F64P),
+                                               n.ctypes.data_as(I64P), b.ctypes.data_as(F64P)))
+        return ms, n, b

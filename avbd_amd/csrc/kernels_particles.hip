@@ -221,6 +221,119 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
   x[i] = st3(sub(xi, mvmul(inv, rhs)));
 }
 
+// The same primal update with G lanes per body: the lanes evaluate the
+// body's rows and tets (the costly part: gathers and the row/tet math) in
+// parallel into LDS, then lanes 0..8 of the group each add one entry of
+// (rhs, lhs) over the items in list order, so every entry is summed in the
+// serial order of kernels.py:400-599 and rounds exactly as it does there;
+// lane 0 then solves. Items are taken in chunks of kChunk.
+constexpr int kChunk = 64;
+
+template <int G>
+__global__ void __launch_bounds__(256)
+k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x,
+               const float4* __restrict__ inertial, const float* __restrict__ mass,
+               const int* __restrict__ type, const int* __restrict__ ra,
+               const int* __restrict__ rb, const float4* __restrict__ anchor,
+               const float* __restrict__ rest, const float* __restrict__ stiff,
+               const float* __restrict__ lam, const float* __restrict__ pen,
+               const float* __restrict__ fmin, const float* __restrict__ fmax,
+               const float* __restrict__ ac0, const int* __restrict__ active,
+               const int* __restrict__ sibling, const float* __restrict__ friction,
+               const int* __restrict__ con_start, const int* __restrict__ con_idx,
+               const int4* __restrict__ tv, const float* __restrict__ tet_rest,
+               const float* __restrict__ tet_pen, const float* __restrict__ tet_ac0,
+               const int* __restrict__ tet_active, const int* __restrict__ tet_start,
+               const int* __restrict__ tet_idx, float dt) {
+  constexpr int BPB = 256 / G;  // bodies per block
+  __shared__ float terms[BPB][kChunk][9];
+  __shared__ int valid[BPB][kChunk];
+  __shared__ float acc[BPB][9];
+  const int g = threadIdx.x % G, lb = threadIdx.x / G;
+  const int k = blockIdx.x * BPB + lb;
+  const bool live = k < count;
+  const int i = live ? bodies[k] : 0;
+  const float m = live ? mass[i] : 0.0f;
+  const bool dyn = live && m > 0.0f;
+  const int r0 = dyn ? con_start[i] : 0, nr = dyn ? con_start[i + 1] - r0 : 0;
+  const int t0 = dyn ? tet_start[i] : 0, ntt = dyn ? tet_start[i + 1] - t0 : 0;
+  const int n_items = nr + ntt;
+  const v3 xi = live ? ld3(x, i) : mk3(0.0f, 0.0f, 0.0f);
+  // lane e < 9 owns entry e: rhs x, y, z, lhs 00 01 02 11 12 22
+  float a = 0.0f;
+  if (dyn && g < 9) {
+    const float mdt2 = m * (1.0f / (dt * dt));
+    const v3 rhs0 = mul(sub(xi, ld3(inertial, i)), mdt2);
+    a = g == 0 ? rhs0.x : g == 1 ? rhs0.y : g == 2 ? rhs0.z
+        : (g == 3 || g == 6 || g == 8) ? mdt2 : 0.0f;
+  }
+  for (int base = 0; __syncthreads_or(base < n_items); base += kChunk) {
+    const int n_here = min(kChunk, n_items - base);
+    for (int q = g; q < n_here; q += G) {
+      const int item = base + q;
+      bool ok;
+      v3 J;
+      float f, kp;
+      if (item < nr) {
+        const int j = con_idx[r0 + item];
+        ok = active[j] != 0;
+        if (ok) {
+          const int t = type[j], ia = ra[j], ib = rb[j];
+          const bool has_b = ib >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
+          const bool i_is_a = ia == i;
+          const int other = has_b ? (i_is_a ? ib : ia) : i;
+          float C;
+          row_CJ(t, i_is_a, xi, ld3(x, other), has_b, ld3(anchor, j), rest[j], C, J);
+          const bool is_t = t == kRowTangent;
+          kp = pen[j];
+          f = row_force(C, stiff[j], ac0[j], lam[j], kp, is_t, friction[j],
+                        is_t ? lam[sibling[j]] : 0.0f, fmin[j], fmax[j]);
+        }
+      } else {
+        const int enc = tet_idx[t0 + item - nr];
+        const int tid = enc >> 2, slot = enc & 3;
+        ok = tet_active[tid] != 0;
+        if (ok) {
+          const int4 qv = tv[tid];
+          kp = tet_pen[tid];
+          tet_jac_force(ld3(x, qv.x), ld3(x, qv.y), ld3(x, qv.z), ld3(x, qv.w), slot,
+                        tet_rest[tid], tet_ac0[tid], kp, J, f);
+        }
+      }
+      valid[lb][q] = ok ? 1 : 0;
+      if (ok) {
+        const Terms tm = make_terms(J, f, kp);
+        float* dst = terms[lb][q];
+        dst[0] = tm.rhs.x;
+        dst[1] = tm.rhs.y;
+        dst[2] = tm.rhs.z;
+        for (int e = 0; e < 6; ++e) dst[3 + e] = tm.l[e];
+      }
+    }
+    __syncthreads();
+    if (g < 9)
+      for (int q = 0; q < n_here; ++q)
+        if (valid[lb][q]) a = a + terms[lb][q][g];
+    __syncthreads();
+  }
+  if (g < 9) acc[lb][g] = a;
+  __syncthreads();
+  if (dyn && g == 0) {
+    const float* c = acc[lb];
+    m33 lhs;
+    lhs.m[0][0] = c[3];
+    lhs.m[0][1] = c[4];
+    lhs.m[0][2] = c[5];
+    lhs.m[1][0] = c[4];
+    lhs.m[1][1] = c[6];
+    lhs.m[1][2] = c[7];
+    lhs.m[2][0] = c[5];
+    lhs.m[2][1] = c[7];
+    lhs.m[2][2] = c[8];
+    x[i] = st3(sub(xi, mvmul(minverse(lhs), mk3(c[0], c[1], c[2]))));
+  }
+}
+
 // dual_update (kernels.py:611-701): a non-tangent row's lane updates it and
 // then the tangent rows that follow it and name it as their sibling.
 __device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
@@ -317,7 +430,40 @@ struct avbd_p_solver {
   hipStream_t st = nullptr;
   bool flushed = false;
   int nb = 0, nc = 0, nt = 0, n_colors = 0;
-  std::vector<int> cstart;  // per-color ranges of `bodies`
+  std::vector<int> cstart;         // per-color ranges of `bodies`
+  std::vector<double> color_bytes;  // algorithmic bytes of one primal launch per color
+  // one step captured as a graph, replayed while the buffers and the
+  // parameters stay the same (a flush or new parameters capture again)
+  bool use_graph = true;
+  int primal_variant = 1;  // 0: lane per body, 1: 16 lanes per body
+  bool graph_ok = false;
+  avbd_p_params gparams{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // launch timing: one event pair per launch, per class (0 primal, 1 dual
+  // and tet dual, 2 the rest)
+  bool prof = false;
+  std::vector<hipEvent_t> events;
+  struct Pending {
+    int cls;
+    double bytes;
+    hipEvent_t e0, e1;
+  };
+  std::vector<Pending> pending;
+  size_t ev_next = 0;
+  double prof_ms[3] = {0, 0, 0}, prof_bytes[3] = {0, 0, 0};
+  long long prof_launch[3] = {0, 0, 0};
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_ok = false;
+  }
+  ~avbd_p_solver() {
+    drop_graph();
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
   PBuf<float4> x, v, prev_v, initial, inertial, anchor;
   PBuf<float> mass, rest, stiff, fmin, fmax, fracture, friction, lam, pen, ac0;
   PBuf<int> bodies, type, ra, rb, active, sibling, con_start, con_idx;
@@ -418,10 +564,27 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
       return perr(AVBD_E_ARG, "tet vertex out of range");
   PHIP(hipSetDevice(s->device));
   PHIP(hipStreamSynchronize(s->st));
+  s->drop_graph();
   s->nb = nb;
   s->nc = nc;
   s->nt = nt;
   s->n_colors = sc->n_colors;
+  // algorithmic bytes of a primal launch: per body x, inertial, mass, list
+  // bounds and the x store; per listed row its index, record and state, the
+  // other body's x for two-body rows, the normal's lambda for friction rows;
+  // per listed tet its index, record, state and four vertex positions
+  s->color_bytes.assign(sc->n_colors, 0.0);
+  for (int i = 0; i < nb; ++i) {
+    double b = 12 + 12 + 4 + 8 + 8 + 12;
+    for (int k = sc->con_start[i]; k < sc->con_start[i + 1]; ++k) {
+      const int j = sc->con_idx[k], t = sc->type[j];
+      b += 4 + 56;
+      if (t == kRowTangent) b += 12;
+      if (sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ) b += 12;
+    }
+    b += (double)(sc->tet_start[i + 1] - sc->tet_start[i]) * (4 + 16 + 16 + 48);
+    s->color_bytes[sc->color[i]] += b;
+  }
   // bodies bucketed by color (ascending body index inside a color)
   s->cstart.assign(sc->n_colors + 1, 0);
   for (int i = 0; i < nb; ++i) s->cstart[sc->color[i] + 1]++;
@@ -473,15 +636,54 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   return AVBD_OK;
 }
 
-// Solver.step without self-collision (solver.py:1124-1400).
-int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
-  if (!s || !p) return perr(AVBD_E_ARG, "null argument");
-  if (!s->flushed) return perr(AVBD_E_STATE, "step before flush");
+}  // extern "C"
+
+namespace {
+
+int prof_collect(avbd_p_solver* s) {
+  if (s->pending.empty()) return AVBD_OK;
+  PHIP(hipStreamSynchronize(s->st));
+  for (const auto& q : s->pending) {
+    float ms = 0.0f;
+    PHIP(hipEventElapsedTime(&ms, q.e0, q.e1));
+    s->prof_ms[q.cls] += ms;
+    s->prof_bytes[q.cls] += q.bytes;
+    s->prof_launch[q.cls] += 1;
+  }
+  s->pending.clear();
+  s->ev_next = 0;
+  return AVBD_OK;
+}
+
+hipEvent_t next_event(avbd_p_solver* s) {
+  if (s->ev_next == s->events.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    s->events.push_back(e);
+  }
+  return s->events[s->ev_next++];
+}
+
+// Issues one step's launches on the solver's stream (timed per launch when
+// profiling; captured into a graph otherwise).
+int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   const int nb = s->nb, nc = s->nc, nt = s->nt;
-  if (nb == 0) return AVBD_OK;
-  PHIP(hipSetDevice(s->device));
   hipStream_t st = s->st;
   const float dt = p->dt;
+  auto timed = [&](int cls, double bytes, auto&& launch) -> int {
+    if (!s->prof) {
+      launch();
+      return AVBD_OK;
+    }
+    if (s->pending.size() >= 8192) PCK(prof_collect(s));
+    hipEvent_t e0 = next_event(s), e1 = next_event(s);
+    if (!e0 || !e1) return perr(AVBD_E_HIP, "hipEventCreate failed");
+    PHIP(hipEventRecord(e0, st));
+    launch();
+    PHIP(hipEventRecord(e1, st));
+    s->pending.push_back({cls, bytes, e0, e1});
+    return AVBD_OK;
+  };
   hipLaunchKernelGGL(k_p_predict, grid(nb), dim3(kB), 0, st, nb, s->x.p, s->v.p, s->prev_v.p,
                      s->mass.p, dt, p->gravity[0], p->gravity[1], p->gravity[2], s->initial.p,
                      s->inertial.p);
@@ -519,26 +721,112 @@ int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
     for (int c = 0; c < s->n_colors; ++c) {
       const int cnt = s->cstart[c + 1] - s->cstart[c];
       if (cnt <= 0) continue;
-      hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt, s->bodies.p + s->cstart[c],
-                         s->x.p, s->inertial.p, s->mass.p, s->type.p, s->ra.p, s->rb.p,
-                         s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p, s->fmin.p,
-                         s->fmax.p, s->ac0.p, s->active.p, s->sibling.p, s->friction.p,
-                         s->con_start.p, s->con_idx.p, s->tv.p, s->tet_rest.p, s->tet_pen.p,
-                         s->tet_ac0.p, s->tet_active.p, s->tet_start.p, s->tet_idx.p, dt);
+      PCK(timed(0, s->color_bytes[c], [&] {
+        if (s->primal_variant == 0)
+          hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt,
+                             s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
+                             s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->stiff.p,
+                             s->lam.p, s->pen.p, s->fmin.p, s->fmax.p, s->ac0.p, s->active.p,
+                             s->sibling.p, s->friction.p, s->con_start.p, s->con_idx.p, s->tv.p,
+                             s->tet_rest.p, s->tet_pen.p, s->tet_ac0.p, s->tet_active.p,
+                             s->tet_start.p, s->tet_idx.p, dt);
+        else
+          hipLaunchKernelGGL(k_p_primal_grp<16>, dim3((cnt + 15) / 16), dim3(kB), 0, st, cnt,
+                             s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
+                             s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->stiff.p,
+                             s->lam.p, s->pen.p, s->fmin.p, s->fmax.p, s->ac0.p, s->active.p,
+                             s->sibling.p, s->friction.p, s->con_start.p, s->con_idx.p, s->tv.p,
+                             s->tet_rest.p, s->tet_pen.p, s->tet_ac0.p, s->tet_active.p,
+                             s->tet_start.p, s->tet_idx.p, dt);
+      }));
     }
     if (nc > 0 && it < I)
-      hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->type.p, s->ra.p,
-                         s->rb.p, s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p,
-                         s->fmin.p, s->fmax.p, s->ac0.p, s->active.p, s->fracture.p,
-                         s->sibling.p, s->friction.p, p->beta);
+      PCK(timed(1, 0.0, [&] {
+        hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->type.p, s->ra.p,
+                           s->rb.p, s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p,
+                           s->fmin.p, s->fmax.p, s->ac0.p, s->active.p, s->fracture.p,
+                           s->sibling.p, s->friction.p, p->beta);
+      }));
     if (nt > 0 && it < I)
-      hipLaunchKernelGGL(k_p_tet_dual, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
-                         s->tet_rest.p, s->tet_stiff.p, s->tet_lam.p, s->tet_pen.p, s->tet_ac0.p,
-                         s->tet_active.p, s->tet_fracture.p, p->beta);
+      PCK(timed(1, 0.0, [&] {
+        hipLaunchKernelGGL(k_p_tet_dual, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
+                           s->tet_rest.p, s->tet_stiff.p, s->tet_lam.p, s->tet_pen.p,
+                           s->tet_ac0.p, s->tet_active.p, s->tet_fracture.p, p->beta);
+      }));
     if (it == I - 1 && nt == 0) finalize();
   }
   if (nt > 0) finalize();
   PHIP(hipGetLastError());
+  return AVBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Solver.step without self-collision (solver.py:1124-1400).
+int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
+  if (!s || !p) return perr(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return perr(AVBD_E_STATE, "step before flush");
+  if (p->iterations < 0) return perr(AVBD_E_ARG, "negative iteration count");
+  if (s->nb == 0) return AVBD_OK;
+  PHIP(hipSetDevice(s->device));
+  if (!s->use_graph || s->prof) return enqueue_step(s, p);
+  if (!s->graph_ok || std::memcmp(&s->gparams, p, sizeof(*p)) != 0) {
+    s->drop_graph();
+    PHIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(s, p);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s->st, &g);
+    if (rc != AVBD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return perr(AVBD_E_HIP, std::string("stream capture: ") + hipGetErrorString(ec));
+    s->graph = g;
+    PHIP(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->gparams = *p;
+    s->graph_ok = true;
+  }
+  PHIP(hipGraphLaunch(s->gexec, s->st));
+  return AVBD_OK;
+}
+
+int avbd_p_set_option(avbd_p_solver* s, int option, int value) {
+  if (!s) return perr(AVBD_E_ARG, "null solver");
+  if (option == AVBD_P_OPT_GRAPH) {
+    s->use_graph = value != 0;
+  } else if (option == AVBD_P_OPT_PRIMAL_VARIANT) {
+    if (value < 0 || value > 1) return perr(AVBD_E_ARG, "primal variant must be 0 or 1");
+    s->primal_variant = value;
+  } else {
+    return perr(AVBD_E_ARG, "unknown option");
+  }
+  s->drop_graph();
+  return AVBD_OK;
+}
+
+int avbd_p_profile_enable(avbd_p_solver* s, int on) {
+  if (!s) return perr(AVBD_E_ARG, "null solver");
+  PCK(prof_collect(s));
+  s->prof = on != 0;
+  for (int k = 0; k < 3; ++k) {
+    s->prof_ms[k] = 0.0;
+    s->prof_bytes[k] = 0.0;
+    s->prof_launch[k] = 0;
+  }
+  return AVBD_OK;
+}
+
+int avbd_p_profile_read(avbd_p_solver* s, double* ms, long long* launches, double* bytes) {
+  if (!s) return perr(AVBD_E_ARG, "null solver");
+  PHIP(hipSetDevice(s->device));
+  PCK(prof_collect(s));
+  for (int k = 0; k < 3; ++k) {
+    if (ms) ms[k] = s->prof_ms[k];
+    if (launches) launches[k] = s->prof_launch[k];
+    if (bytes) bytes[k] = s->prof_bytes[k];
+  }
   return AVBD_OK;
 }
 

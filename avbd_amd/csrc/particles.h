@@ -88,22 +88,55 @@ AVBD_HD void warmstart_row(int t, float stiff, float alpha, float gamma, int pos
   pen = p;
 }
 
+// Row force f = clamp(k C~ + lambda_eff, bounds) (kernels.py:507-530).
+AVBD_HD float row_force(float C, float stiff, float ac0, float lam, float pen, bool is_t,
+                        float mu, float lam_sib, float fmin, float fmax) {
+  if (stiff >= kInf) C = C - ac0;
+  const float lam_eff = stiff >= kInf ? lam : 0.0f;
+  if (is_t) {
+    const float bound = mu * fabs_(lam_sib);
+    return clampf_(pen * C + lam_eff, -bound, bound);
+  }
+  return clampf_(pen * C + lam_eff, fmin, fmax);
+}
+
+// A row's or a tet's terms of the body system: J f for rhs and
+// (J_r J_c) k for lhs (kernels.py:532-541, 595-599). lhs is symmetric (both
+// products of an off-diagonal pair round alike), so six entries are kept:
+// 00 01 02 11 12 22.
+struct Terms {
+  v3 rhs;
+  float l[6];
+};
+AVBD_HD Terms make_terms(v3 J, float f, float pen) {
+  Terms t;
+  t.rhs = mul(J, f);
+  t.l[0] = (J.x * J.x) * pen;
+  t.l[1] = (J.x * J.y) * pen;
+  t.l[2] = (J.x * J.z) * pen;
+  t.l[3] = (J.y * J.y) * pen;
+  t.l[4] = (J.y * J.z) * pen;
+  t.l[5] = (J.z * J.z) * pen;
+  return t;
+}
+AVBD_HD void add_terms(m33& lhs, v3& rhs, const Terms& t) {
+  rhs = add(rhs, t.rhs);
+  lhs.m[0][0] = lhs.m[0][0] + t.l[0];
+  lhs.m[0][1] = lhs.m[0][1] + t.l[1];
+  lhs.m[0][2] = lhs.m[0][2] + t.l[2];
+  lhs.m[1][0] = lhs.m[1][0] + t.l[1];
+  lhs.m[1][1] = lhs.m[1][1] + t.l[3];
+  lhs.m[1][2] = lhs.m[1][2] + t.l[4];
+  lhs.m[2][0] = lhs.m[2][0] + t.l[2];
+  lhs.m[2][1] = lhs.m[2][1] + t.l[4];
+  lhs.m[2][2] = lhs.m[2][2] + t.l[5];
+}
+
 // Row force and accumulation into (lhs, rhs) (kernels.py:507-541).
 AVBD_HD void primal_add(m33& lhs, v3& rhs, v3 J, float C, float stiff, float ac0, float lam,
                         float pen, bool is_t, float mu, float lam_sib, float fmin, float fmax) {
-  if (stiff >= kInf) C = C - ac0;
-  const float lam_eff = stiff >= kInf ? lam : 0.0f;
-  float f;
-  if (is_t) {
-    const float bound = mu * fabs_(lam_sib);
-    f = clampf_(pen * C + lam_eff, -bound, bound);
-  } else {
-    f = clampf_(pen * C + lam_eff, fmin, fmax);
-  }
-  rhs = add(rhs, mul(J, f));
-  const float Jv[3] = {J.x, J.y, J.z};
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) lhs.m[r][c] = lhs.m[r][c] + (Jv[r] * Jv[c]) * pen;
+  const float f = row_force(C, stiff, ac0, lam, pen, is_t, mu, lam_sib, fmin, fmax);
+  add_terms(lhs, rhs, make_terms(J, f, pen));
 }
 
 // Signed volume ratio pieces of a tet (kernels.py:552-575).
@@ -111,15 +144,14 @@ AVBD_HD float tet_volume(v3 p0, v3 p1, v3 p2, v3 p3) {
   return dot(sub(p1, p0), cross(sub(p2, p0), sub(p3, p0))) * (1.0f / 6.0f);
 }
 
-// A tet's contribution to body `slot`'s system (kernels.py:552-599).
-AVBD_HD void tet_primal_add(m33& lhs, v3& rhs, v3 p0, v3 p1, v3 p2, v3 p3, int slot, float V0,
-                            float ac0, float pen) {
+// A tet's gradient in vertex `slot` and its force k C~ (kernels.py:552-594).
+AVBD_HD void tet_jac_force(v3 p0, v3 p1, v3 p2, v3 p3, int slot, float V0, float ac0, float pen,
+                           v3& J, float& f) {
   const v3 a_e = sub(p1, p0), b_e = sub(p2, p0), c_e = sub(p3, p0);
   const float inv_V0 = 1.0f / V0;
   const float V = dot(a_e, cross(b_e, c_e)) * (1.0f / 6.0f);
   const float C_t = V * inv_V0 - 1.0f - ac0;
   const float s_grad = (1.0f / 6.0f) * inv_V0;
-  v3 J;
   if (slot == 1) {
     J = mul(cross(b_e, c_e), s_grad);
   } else if (slot == 2) {
@@ -131,11 +163,16 @@ AVBD_HD void tet_primal_add(m33& lhs, v3& rhs, v3 p0, v3 p1, v3 p2, v3 p3, int s
              g3 = mul(cross(a_e, b_e), s_grad);
     J = neg(add(add(g1, g2), g3));
   }
-  const float f_t = pen * C_t;
-  rhs = add(rhs, mul(J, f_t));
-  const float Jv[3] = {J.x, J.y, J.z};
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) lhs.m[r][c] = lhs.m[r][c] + (Jv[r] * Jv[c]) * pen;
+  f = pen * C_t;
+}
+
+// A tet's contribution to body `slot`'s system (kernels.py:552-599).
+AVBD_HD void tet_primal_add(m33& lhs, v3& rhs, v3 p0, v3 p1, v3 p2, v3 p3, int slot, float V0,
+                            float ac0, float pen) {
+  v3 J;
+  float f;
+  tet_jac_force(p0, p1, p2, p3, slot, V0, ac0, pen, J, f);
+  add_terms(lhs, rhs, make_terms(J, f, pen));
 }
 
 // Dual update of one row (kernels.py:668-701): returns false when the row

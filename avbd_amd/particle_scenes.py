@@ -2,12 +2,16 @@
 
     chain(s, n)            pinned chain of n distance links (tests/test_solver.py:103-132)
     cloth(s, nx, nz)       grid of particles with structural and shear links, two
-                           pinned corners
+                           pinned corners (soft links: with hard links a flat
+                           sheet pulled perpendicular to its links diverges,
+                           the primal having no geometric-stiffness term,
+                           kernels.py:522-526)
     soft_block(s, n)       tet lattice of a box, bound like bind_tet_mesh_to_solver
                            (src/avbd3d/deformable.py:552-659): one particle per
                            lattice vertex, one distance row per tet edge, floor
                            rows with friction on the surface vertices, one
                            tet-volume row per tet
+    soft_blocks(s, k)      k such blocks side by side on one floor
 
 The block is a regular grid of cells, each split into six tets around the
 cell's main diagonal (the Freudenthal/Kuhn split), in place of the
@@ -33,7 +37,7 @@ def chain(s, n: int = 5, link: float = 0.5, top=(0.0, 5.0, 0.0), fracture: float
 
 
 def cloth(s, nx: int, nz: int, spacing: float = 0.05, height: float = 2.0,
-          mass: float = 0.01, stiffness: float = math.inf, shear: bool = True):
+          mass: float = 0.05, stiffness: float = 2.0e3, shear: bool = True):
     """nx x nz particles in the y = height plane; rows: structural links along
     x and z, shear links on both diagonals; the two corners of the first row
     pinned. Returns the bodies as an (nx, nz) index array."""
@@ -131,3 +135,16 @@ def soft_block(s, n=(3, 3, 3), size=(0.3, 0.3, 0.3), center=(0.0, 0.7, 0.0),
         s.add_tet_volume(*(bodies[v] for v in t), stiffness=float(volume_stiffness))
         for t in lat.tets.tolist()]
     return SoftBlock(lat, np.array([b.index for b in bodies], np.int32), edges, floors, vols)
+
+
+def soft_blocks(s, count: int, n=(6, 6, 6), size=(0.4, 0.4, 0.4), gap: float = 0.2,
+                height: float = 0.7, **kw) -> list[SoftBlock]:
+    """`count` soft blocks on a square grid in x/z, `gap` apart, each
+    dropped from `height` (block centre) onto the floor y = 0."""
+    side = int(math.ceil(math.sqrt(count)))
+    pitch = max(size[0], size[2]) + gap
+    out = []
+    for k in range(count):
+        cx, cz = (k % side) * pitch, (k // side) * pitch
+        out.append(soft_block(s, n=n, size=size, center=(cx, height, cz), **kw))
+    return out

@@ -342,6 +342,22 @@ int avbd_p_write_bodies(avbd_p_solver* s, int field, int first, int n, const flo
 /* set_plane_contact_bulk (solver.py:868-897): rows [first, first+n); anchor
  * (3 per row) and rest may be NULL. */
 int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, const float* rest);
+/* No counterpart in the reference (Warp launches each kernel from Python).
+ * AVBD_P_OPT_GRAPH: 1 (default) captures a step once as a HIP graph and
+ * replays it while the flushed buffers and the parameters stay the same;
+ * 0 launches every kernel directly.
+ * AVBD_P_OPT_PRIMAL_VARIANT: 0 one lane per body, 1 (default) 16 lanes per
+ * body evaluating its rows and tets in parallel.
+ * Results are bit-identical for every setting. */
+#define AVBD_P_OPT_GRAPH 0
+#define AVBD_P_OPT_PRIMAL_VARIANT 1
+int avbd_p_set_option(avbd_p_solver* s, int option, int value);
+/* No counterpart in the reference: per-launch timing with HIP events, by
+ * class (0 primal, 1 dual and tet dual, 2 the rest, untimed). Enabling
+ * resets the totals and launches directly (no graph) while on. */
+int avbd_p_profile_enable(avbd_p_solver* s, int on);
+/* ms, launches, algorithmic bytes per class (3 entries each; NULL skips). */
+int avbd_p_profile_read(avbd_p_solver* s, double* ms, long long* launches, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,7 @@ def load() -> C.CDLL:
                                                                         "set_option",
                                                                         "profile_enable",
                                                                         "profile_read")
-                                   and k not in N.ML_NAMES])
+                                   and k not in N.ML_NAMES and k not in N.P_GPU_ONLY])
         lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.orc_create.restype = C.c_int
         lib.orc_set_capacities.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
