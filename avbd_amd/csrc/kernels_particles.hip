@@ -226,28 +226,28 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
 // parallel into LDS, then lanes 0..8 of the group each add one entry of
 // (rhs, lhs) over the items in list order, so every entry is summed in the
 // serial order of kernels.py:400-599 and rounds exactly as it does there;
-// lane 0 then solves. Items are taken in chunks of kChunk.
+// lane 0 then solves. A skipped item (inactive row or tet) is stored as
+// -0.0 terms: x + (-0.0) == x bit for bit for every x (signed zeros, inf
+// and NaN included), so the sums take no branch. Items are taken in chunks
+// of kChunk. The rows' static fields come packed: int4 {type, body_a,
+// body_b, sibling}, float4 {anchor, rest}, float4 {stiffness, fmin, fmax,
+// friction}.
 constexpr int kChunk = 64;
 
 template <int G>
 __global__ void __launch_bounds__(256)
 k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
-               const int* __restrict__ type, const int* __restrict__ ra,
-               const int* __restrict__ rb, const float4* __restrict__ anchor,
-               const float* __restrict__ rest, const float* __restrict__ stiff,
-               const float* __restrict__ lam, const float* __restrict__ pen,
-               const float* __restrict__ fmin, const float* __restrict__ fmax,
-               const float* __restrict__ ac0, const int* __restrict__ active,
-               const int* __restrict__ sibling, const float* __restrict__ friction,
-               const int* __restrict__ con_start, const int* __restrict__ con_idx,
-               const int4* __restrict__ tv, const float* __restrict__ tet_rest,
-               const float* __restrict__ tet_pen, const float* __restrict__ tet_ac0,
-               const int* __restrict__ tet_active, const int* __restrict__ tet_start,
-               const int* __restrict__ tet_idx, float dt) {
+               const int4* __restrict__ rec_i, const float4* __restrict__ rec_ar,
+               const float4* __restrict__ rec_f, const float* __restrict__ lam,
+               const float* __restrict__ pen, const float* __restrict__ ac0,
+               const int* __restrict__ active, const int* __restrict__ con_start,
+               const int* __restrict__ con_idx, const int4* __restrict__ tv,
+               const float* __restrict__ tet_rest, const float* __restrict__ tet_pen,
+               const float* __restrict__ tet_ac0, const int* __restrict__ tet_active,
+               const int* __restrict__ tet_start, const int* __restrict__ tet_idx, float dt) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
-  __shared__ int valid[BPB][kChunk];
   __shared__ float acc[BPB][9];
   const int g = threadIdx.x % G, lb = threadIdx.x / G;
   const int k = blockIdx.x * BPB + lb;
@@ -278,16 +278,18 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
         const int j = con_idx[r0 + item];
         ok = active[j] != 0;
         if (ok) {
-          const int t = type[j], ia = ra[j], ib = rb[j];
-          const bool has_b = ib >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
-          const bool i_is_a = ia == i;
-          const int other = has_b ? (i_is_a ? ib : ia) : i;
+          const int4 ri = rec_i[j];
+          const float4 ar = rec_ar[j], rf = rec_f[j];
+          const int t = ri.x;
+          const bool has_b = ri.z >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
+          const bool i_is_a = ri.y == i;
+          const int other = has_b ? (i_is_a ? ri.z : ri.y) : i;
           float C;
-          row_CJ(t, i_is_a, xi, ld3(x, other), has_b, ld3(anchor, j), rest[j], C, J);
+          row_CJ(t, i_is_a, xi, ld3(x, other), has_b, mk3(ar.x, ar.y, ar.z), ar.w, C, J);
           const bool is_t = t == kRowTangent;
           kp = pen[j];
-          f = row_force(C, stiff[j], ac0[j], lam[j], kp, is_t, friction[j],
-                        is_t ? lam[sibling[j]] : 0.0f, fmin[j], fmax[j]);
+          f = row_force(C, rf.x, ac0[j], lam[j], kp, is_t, rf.w, is_t ? lam[ri.w] : 0.0f, rf.y,
+                        rf.z);
         }
       } else {
         const int enc = tet_idx[t0 + item - nr];
@@ -300,20 +302,20 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
                         tet_rest[tid], tet_ac0[tid], kp, J, f);
         }
       }
-      valid[lb][q] = ok ? 1 : 0;
+      float* dst = terms[lb][q];
       if (ok) {
         const Terms tm = make_terms(J, f, kp);
-        float* dst = terms[lb][q];
         dst[0] = tm.rhs.x;
         dst[1] = tm.rhs.y;
         dst[2] = tm.rhs.z;
         for (int e = 0; e < 6; ++e) dst[3 + e] = tm.l[e];
+      } else {
+        for (int e = 0; e < 9; ++e) dst[e] = -0.0f;
       }
     }
     __syncthreads();
     if (g < 9)
-      for (int q = 0; q < n_here; ++q)
-        if (valid[lb][q]) a = a + terms[lb][q][g];
+      for (int q = 0; q < n_here; ++q) a = a + terms[lb][q][g];
     __syncthreads();
   }
   if (g < 9) acc[lb][g] = a;
@@ -435,7 +437,7 @@ struct avbd_p_solver {
   // one step captured as a graph, replayed while the buffers and the
   // parameters stay the same (a flush or new parameters capture again)
   bool use_graph = true;
-  int primal_variant = 1;  // 0: lane per body, 1: 16 lanes per body
+  int primal_variant = 2;  // 0: lane per body, 1/2/3: 16/32/64 lanes per body
   bool graph_ok = false;
   avbd_p_params gparams{};
   hipGraph_t graph = nullptr;
@@ -465,6 +467,10 @@ struct avbd_p_solver {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
   }
   PBuf<float4> x, v, prev_v, initial, inertial, anchor;
+  // packed static row fields for the grouped primal
+  PBuf<int4> rec_i;
+  PBuf<float4> rec_ar, rec_f;
+  std::vector<float4> h_ar;  // host copy of rec_ar (plane rows move)
   PBuf<float> mass, rest, stiff, fmin, fmax, fracture, friction, lam, pen, ac0;
   PBuf<int> bodies, type, ra, rb, active, sibling, con_start, con_idx;
   PBuf<int4> tv;
@@ -601,6 +607,19 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   PCK(s->mass.upload(sc->mass, nb));
   const auto an = pack3(sc->anchor, nc);
   PCK(s->anchor.upload(an.data(), nc));
+  {
+    std::vector<int4> ri((size_t)nc);
+    std::vector<float4> rf((size_t)nc);
+    s->h_ar.assign((size_t)nc, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int j = 0; j < nc; ++j) {
+      ri[j] = make_int4(sc->type[j], sc->body_a[j], sc->body_b[j], sc->sibling[j]);
+      rf[j] = make_float4(sc->stiffness[j], sc->fmin[j], sc->fmax[j], sc->friction[j]);
+      s->h_ar[j] = make_float4(an[j].x, an[j].y, an[j].z, sc->rest[j]);
+    }
+    PCK(s->rec_i.upload(ri.data(), nc));
+    PCK(s->rec_f.upload(rf.data(), nc));
+    PCK(s->rec_ar.upload(s->h_ar.data(), nc));
+  }
   PCK(s->type.upload(sc->type, nc));
   PCK(s->ra.upload(sc->body_a, nc));
   PCK(s->rb.upload(sc->body_b, nc));
@@ -730,14 +749,18 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
                              s->sibling.p, s->friction.p, s->con_start.p, s->con_idx.p, s->tv.p,
                              s->tet_rest.p, s->tet_pen.p, s->tet_ac0.p, s->tet_active.p,
                              s->tet_start.p, s->tet_idx.p, dt);
-        else
-          hipLaunchKernelGGL(k_p_primal_grp<16>, dim3((cnt + 15) / 16), dim3(kB), 0, st, cnt,
+        else {
+          const int G = s->primal_variant == 1 ? 16 : s->primal_variant == 2 ? 32 : 64;
+          const int bpb = kB / G;
+          auto kern = G == 16 ? k_p_primal_grp<16> : G == 32 ? k_p_primal_grp<32>
+                                                              : k_p_primal_grp<64>;
+          hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
                              s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
-                             s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->stiff.p,
-                             s->lam.p, s->pen.p, s->fmin.p, s->fmax.p, s->ac0.p, s->active.p,
-                             s->sibling.p, s->friction.p, s->con_start.p, s->con_idx.p, s->tv.p,
-                             s->tet_rest.p, s->tet_pen.p, s->tet_ac0.p, s->tet_active.p,
-                             s->tet_start.p, s->tet_idx.p, dt);
+                             s->rec_i.p, s->rec_ar.p, s->rec_f.p, s->lam.p, s->pen.p, s->ac0.p,
+                             s->active.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tet_rest.p,
+                             s->tet_pen.p, s->tet_ac0.p, s->tet_active.p, s->tet_start.p,
+                             s->tet_idx.p, dt);
+        }
       }));
     }
     if (nc > 0 && it < I)
@@ -797,7 +820,7 @@ int avbd_p_set_option(avbd_p_solver* s, int option, int value) {
   if (option == AVBD_P_OPT_GRAPH) {
     s->use_graph = value != 0;
   } else if (option == AVBD_P_OPT_PRIMAL_VARIANT) {
-    if (value < 0 || value > 1) return perr(AVBD_E_ARG, "primal variant must be 0 or 1");
+    if (value < 0 || value > 3) return perr(AVBD_E_ARG, "primal variant must be 0 to 3");
     s->primal_variant = value;
   } else {
     return perr(AVBD_E_ARG, "unknown option");
@@ -887,9 +910,19 @@ int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, c
   if (anchor && n > 0) {
     const auto packed = pack3(anchor, n);
     PHIP(hipMemcpy(s->anchor.p + first, packed.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (int k = 0; k < n; ++k) {
+      s->h_ar[first + k].x = packed[k].x;
+      s->h_ar[first + k].y = packed[k].y;
+      s->h_ar[first + k].z = packed[k].z;
+    }
   }
-  if (rest && n > 0)
+  if (rest && n > 0) {
     PHIP(hipMemcpy(s->rest.p + first, rest, n * sizeof(float), hipMemcpyHostToDevice));
+    for (int k = 0; k < n; ++k) s->h_ar[first + k].w = rest[k];
+  }
+  if ((anchor || rest) && n > 0)
+    PHIP(hipMemcpy(s->rec_ar.p + first, s->h_ar.data() + first, n * sizeof(float4),
+                   hipMemcpyHostToDevice));
   return AVBD_OK;
 }
 

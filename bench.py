@@ -13,6 +13,11 @@ timed region starts. The line also carries iterations to the dual residual
 ln 1024))) boxes each (the 10k-box pile's generator per scene), 10
 iterations, stepped as one batch; same metric over all scenes.
 
+--workload particles: the 3-DOF particle Solver on 256 soft blocks (tet
+lattices of 6^3 cells: 87,808 particles, 642,048 rows, 331,776 tets) dropped
+on a floor, dt 1/60, 20 iterations + post-stabilisation. Metric:
+particle-steps/s.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -109,10 +114,104 @@ def iters_to_residual(frames: int = 30) -> dict:
             "scope": scope}
 
 
+PARTICLE_TRAFFIC_FILE = ROOT / "profiles" / "k_p_primal_grp_softblocks256_pmc.md"
+PARTICLE_KERNELS = ["k_p_primal (lane per body)", "k_p_primal_grp<16>", "k_p_primal_grp<32>",
+                    "k_p_primal_grp<64>"]
+
+
+def particle_traffic(blocks: int, variant: int) -> float | None:
+    if blocks != 256 or not PARTICLE_TRAFFIC_FILE.exists():
+        return None
+    fields = dict(line.split(":", 1) for line in PARTICLE_TRAFFIC_FILE.read_text().splitlines()
+                  if line.startswith(("traffic_bytes_per_launch:", "primal_variant:")))
+    if int(fields.get("primal_variant", "-1")) != variant:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
+
+
+def particle_bench(args) -> dict:
+    """The particle Solver's step on soft blocks; timed steps with the step
+    graph replayed, then a few steps with per-launch event timing for the
+    primal kernel's roofline line."""
+    import torch
+
+    from avbd_amd import particle_scenes as PS
+    from avbd_amd.solver import Solver
+    t0 = time.perf_counter()
+    s = Solver(dt=1 / 60, iterations=20)
+    PS.soft_blocks(s, args.blocks)
+    s._flush()
+    if args.primal_variant is not None:
+        s._h.set_option(1, args.primal_variant)
+    variant = args.primal_variant if args.primal_variant is not None else 2
+    s._h.set_option(1, variant)
+    nb = len(s._bodies_x)
+    print(f"# scene: {nb} particles, {len(s._constraints)} rows, {len(s._tets)} tets, "
+          f"{s.num_colors} colors, build+flush {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s._h.profile_enable(True)
+    for _ in range(3):
+        s.step()
+    ms, launches, nbytes = s._h.profile_read()
+    s._h.profile_enable(False)
+    assert bool(__import__("numpy").isfinite(s.positions()).all()), "particle state diverged"
+    prim_n = int(launches[0])
+    avg_s = float(ms[0]) / prim_n * 1e-3
+    bytes_per = float(nbytes[0]) / prim_n
+    achieved = bytes_per / avg_s / 1e9
+    out = {
+        "metric": f"particle-steps/s, {args.blocks} soft blocks",
+        "value": nb * args.steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.blocks} soft blocks (6^3-cell tet lattices, edge rows "
+                               f"5e4, tet rows 1e4, floor + friction rows) falling on a floor, "
+                               f"dt 1/60, 20 iterations + post-stabilisation",
+                   "particles": nb, "rows": len(s._constraints), "tets": len(s._tets),
+                   "colors": s.num_colors, "parallelism": "single GPU",
+                   "primal_variant": variant},
+        "roofline": {"bound": "hbm", "kernel": PARTICLE_KERNELS[variant] + " (one launch per color)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": particle_traffic(args.blocks, variant),
+                     "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per},
+        "kernel_ms_3_steps": {"primal": round(float(ms[0]), 3), "dual": round(float(ms[1]), 3)},
+    }
+    if args.cpu_baseline:
+        from oracle.oracle import particle_solver_class
+        so = particle_solver_class()(dt=1 / 60, iterations=20)
+        PS.soft_blocks(so, args.blocks)
+        so._flush()
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_substeps):
+            so.step()
+        so.positions()
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nb * args.cpu_substeps / el, "unit": "particle-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the same scene, {args.cpu_substeps} steps from the "
+                                         f"initial state, C++ restatement on one host thread "
+                                         f"({el:.1f} s)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("pyramid", "batched"), default="pyramid")
+    ap.add_argument("--workload", choices=("pyramid", "batched", "particles"),
+                    default="pyramid")
+    ap.add_argument("--blocks", type=int, default=256, help="particles: number of soft blocks")
     ap.add_argument("--scenes", type=int, default=2048, help="batched: number of scenes")
     ap.add_argument("--residual", type=int, default=1,
                     help="1: add iterations to residual, plain vs learned warm start")
@@ -147,6 +246,10 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"# copied {n} bytes 5 times", file=sys.stderr, flush=True)
+        return 0
+
+    if args.workload == "particles":
+        print(json.dumps(particle_bench(args)), flush=True)
         return 0
 
     from avbd_amd import scenes
