@@ -482,17 +482,24 @@ class ParticleHandle:
     def step(self, params: PParams) -> None:
         self._check(self._fn("p_step")(self.h, C.byref(params)))
 
-    def read(self) -> dict[str, np.ndarray]:
-        nb, nc, nt = self.nb, self.nc, self.nt
-        out = {"x": np.zeros((nb, 3), np.float32), "v": np.zeros((nb, 3), np.float32),
-               "prev_v": np.zeros((nb, 3), np.float32), "lambda": np.zeros(nc, np.float32),
-               "penalty": np.zeros(nc, np.float32), "active": np.zeros(nc, np.int32),
-               "tet_lambda": np.zeros(nt, np.float32), "tet_penalty": np.zeros(nt, np.float32),
-               "tet_active": np.zeros(nt, np.int32)}
-        self._check(self._fn("p_read")(
-            self.h, fptr(out["x"]), fptr(out["v"]), fptr(out["prev_v"]), fptr(out["lambda"]),
-            fptr(out["penalty"]), iptr(out["active"]), fptr(out["tet_lambda"]),
-            fptr(out["tet_penalty"]), iptr(out["tet_active"])))
+    _READ = (("x", 3, np.float32), ("v", 3, np.float32), ("prev_v", 3, np.float32),
+             ("lambda", 0, np.float32), ("penalty", 0, np.float32), ("active", 0, np.int32),
+             ("tet_lambda", 0, np.float32), ("tet_penalty", 0, np.float32),
+             ("tet_active", 0, np.int32))
+
+    def read(self, keys=None) -> dict[str, np.ndarray]:
+        """Device state; `keys` limits it to some of x, v, prev_v, lambda,
+        penalty, active, tet_lambda, tet_penalty, tet_active."""
+        out, args = {}, []
+        for key, width, dt in self._READ:
+            n = self.nb if width else (self.nt if key.startswith("tet") else self.nc)
+            if keys is not None and key not in keys:
+                args.append(None)
+                continue
+            a = np.zeros((n, 3) if width else n, dt)
+            out[key] = a
+            args.append(fptr(a) if dt == np.float32 else iptr(a))
+        self._check(self._fn("p_read")(self.h, *args))
         return out
 
     def write_bodies(self, field: int, first: int, values: np.ndarray) -> None:

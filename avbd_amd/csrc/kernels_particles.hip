@@ -6,10 +6,16 @@
 // body -> row / body -> tet lists; the library holds the device state and
 // issues one step: predict, warm start, alpha C0, then per iteration one
 // primal launch per color (over that color's bodies only), the dual and the
-// tet dual, and the velocity finalize. Rows are one record per row, as in
-// the reference; a contact's tangent rows follow their normal row and are
-// updated in the normal row's lane after it, which is the order a serial
-// dual gives them (a tangent's friction bound reads its normal's new lambda).
+// tet dual, and the velocity finalize.
+//
+// Layout: a row is one 64-byte record (PRow) holding its static fields and
+// its state, so the primal's gather of a listed row touches one half cache
+// line; a tet is an int4 of vertices plus a 16-byte state record. Rows keep
+// the reference's indices. A contact's friction rows follow their normal
+// row and are updated in the normal row's lane after it, which is the order
+// a serial dual gives them (a friction row's bound reads its normal's new
+// lambda).
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +39,12 @@ int perr(int code, const std::string& m) {
     hipError_t e_ = (expr);                                                          \
     if (e_ != hipSuccess)                                                            \
       return perr(AVBD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define PCK(expr)                 \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != AVBD_OK) return r_; \
   } while (0)
 
 template <class T>
@@ -62,11 +74,25 @@ struct PBuf {
   }
 };
 
+// One row: meta {type, body_a, body_b, sibling}, ar {anchor, rest},
+// lim {stiffness, fmin, fmax, friction}, st {lambda, penalty, alpha C0,
+// active (1 or 0)}.
+struct alignas(64) PRow {
+  int4 meta;
+  float4 ar;
+  float4 lim;
+  float4 st;
+};
+static_assert(sizeof(PRow) == 64, "row record is 64 bytes");
+
 __device__ __forceinline__ v3 ld3(const float4* a, int i) {
   const float4 t = a[i];
   return mk3(t.x, t.y, t.z);
 }
 __device__ __forceinline__ float4 st3(v3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+__device__ __forceinline__ bool is_pin(int t) {
+  return t == kRowPinX || t == kRowPinY || t == kRowPinZ;
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -101,68 +127,103 @@ k_p_predict(int nb, float4* __restrict__ x, const float4* __restrict__ v,
 
 // warmstart_duals (kernels.py:210-236)
 __global__ void __launch_bounds__(256)
-k_p_warmstart(int nc, const int* __restrict__ type, const float* __restrict__ stiff, float alpha,
-              float gamma, int post, float* __restrict__ lam, float* __restrict__ pen) {
+k_p_warmstart(int nc, PRow* __restrict__ rows, float alpha, float gamma, int post) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nc) return;
-  float l = lam[j], p = pen[j];
-  warmstart_row(type[j], stiff[j], alpha, gamma, post, l, p);
-  lam[j] = l;
-  pen[j] = p;
+  const int t = rows[j].meta.x;
+  const float stiff = rows[j].lim.x;
+  float4 st = rows[j].st;
+  warmstart_row(t, stiff, alpha, gamma, post, st.x, st.y);
+  rows[j].st = st;
 }
 
-// tet_warmstart_duals (kernels.py:815-836)
+// tet_warmstart_duals (kernels.py:815-836); tst = {rest, alpha C0, penalty, active}
 __global__ void __launch_bounds__(256)
-k_p_tet_warmstart(int nt, const float* __restrict__ stiff, float alpha, float gamma, int post,
-                  float* __restrict__ lam, float* __restrict__ pen) {
+k_p_tet_warmstart(int nt, float4* __restrict__ tst, float* __restrict__ lam,
+                  const float* __restrict__ stiff, float alpha, float gamma, int post) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  float p = clampf_(pen[t] * gamma, kPenaltyMinTet, kPenaltyMax);
+  float p = clampf_(tst[t].z * gamma, kPenaltyMinTet, kPenaltyMax);
   if (post == 0) lam[t] = lam[t] * alpha * gamma;
   const float s = stiff[t];
   if (!(s != s) && s < kInf) p = fmin_(p, s);
-  pen[t] = p;
+  tst[t].z = p;
 }
 
 // cache_alpha_C0 (kernels.py:740-802)
 __global__ void __launch_bounds__(256)
 k_p_cache_c0(int nc, const float4* __restrict__ x, const float4* __restrict__ initial,
-             const int* __restrict__ type, const int* __restrict__ ra, const int* __restrict__ rb,
-             const float4* __restrict__ anchor, const float* __restrict__ rest,
-             const int* __restrict__ active, float alpha, float* __restrict__ ac0) {
+             PRow* __restrict__ rows, float alpha) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nc) return;
-  if (active[j] == 0) {
-    ac0[j] = 0.0f;
+  if (rows[j].st.w == 0.0f) {
+    rows[j].st.z = 0.0f;
     return;
   }
-  const int t = type[j], a = ra[j], b = rb[j];
-  const bool has_b = b >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
-  const v3 an = ld3(anchor, j);
+  const int4 m = rows[j].meta;
+  const float4 ar = rows[j].ar;
+  const int t = m.x;
+  const bool has_b = m.z >= 0 && !is_pin(t);
+  const v3 an = mk3(ar.x, ar.y, ar.z);
   if (t == kRowTangent) {  // frame-start positions, alpha 1 (kernels.py:785-800)
-    ac0[j] = row_C(t, ld3(initial, a), has_b ? ld3(initial, b) : mk3(0.f, 0.f, 0.f), has_b, an,
-                   rest[j]);
+    rows[j].st.z = row_C(t, ld3(initial, m.y), has_b ? ld3(initial, m.z) : mk3(0.f, 0.f, 0.f),
+                         has_b, an, ar.w);
     return;
   }
-  const float C0 = row_C(t, ld3(x, a), has_b ? ld3(x, b) : mk3(0.f, 0.f, 0.f), has_b, an, rest[j]);
-  ac0[j] = alpha * C0;
+  const float C0 = row_C(t, ld3(x, m.y), has_b ? ld3(x, m.z) : mk3(0.f, 0.f, 0.f), has_b, an, ar.w);
+  rows[j].st.z = alpha * C0;
 }
 
 // tet_cache_alpha_C0 (kernels.py:838-864)
 __global__ void __launch_bounds__(256)
 k_p_tet_cache_c0(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
-                 const float* __restrict__ rest_vol, const int* __restrict__ active, float alpha,
-                 float* __restrict__ ac0) {
+                 float4* __restrict__ tst, float alpha) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  if (active[t] == 0) {
-    ac0[t] = 0.0f;
+  const float4 ts = tst[t];
+  if (ts.w == 0.0f) {
+    tst[t].y = 0.0f;
     return;
   }
   const int4 q = tv[t];
   const float V = tet_volume(ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w));
-  const float C0 = V / rest_vol[t] - 1.0f;
-  ac0[t] = alpha * C0;
+  const float C0 = V / ts.x - 1.0f;
+  tst[t].y = alpha * C0;
+}
+
+// One listed row's J, force and penalty seen from body i (kernels.py:395-530);
+// false for an inactive row.
+__device__ __forceinline__ bool row_item(const PRow* __restrict__ rows,
+                                         const float4* __restrict__ x, int j, int i, v3 xi,
+                                         v3& J, float& f, float& kp) {
+  const PRow r = rows[j];
+  if (r.st.w == 0.0f) return false;
+  const int t = r.meta.x;
+  const bool has_b = r.meta.z >= 0 && !is_pin(t);
+  const bool i_is_a = r.meta.y == i;
+  const int other = has_b ? (i_is_a ? r.meta.z : r.meta.y) : i;
+  float C;
+  row_CJ(t, i_is_a, xi, ld3(x, other), has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, C, J);
+  const bool is_t = t == kRowTangent;
+  kp = r.st.y;
+  f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w,
+                is_t ? rows[r.meta.w].st.x : 0.0f, r.lim.y, r.lim.z);
+  return true;
+}
+
+// One listed tet's J, force and penalty for its vertex `slot`
+// (kernels.py:552-594); false for an inactive tet.
+__device__ __forceinline__ bool tet_item(const int4* __restrict__ tv,
+                                         const float4* __restrict__ tst,
+                                         const float4* __restrict__ x, int enc, v3& J, float& f,
+                                         float& kp) {
+  const int tid = enc >> 2, slot = enc & 3;
+  const float4 ts = tst[tid];
+  if (ts.w == 0.0f) return false;
+  const int4 q = tv[tid];
+  kp = ts.z;
+  tet_jac_force(ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w), slot, ts.x, ts.y, kp, J, f);
+  return true;
 }
 
 // primal_update for the bodies of one color (kernels.py:335-608): one lane
@@ -170,55 +231,29 @@ k_p_tet_cache_c0(int nt, const float4* __restrict__ x, const int4* __restrict__ 
 __global__ void __launch_bounds__(256)
 k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
            const float4* __restrict__ inertial, const float* __restrict__ mass,
-           const int* __restrict__ type, const int* __restrict__ ra, const int* __restrict__ rb,
-           const float4* __restrict__ anchor, const float* __restrict__ rest,
-           const float* __restrict__ stiff, const float* __restrict__ lam,
-           const float* __restrict__ pen, const float* __restrict__ fmin,
-           const float* __restrict__ fmax, const float* __restrict__ ac0,
-           const int* __restrict__ active, const int* __restrict__ sibling,
-           const float* __restrict__ friction, const int* __restrict__ con_start,
+           const PRow* __restrict__ rows, const int* __restrict__ con_start,
            const int* __restrict__ con_idx, const int4* __restrict__ tv,
-           const float* __restrict__ tet_rest, const float* __restrict__ tet_pen,
-           const float* __restrict__ tet_ac0, const int* __restrict__ tet_active,
-           const int* __restrict__ tet_start, const int* __restrict__ tet_idx, float dt) {
+           const float4* __restrict__ tst, const int* __restrict__ tet_start,
+           const int* __restrict__ tet_idx, float dt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const int i = bodies[k];
   const float m = mass[i];
   if (m <= 0.0f) return;
-  const float inv_dt2 = 1.0f / (dt * dt);
-  const float mdt2 = m * inv_dt2;
+  const float mdt2 = m * (1.0f / (dt * dt));
   m33 lhs = mzero();
   lhs.m[0][0] = mdt2;
   lhs.m[1][1] = mdt2;
   lhs.m[2][2] = mdt2;
   const v3 xi = ld3(x, i);
   v3 rhs = mul(sub(xi, ld3(inertial, i)), mdt2);
-  for (int e = con_start[i]; e < con_start[i + 1]; ++e) {
-    const int j = con_idx[e];
-    if (active[j] == 0) continue;
-    const int t = type[j], a = ra[j], b = rb[j];
-    const bool is_pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
-    const bool has_b = b >= 0 && !is_pin;
-    const bool i_is_a = a == i;
-    const int other = has_b ? (i_is_a ? b : a) : i;
-    float C;
-    v3 J;
-    row_CJ(t, i_is_a, xi, ld3(x, other), has_b, ld3(anchor, j), rest[j], C, J);
-    const bool is_t = t == kRowTangent;
-    primal_add(lhs, rhs, J, C, stiff[j], ac0[j], lam[j], pen[j], is_t, friction[j],
-               is_t ? lam[sibling[j]] : 0.0f, fmin[j], fmax[j]);
-  }
-  for (int e = tet_start[i]; e < tet_start[i + 1]; ++e) {
-    const int enc = tet_idx[e];
-    const int tid = enc >> 2, slot = enc & 3;
-    if (tet_active[tid] == 0) continue;
-    const int4 q = tv[tid];
-    tet_primal_add(lhs, rhs, ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w), slot,
-                   tet_rest[tid], tet_ac0[tid], tet_pen[tid]);
-  }
-  const m33 inv = minverse(lhs);
-  x[i] = st3(sub(xi, mvmul(inv, rhs)));
+  v3 J;
+  float f, kp;
+  for (int e = con_start[i]; e < con_start[i + 1]; ++e)
+    if (row_item(rows, x, con_idx[e], i, xi, J, f, kp)) add_terms(lhs, rhs, make_terms(J, f, kp));
+  for (int e = tet_start[i]; e < tet_start[i + 1]; ++e)
+    if (tet_item(tv, tst, x, tet_idx[e], J, f, kp)) add_terms(lhs, rhs, make_terms(J, f, kp));
+  x[i] = st3(sub(xi, mvmul(minverse(lhs), rhs)));
 }
 
 // The same primal update with G lanes per body: the lanes evaluate the
@@ -229,23 +264,17 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
 // lane 0 then solves. A skipped item (inactive row or tet) is stored as
 // -0.0 terms: x + (-0.0) == x bit for bit for every x (signed zeros, inf
 // and NaN included), so the sums take no branch. Items are taken in chunks
-// of kChunk. The rows' static fields come packed: int4 {type, body_a,
-// body_b, sibling}, float4 {anchor, rest}, float4 {stiffness, fmin, fmax,
-// friction}.
+// of kChunk.
 constexpr int kChunk = 64;
 
 template <int G>
 __global__ void __launch_bounds__(256)
 k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
-               const int4* __restrict__ rec_i, const float4* __restrict__ rec_ar,
-               const float4* __restrict__ rec_f, const float* __restrict__ lam,
-               const float* __restrict__ pen, const float* __restrict__ ac0,
-               const int* __restrict__ active, const int* __restrict__ con_start,
+               const PRow* __restrict__ rows, const int* __restrict__ con_start,
                const int* __restrict__ con_idx, const int4* __restrict__ tv,
-               const float* __restrict__ tet_rest, const float* __restrict__ tet_pen,
-               const float* __restrict__ tet_ac0, const int* __restrict__ tet_active,
-               const int* __restrict__ tet_start, const int* __restrict__ tet_idx, float dt) {
+               const float4* __restrict__ tst, const int* __restrict__ tet_start,
+               const int* __restrict__ tet_idx, float dt) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
   __shared__ float acc[BPB][9];
@@ -271,37 +300,10 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
     const int n_here = min(kChunk, n_items - base);
     for (int q = g; q < n_here; q += G) {
       const int item = base + q;
-      bool ok;
       v3 J;
       float f, kp;
-      if (item < nr) {
-        const int j = con_idx[r0 + item];
-        ok = active[j] != 0;
-        if (ok) {
-          const int4 ri = rec_i[j];
-          const float4 ar = rec_ar[j], rf = rec_f[j];
-          const int t = ri.x;
-          const bool has_b = ri.z >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
-          const bool i_is_a = ri.y == i;
-          const int other = has_b ? (i_is_a ? ri.z : ri.y) : i;
-          float C;
-          row_CJ(t, i_is_a, xi, ld3(x, other), has_b, mk3(ar.x, ar.y, ar.z), ar.w, C, J);
-          const bool is_t = t == kRowTangent;
-          kp = pen[j];
-          f = row_force(C, rf.x, ac0[j], lam[j], kp, is_t, rf.w, is_t ? lam[ri.w] : 0.0f, rf.y,
-                        rf.z);
-        }
-      } else {
-        const int enc = tet_idx[t0 + item - nr];
-        const int tid = enc >> 2, slot = enc & 3;
-        ok = tet_active[tid] != 0;
-        if (ok) {
-          const int4 qv = tv[tid];
-          kp = tet_pen[tid];
-          tet_jac_force(ld3(x, qv.x), ld3(x, qv.y), ld3(x, qv.z), ld3(x, qv.w), slot,
-                        tet_rest[tid], tet_ac0[tid], kp, J, f);
-        }
-      }
+      const bool ok = item < nr ? row_item(rows, x, con_idx[r0 + item], i, xi, J, f, kp)
+                                : tet_item(tv, tst, x, tet_idx[t0 + item - nr], J, f, kp);
       float* dst = terms[lb][q];
       if (ok) {
         const Terms tm = make_terms(J, f, kp);
@@ -336,73 +338,58 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
   }
 }
 
-// dual_update (kernels.py:611-701): a non-tangent row's lane updates it and
-// then the tangent rows that follow it and name it as their sibling.
+// dual_update of one row (kernels.py:611-701)
 __device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
-                                           const int* __restrict__ type,
-                                           const int* __restrict__ ra, const int* __restrict__ rb,
-                                           const float4* __restrict__ anchor,
-                                           const float* __restrict__ rest,
-                                           const float* __restrict__ stiff,
-                                           float* __restrict__ lam, float* __restrict__ pen,
-                                           const float* __restrict__ fmin,
-                                           const float* __restrict__ fmax,
-                                           const float* __restrict__ ac0,
-                                           int* __restrict__ active,
-                                           const float* __restrict__ fracture,
-                                           const int* __restrict__ sibling,
-                                           const float* __restrict__ friction, float beta) {
-  if (active[j] == 0) return;
-  const int t = type[j], a = ra[j], b = rb[j];
-  const bool has_b = b >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
-  const float C = row_C(t, ld3(x, a), has_b ? ld3(x, b) : mk3(0.f, 0.f, 0.f), has_b,
-                        ld3(anchor, j), rest[j]);
-  float l = lam[j], p = pen[j];
-  const bool alive = dual_row(t, C, stiff[j], ac0[j], l, p, fmin[j], fmax[j], fracture[j],
-                              friction[j], t == kRowTangent ? lam[sibling[j]] : 0.0f, beta);
-  lam[j] = l;
-  pen[j] = p;
-  if (!alive) active[j] = 0;
+                                           PRow* __restrict__ rows,
+                                           const float* __restrict__ fracture, float beta) {
+  const PRow r = rows[j];
+  if (r.st.w == 0.0f) return;
+  const int t = r.meta.x;
+  const bool has_b = r.meta.z >= 0 && !is_pin(t);
+  const float C = row_C(t, ld3(x, r.meta.y), has_b ? ld3(x, r.meta.z) : mk3(0.f, 0.f, 0.f),
+                        has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w);
+  float l = r.st.x, p = r.st.y;
+  const bool alive = dual_row(t, C, r.lim.x, r.st.z, l, p, r.lim.y, r.lim.z, fracture[j], r.lim.w,
+                              t == kRowTangent ? rows[r.meta.w].st.x : 0.0f, beta);
+  rows[j].st = make_float4(l, p, r.st.z, alive ? 1.0f : 0.0f);
 }
 
+// dual_update (kernels.py:611-701): a non-friction row's lane updates it and
+// then the friction rows that follow it and name it as their sibling.
 __global__ void __launch_bounds__(256)
-k_p_dual(int nc, const float4* __restrict__ x, const int* __restrict__ type,
-         const int* __restrict__ ra, const int* __restrict__ rb, const float4* __restrict__ anchor,
-         const float* __restrict__ rest, const float* __restrict__ stiff, float* __restrict__ lam,
-         float* __restrict__ pen, const float* __restrict__ fmin, const float* __restrict__ fmax,
-         const float* __restrict__ ac0, int* __restrict__ active,
-         const float* __restrict__ fracture, const int* __restrict__ sibling,
-         const float* __restrict__ friction, float beta) {
+k_p_dual(int nc, const float4* __restrict__ x, PRow* __restrict__ rows,
+         const float* __restrict__ fracture, float beta) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nc || type[j] == kRowTangent) return;
-  p_dual_one(j, x, type, ra, rb, anchor, rest, stiff, lam, pen, fmin, fmax, ac0, active, fracture,
-             sibling, friction, beta);
-  for (int k = j + 1; k < nc && type[k] == kRowTangent && sibling[k] == j; ++k)
-    p_dual_one(k, x, type, ra, rb, anchor, rest, stiff, lam, pen, fmin, fmax, ac0, active,
-               fracture, sibling, friction, beta);
+  if (j >= nc || rows[j].meta.x == kRowTangent) return;
+  p_dual_one(j, x, rows, fracture, beta);
+  for (int k = j + 1; k < nc; ++k) {
+    const int4 m = rows[k].meta;
+    if (m.x != kRowTangent || m.w != j) break;
+    p_dual_one(k, x, rows, fracture, beta);
+  }
 }
 
 // tet_dual_update (kernels.py:866-912)
 __global__ void __launch_bounds__(256)
 k_p_tet_dual(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
-             const float* __restrict__ rest_vol, const float* __restrict__ stiff,
-             float* __restrict__ lam, float* __restrict__ pen, const float* __restrict__ ac0,
-             int* __restrict__ active, const float* __restrict__ fracture, float beta) {
+             float4* __restrict__ tst, float* __restrict__ lam, const float* __restrict__ stiff,
+             const float* __restrict__ fracture, float beta) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt || active[t] == 0) return;
+  if (t >= nt) return;
+  const float4 ts = tst[t];
+  if (ts.w == 0.0f) return;
   const int4 q = tv[t];
   const float V = tet_volume(ld3(x, q.x), ld3(x, q.y), ld3(x, q.z), ld3(x, q.w));
-  const float C = V / rest_vol[t] - 1.0f - ac0[t];
-  const float nl = pen[t] * C;
+  const float C = V / ts.x - 1.0f - ts.y;
+  const float nl = ts.z * C;
   lam[t] = nl;
   if (fabs_(nl) >= fracture[t]) {
-    active[t] = 0;
     lam[t] = 0.0f;
-    pen[t] = 0.0f;
+    tst[t] = make_float4(ts.x, ts.y, 0.0f, 0.0f);
     return;
   }
   const float upper = fmin_(kPenaltyMax, stiff[t]);
-  pen[t] = fmin_(pen[t] + beta * fabs_(C), upper);
+  tst[t].z = fmin_(ts.z + beta * fabs_(C), upper);
 }
 
 // finalize_velocity + cap_velocity (kernels.py:707-737)
@@ -420,6 +407,20 @@ k_p_finalize(int nb, const float4* __restrict__ x, const float4* __restrict__ in
     if (speed > max_speed) vi = mul(vi, max_speed / speed);
   }
   v[i] = st3(vi);
+}
+
+// Row state out to a contiguous staging array (for reads), and moved plane
+// rows' {anchor, rest} in from one (for writes).
+__global__ void __launch_bounds__(256)
+k_p_rows_get_state(int nc, const PRow* __restrict__ rows, float4* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nc) out[j] = rows[j].st;
+}
+
+__global__ void __launch_bounds__(256)
+k_p_rows_set_ar(int n, int first, const float4* __restrict__ in, PRow* __restrict__ rows) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) rows[first + k].ar = in[k];
 }
 
 constexpr int kB = 256;
@@ -466,16 +467,16 @@ struct avbd_p_solver {
     drop_graph();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
   }
-  PBuf<float4> x, v, prev_v, initial, inertial, anchor;
-  // packed static row fields for the grouped primal
-  PBuf<int4> rec_i;
-  PBuf<float4> rec_ar, rec_f;
-  std::vector<float4> h_ar;  // host copy of rec_ar (plane rows move)
-  PBuf<float> mass, rest, stiff, fmin, fmax, fracture, friction, lam, pen, ac0;
-  PBuf<int> bodies, type, ra, rb, active, sibling, con_start, con_idx;
+  PBuf<float4> x, v, prev_v, initial, inertial;
+  PBuf<float> mass, fracture;
+  PBuf<PRow> rows;
+  PBuf<float4> stage;        // n_rows entries: row state out, {anchor, rest} in
+  std::vector<float4> h_ar;  // host copy of every row's {anchor, rest} (plane rows move)
+  PBuf<int> bodies, con_start, con_idx;
   PBuf<int4> tv;
-  PBuf<float> tet_rest, tet_stiff, tet_fracture, tet_lam, tet_pen, tet_ac0;
-  PBuf<int> tet_active, tet_start, tet_idx;
+  PBuf<float4> tst;  // {rest volume, alpha C0, penalty, active}
+  PBuf<float> tet_lam, tet_stiff, tet_fracture;
+  PBuf<int> tet_start, tet_idx;
 };
 
 namespace {
@@ -485,179 +486,6 @@ std::vector<float4> pack3(const float* a, int n) {
   for (int i = 0; i < n; ++i) o[i] = make_float4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0f);
   return o;
 }
-
-}  // namespace
-
-extern "C" {
-
-const char* avbd_p_last_error(void) { return p_err.c_str(); }
-
-int avbd_p_create(int device, avbd_p_solver** out) {
-  int n = 0;
-  if (!out) return perr(AVBD_E_ARG, "null output");
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
-    return perr(AVBD_E_HIP, "no HIP device available");
-  if (device < 0 || device >= n) return perr(AVBD_E_ARG, "device ordinal out of range");
-  PHIP(hipSetDevice(device));
-  avbd_p_solver* s = new avbd_p_solver();
-  s->device = device;
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
-    delete s;
-    return perr(AVBD_E_HIP, "hipStreamCreate failed");
-  }
-  *out = s;
-  return AVBD_OK;
-}
-
-int avbd_p_destroy(avbd_p_solver* s) {
-  if (!s) return AVBD_OK;
-  (void)hipSetDevice(s->device);
-  (void)hipStreamSynchronize(s->st);
-  (void)hipStreamDestroy(s->st);
-  delete s;
-  return AVBD_OK;
-}
-
-#define PCK(expr)                 \
-  do {                            \
-    int r_ = (expr);              \
-    if (r_ != AVBD_OK) return r_; \
-  } while (0)
-
-int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
-  if (!s || !sc) return perr(AVBD_E_ARG, "null argument");
-  if (sc->n_bodies < 0 || sc->n_rows < 0 || sc->n_tets < 0 || sc->n_colors < 0)
-    return perr(AVBD_E_ARG, "negative size");
-  const int nb = sc->n_bodies, nc = sc->n_rows, nt = sc->n_tets;
-  for (int j = 0; j < nc; ++j) {
-    const int t = sc->type[j], a = sc->body_a[j], b = sc->body_b[j];
-    const bool pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
-    if (a < 0 || a >= nb || (!pin && b >= nb) || (t == kRowDistance && b < 0))
-      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " names a body out of range");
-    if (t == kRowTangent) {
-      // a tangent row follows its normal row, possibly after other tangents
-      // of the same normal (add_floor_contact's layout)
-      const int sib = sc->sibling[j];
-      bool ok = sib >= 0 && sib < j && sc->type[sib] != kRowTangent;
-      for (int m = sib + 1; ok && m < j; ++m)
-        ok = sc->type[m] == kRowTangent && sc->sibling[m] == sib;
-      if (!ok)
-        return perr(AVBD_E_ARG, "tangent row " + std::to_string(j) +
-                                    " does not follow its normal row");
-    }
-    if (t != kRowPinX && t != kRowPinY && t != kRowPinZ && t != kRowDistance && t != kRowFloor &&
-        t != kRowTangent && t != kRowPlane)
-      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " has an unsupported type");
-  }
-  for (int i = 0; i < nb; ++i)
-    if (sc->color[i] < 0 || sc->color[i] >= sc->n_colors)
-      return perr(AVBD_E_ARG, "body color out of range");
-  // body -> row and body -> tet lists: every entry the primal reads must
-  // name a row / tet / vertex that exists
-  if (sc->con_start[0] != 0 || sc->tet_start[0] != 0)
-    return perr(AVBD_E_ARG, "body lists must start at 0");
-  for (int i = 0; i < nb; ++i)
-    if (sc->con_start[i + 1] < sc->con_start[i] || sc->tet_start[i + 1] < sc->tet_start[i])
-      return perr(AVBD_E_ARG, "body lists are not monotone");
-  for (int k = 0; k < sc->con_start[nb]; ++k)
-    if (sc->con_idx[k] < 0 || sc->con_idx[k] >= nc)
-      return perr(AVBD_E_ARG, "body -> row list names a row out of range");
-  for (int k = 0; k < sc->tet_start[nb]; ++k)
-    if (sc->tet_idx[k] < 0 || (sc->tet_idx[k] >> 2) >= nt)
-      return perr(AVBD_E_ARG, "body -> tet list names a tet out of range");
-  for (int k = 0; k < 4 * nt; ++k)
-    if (sc->tet_v[k] < 0 || sc->tet_v[k] >= nb)
-      return perr(AVBD_E_ARG, "tet vertex out of range");
-  PHIP(hipSetDevice(s->device));
-  PHIP(hipStreamSynchronize(s->st));
-  s->drop_graph();
-  s->nb = nb;
-  s->nc = nc;
-  s->nt = nt;
-  s->n_colors = sc->n_colors;
-  // algorithmic bytes of a primal launch: per body x, inertial, mass, list
-  // bounds and the x store; per listed row its index, record and state, the
-  // other body's x for two-body rows, the normal's lambda for friction rows;
-  // per listed tet its index, record, state and four vertex positions
-  s->color_bytes.assign(sc->n_colors, 0.0);
-  for (int i = 0; i < nb; ++i) {
-    double b = 12 + 12 + 4 + 8 + 8 + 12;
-    for (int k = sc->con_start[i]; k < sc->con_start[i + 1]; ++k) {
-      const int j = sc->con_idx[k], t = sc->type[j];
-      b += 4 + 56;
-      if (t == kRowTangent) b += 12;
-      if (sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ) b += 12;
-    }
-    b += (double)(sc->tet_start[i + 1] - sc->tet_start[i]) * (4 + 16 + 16 + 48);
-    s->color_bytes[sc->color[i]] += b;
-  }
-  // bodies bucketed by color (ascending body index inside a color)
-  s->cstart.assign(sc->n_colors + 1, 0);
-  for (int i = 0; i < nb; ++i) s->cstart[sc->color[i] + 1]++;
-  for (int c = 0; c < sc->n_colors; ++c) s->cstart[c + 1] += s->cstart[c];
-  std::vector<int> order((size_t)nb), cur(s->cstart.begin(), s->cstart.end() - 1);
-  for (int i = 0; i < nb; ++i) order[cur[sc->color[i]]++] = i;
-  PCK(s->bodies.upload(order.data(), nb));
-  const auto x = pack3(sc->x, nb), v = pack3(sc->v, nb), pv = pack3(sc->prev_v, nb);
-  PCK(s->x.upload(x.data(), nb));
-  PCK(s->v.upload(v.data(), nb));
-  PCK(s->prev_v.upload(pv.data(), nb));
-  PCK(s->initial.upload(x.data(), nb));
-  PCK(s->inertial.upload(x.data(), nb));
-  PCK(s->mass.upload(sc->mass, nb));
-  const auto an = pack3(sc->anchor, nc);
-  PCK(s->anchor.upload(an.data(), nc));
-  {
-    std::vector<int4> ri((size_t)nc);
-    std::vector<float4> rf((size_t)nc);
-    s->h_ar.assign((size_t)nc, make_float4(0.f, 0.f, 0.f, 0.f));
-    for (int j = 0; j < nc; ++j) {
-      ri[j] = make_int4(sc->type[j], sc->body_a[j], sc->body_b[j], sc->sibling[j]);
-      rf[j] = make_float4(sc->stiffness[j], sc->fmin[j], sc->fmax[j], sc->friction[j]);
-      s->h_ar[j] = make_float4(an[j].x, an[j].y, an[j].z, sc->rest[j]);
-    }
-    PCK(s->rec_i.upload(ri.data(), nc));
-    PCK(s->rec_f.upload(rf.data(), nc));
-    PCK(s->rec_ar.upload(s->h_ar.data(), nc));
-  }
-  PCK(s->type.upload(sc->type, nc));
-  PCK(s->ra.upload(sc->body_a, nc));
-  PCK(s->rb.upload(sc->body_b, nc));
-  PCK(s->rest.upload(sc->rest, nc));
-  PCK(s->stiff.upload(sc->stiffness, nc));
-  PCK(s->fmin.upload(sc->fmin, nc));
-  PCK(s->fmax.upload(sc->fmax, nc));
-  PCK(s->fracture.upload(sc->fracture, nc));
-  PCK(s->sibling.upload(sc->sibling, nc));
-  PCK(s->friction.upload(sc->friction, nc));
-  PCK(s->lam.upload(sc->lambda, nc));
-  PCK(s->pen.upload(sc->penalty, nc));
-  PCK(s->active.upload(sc->active, nc));
-  std::vector<float> zeros((size_t)std::max(nc, nt), 0.0f);
-  PCK(s->ac0.upload(zeros.data(), nc));
-  PCK(s->con_start.upload(sc->con_start, nb + 1));
-  PCK(s->con_idx.upload(sc->con_idx, sc->con_start[nb]));
-  std::vector<int4> tv((size_t)nt);
-  for (int t = 0; t < nt; ++t)
-    tv[t] = make_int4(sc->tet_v[4 * t], sc->tet_v[4 * t + 1], sc->tet_v[4 * t + 2],
-                      sc->tet_v[4 * t + 3]);
-  PCK(s->tv.upload(tv.data(), nt));
-  PCK(s->tet_rest.upload(sc->tet_rest_vol, nt));
-  PCK(s->tet_stiff.upload(sc->tet_stiffness, nt));
-  PCK(s->tet_fracture.upload(sc->tet_fracture, nt));
-  PCK(s->tet_lam.upload(sc->tet_lambda, nt));
-  PCK(s->tet_pen.upload(sc->tet_penalty, nt));
-  PCK(s->tet_active.upload(sc->tet_active, nt));
-  PCK(s->tet_ac0.upload(zeros.data(), nt));
-  PCK(s->tet_start.upload(sc->tet_start, nb + 1));
-  PCK(s->tet_idx.upload(sc->tet_idx, sc->tet_start[nb]));
-  s->flushed = true;
-  return AVBD_OK;
-}
-
-}  // extern "C"
-
-namespace {
 
 int prof_collect(avbd_p_solver* s) {
   if (s->pending.empty()) return AVBD_OK;
@@ -710,24 +538,22 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   auto cache = [&](float alpha) {
     if (nc > 0)
       hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
-                         s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->active.p, alpha,
-                         s->ac0.p);
+                         s->rows.p, alpha);
     if (nt > 0)
       hipLaunchKernelGGL(k_p_tet_cache_c0, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
-                         s->tet_rest.p, s->tet_active.p, alpha, s->tet_ac0.p);
+                         s->tst.p, alpha);
   };
-  if (nc > 0)
-    hipLaunchKernelGGL(k_p_warmstart, grid(nc), dim3(kB), 0, st, nc, s->type.p, s->stiff.p,
-                       p->alpha, p->gamma, post, s->lam.p, s->pen.p);
-  if (nc > 0)
+  if (nc > 0) {
+    hipLaunchKernelGGL(k_p_warmstart, grid(nc), dim3(kB), 0, st, nc, s->rows.p, p->alpha,
+                       p->gamma, post);
     hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
-                       s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->active.p,
-                       p->alpha, s->ac0.p);
+                       s->rows.p, p->alpha);
+  }
   if (nt > 0) {
-    hipLaunchKernelGGL(k_p_tet_warmstart, grid(nt), dim3(kB), 0, st, nt, s->tet_stiff.p, p->alpha,
-                       p->gamma, post, s->tet_lam.p, s->tet_pen.p);
+    hipLaunchKernelGGL(k_p_tet_warmstart, grid(nt), dim3(kB), 0, st, nt, s->tst.p, s->tet_lam.p,
+                       s->tet_stiff.p, p->alpha, p->gamma, post);
     hipLaunchKernelGGL(k_p_tet_cache_c0, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
-                       s->tet_rest.p, s->tet_active.p, p->alpha, s->tet_ac0.p);
+                       s->tst.p, p->alpha);
   }
   const int I = p->iterations;
   const bool capv = p->max_speed > 0.0f && p->max_speed < kInf;
@@ -741,40 +567,32 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
       const int cnt = s->cstart[c + 1] - s->cstart[c];
       if (cnt <= 0) continue;
       PCK(timed(0, s->color_bytes[c], [&] {
-        if (s->primal_variant == 0)
+        if (s->primal_variant == 0) {
           hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt,
                              s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
-                             s->type.p, s->ra.p, s->rb.p, s->anchor.p, s->rest.p, s->stiff.p,
-                             s->lam.p, s->pen.p, s->fmin.p, s->fmax.p, s->ac0.p, s->active.p,
-                             s->sibling.p, s->friction.p, s->con_start.p, s->con_idx.p, s->tv.p,
-                             s->tet_rest.p, s->tet_pen.p, s->tet_ac0.p, s->tet_active.p,
+                             s->rows.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tst.p,
                              s->tet_start.p, s->tet_idx.p, dt);
-        else {
+        } else {
           const int G = s->primal_variant == 1 ? 16 : s->primal_variant == 2 ? 32 : 64;
           const int bpb = kB / G;
           auto kern = G == 16 ? k_p_primal_grp<16> : G == 32 ? k_p_primal_grp<32>
                                                               : k_p_primal_grp<64>;
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
                              s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
-                             s->rec_i.p, s->rec_ar.p, s->rec_f.p, s->lam.p, s->pen.p, s->ac0.p,
-                             s->active.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tet_rest.p,
-                             s->tet_pen.p, s->tet_ac0.p, s->tet_active.p, s->tet_start.p,
-                             s->tet_idx.p, dt);
+                             s->rows.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tst.p,
+                             s->tet_start.p, s->tet_idx.p, dt);
         }
       }));
     }
     if (nc > 0 && it < I)
       PCK(timed(1, 0.0, [&] {
-        hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->type.p, s->ra.p,
-                           s->rb.p, s->anchor.p, s->rest.p, s->stiff.p, s->lam.p, s->pen.p,
-                           s->fmin.p, s->fmax.p, s->ac0.p, s->active.p, s->fracture.p,
-                           s->sibling.p, s->friction.p, p->beta);
+        hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->rows.p,
+                           s->fracture.p, p->beta);
       }));
     if (nt > 0 && it < I)
       PCK(timed(1, 0.0, [&] {
         hipLaunchKernelGGL(k_p_tet_dual, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
-                           s->tet_rest.p, s->tet_stiff.p, s->tet_lam.p, s->tet_pen.p,
-                           s->tet_ac0.p, s->tet_active.p, s->tet_fracture.p, p->beta);
+                           s->tst.p, s->tet_lam.p, s->tet_stiff.p, s->tet_fracture.p, p->beta);
       }));
     if (it == I - 1 && nt == 0) finalize();
   }
@@ -786,6 +604,150 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
 }  // namespace
 
 extern "C" {
+
+const char* avbd_p_last_error(void) { return p_err.c_str(); }
+
+int avbd_p_create(int device, avbd_p_solver** out) {
+  int n = 0;
+  if (!out) return perr(AVBD_E_ARG, "null output");
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return perr(AVBD_E_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return perr(AVBD_E_ARG, "device ordinal out of range");
+  PHIP(hipSetDevice(device));
+  avbd_p_solver* s = new avbd_p_solver();
+  s->device = device;
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return perr(AVBD_E_HIP, "hipStreamCreate failed");
+  }
+  *out = s;
+  return AVBD_OK;
+}
+
+int avbd_p_destroy(avbd_p_solver* s) {
+  if (!s) return AVBD_OK;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->st);
+  (void)hipStreamDestroy(s->st);
+  delete s;
+  return AVBD_OK;
+}
+
+int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
+  if (!s || !sc) return perr(AVBD_E_ARG, "null argument");
+  if (sc->n_bodies < 0 || sc->n_rows < 0 || sc->n_tets < 0 || sc->n_colors < 0)
+    return perr(AVBD_E_ARG, "negative size");
+  const int nb = sc->n_bodies, nc = sc->n_rows, nt = sc->n_tets;
+  for (int j = 0; j < nc; ++j) {
+    const int t = sc->type[j], a = sc->body_a[j], b = sc->body_b[j];
+    const bool pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
+    if (a < 0 || a >= nb || (!pin && b >= nb) || (t == kRowDistance && b < 0))
+      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " names a body out of range");
+    if (t == kRowTangent) {
+      // a friction row follows its normal row, possibly after other friction
+      // rows of the same normal (add_floor_contact's layout)
+      const int sib = sc->sibling[j];
+      bool ok = sib >= 0 && sib < j && sc->type[sib] != kRowTangent;
+      for (int m = sib + 1; ok && m < j; ++m)
+        ok = sc->type[m] == kRowTangent && sc->sibling[m] == sib;
+      if (!ok)
+        return perr(AVBD_E_ARG, "tangent row " + std::to_string(j) +
+                                    " does not follow its normal row");
+    }
+    if (t != kRowPinX && t != kRowPinY && t != kRowPinZ && t != kRowDistance && t != kRowFloor &&
+        t != kRowTangent && t != kRowPlane)
+      return perr(AVBD_E_ARG, "row " + std::to_string(j) + " has an unsupported type");
+  }
+  for (int i = 0; i < nb; ++i)
+    if (sc->color[i] < 0 || sc->color[i] >= sc->n_colors)
+      return perr(AVBD_E_ARG, "body color out of range");
+  // body -> row and body -> tet lists: every entry the primal reads must
+  // name a row / tet / vertex that exists
+  if (sc->con_start[0] != 0 || sc->tet_start[0] != 0)
+    return perr(AVBD_E_ARG, "body lists must start at 0");
+  for (int i = 0; i < nb; ++i)
+    if (sc->con_start[i + 1] < sc->con_start[i] || sc->tet_start[i + 1] < sc->tet_start[i])
+      return perr(AVBD_E_ARG, "body lists are not monotone");
+  for (int k = 0; k < sc->con_start[nb]; ++k)
+    if (sc->con_idx[k] < 0 || sc->con_idx[k] >= nc)
+      return perr(AVBD_E_ARG, "body -> row list names a row out of range");
+  for (int k = 0; k < sc->tet_start[nb]; ++k)
+    if (sc->tet_idx[k] < 0 || (sc->tet_idx[k] >> 2) >= nt)
+      return perr(AVBD_E_ARG, "body -> tet list names a tet out of range");
+  for (int k = 0; k < 4 * nt; ++k)
+    if (sc->tet_v[k] < 0 || sc->tet_v[k] >= nb)
+      return perr(AVBD_E_ARG, "tet vertex out of range");
+  PHIP(hipSetDevice(s->device));
+  PHIP(hipStreamSynchronize(s->st));
+  s->drop_graph();
+  s->nb = nb;
+  s->nc = nc;
+  s->nt = nt;
+  s->n_colors = sc->n_colors;
+  // algorithmic bytes of a primal launch: per body x, inertial, mass, list
+  // bounds and the x store; per listed row its index and record, the other
+  // body's x for two-body rows, the normal's lambda for friction rows; per
+  // listed tet its index, vertices, state and four vertex positions
+  s->color_bytes.assign(sc->n_colors, 0.0);
+  for (int i = 0; i < nb; ++i) {
+    double b = 12 + 12 + 4 + 8 + 8 + 12;
+    for (int k = sc->con_start[i]; k < sc->con_start[i + 1]; ++k) {
+      const int j = sc->con_idx[k], t = sc->type[j];
+      b += 4 + 64;
+      if (t == kRowTangent) b += 4;
+      if (sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ) b += 12;
+    }
+    b += (double)(sc->tet_start[i + 1] - sc->tet_start[i]) * (4 + 16 + 16 + 48);
+    s->color_bytes[sc->color[i]] += b;
+  }
+  // bodies bucketed by color (ascending body index inside a color)
+  s->cstart.assign(sc->n_colors + 1, 0);
+  for (int i = 0; i < nb; ++i) s->cstart[sc->color[i] + 1]++;
+  for (int c = 0; c < sc->n_colors; ++c) s->cstart[c + 1] += s->cstart[c];
+  std::vector<int> order((size_t)nb), cur(s->cstart.begin(), s->cstart.end() - 1);
+  for (int i = 0; i < nb; ++i) order[cur[sc->color[i]]++] = i;
+  PCK(s->bodies.upload(order.data(), nb));
+  const auto x = pack3(sc->x, nb), v = pack3(sc->v, nb), pv = pack3(sc->prev_v, nb);
+  PCK(s->x.upload(x.data(), nb));
+  PCK(s->v.upload(v.data(), nb));
+  PCK(s->prev_v.upload(pv.data(), nb));
+  PCK(s->initial.upload(x.data(), nb));
+  PCK(s->inertial.upload(x.data(), nb));
+  PCK(s->mass.upload(sc->mass, nb));
+  std::vector<PRow> rows((size_t)nc);
+  s->h_ar.assign((size_t)nc, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int j = 0; j < nc; ++j) {
+    PRow& r = rows[j];
+    r.meta = make_int4(sc->type[j], sc->body_a[j], sc->body_b[j], sc->sibling[j]);
+    r.ar = make_float4(sc->anchor[3 * j], sc->anchor[3 * j + 1], sc->anchor[3 * j + 2],
+                       sc->rest[j]);
+    r.lim = make_float4(sc->stiffness[j], sc->fmin[j], sc->fmax[j], sc->friction[j]);
+    r.st = make_float4(sc->lambda[j], sc->penalty[j], 0.0f, sc->active[j] != 0 ? 1.0f : 0.0f);
+    s->h_ar[j] = r.ar;
+  }
+  PCK(s->rows.upload(rows.data(), nc));
+  PCK(s->stage.upload(nullptr, nc));
+  PCK(s->fracture.upload(sc->fracture, nc));
+  PCK(s->con_start.upload(sc->con_start, nb + 1));
+  PCK(s->con_idx.upload(sc->con_idx, sc->con_start[nb]));
+  std::vector<int4> tv((size_t)nt);
+  std::vector<float4> tst((size_t)nt);
+  for (int t = 0; t < nt; ++t) {
+    tv[t] = make_int4(sc->tet_v[4 * t], sc->tet_v[4 * t + 1], sc->tet_v[4 * t + 2],
+                      sc->tet_v[4 * t + 3]);
+    tst[t] = make_float4(sc->tet_rest_vol[t], 0.0f, sc->tet_penalty[t],
+                         sc->tet_active[t] != 0 ? 1.0f : 0.0f);
+  }
+  PCK(s->tv.upload(tv.data(), nt));
+  PCK(s->tst.upload(tst.data(), nt));
+  PCK(s->tet_lam.upload(sc->tet_lambda, nt));
+  PCK(s->tet_stiff.upload(sc->tet_stiffness, nt));
+  PCK(s->tet_fracture.upload(sc->tet_fracture, nt));
+  PCK(s->tet_start.upload(sc->tet_start, nb + 1));
+  PCK(s->tet_idx.upload(sc->tet_idx, sc->tet_start[nb]));
+  s->flushed = true;
+  return AVBD_OK;
+}
 
 // Solver.step without self-collision (solver.py:1124-1400).
 int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
@@ -805,7 +767,8 @@ int avbd_p_step(avbd_p_solver* s, const avbd_p_params* p) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
-    if (ec != hipSuccess) return perr(AVBD_E_HIP, std::string("stream capture: ") + hipGetErrorString(ec));
+    if (ec != hipSuccess)
+      return perr(AVBD_E_HIP, std::string("stream capture: ") + hipGetErrorString(ec));
     s->graph = g;
     PHIP(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
     s->gparams = *p;
@@ -873,18 +836,31 @@ int avbd_p_read(avbd_p_solver* s, float* x, float* v, float* prev_v, float* lamb
   PCK(unpack(s->x, x));
   PCK(unpack(s->v, v));
   PCK(unpack(s->prev_v, prev_v));
-  if (s->nc > 0) {
-    if (lambda) PHIP(hipMemcpy(lambda, s->lam.p, s->nc * sizeof(float), hipMemcpyDeviceToHost));
-    if (penalty) PHIP(hipMemcpy(penalty, s->pen.p, s->nc * sizeof(float), hipMemcpyDeviceToHost));
-    if (active) PHIP(hipMemcpy(active, s->active.p, s->nc * sizeof(int), hipMemcpyDeviceToHost));
+  if (s->nc > 0 && (lambda || penalty || active)) {
+    std::vector<float4> st((size_t)s->nc);
+    hipLaunchKernelGGL(k_p_rows_get_state, grid(s->nc), dim3(kB), 0, s->st, s->nc, s->rows.p,
+                       s->stage.p);
+    PHIP(hipGetLastError());
+    PHIP(hipMemcpyAsync(st.data(), s->stage.p, s->nc * sizeof(float4), hipMemcpyDeviceToHost,
+                        s->st));
+    PHIP(hipStreamSynchronize(s->st));
+    for (int j = 0; j < s->nc; ++j) {
+      if (lambda) lambda[j] = st[j].x;
+      if (penalty) penalty[j] = st[j].y;
+      if (active) active[j] = st[j].w != 0.0f ? 1 : 0;
+    }
   }
   if (s->nt > 0) {
     if (tet_lambda)
       PHIP(hipMemcpy(tet_lambda, s->tet_lam.p, s->nt * sizeof(float), hipMemcpyDeviceToHost));
-    if (tet_penalty)
-      PHIP(hipMemcpy(tet_penalty, s->tet_pen.p, s->nt * sizeof(float), hipMemcpyDeviceToHost));
-    if (tet_active)
-      PHIP(hipMemcpy(tet_active, s->tet_active.p, s->nt * sizeof(int), hipMemcpyDeviceToHost));
+    if (tet_penalty || tet_active) {
+      std::vector<float4> ts((size_t)s->nt);
+      PHIP(hipMemcpy(ts.data(), s->tst.p, s->nt * sizeof(float4), hipMemcpyDeviceToHost));
+      for (int t = 0; t < s->nt; ++t) {
+        if (tet_penalty) tet_penalty[t] = ts[t].z;
+        if (tet_active) tet_active[t] = ts[t].w != 0.0f ? 1 : 0;
+      }
+    }
   }
   return AVBD_OK;
 }
@@ -905,24 +881,26 @@ int avbd_p_write_bodies(avbd_p_solver* s, int field, int first, int n, const flo
 int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, const float* rest) {
   if (!s || !s->flushed) return perr(AVBD_E_STATE, "write before flush");
   if (first < 0 || n < 0 || first + n > s->nc) return perr(AVBD_E_ARG, "bad range");
+  if (n == 0 || (!anchor && !rest)) return AVBD_OK;
   PHIP(hipSetDevice(s->device));
   PHIP(hipStreamSynchronize(s->st));
-  if (anchor && n > 0) {
-    const auto packed = pack3(anchor, n);
-    PHIP(hipMemcpy(s->anchor.p + first, packed.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-    for (int k = 0; k < n; ++k) {
-      s->h_ar[first + k].x = packed[k].x;
-      s->h_ar[first + k].y = packed[k].y;
-      s->h_ar[first + k].z = packed[k].z;
+  for (int k = 0; k < n; ++k) {
+    float4& a = s->h_ar[first + k];
+    if (anchor) {
+      a.x = anchor[3 * k];
+      a.y = anchor[3 * k + 1];
+      a.z = anchor[3 * k + 2];
     }
+    if (rest) a.w = rest[k];
   }
-  if (rest && n > 0) {
-    PHIP(hipMemcpy(s->rest.p + first, rest, n * sizeof(float), hipMemcpyHostToDevice));
-    for (int k = 0; k < n; ++k) s->h_ar[first + k].w = rest[k];
-  }
-  if ((anchor || rest) && n > 0)
-    PHIP(hipMemcpy(s->rec_ar.p + first, s->h_ar.data() + first, n * sizeof(float4),
-                   hipMemcpyHostToDevice));
+  // the {anchor, rest} field of rows [first, first + n): staged, then
+  // scattered into the 64-byte records
+  PHIP(hipMemcpyAsync(s->stage.p, s->h_ar.data() + first, n * sizeof(float4),
+                      hipMemcpyHostToDevice, s->st));
+  hipLaunchKernelGGL(k_p_rows_set_ar, grid(n), dim3(kB), 0, s->st, n, first, s->stage.p,
+                     s->rows.p);
+  PHIP(hipGetLastError());
+  PHIP(hipStreamSynchronize(s->st));
   return AVBD_OK;
 }
 

@@ -296,7 +296,7 @@ class Solver:
 
     # ---- runtime perturbations (solver.py:899-920) -----------------------------
     def _body_row(self, field: str, index: int) -> np.ndarray:
-        return self._h.read()[field][index]
+        return self._h.read((field,))[field][index]
 
     def set_velocity(self, body: Body, v) -> None:
         self._flush()
@@ -427,7 +427,7 @@ class Solver:
         positions, then the contact pool (empty here) is rebuilt, which
         leaves every row fixed and re-flushes the scene."""
         self._flush()
-        st = self._h.read()
+        st = self._h.read(("x", "v"))
         xs = st["x"]
         bad = ~np.all(np.isfinite(xs), axis=1)
         if bad.any():
@@ -460,22 +460,22 @@ class Solver:
     def positions(self) -> np.ndarray:
         if self._h is None:
             return np.array(self._bodies_x, np.float32).reshape(-1, 3)
-        return self._h.read()["x"]
+        return self._h.read(("x",))["x"]
 
     def velocities(self) -> np.ndarray:
         if self._h is None:
             return np.array(self._bodies_v, np.float32).reshape(-1, 3)
-        return self._h.read()["v"]
+        return self._h.read(("v",))["v"]
 
     def lambdas(self) -> np.ndarray:
         if self._h is None:
             return np.zeros(len(self._constraints), np.float32)
-        return self._h.read()["lambda"]
+        return self._h.read(("lambda",))["lambda"]
 
     def active(self) -> np.ndarray:
         if self._h is None:
             return np.ones(len(self._constraints), np.int32)
-        return self._h.read()["active"]
+        return self._h.read(("active",))["active"]
 
     def state(self) -> dict[str, np.ndarray]:
         """Every device array the step updates (bodies, rows, tets)."""
