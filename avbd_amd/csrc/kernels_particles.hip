@@ -267,14 +267,49 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
 // of kChunk.
 constexpr int kChunk = 64;
 
+// A body's items in slot order (the color-bucketed body order), one int4
+// each, rows first then tets, each in the body's list order:
+//   row: {-(row + 1), other body (the body itself for one-body rows), 0, 0}
+//   tet: {(tet << 2) | slot, the other three vertices in slot order}
+// so the row record or tet state and the other bodies' positions are
+// gathered side by side, one dependent load after the item.
+__device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
+                                          const float4* __restrict__ tst,
+                                          const float4* __restrict__ x, int4 it, int i, v3 xi,
+                                          v3& J, float& f, float& kp) {
+  if (it.x < 0) {
+    const int j = -it.x - 1;
+    const PRow r = rows[j];
+    const v3 xo = ld3(x, it.y);
+    if (r.st.w == 0.0f) return false;
+    const int t = r.meta.x;
+    const bool has_b = r.meta.z >= 0 && !is_pin(t);
+    float C;
+    row_CJ(t, r.meta.y == i, xi, xo, has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, C, J);
+    const bool is_t = t == kRowTangent;
+    kp = r.st.y;
+    f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w,
+                  is_t ? rows[r.meta.w].st.x : 0.0f, r.lim.y, r.lim.z);
+    return true;
+  }
+  const int tid = it.x >> 2, slot = it.x & 3;
+  const float4 ts = tst[tid];
+  const v3 o1 = ld3(x, it.y), o2 = ld3(x, it.z), o3 = ld3(x, it.w);
+  if (ts.w == 0.0f) return false;
+  v3 p[4];
+  int n = 0;
+  for (int v = 0; v < 4; ++v) p[v] = v == slot ? xi : (n++ == 0 ? o1 : (n == 2 ? o2 : o3));
+  kp = ts.z;
+  tet_jac_force(p[0], p[1], p[2], p[3], slot, ts.x, ts.y, kp, J, f);
+  return true;
+}
+
 template <int G>
 __global__ void __launch_bounds__(256)
-k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x,
+k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict__ slot_start,
+               const int4* __restrict__ items, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
-               const PRow* __restrict__ rows, const int* __restrict__ con_start,
-               const int* __restrict__ con_idx, const int4* __restrict__ tv,
-               const float4* __restrict__ tst, const int* __restrict__ tet_start,
-               const int* __restrict__ tet_idx, float dt) {
+               const PRow* __restrict__ rows, const float4* __restrict__ tst, float dt) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
   __shared__ float acc[BPB][9];
@@ -282,11 +317,10 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
   const int k = blockIdx.x * BPB + lb;
   const bool live = k < count;
   const int i = live ? bodies[k] : 0;
+  const int s0 = live ? slot_start[k] : 0, s1 = live ? slot_start[k + 1] : 0;
   const float m = live ? mass[i] : 0.0f;
   const bool dyn = live && m > 0.0f;
-  const int r0 = dyn ? con_start[i] : 0, nr = dyn ? con_start[i + 1] - r0 : 0;
-  const int t0 = dyn ? tet_start[i] : 0, ntt = dyn ? tet_start[i + 1] - t0 : 0;
-  const int n_items = nr + ntt;
+  const int n_items = dyn ? s1 - s0 : 0;
   const v3 xi = live ? ld3(x, i) : mk3(0.0f, 0.0f, 0.0f);
   // lane e < 9 owns entry e: rhs x, y, z, lhs 00 01 02 11 12 22
   float a = 0.0f;
@@ -299,11 +333,9 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, float4* __restrict__ x
   for (int base = 0; __syncthreads_or(base < n_items); base += kChunk) {
     const int n_here = min(kChunk, n_items - base);
     for (int q = g; q < n_here; q += G) {
-      const int item = base + q;
       v3 J;
       float f, kp;
-      const bool ok = item < nr ? row_item(rows, x, con_idx[r0 + item], i, xi, J, f, kp)
-                                : tet_item(tv, tst, x, tet_idx[t0 + item - nr], J, f, kp);
+      const bool ok = slot_item(rows, tst, x, items[s0 + base + q], i, xi, J, f, kp);
       float* dst = terms[lb][q];
       if (ok) {
         const Terms tm = make_terms(J, f, kp);
@@ -473,6 +505,8 @@ struct avbd_p_solver {
   PBuf<float4> stage;        // n_rows entries: row state out, {anchor, rest} in
   std::vector<float4> h_ar;  // host copy of every row's {anchor, rest} (plane rows move)
   PBuf<int> bodies, con_start, con_idx;
+  PBuf<int> slot_start;  // item ranges per slot of `bodies`
+  PBuf<int4> items;      // per-body rows and tets in slot order (k_p_primal_grp)
   PBuf<int4> tv;
   PBuf<float4> tst;  // {rest volume, alpha C0, penalty, active}
   PBuf<float> tet_lam, tet_stiff, tet_fracture;
@@ -578,9 +612,9 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
           auto kern = G == 16 ? k_p_primal_grp<16> : G == 32 ? k_p_primal_grp<32>
                                                               : k_p_primal_grp<64>;
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
-                             s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
-                             s->rows.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tst.p,
-                             s->tet_start.p, s->tet_idx.p, dt);
+                             s->bodies.p + s->cstart[c], s->slot_start.p + s->cstart[c],
+                             s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p, s->tst.p,
+                             dt);
         }
       }));
     }
@@ -684,20 +718,21 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   s->nc = nc;
   s->nt = nt;
   s->n_colors = sc->n_colors;
-  // algorithmic bytes of a primal launch: per body x, inertial, mass, list
-  // bounds and the x store; per listed row its index and record, the other
-  // body's x for two-body rows, the normal's lambda for friction rows; per
-  // listed tet its index, vertices, state and four vertex positions
+  // algorithmic bytes of a primal launch of k_p_primal_grp: per body its
+  // slot entry, x, inertial, mass, item bounds and the x store; per row item
+  // the item, the row record, the other body's x for two-body rows and the
+  // normal's lambda for friction rows; per tet item the item, the tet state
+  // and the other three vertices' x
   s->color_bytes.assign(sc->n_colors, 0.0);
   for (int i = 0; i < nb; ++i) {
-    double b = 12 + 12 + 4 + 8 + 8 + 12;
+    double b = 4 + 12 + 12 + 4 + 8 + 12;
     for (int k = sc->con_start[i]; k < sc->con_start[i + 1]; ++k) {
       const int j = sc->con_idx[k], t = sc->type[j];
-      b += 4 + 64;
+      b += 16 + 64;
       if (t == kRowTangent) b += 4;
       if (sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ) b += 12;
     }
-    b += (double)(sc->tet_start[i + 1] - sc->tet_start[i]) * (4 + 16 + 16 + 48);
+    b += (double)(sc->tet_start[i + 1] - sc->tet_start[i]) * (16 + 16 + 36);
     s->color_bytes[sc->color[i]] += b;
   }
   // bodies bucketed by color (ascending body index inside a color)
@@ -745,6 +780,30 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   PCK(s->tet_fracture.upload(sc->tet_fracture, nt));
   PCK(s->tet_start.upload(sc->tet_start, nb + 1));
   PCK(s->tet_idx.upload(sc->tet_idx, sc->tet_start[nb]));
+  {
+    std::vector<int> ss((size_t)nb + 1, 0);
+    std::vector<int4> it;
+    it.reserve((size_t)sc->con_start[nb] + (size_t)sc->tet_start[nb]);
+    for (int k = 0; k < nb; ++k) {
+      const int i = order[k];
+      for (int e = sc->con_start[i]; e < sc->con_start[i + 1]; ++e) {
+        const int j = sc->con_idx[e], t = sc->type[j];
+        const bool has_b = sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
+        const int other = has_b ? (sc->body_a[j] == i ? sc->body_b[j] : sc->body_a[j]) : i;
+        it.push_back(make_int4(-j - 1, other, 0, 0));
+      }
+      for (int e = sc->tet_start[i]; e < sc->tet_start[i + 1]; ++e) {
+        const int enc = sc->tet_idx[e], tid = enc >> 2, slot = enc & 3;
+        int o[3], n = 0;
+        for (int v = 0; v < 4; ++v)
+          if (v != slot) o[n++] = sc->tet_v[4 * tid + v];
+        it.push_back(make_int4(enc, o[0], o[1], o[2]));
+      }
+      ss[k + 1] = (int)it.size();
+    }
+    PCK(s->slot_start.upload(ss.data(), nb + 1));
+    PCK(s->items.upload(it.data(), it.size()));
+  }
   s->flushed = true;
   return AVBD_OK;
 }
