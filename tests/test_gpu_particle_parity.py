@@ -67,10 +67,10 @@ def soft_block(s):
     PS.soft_block(s, n=(3, 3, 3), size=(0.4, 0.4, 0.4), center=(0.0, 0.5, 0.0))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_soft_block_on_floor_lockstep(variant):
     """tets, soft distance rows, floor rows with friction rows, post-stab and
-    the finalize after it (solver.py:1374-1390); both primal kernels."""
+    the finalize after it (solver.py:1374-1390); every primal kernel."""
     _lockstep(soft_block, 90, variant=variant, dt=1 / 60, iterations=12)
 
 
