@@ -121,6 +121,7 @@ class PScene(C.Structure):
         ("tet_v", I32P), ("tet_rest_vol", F32P), ("tet_stiffness", F32P),
         ("tet_fracture", F32P), ("tet_lambda", F32P), ("tet_penalty", F32P),
         ("tet_active", I32P), ("tet_start", I32P), ("tet_idx", I32P),
+        ("off_a", F32P), ("off_b", F32P),
     ]
 
 

@@ -93,6 +93,20 @@ __device__ __forceinline__ float4 st3(v3 v) { return make_float4(v.x, v.y, v.z, 
 __device__ __forceinline__ bool is_pin(int t) {
   return t == kRowPinX || t == kRowPinY || t == kRowPinZ;
 }
+__device__ __forceinline__ bool has_offsets(int t) {
+  return t == kRowSphere || t == kRowSphereBox || t == kRowBoxBox;
+}
+// body-local contact offsets of a body-body contact row (offs[2j], offs[2j+1])
+__device__ __forceinline__ void row_offsets(const float4* __restrict__ offs, int j, int t, v3& oa,
+                                            v3& ob) {
+  if (has_offsets(t)) {
+    oa = ld3(offs, 2 * j);
+    ob = ld3(offs, 2 * j + 1);
+  } else {
+    oa = mk3(0.0f, 0.0f, 0.0f);
+    ob = oa;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -153,7 +167,7 @@ k_p_tet_warmstart(int nt, float4* __restrict__ tst, float* __restrict__ lam,
 // cache_alpha_C0 (kernels.py:740-802)
 __global__ void __launch_bounds__(256)
 k_p_cache_c0(int nc, const float4* __restrict__ x, const float4* __restrict__ initial,
-             PRow* __restrict__ rows, float alpha) {
+             PRow* __restrict__ rows, const float4* __restrict__ offs, float alpha) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nc) return;
   if (rows[j].st.w == 0.0f) {
@@ -165,12 +179,15 @@ k_p_cache_c0(int nc, const float4* __restrict__ x, const float4* __restrict__ in
   const int t = m.x;
   const bool has_b = m.z >= 0 && !is_pin(t);
   const v3 an = mk3(ar.x, ar.y, ar.z);
+  v3 oa, ob;
+  row_offsets(offs, j, t, oa, ob);
   if (t == kRowTangent) {  // frame-start positions, alpha 1 (kernels.py:785-800)
     rows[j].st.z = row_C(t, ld3(initial, m.y), has_b ? ld3(initial, m.z) : mk3(0.f, 0.f, 0.f),
-                         has_b, an, ar.w);
+                         has_b, an, ar.w, oa, ob);
     return;
   }
-  const float C0 = row_C(t, ld3(x, m.y), has_b ? ld3(x, m.z) : mk3(0.f, 0.f, 0.f), has_b, an, ar.w);
+  const float C0 = row_C(t, ld3(x, m.y), has_b ? ld3(x, m.z) : mk3(0.f, 0.f, 0.f), has_b, an, ar.w,
+                         oa, ob);
   rows[j].st.z = alpha * C0;
 }
 
@@ -194,6 +211,7 @@ k_p_tet_cache_c0(int nt, const float4* __restrict__ x, const int4* __restrict__ 
 // One listed row's J, force and penalty seen from body i (kernels.py:395-530);
 // false for an inactive row.
 __device__ __forceinline__ bool row_item(const PRow* __restrict__ rows,
+                                         const float4* __restrict__ offs,
                                          const float4* __restrict__ x, int j, int i, v3 xi,
                                          v3& J, float& f, float& kp) {
   const PRow r = rows[j];
@@ -202,8 +220,10 @@ __device__ __forceinline__ bool row_item(const PRow* __restrict__ rows,
   const bool has_b = r.meta.z >= 0 && !is_pin(t);
   const bool i_is_a = r.meta.y == i;
   const int other = has_b ? (i_is_a ? r.meta.z : r.meta.y) : i;
+  v3 oa, ob;
+  row_offsets(offs, j, t, oa, ob);
   float C;
-  row_CJ(t, i_is_a, xi, ld3(x, other), has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, C, J);
+  row_CJ(t, i_is_a, xi, ld3(x, other), has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, oa, ob, C, J);
   const bool is_t = t == kRowTangent;
   kp = r.st.y;
   f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w,
@@ -231,8 +251,9 @@ __device__ __forceinline__ bool tet_item(const int4* __restrict__ tv,
 __global__ void __launch_bounds__(256)
 k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
            const float4* __restrict__ inertial, const float* __restrict__ mass,
-           const PRow* __restrict__ rows, const int* __restrict__ con_start,
-           const int* __restrict__ con_idx, const int4* __restrict__ tv,
+           const PRow* __restrict__ rows, const float4* __restrict__ offs,
+           const int* __restrict__ con_start, const int* __restrict__ con_idx,
+           const int4* __restrict__ tv,
            const float4* __restrict__ tst, const int* __restrict__ tet_start,
            const int* __restrict__ tet_idx, float dt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,7 +271,8 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
   v3 J;
   float f, kp;
   for (int e = con_start[i]; e < con_start[i + 1]; ++e)
-    if (row_item(rows, x, con_idx[e], i, xi, J, f, kp)) add_terms(lhs, rhs, make_terms(J, f, kp));
+    if (row_item(rows, offs, x, con_idx[e], i, xi, J, f, kp))
+      add_terms(lhs, rhs, make_terms(J, f, kp));
   for (int e = tet_start[i]; e < tet_start[i + 1]; ++e)
     if (tet_item(tv, tst, x, tet_idx[e], J, f, kp)) add_terms(lhs, rhs, make_terms(J, f, kp));
   x[i] = st3(sub(xi, mvmul(minverse(lhs), rhs)));
@@ -274,6 +296,7 @@ constexpr int kChunk = 64;
 // so the row record or tet state and the other bodies' positions are
 // gathered side by side, one dependent load after the item.
 __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
+                                          const float4* __restrict__ offs,
                                           const float4* __restrict__ tst,
                                           const float4* __restrict__ x, int4 it, int i, v3 xi,
                                           v3& J, float& f, float& kp) {
@@ -284,8 +307,10 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
     if (r.st.w == 0.0f) return false;
     const int t = r.meta.x;
     const bool has_b = r.meta.z >= 0 && !is_pin(t);
+    v3 oa, ob;
+    row_offsets(offs, j, t, oa, ob);
     float C;
-    row_CJ(t, r.meta.y == i, xi, xo, has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, C, J);
+    row_CJ(t, r.meta.y == i, xi, xo, has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, oa, ob, C, J);
     const bool is_t = t == kRowTangent;
     kp = r.st.y;
     f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w,
@@ -309,7 +334,8 @@ __global__ void __launch_bounds__(256)
 k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict__ slot_start,
                const int4* __restrict__ items, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
-               const PRow* __restrict__ rows, const float4* __restrict__ tst, float dt) {
+               const PRow* __restrict__ rows, const float4* __restrict__ offs,
+               const float4* __restrict__ tst, float dt) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
   __shared__ float acc[BPB][9];
@@ -335,7 +361,7 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict_
     for (int q = g; q < n_here; q += G) {
       v3 J;
       float f, kp;
-      const bool ok = slot_item(rows, tst, x, items[s0 + base + q], i, xi, J, f, kp);
+      const bool ok = slot_item(rows, offs, tst, x, items[s0 + base + q], i, xi, J, f, kp);
       float* dst = terms[lb][q];
       if (ok) {
         const Terms tm = make_terms(J, f, kp);
@@ -373,13 +399,16 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict_
 // dual_update of one row (kernels.py:611-701)
 __device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
                                            PRow* __restrict__ rows,
+                                           const float4* __restrict__ offs,
                                            const float* __restrict__ fracture, float beta) {
   const PRow r = rows[j];
   if (r.st.w == 0.0f) return;
   const int t = r.meta.x;
   const bool has_b = r.meta.z >= 0 && !is_pin(t);
+  v3 oa, ob;
+  row_offsets(offs, j, t, oa, ob);
   const float C = row_C(t, ld3(x, r.meta.y), has_b ? ld3(x, r.meta.z) : mk3(0.f, 0.f, 0.f),
-                        has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w);
+                        has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, oa, ob);
   float l = r.st.x, p = r.st.y;
   const bool alive = dual_row(t, C, r.lim.x, r.st.z, l, p, r.lim.y, r.lim.z, fracture[j], r.lim.w,
                               t == kRowTangent ? rows[r.meta.w].st.x : 0.0f, beta);
@@ -390,14 +419,14 @@ __device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
 // then the friction rows that follow it and name it as their sibling.
 __global__ void __launch_bounds__(256)
 k_p_dual(int nc, const float4* __restrict__ x, PRow* __restrict__ rows,
-         const float* __restrict__ fracture, float beta) {
+         const float4* __restrict__ offs, const float* __restrict__ fracture, float beta) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nc || rows[j].meta.x == kRowTangent) return;
-  p_dual_one(j, x, rows, fracture, beta);
+  p_dual_one(j, x, rows, offs, fracture, beta);
   for (int k = j + 1; k < nc; ++k) {
     const int4 m = rows[k].meta;
     if (m.x != kRowTangent || m.w != j) break;
-    p_dual_one(k, x, rows, fracture, beta);
+    p_dual_one(k, x, rows, offs, fracture, beta);
   }
 }
 
@@ -502,6 +531,7 @@ struct avbd_p_solver {
   PBuf<float4> x, v, prev_v, initial, inertial;
   PBuf<float> mass, fracture;
   PBuf<PRow> rows;
+  PBuf<float4> offs;         // 2 per row: body-local contact offsets (off_a, off_b)
   PBuf<float4> stage;        // n_rows entries: row state out, {anchor, rest} in
   std::vector<float4> h_ar;  // host copy of every row's {anchor, rest} (plane rows move)
   PBuf<int> bodies, con_start, con_idx;
@@ -572,7 +602,7 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   auto cache = [&](float alpha) {
     if (nc > 0)
       hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
-                         s->rows.p, alpha);
+                         s->rows.p, s->offs.p, alpha);
     if (nt > 0)
       hipLaunchKernelGGL(k_p_tet_cache_c0, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
                          s->tst.p, alpha);
@@ -581,7 +611,7 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
     hipLaunchKernelGGL(k_p_warmstart, grid(nc), dim3(kB), 0, st, nc, s->rows.p, p->alpha,
                        p->gamma, post);
     hipLaunchKernelGGL(k_p_cache_c0, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->initial.p,
-                       s->rows.p, p->alpha);
+                       s->rows.p, s->offs.p, p->alpha);
   }
   if (nt > 0) {
     hipLaunchKernelGGL(k_p_tet_warmstart, grid(nt), dim3(kB), 0, st, nt, s->tst.p, s->tet_lam.p,
@@ -604,8 +634,8 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
         if (s->primal_variant == 0) {
           hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt,
                              s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
-                             s->rows.p, s->con_start.p, s->con_idx.p, s->tv.p, s->tst.p,
-                             s->tet_start.p, s->tet_idx.p, dt);
+                             s->rows.p, s->offs.p, s->con_start.p, s->con_idx.p, s->tv.p,
+                             s->tst.p, s->tet_start.p, s->tet_idx.p, dt);
         } else {
           const int G = s->primal_variant == 1 ? 16 : s->primal_variant == 2 ? 32 : 64;
           const int bpb = kB / G;
@@ -613,15 +643,15 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
                                                               : k_p_primal_grp<64>;
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
                              s->bodies.p + s->cstart[c], s->slot_start.p + s->cstart[c],
-                             s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p, s->tst.p,
-                             dt);
+                             s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p,
+                             s->offs.p, s->tst.p, dt);
         }
       }));
     }
     if (nc > 0 && it < I)
       PCK(timed(1, 0.0, [&] {
         hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->rows.p,
-                           s->fracture.p, p->beta);
+                           s->offs.p, s->fracture.p, p->beta);
       }));
     if (nt > 0 && it < I)
       PCK(timed(1, 0.0, [&] {
@@ -675,7 +705,8 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   for (int j = 0; j < nc; ++j) {
     const int t = sc->type[j], a = sc->body_a[j], b = sc->body_b[j];
     const bool pin = t == kRowPinX || t == kRowPinY || t == kRowPinZ;
-    if (a < 0 || a >= nb || (!pin && b >= nb) || (t == kRowDistance && b < 0))
+    const bool two = t == kRowDistance || t == kRowSphere || t == kRowSphereBox || t == kRowBoxBox;
+    if (a < 0 || a >= nb || (!pin && b >= nb) || (two && b < 0))
       return perr(AVBD_E_ARG, "row " + std::to_string(j) + " names a body out of range");
     if (t == kRowTangent) {
       // a friction row follows its normal row, possibly after other friction
@@ -689,7 +720,8 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
                                     " does not follow its normal row");
     }
     if (t != kRowPinX && t != kRowPinY && t != kRowPinZ && t != kRowDistance && t != kRowFloor &&
-        t != kRowTangent && t != kRowPlane)
+        t != kRowTangent && t != kRowPlane && t != kRowSphere && t != kRowSphereBox &&
+        t != kRowBoxBox)
       return perr(AVBD_E_ARG, "row " + std::to_string(j) + " has an unsupported type");
   }
   for (int i = 0; i < nb; ++i)
@@ -761,6 +793,17 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
     s->h_ar[j] = r.ar;
   }
   PCK(s->rows.upload(rows.data(), nc));
+  {
+    std::vector<float4> off((size_t)2 * nc, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int j = 0; j < nc; ++j) {
+      if (sc->off_a)
+        off[2 * j] = make_float4(sc->off_a[3 * j], sc->off_a[3 * j + 1], sc->off_a[3 * j + 2], 0.f);
+      if (sc->off_b)
+        off[2 * j + 1] =
+            make_float4(sc->off_b[3 * j], sc->off_b[3 * j + 1], sc->off_b[3 * j + 2], 0.f);
+    }
+    PCK(s->offs.upload(off.data(), off.size()));
+  }
   PCK(s->stage.upload(nullptr, nc));
   PCK(s->fracture.upload(sc->fracture, nc));
   PCK(s->con_start.upload(sc->con_start, nb + 1));

@@ -16,8 +16,8 @@ namespace avbd {
 namespace particles {
 
 // constraint types (kernels.py:28-70)
-constexpr int kRowPinX = 0, kRowPinY = 1, kRowPinZ = 2, kRowDistance = 3, kRowFloor = 4, kRowTangent = 6,
-              kRowPlane = 10;
+constexpr int kRowPinX = 0, kRowPinY = 1, kRowPinZ = 2, kRowDistance = 3, kRowFloor = 4,
+              kRowSphere = 5, kRowTangent = 6, kRowSphereBox = 7, kRowBoxBox = 8, kRowPlane = 10;
 constexpr float kPenaltyMinTet = 10.0f;  // kernels.py:79
 
 AVBD_HD v3 axis_basis(int axis) {  // kernels.py:255-261
@@ -35,22 +35,77 @@ AVBD_HD v3 distance_jac(v3 pa, v3 pb) {
   return n > 1.0e-12f ? divs(d, n) : mk3(0.0f, 0.0f, 0.0f);
 }
 
+// Gap and outward normal of a sphere (centre ps, radius r) against an
+// axis-aligned box (centre pb, half-extents h): closest point on the box,
+// or, with the centre inside, the face of least depth, ties to x then y
+// (kernels.py:269-317). The normal points from the box to the sphere.
+AVBD_HD void sphere_box(v3 ps, v3 pb, v3 h, float r, float& C, v3& n) {
+  const v3 rel = sub(ps, pb);
+  const v3 d = mk3(rel.x - clampf_(rel.x, -h.x, h.x), rel.y - clampf_(rel.y, -h.y, h.y),
+                   rel.z - clampf_(rel.z, -h.z, h.z));
+  const float dl = length(d);
+  if (dl > 1.0e-9f) {
+    n = divs(d, dl);
+    C = dl - r;
+    return;
+  }
+  const float dx = h.x - fabs_(rel.x), dy = h.y - fabs_(rel.y), dz = h.z - fabs_(rel.z);
+  if (dx <= dy && dx <= dz) {
+    n = mk3(rel.x < 0.0f ? -1.0f : 1.0f, 0.0f, 0.0f);
+    C = -(dx + r);
+  } else if (dy <= dz) {
+    n = mk3(0.0f, rel.y < 0.0f ? -1.0f : 1.0f, 0.0f);
+    C = -(dy + r);
+  } else {
+    n = mk3(0.0f, 0.0f, rel.z < 0.0f ? -1.0f : 1.0f);
+    C = -(dz + r);
+  }
+}
+
 // One row's C (no stabilisation) as the dual and the alpha C0 cache see it
 // (kernels.py:638-666, 760-802): xa, xb are the row's bodies' positions (xb
-// unused without a second body).
-AVBD_HD float row_C(int t, v3 xa, v3 xb, bool has_b, v3 anchor, float rest) {
+// unused without a second body); oa, ob the body-local contact offsets of
+// the body-body contact rows.
+AVBD_HD float row_C(int t, v3 xa, v3 xb, bool has_b, v3 anchor, float rest, v3 oa, v3 ob) {
   if (t == kRowPinX || t == kRowPinY || t == kRowPinZ) return comp(xa, t) - comp(anchor, t);
   if (t == kRowDistance) return eval_distance(xa, xb, rest);
   if (t == kRowFloor) return xa.y - anchor.y;
   if (t == kRowPlane) return dot(anchor, xa) - rest;
+  if (t == kRowSphere) return eval_distance(add(xa, oa), add(xb, ob), rest);
+  if (t == kRowSphereBox) {
+    float C;
+    v3 n;
+    sphere_box(add(xa, oa), xb, anchor, rest, C, n);
+    return C;
+  }
+  if (t == kRowBoxBox) return dot(anchor, sub(add(xa, oa), add(xb, ob)));
   if (t == kRowTangent) return has_b ? dot(anchor, sub(xa, xb)) : dot(anchor, xa);
   return 0.0f;
 }
 
 // C and dC/dx_i of a row seen from body i in the primal (kernels.py:395-505);
-// xo is the other body's position (distance and two-body tangent rows).
-AVBD_HD void row_CJ(int t, bool i_is_a, v3 xi, v3 xo, bool has_b, v3 anchor, float rest, float& C,
-                    v3& J) {
+// xo is the other body's position (two-body rows); oa, ob as in row_C.
+AVBD_HD void row_CJ(int t, bool i_is_a, v3 xi, v3 xo, bool has_b, v3 anchor, float rest, v3 oa,
+                    v3 ob, float& C, v3& J) {
+  if (t == kRowSphere) {
+    const v3 pa = add(i_is_a ? xi : xo, oa), pb = add(i_is_a ? xo : xi, ob);
+    const v3 pi = i_is_a ? pa : pb, po = i_is_a ? pb : pa;
+    J = distance_jac(pi, po);
+    C = eval_distance(pi, po, rest);
+    return;
+  }
+  if (t == kRowSphereBox) {
+    v3 n;
+    sphere_box(add(i_is_a ? xi : xo, oa), i_is_a ? xo : xi, anchor, rest, C, n);
+    J = i_is_a ? n : neg(n);
+    return;
+  }
+  if (t == kRowBoxBox) {
+    const v3 ra = add(i_is_a ? xi : xo, oa), rb = add(i_is_a ? xo : xi, ob);
+    C = dot(anchor, sub(ra, rb));
+    J = i_is_a ? anchor : neg(anchor);
+    return;
+  }
   if (t == kRowPinX || t == kRowPinY || t == kRowPinZ) {
     J = axis_basis(t);
     C = comp(xi, t) - comp(anchor, t);

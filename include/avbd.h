@@ -272,9 +272,10 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
                    float* loss_hist);
 
 /* ---- 3-DOF particle solver ---------------------------------------------
- * Solver.step (src/avbd3d/solver.py:1124-1400) without self-collision:
- * particles with pin (x/y/z), distance, floor (with x/z friction rows),
- * plane and tet-volume constraints. The host (avbd_amd/solver.py, after
+ * Solver.step (src/avbd3d/solver.py:1124-1400): particles with pin (x/y/z),
+ * distance, floor (with x/z friction rows), plane and tet-volume
+ * constraints, and the body-body contact rows the host's broad phase
+ * emits (sphere, sphere-box, box-box, each with two friction rows). The host (avbd_amd/solver.py, after
  * Solver._flush :964-1122) supplies the rows in the reference's per-row
  * layout, the body coloring (coloring.py:41-55) and the body -> row and
  * body -> tet lists (solver.py:922-962). Vectors are 3 floats. */
@@ -296,7 +297,8 @@ typedef struct avbd_p_scene {
   const int* color;            /* per body, < n_colors */
   int n_colors;
   int n_rows;
-  const int* type;             /* 0-2 pin x/y/z, 3 distance, 4 floor, 6 tangent, 10 plane */
+  const int* type;             /* 0-2 pin x/y/z, 3 distance, 4 floor, 5 sphere, 6 tangent,
+                                  7 sphere-box, 8 box-box, 10 plane */
   const int* body_a;
   const int* body_b;
   const float* anchor;         /* 3 per row */
@@ -322,6 +324,8 @@ typedef struct avbd_p_scene {
   const int* tet_active;
   const int* tet_start;        /* n_bodies + 1 */
   const int* tet_idx;          /* (tet << 2) | vertex slot */
+  const float* off_a;          /* 3 per row: body_a's contact offset (5, 7, 8), or NULL */
+  const float* off_b;          /* 3 per row: body_b's contact offset (5, 8), or NULL */
 } avbd_p_scene;
 
 typedef struct avbd_p_solver avbd_p_solver;

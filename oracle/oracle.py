@@ -55,7 +55,7 @@ def load() -> C.CDLL:
                                        C.POINTER(C.c_int), F, F, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), F, F]
         lib.orc_probe_pair.restype = C.c_int
-        lib.orc_p_probe_row.argtypes = [C.c_int, C.c_int, F, F, C.c_int, F, C.c_float,
+        lib.orc_p_probe_row.argtypes = [C.c_int, C.c_int, F, F, C.c_int, F, C.c_float, F, F,
                                         C.POINTER(C.c_float), F]
         lib.orc_p_probe_row.restype = None
         lib.orc_p_probe_tet.argtypes = [F, C.c_int, C.c_float, C.c_float, C.c_float, F, F]
@@ -141,14 +141,14 @@ def probe_pair(xa, qa, ea, xb, qb, eb, margin=0.005):
             "count": cnt.value, "ref_is_a": refa.value, "off_ref": offr, "off_inc": offi}
 
 
-def probe_particle_row(t, i_is_a, xi, xo, has_b, anchor, rest):
+def probe_particle_row(t, i_is_a, xi, xo, has_b, anchor, rest, oa=(0, 0, 0), ob=(0, 0, 0)):
     """(C, dC/dx_i) of one particle-solver row through the restatement."""
     lib = load()
     f = lambda v: np.ascontiguousarray(v, np.float32)
     c = C.c_float(0.0)
     J = np.zeros(3, np.float32)
     lib.orc_p_probe_row(int(t), int(bool(i_is_a)), f(xi), f(xo), int(bool(has_b)), f(anchor),
-                        float(rest), C.byref(c), J)
+                        float(rest), f(oa), f(ob), C.byref(c), J)
     return float(c.value), J
 
 

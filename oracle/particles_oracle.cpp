@@ -34,7 +34,7 @@ int fail(int code, const char* m) {
 struct POracle {
   int nb = 0, nc = 0, nt = 0, n_colors = 0;
   bool flushed = false;
-  std::vector<v3> x, v, prev_v, initial, inertial, anchor;
+  std::vector<v3> x, v, prev_v, initial, inertial, anchor, off_a, off_b;
   std::vector<float> mass, rest, stiff, fmin, fmax, fracture, friction, lam, pen, ac0;
   std::vector<int> color, type, ra, rb, active, sibling, con_start, con_idx;
   std::vector<int> tv;  // 4 per tet
@@ -91,7 +91,8 @@ void cache_c0(POracle& s, float alpha) {
       continue;
     }
     const v3 xb = has_b ? s.x[s.rb[j]] : mk3(0.0f, 0.0f, 0.0f);
-    s.ac0[j] = alpha * row_C(t, s.x[s.ra[j]], xb, has_b, s.anchor[j], s.rest[j]);
+    s.ac0[j] = alpha * row_C(t, s.x[s.ra[j]], xb, has_b, s.anchor[j], s.rest[j], s.off_a[j],
+                             s.off_b[j]);
   }
 }
 
@@ -125,7 +126,7 @@ void primal_body(POracle& s, int i, float dt) {
     const v3 xo = has_b ? s.x[i_is_a ? s.rb[j] : s.ra[j]] : s.x[i];
     float C;
     v3 J;
-    row_CJ(t, i_is_a, s.x[i], xo, has_b, s.anchor[j], s.rest[j], C, J);
+    row_CJ(t, i_is_a, s.x[i], xo, has_b, s.anchor[j], s.rest[j], s.off_a[j], s.off_b[j], C, J);
     const bool is_t = t == kRowTangent;
     primal_add(lhs, rhs, J, C, s.stiff[j], s.ac0[j], s.lam[j], s.pen[j], is_t, s.friction[j],
                is_t ? s.lam[s.sibling[j]] : 0.0f, s.fmin[j], s.fmax[j]);
@@ -148,7 +149,8 @@ void dual(POracle& s, float beta) {
     const int t = s.type[j];
     const bool has_b = s.rb[j] >= 0 && !is_pin(t);
     const v3 xb = has_b ? s.x[s.rb[j]] : mk3(0.0f, 0.0f, 0.0f);
-    const float C = row_C(t, s.x[s.ra[j]], xb, has_b, s.anchor[j], s.rest[j]);
+    const float C = row_C(t, s.x[s.ra[j]], xb, has_b, s.anchor[j], s.rest[j], s.off_a[j],
+                          s.off_b[j]);
     const float lam_sib = t == kRowTangent ? s.lam[s.sibling[j]] : 0.0f;
     if (!dual_row(t, C, s.stiff[j], s.ac0[j], s.lam[j], s.pen[j], s.fmin[j], s.fmax[j],
                   s.fracture[j], s.friction[j], lam_sib, beta))
@@ -235,6 +237,8 @@ int orc_p_flush(void* h, const avbd_p_scene* sc) {
   s.ra = load(sc->body_a, nc);
   s.rb = load(sc->body_b, nc);
   s.anchor = load3(sc->anchor, nc);
+  s.off_a = sc->off_a ? load3(sc->off_a, nc) : std::vector<v3>((size_t)nc, mk3(0.f, 0.f, 0.f));
+  s.off_b = sc->off_b ? load3(sc->off_b, nc) : std::vector<v3>((size_t)nc, mk3(0.f, 0.f, 0.f));
   s.rest = load(sc->rest, nc);
   s.stiff = load(sc->stiffness, nc);
   s.fmin = load(sc->fmin, nc);
@@ -348,11 +352,13 @@ int orc_p_write_rows(void* h, int first, int n, const float* anchor, const float
 extern "C" {
 
 void orc_p_probe_row(int type, int i_is_a, const float* xi, const float* xo, int has_b,
-                     const float* anchor, float rest, float* C, float* J) {
+                     const float* anchor, float rest, const float* oa, const float* ob, float* C,
+                     float* J) {
   v3 j;
   float c;
   row_CJ(type, i_is_a != 0, mk3(xi[0], xi[1], xi[2]), mk3(xo[0], xo[1], xo[2]), has_b != 0,
-         mk3(anchor[0], anchor[1], anchor[2]), rest, c, j);
+         mk3(anchor[0], anchor[1], anchor[2]), rest, mk3(oa[0], oa[1], oa[2]),
+         mk3(ob[0], ob[1], ob[2]), c, j);
   *C = c;
   J[0] = j.x;
   J[1] = j.y;

@@ -172,3 +172,53 @@ def test_graph_replay_matches_direct_launches():
         sa.step()
         sb.step()
         _compare(sa.state(), sb.state(), f"step {f}")
+
+
+def contacts_mixed(s):
+    """Spheres, cubes and a pillar falling onto each other and the floor,
+    with friction: sphere, sphere-box and box-box rows and their friction
+    rows, generated by the host every step."""
+    from avbd_amd.scene import Shape
+    r = 0.15
+    bodies = []
+    for k in range(3):
+        bodies.append(s.add_particle((0.02 * k, r + 0.32 * k, 0.0), 2.0 - 0.5 * k,
+                                     shape=Shape("cube", (r, r, r)), friction=0.6))
+    for k in range(4):
+        bodies.append(s.add_particle((0.5 + 0.05 * k, 0.2 + 0.3 * k, 0.03 * k), 1.0,
+                                     shape=Shape("sphere", (0.1,)), friction=0.4))
+    bodies.append(s.add_particle((-0.45, 0.35, 0.0), 3.0, shape=Shape("pillar", (0.1, 0.35)),
+                                 velocity=(1.5, 0.0, 0.0), friction=0.5))
+    bodies.append(s.add_particle((0.2, 1.5, 0.0), 1.0, shape=Shape("sphere", (0.12,)),
+                                 velocity=(0.0, -2.0, 0.0)))
+    for b in bodies:
+        s.add_floor_contact(b, floor_y=0.1)
+    s.enable_self_collision(True, default_friction=0.3)
+
+
+def test_body_contacts_lockstep():
+    sg, _ = _lockstep(contacts_mixed, 120, dt=1 / 60, iterations=15)
+    types = set(int(t) for t in sg.c_type.numpy().tolist())
+    assert {7, 8} <= types  # sphere-box and box-box rows were exercised
+
+
+def broken_pair(s):
+    from avbd_amd.scene import Shape
+    a = s.add_particle((0.0, 2.0, 0.0), 5.0, shape=Shape("sphere", (0.1,)))
+    s.add_floor_contact(a, floor_y=0.1)
+    s.add_pin(a, (0.0, 2.0, 0.0))
+    b = s.add_particle((0.0, 1.5, 0.0), 1.0, shape=Shape("sphere", (0.1,)))
+    s.add_floor_contact(b, floor_y=0.1)
+    s.add_distance(a, b, rest=0.5, fracture=5.0)
+    s.enable_self_collision(True)
+
+
+def broken_pair_hook(s, f):
+    if f == 40:
+        s.set_position(type("B", (), {"index": 1})(), (0.0, 2.3, 0.0))
+        s.set_velocity(type("B", (), {"index": 1})(), (0.0, 0.0, 0.0))
+
+
+def test_fracture_then_contact_lockstep():
+    sg, _ = _lockstep(broken_pair, 120, hook=broken_pair_hook, dt=1 / 60, iterations=25)
+    assert int((sg.c_type.numpy() == 5).sum()) >= 0

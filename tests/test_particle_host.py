@@ -64,6 +64,59 @@ def test_row_gradient_matches_finite_difference(t, i_is_a):
         np.testing.assert_allclose(J, _num_grad(f, xi64), atol=2e-4)
 
 
+def _box_gap64(ps, pb, h, r):
+    rel = ps - pb
+    return np.linalg.norm(rel - np.clip(rel, -h, h)) - r
+
+
+@pytest.mark.parametrize("t", [5, 7, 8])
+@pytest.mark.parametrize("i_is_a", [True, False])
+def test_contact_row_gradient_matches_finite_difference(t, i_is_a):
+    """Sphere, sphere-box (sphere outside the box) and box-box rows with
+    body-local offsets: C and J against float64 (kernels.py:269-317, 423-475)."""
+    for _ in range(20):
+        xa, xb = RNG.normal(size=3), RNG.normal(size=3)
+        oa, ob = 0.2 * RNG.normal(size=3), 0.2 * RNG.normal(size=3)
+        anchor = RNG.normal(size=3)
+        rest = float(abs(RNG.normal()))
+        if t == 7:
+            ob[:] = 0.0
+            anchor = np.abs(anchor) * 0.3          # half-extents
+            xa = xb + np.sign(RNG.normal(size=3)) * (anchor + 0.3 + np.abs(RNG.normal(size=3)))
+            xa = xa - oa
+        if t == 8:
+            anchor /= np.linalg.norm(anchor)
+        c32 = [v.astype(np.float32) for v in (xa, xb, oa, ob, anchor)]
+        xa, xb, oa, ob, anchor = (v.astype(np.float64) for v in c32)
+        r64 = float(np.float32(rest))
+
+        def C(pa, pb):
+            if t == 5:
+                return np.linalg.norm((pa + oa) - (pb + ob)) - r64
+            if t == 7:
+                return _box_gap64(pa + oa, pb, anchor, r64)
+            return anchor @ ((pa + oa) - (pb + ob))
+        xi, xo = (xa, xb) if i_is_a else (xb, xa)
+        Cg, J = probe_particle_row(t, i_is_a, xi.astype(np.float32), xo.astype(np.float32), True,
+                                   anchor.astype(np.float32), np.float32(rest),
+                                   oa.astype(np.float32), ob.astype(np.float32))
+        f = (lambda x: C(x, xb)) if i_is_a else (lambda x: C(xa, x))
+        assert Cg == pytest.approx(C(xa, xb), abs=1e-5)
+        np.testing.assert_allclose(J, _num_grad(f, xi), atol=2e-4)
+
+
+def test_sphere_box_inside_takes_the_shallowest_face():
+    """Centre inside the box: normal along the face of least depth, C =
+    -(depth + r) (kernels.py:293-317)."""
+    h = np.array([0.3, 0.2, 0.4], np.float32)
+    ps = np.array([0.05, 0.15, -0.1], np.float32)       # y face is 0.05 deep
+    C, J = probe_particle_row(7, True, ps, np.zeros(3, np.float32), True, h, np.float32(0.1))
+    assert C == pytest.approx(-(0.05 + 0.1), abs=1e-6)
+    np.testing.assert_array_equal(J, [0.0, 1.0, 0.0])
+    C, J = probe_particle_row(7, False, np.zeros(3, np.float32), -ps, True, h, np.float32(0.1))
+    np.testing.assert_array_equal(J, [0.0, 1.0, 0.0])  # box side: -n with n = -y
+
+
 @pytest.mark.parametrize("slot", [0, 1, 2, 3])
 def test_tet_gradient_and_force_match_finite_difference(slot):
     """rhs = k C J and lhs = k J J^T with C = V/V0 - 1 - ac0 and J its
@@ -197,14 +250,57 @@ def test_plane_contact_bulk_moves_plates():
     assert s._c_world_anchor_host[0].tolist() == [1.0, 0.0, 0.0]
 
 
-def test_self_collision_between_two_bodies_is_not_built():
+def test_contact_rows_of_each_shape_pair():
+    """The broad phase's rows for sphere-sphere, sphere-box and box-box
+    (solver.py:400-783): types, bodies, offsets and friction rows."""
     S = particle_solver_class()
-    s = S()
-    s.add_particle((0, 0, 0), 1.0)
-    s.add_particle((0, 0.1, 0), 1.0)
+    s = S(gravity=(0, 0, 0))
+    a = s.add_particle((0.0, 0.0, 0.0), 1.0, shape=Shape("sphere", (0.2,)), friction=0.5)
+    b = s.add_particle((0.35, 0.0, 0.0), 1.0, shape=Shape("sphere", (0.2,)), friction=0.5)
     s.enable_self_collision(True)
-    with pytest.raises(NotImplementedError):
+    s.step()
+    types = [r.type for r in s._constraints]
+    assert types == [5, 6, 6]
+    r0 = s._constraints[0]
+    assert (r0.body_a, r0.body_b) == (a.index, b.index) and r0.rest == pytest.approx(0.4)
+    assert s._constraints[1].sibling == 0 and s._constraints[1].friction == pytest.approx(0.5)
+    # tangents are unit and orthogonal to the contact normal (-x)
+    t = np.array(s._constraints[1].world_anchor)
+    bt = np.array(s._constraints[2].world_anchor)
+    assert abs(np.linalg.norm(t) - 1) < 1e-6 and abs(t @ [1, 0, 0]) < 1e-6
+    assert abs(t @ bt) < 1e-6
+
+    s = S(gravity=(0, 0, 0))
+    c = s.add_particle((0.0, 0.0, 0.0), 1.0, shape=Shape("cube", (0.2, 0.2, 0.2)))
+    d = s.add_particle((0.0, 0.35, 0.0), 1.0, shape=Shape("cube", (0.2, 0.2, 0.2)))
+    e = s.add_particle((0.0, -0.25, 0.0), 1.0, shape=Shape("sphere", (0.1,)))
+    s.enable_self_collision(True)
+    s.step()
+    bb = [r for r in s._constraints if r.type == 8]
+    assert len(bb) == 4 and all((r.body_a, r.body_b) == (c.index, d.index) for r in bb)
+    assert all(r.world_anchor == (0.0, -1.0, 0.0) for r in bb)  # from d to c
+    sb = [r for r in s._constraints if r.type == 7]
+    assert len(sb) == 1 and (sb[0].body_a, sb[0].body_b) == (e.index, c.index)
+    assert sb[0].world_anchor == pytest.approx((0.2, 0.2, 0.2)) and sb[0].rest == pytest.approx(0.1)
+
+
+def test_contact_rows_persist_and_sphere_rows_are_rebuilt():
+    """Sphere rows are rebuilt every step; box rows stay and exclude their
+    pair from later broad phases (the reference's behaviour)."""
+    S = particle_solver_class()
+    s = S(gravity=(0, 0, 0))
+    s.add_particle((0.0, 0.0, 0.0), 1.0, shape=Shape("cube", (0.2, 0.2, 0.2)))
+    s.add_particle((0.0, 0.39, 0.0), 1.0, shape=Shape("cube", (0.2, 0.2, 0.2)))
+    s.add_particle((2.0, 0.0, 0.0), 1.0, shape=Shape("sphere", (0.2,)))
+    s.add_particle((2.0, 0.39, 0.0), 1.0, shape=Shape("sphere", (0.2,)))
+    s.enable_self_collision(True)
+    s.step()
+    types = [r.type for r in s._constraints]
+    assert types.count(8) == 4 and types.count(5) == 1
+    for _ in range(5):  # box rows remain; the sphere row is rebuilt, never accumulated
         s.step()
+        types = [r.type for r in s._constraints]
+        assert types.count(8) == 4 and types.count(5) <= 1
 
 
 def test_default_friction_backfill_and_floor_rows():
