@@ -1,8 +1,9 @@
-"""avbd_amd — the AVBD Solver6DOF substep on AMD Instinct MI355X (gfx950).
+"""avbd_amd — the AVBD solvers on AMD Instinct MI355X (gfx950).
 
-Drop-in for ``avbd3d``'s 6-DOF path: ``from avbd_amd import Solver6DOF``
-(or ``import avbd_amd as avbd3d``). The substep runs as hand-written HIP
-kernels in libavbd_hip.so, called through a ctypes C ABI (include/avbd.h).
+Drop-in for ``avbd3d``: ``from avbd_amd import Solver6DOF, Solver`` (or
+``import avbd_amd as avbd3d``). The 6-DOF substep and the 3-DOF particle
+step run as hand-written HIP kernels in libavbd_hip.so, called through a
+ctypes C ABI (include/avbd.h).
 """
 from .solver_6dof import (  # noqa: F401
     BOX_BOX_CONTACT_6DOF,
@@ -15,7 +16,19 @@ from .solver_6dof import (  # noqa: F401
     box_inv_inertia_local,
 )
 
+from .deformable import DeformableBody, TetMesh, bind_tet_mesh_to_solver, box_tet_mesh  # noqa: F401
+from .scene import Body, ConstraintHandle, Shape  # noqa: F401
+from .solver import Solver  # noqa: F401
+
 __all__ = [
+    "Solver",
+    "Body",
+    "ConstraintHandle",
+    "Shape",
+    "TetMesh",
+    "DeformableBody",
+    "bind_tet_mesh_to_solver",
+    "box_tet_mesh",
     "Solver6DOF",
     "RigidBody",
     "box_inertia_local",

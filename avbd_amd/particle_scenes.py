@@ -20,10 +20,9 @@ reference's voxelised bunny (which needs a downloaded mesh).
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
-
 import numpy as np
 
+from .deformable import DeformableBody, TetMesh, bind_tet_mesh_to_solver, box_tet_mesh
 from .scene import Body
 
 
@@ -62,83 +61,23 @@ def cloth(s, nx: int, nz: int, spacing: float = 0.05, height: float = 2.0,
     return idx
 
 
-@dataclass
-class TetLattice:
-    vertices: np.ndarray      # (V, 3) float32
-    tets: np.ndarray          # (T, 4) int32
-    volumes: np.ndarray       # (T,) signed rest volumes
-    edges: np.ndarray         # (E, 2) unique, a < b
-    surface_verts: np.ndarray
-
-
-def box_lattice(n: tuple[int, int, int], size: tuple[float, float, float],
-                center: tuple[float, float, float]) -> TetLattice:
-    nx, ny, nz = n
-    h = np.asarray(size, np.float64) / np.asarray(n, np.float64)
-    lo = np.asarray(center, np.float64) - 0.5 * np.asarray(size, np.float64)
-    vid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
-    gi, gj, gk = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
-                             indexing="ij")
-    verts = (lo + np.stack([gi, gj, gk], -1).reshape(-1, 3) * h).astype(np.float32)
-    # the six monotone lattice paths from corner (0,0,0) to (1,1,1)
-    paths = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
-    tets = []
-    for i in range(nx):
-        for j in range(ny):
-            for k in range(nz):
-                for p in paths:
-                    c = [0, 0, 0]
-                    q = [vid(i, j, k)]
-                    for ax in p:
-                        c[ax] += 1
-                        q.append(vid(i + c[0], j + c[1], k + c[2]))
-                    tets.append(q)
-    tets = np.asarray(tets, np.int32)
-    P = verts.astype(np.float64)
-    vols = np.einsum("ij,ij->i", P[tets[:, 1]] - P[tets[:, 0]],
-                     np.cross(P[tets[:, 2]] - P[tets[:, 0]], P[tets[:, 3]] - P[tets[:, 0]])) / 6.0
-    e = tets[:, [0, 0, 0, 1, 1, 2, 1, 2, 3, 2, 3, 3]].reshape(-1, 2, 6).transpose(0, 2, 1)
-    e = np.sort(e.reshape(-1, 2), axis=1)
-    edges = np.unique(e, axis=0).astype(np.int32)
-    on = (gi == 0) | (gi == nx) | (gj == 0) | (gj == ny) | (gk == 0) | (gk == nz)
-    surface = np.nonzero(on.reshape(-1))[0].astype(np.int32)
-    return TetLattice(verts, tets, vols, edges, surface)
-
-
-@dataclass
-class SoftBlock:
-    lattice: TetLattice
-    indices: np.ndarray
-    edge_constraints: list
-    floor_constraints: list
-    volume_constraints: list
+def box_lattice(n, size, center) -> TetMesh:
+    """The tet mesh of a box (deformable.box_tet_mesh)."""
+    return box_tet_mesh(n, size, center)
 
 
 def soft_block(s, n=(3, 3, 3), size=(0.3, 0.3, 0.3), center=(0.0, 0.7, 0.0),
                density: float = 1000.0, edge_stiffness: float = 5.0e4,
                volume_stiffness: float = 1.0e4, friction: float = 0.5,
-               floor_y: float | None = 0.0) -> SoftBlock:
-    lat = box_lattice(n, size, center)
-    masses = np.zeros(len(lat.vertices), np.float32)
-    for k in range(4):
-        np.add.at(masses, lat.tets[:, k], np.abs(lat.volumes).astype(np.float32))
-    masses = np.maximum(masses * np.float32(0.25 * density), np.float32(1e-6))
-    bodies = [s.add_particle(tuple(float(c) for c in p), float(m), collide=False,
-                             friction=friction) for p, m in zip(lat.vertices, masses)]
-    edges = [s.add_distance(bodies[a], bodies[b],
-                            rest=float(np.linalg.norm(lat.vertices[a] - lat.vertices[b])),
-                            stiffness=edge_stiffness) for a, b in lat.edges.tolist()]
-    floors = [] if floor_y is None else [
-        s.add_floor_contact(bodies[v], floor_y=float(floor_y), friction=friction)
-        for v in lat.surface_verts.tolist()]
-    vols = [] if volume_stiffness <= 0.0 else [
-        s.add_tet_volume(*(bodies[v] for v in t), stiffness=float(volume_stiffness))
-        for t in lat.tets.tolist()]
-    return SoftBlock(lat, np.array([b.index for b in bodies], np.int32), edges, floors, vols)
+               floor_y: float | None = 0.0) -> DeformableBody:
+    return bind_tet_mesh_to_solver(s, box_tet_mesh(n, size, center), density=density,
+                                   edge_stiffness=edge_stiffness,
+                                   volume_stiffness=volume_stiffness, floor_y=floor_y,
+                                   friction=friction)
 
 
 def soft_blocks(s, count: int, n=(6, 6, 6), size=(0.4, 0.4, 0.4), gap: float = 0.2,
-                height: float = 0.7, **kw) -> list[SoftBlock]:
+                height: float = 0.7, **kw) -> list[DeformableBody]:
     """`count` soft blocks on a square grid in x/z, `gap` apart, each
     dropped from `height` (block centre) onto the floor y = 0."""
     side = int(math.ceil(math.sqrt(count)))

@@ -188,9 +188,9 @@ def test_volume_preservation_prevents_tet_inversion_on_floor(S):
         s.step()
     pos = s.positions()[blk.indices]
     assert np.all(np.isfinite(pos))
-    V = _tet_volumes(pos, blk.lattice.tets)
-    assert int((np.sign(V) != np.sign(blk.lattice.volumes)).sum()) == 0
-    ratio = V / blk.lattice.volumes
+    V = _tet_volumes(pos, blk.tet.tets)
+    assert int((np.sign(V) != np.sign(blk.tet.volumes)).sum()) == 0
+    ratio = V / blk.tet.volumes
     assert ratio.min() > 0.7 and ratio.max() < 1.3
     assert pos[:, 1].min() > -0.02  # rests on the floor
 
@@ -198,7 +198,7 @@ def test_volume_preservation_prevents_tet_inversion_on_floor(S):
 def test_tet_pool_has_one_row_per_tet(S):
     s = S(dt=1.0 / 60.0, iterations=5)
     blk = PS.soft_block(s, n=(2, 2, 2), volume_stiffness=1.0e4, floor_y=None)
-    assert len(blk.volume_constraints) == len(blk.lattice.tets) == len(s._tets) == 48
+    assert len(blk.volume_constraints) == len(blk.tet.tets) == len(s._tets) == 48
     s2 = S(dt=1.0 / 60.0, iterations=5)
     blk2 = PS.soft_block(s2, n=(2, 2, 2), volume_stiffness=0.0, floor_y=None)
     assert len(blk2.volume_constraints) == 0 and len(s2._tets) == 0

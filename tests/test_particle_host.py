@@ -180,7 +180,7 @@ def test_flush_arrays_match_reference_loops():
     for c in s._constraints:
         if c.body_b >= 0:
             assert col[c.body_a] != col[c.body_b]
-    for t in blk.lattice.tets:
+    for t in blk.tet.tets:
         assert len({int(col[v]) for v in t}) == 4
     assert a["n_colors"] == col.max() + 1
 
@@ -330,3 +330,22 @@ def test_box_lattice_is_consistent():
     for t in lat.tets:
         assert len(set(t.tolist())) == 4
     assert len({tuple(e) for e in lat.edges.tolist()}) == len(lat.edges)
+
+
+def test_box_tet_mesh_surface_and_binding():
+    """Surface triangles are the faces owned by one tet (2 per boundary cell
+    face); binding gives one particle per vertex, one row per edge, floor
+    rows with friction on the surface, one tet row per tet, and vertex
+    masses summing to density x volume (deformable.py:552-659)."""
+    from avbd_amd.deformable import bind_tet_mesh_to_solver, box_tet_mesh
+    m = box_tet_mesh((2, 3, 2), (0.2, 0.3, 0.2), (0.0, 1.0, 0.0))
+    assert len(m.surface_tris) == 2 * 2 * (2 * 3 + 3 * 2 + 2 * 2)
+    assert set(m.surface_tris.reshape(-1).tolist()) <= set(m.surface_verts.tolist())
+    S = particle_solver_class()
+    s = S()
+    d = bind_tet_mesh_to_solver(s, m, density=500.0, friction=0.4)
+    assert len(s._bodies_x) == len(m.vertices)
+    assert len(d.edge_constraints) == len(m.edges) and len(d.volume_constraints) == len(m.tets)
+    assert len(d.floor_constraints) == len(m.surface_verts)
+    assert sum(s._bodies_mass) == pytest.approx(500.0 * 0.2 * 0.3 * 0.2, rel=1e-5)
+    assert all(h.rows == 3 for h in d.floor_constraints)
