@@ -8,7 +8,11 @@
 // primal launch per color (over that color's bodies only), the dual and the
 // tet dual, and the velocity finalize.
 //
-// Layout: a row is one 64-byte record (PRow) holding its static fields and
+// Layout: bodies are stored color by color (device index = position in the
+// color-bucketed order), so a color's bodies are one contiguous index range
+// and the primal indexes x, mass and its item lists without an indirection;
+// rows, tets and items name bodies by device index, and the host-facing
+// reads and writes map between the two orders. A row is one 64-byte record (PRow) holding its static fields and
 // its state, so the primal's gather of a listed row touches one half cache
 // line; a tet is an int4 of vertices plus a 16-byte state record. Rows keep
 // the reference's indices. A contact's friction rows follow their normal
@@ -249,7 +253,7 @@ __device__ __forceinline__ bool tet_item(const int4* __restrict__ tv,
 // primal_update for the bodies of one color (kernels.py:335-608): one lane
 // per body, rows in the body's list order, then its tets, then the 3x3 solve.
 __global__ void __launch_bounds__(256)
-k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
+k_p_primal(int count, int first, float4* __restrict__ x,
            const float4* __restrict__ inertial, const float* __restrict__ mass,
            const PRow* __restrict__ rows, const float4* __restrict__ offs,
            const int* __restrict__ con_start, const int* __restrict__ con_idx,
@@ -258,7 +262,7 @@ k_p_primal(int count, const int* __restrict__ bodies, float4* __restrict__ x,
            const int* __restrict__ tet_idx, float dt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
-  const int i = bodies[k];
+  const int i = first + k;
   const float m = mass[i];
   if (m <= 0.0f) return;
   const float mdt2 = m * (1.0f / (dt * dt));
@@ -291,7 +295,8 @@ constexpr int kChunk = 64;
 
 // A body's items in slot order (the color-bucketed body order), one int4
 // each, rows first then tets, each in the body's list order:
-//   row: {-(row + 1), other body (the body itself for one-body rows), 0, 0}
+//   row: {-(row + 1), other body (the body itself for one-body rows),
+//         the normal row of a friction row (else -1), 0}
 //   tet: {(tet << 2) | slot, the other three vertices in slot order}
 // so the row record or tet state and the other bodies' positions are
 // gathered side by side, one dependent load after the item.
@@ -304,6 +309,7 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
     const int j = -it.x - 1;
     const PRow r = rows[j];
     const v3 xo = ld3(x, it.y);
+    const float lam_sib = it.z >= 0 ? rows[it.z].st.x : 0.0f;
     if (r.st.w == 0.0f) return false;
     const int t = r.meta.x;
     const bool has_b = r.meta.z >= 0 && !is_pin(t);
@@ -313,8 +319,7 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
     row_CJ(t, r.meta.y == i, xi, xo, has_b, mk3(r.ar.x, r.ar.y, r.ar.z), r.ar.w, oa, ob, C, J);
     const bool is_t = t == kRowTangent;
     kp = r.st.y;
-    f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w,
-                  is_t ? rows[r.meta.w].st.x : 0.0f, r.lim.y, r.lim.z);
+    f = row_force(C, r.lim.x, r.st.z, r.st.x, kp, is_t, r.lim.w, lam_sib, r.lim.y, r.lim.z);
     return true;
   }
   const int tid = it.x >> 2, slot = it.x & 3;
@@ -331,7 +336,7 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
 
 template <int G>
 __global__ void __launch_bounds__(256)
-k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict__ slot_start,
+k_p_primal_grp(int count, int first, const int* __restrict__ slot_start,
                const int4* __restrict__ items, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
                const PRow* __restrict__ rows, const float4* __restrict__ offs,
@@ -342,8 +347,8 @@ k_p_primal_grp(int count, const int* __restrict__ bodies, const int* __restrict_
   const int g = threadIdx.x % G, lb = threadIdx.x / G;
   const int k = blockIdx.x * BPB + lb;
   const bool live = k < count;
-  const int i = live ? bodies[k] : 0;
-  const int s0 = live ? slot_start[k] : 0, s1 = live ? slot_start[k + 1] : 0;
+  const int i = first + (live ? k : 0);
+  const int s0 = live ? slot_start[i] : 0, s1 = live ? slot_start[i + 1] : 0;
   const float m = live ? mass[i] : 0.0f;
   const bool dyn = live && m > 0.0f;
   const int n_items = dyn ? s1 - s0 : 0;
@@ -494,7 +499,9 @@ struct avbd_p_solver {
   hipStream_t st = nullptr;
   bool flushed = false;
   int nb = 0, nc = 0, nt = 0, n_colors = 0;
-  std::vector<int> cstart;         // per-color ranges of `bodies`
+  std::vector<int> cstart;         // per-color device index ranges
+  std::vector<int> order;          // device index -> body
+  std::vector<int> pos;            // body -> device index
   std::vector<double> color_bytes;  // algorithmic bytes of one primal launch per color
   // one step captured as a graph, replayed while the buffers and the
   // parameters stay the same (a flush or new parameters capture again)
@@ -534,8 +541,8 @@ struct avbd_p_solver {
   PBuf<float4> offs;         // 2 per row: body-local contact offsets (off_a, off_b)
   PBuf<float4> stage;        // n_rows entries: row state out, {anchor, rest} in
   std::vector<float4> h_ar;  // host copy of every row's {anchor, rest} (plane rows move)
-  PBuf<int> bodies, con_start, con_idx;
-  PBuf<int> slot_start;  // item ranges per slot of `bodies`
+  PBuf<int> con_start, con_idx;  // per device index: its rows (row indices)
+  PBuf<int> slot_start;          // per device index: its range of `items`
   PBuf<int4> items;      // per-body rows and tets in slot order (k_p_primal_grp)
   PBuf<int4> tv;
   PBuf<float4> tst;  // {rest volume, alpha C0, penalty, active}
@@ -632,8 +639,8 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
       if (cnt <= 0) continue;
       PCK(timed(0, s->color_bytes[c], [&] {
         if (s->primal_variant == 0) {
-          hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt,
-                             s->bodies.p + s->cstart[c], s->x.p, s->inertial.p, s->mass.p,
+          hipLaunchKernelGGL(k_p_primal, grid(cnt), dim3(kB), 0, st, cnt, s->cstart[c],
+                             s->x.p, s->inertial.p, s->mass.p,
                              s->rows.p, s->offs.p, s->con_start.p, s->con_idx.p, s->tv.p,
                              s->tst.p, s->tet_start.p, s->tet_idx.p, dt);
         } else {
@@ -642,8 +649,7 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
           auto kern = G == 16 ? k_p_primal_grp<16> : G == 32 ? k_p_primal_grp<32>
                                                               : k_p_primal_grp<64>;
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
-                             s->bodies.p + s->cstart[c], s->slot_start.p + s->cstart[c],
-                             s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p,
+                             s->cstart[c], s->slot_start.p, s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p,
                              s->offs.p, s->tst.p, dt);
         }
       }));
@@ -771,21 +777,40 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   s->cstart.assign(sc->n_colors + 1, 0);
   for (int i = 0; i < nb; ++i) s->cstart[sc->color[i] + 1]++;
   for (int c = 0; c < sc->n_colors; ++c) s->cstart[c + 1] += s->cstart[c];
-  std::vector<int> order((size_t)nb), cur(s->cstart.begin(), s->cstart.end() - 1);
-  for (int i = 0; i < nb; ++i) order[cur[sc->color[i]]++] = i;
-  PCK(s->bodies.upload(order.data(), nb));
-  const auto x = pack3(sc->x, nb), v = pack3(sc->v, nb), pv = pack3(sc->prev_v, nb);
+  std::vector<int>& order = s->order;
+  std::vector<int>& pos = s->pos;
+  order.assign((size_t)nb, 0);
+  pos.assign((size_t)nb, 0);
+  {
+    std::vector<int> cur(s->cstart.begin(), s->cstart.end() - 1);
+    for (int i = 0; i < nb; ++i) {
+      pos[i] = cur[sc->color[i]]++;
+      order[pos[i]] = i;
+    }
+  }
+  auto dev3 = [&](const float* a) {  // a body array in device order
+    std::vector<float4> o((size_t)nb);
+    for (int k = 0; k < nb; ++k)
+      o[k] = make_float4(a[3 * order[k]], a[3 * order[k] + 1], a[3 * order[k] + 2], 0.0f);
+    return o;
+  };
+  const auto x = dev3(sc->x), v = dev3(sc->v), pv = dev3(sc->prev_v);
   PCK(s->x.upload(x.data(), nb));
   PCK(s->v.upload(v.data(), nb));
   PCK(s->prev_v.upload(pv.data(), nb));
   PCK(s->initial.upload(x.data(), nb));
   PCK(s->inertial.upload(x.data(), nb));
-  PCK(s->mass.upload(sc->mass, nb));
+  {
+    std::vector<float> m((size_t)nb);
+    for (int k = 0; k < nb; ++k) m[k] = sc->mass[order[k]];
+    PCK(s->mass.upload(m.data(), nb));
+  }
   std::vector<PRow> rows((size_t)nc);
   s->h_ar.assign((size_t)nc, make_float4(0.f, 0.f, 0.f, 0.f));
   for (int j = 0; j < nc; ++j) {
     PRow& r = rows[j];
-    r.meta = make_int4(sc->type[j], sc->body_a[j], sc->body_b[j], sc->sibling[j]);
+    r.meta = make_int4(sc->type[j], pos[sc->body_a[j]],
+                       sc->body_b[j] >= 0 ? pos[sc->body_b[j]] : sc->body_b[j], sc->sibling[j]);
     r.ar = make_float4(sc->anchor[3 * j], sc->anchor[3 * j + 1], sc->anchor[3 * j + 2],
                        sc->rest[j]);
     r.lim = make_float4(sc->stiffness[j], sc->fmin[j], sc->fmax[j], sc->friction[j]);
@@ -806,13 +831,24 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   }
   PCK(s->stage.upload(nullptr, nc));
   PCK(s->fracture.upload(sc->fracture, nc));
-  PCK(s->con_start.upload(sc->con_start, nb + 1));
-  PCK(s->con_idx.upload(sc->con_idx, sc->con_start[nb]));
+  // per-device-index row and tet lists (each body's lists in their order)
+  auto regroup = [&](const int* start, const int* idx, PBuf<int>& d_start,
+                     PBuf<int>& d_idx) -> int {
+    std::vector<int> st((size_t)nb + 1, 0), ix;
+    ix.reserve((size_t)start[nb]);
+    for (int k = 0; k < nb; ++k) {
+      ix.insert(ix.end(), idx + start[order[k]], idx + start[order[k] + 1]);
+      st[k + 1] = (int)ix.size();
+    }
+    PCK(d_start.upload(st.data(), nb + 1));
+    return d_idx.upload(ix.data(), ix.size());
+  };
+  PCK(regroup(sc->con_start, sc->con_idx, s->con_start, s->con_idx));
   std::vector<int4> tv((size_t)nt);
   std::vector<float4> tst((size_t)nt);
   for (int t = 0; t < nt; ++t) {
-    tv[t] = make_int4(sc->tet_v[4 * t], sc->tet_v[4 * t + 1], sc->tet_v[4 * t + 2],
-                      sc->tet_v[4 * t + 3]);
+    tv[t] = make_int4(pos[sc->tet_v[4 * t]], pos[sc->tet_v[4 * t + 1]],
+                      pos[sc->tet_v[4 * t + 2]], pos[sc->tet_v[4 * t + 3]]);
     tst[t] = make_float4(sc->tet_rest_vol[t], 0.0f, sc->tet_penalty[t],
                          sc->tet_active[t] != 0 ? 1.0f : 0.0f);
   }
@@ -821,8 +857,7 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
   PCK(s->tet_lam.upload(sc->tet_lambda, nt));
   PCK(s->tet_stiff.upload(sc->tet_stiffness, nt));
   PCK(s->tet_fracture.upload(sc->tet_fracture, nt));
-  PCK(s->tet_start.upload(sc->tet_start, nb + 1));
-  PCK(s->tet_idx.upload(sc->tet_idx, sc->tet_start[nb]));
+  PCK(regroup(sc->tet_start, sc->tet_idx, s->tet_start, s->tet_idx));
   {
     std::vector<int> ss((size_t)nb + 1, 0);
     std::vector<int4> it;
@@ -833,13 +868,13 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
         const int j = sc->con_idx[e], t = sc->type[j];
         const bool has_b = sc->body_b[j] >= 0 && t != kRowPinX && t != kRowPinY && t != kRowPinZ;
         const int other = has_b ? (sc->body_a[j] == i ? sc->body_b[j] : sc->body_a[j]) : i;
-        it.push_back(make_int4(-j - 1, other, 0, 0));
+        it.push_back(make_int4(-j - 1, pos[other], t == kRowTangent ? sc->sibling[j] : -1, 0));
       }
       for (int e = sc->tet_start[i]; e < sc->tet_start[i + 1]; ++e) {
         const int enc = sc->tet_idx[e], tid = enc >> 2, slot = enc & 3;
         int o[3], n = 0;
         for (int v = 0; v < 4; ++v)
-          if (v != slot) o[n++] = sc->tet_v[4 * tid + v];
+          if (v != slot) o[n++] = pos[sc->tet_v[4 * tid + v]];
         it.push_back(make_int4(enc, o[0], o[1], o[2]));
       }
       ss[k + 1] = (int)it.size();
@@ -928,10 +963,11 @@ int avbd_p_read(avbd_p_solver* s, float* x, float* v, float* prev_v, float* lamb
   auto unpack = [&](const PBuf<float4>& b, float* out) -> int {
     if (!out || s->nb == 0) return AVBD_OK;
     PHIP(hipMemcpy(tmp.data(), b.p, s->nb * sizeof(float4), hipMemcpyDeviceToHost));
-    for (int i = 0; i < s->nb; ++i) {
-      out[3 * i] = tmp[i].x;
-      out[3 * i + 1] = tmp[i].y;
-      out[3 * i + 2] = tmp[i].z;
+    for (int k = 0; k < s->nb; ++k) {
+      const int i = s->order[k];
+      out[3 * i] = tmp[k].x;
+      out[3 * i + 1] = tmp[k].y;
+      out[3 * i + 2] = tmp[k].z;
     }
     return AVBD_OK;
   };
@@ -974,9 +1010,14 @@ int avbd_p_write_bodies(avbd_p_solver* s, int field, int first, int n, const flo
   if (!dst) return perr(AVBD_E_ARG, "field must be 0 (x) or 1 (v)");
   PHIP(hipSetDevice(s->device));
   PHIP(hipStreamSynchronize(s->st));
-  const auto packed = pack3(vals, n);
-  if (n > 0)
-    PHIP(hipMemcpy(dst->p + first, packed.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  if (n == 0) return AVBD_OK;
+  // bodies [first, first + n) sit at scattered device indices: rewrite the
+  // whole array
+  std::vector<float4> all((size_t)s->nb);
+  PHIP(hipMemcpy(all.data(), dst->p, s->nb * sizeof(float4), hipMemcpyDeviceToHost));
+  for (int m = 0; m < n; ++m)
+    all[s->pos[first + m]] = make_float4(vals[3 * m], vals[3 * m + 1], vals[3 * m + 2], 0.0f);
+  PHIP(hipMemcpy(dst->p, all.data(), s->nb * sizeof(float4), hipMemcpyHostToDevice));
   return AVBD_OK;
 }
 
