@@ -154,7 +154,8 @@ class Solver:
         self.num_colors = 0
         self.color_counts: dict[int, int] = {}
         self.body_color = self.c_type = self.c_body_a = self.c_body_b = None
-        self.c_active = None
+        self.c_active = self.c_lambda = self.c_penalty = None
+        self.x = self.v = self.prev_v = self.tet_lambda = self.tet_active = None
 
     # ---- scene building ------------------------------------------------------
     def add_particle(self, position, mass: float, velocity=(0.0, 0.0, 0.0),
@@ -428,6 +429,14 @@ class Solver:
         for name, key in (("c_type", "type"), ("c_body_a", "body_a"), ("c_body_b", "body_b")):
             setattr(self, name, DeviceArray(lambda a=arrays[key]: a.copy()))
         self.c_active = DeviceArray(lambda: self._h.read(("active",))["active"])
+        # the reference's device arrays the viewer and tests read (solver.py:213-225)
+        self.x = DeviceArray(lambda: self._h.read(("x",))["x"])
+        self.v = DeviceArray(lambda: self._h.read(("v",))["v"])
+        self.prev_v = DeviceArray(lambda: self._h.read(("prev_v",))["prev_v"])
+        self.c_lambda = DeviceArray(lambda: self._h.read(("lambda",))["lambda"])
+        self.c_penalty = DeviceArray(lambda: self._h.read(("penalty",))["penalty"])
+        self.tet_lambda = DeviceArray(lambda: self._h.read(("tet_lambda",))["tet_lambda"])
+        self.tet_active = DeviceArray(lambda: self._h.read(("tet_active",))["tet_active"])
         self._dirty = False
 
     def _seed_pool(self, arrays: dict) -> None:

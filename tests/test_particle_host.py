@@ -349,3 +349,18 @@ def test_box_tet_mesh_surface_and_binding():
     assert len(d.floor_constraints) == len(m.surface_verts)
     assert sum(s._bodies_mass) == pytest.approx(500.0 * 0.2 * 0.3 * 0.2, rel=1e-5)
     assert all(h.rows == 3 for h in d.floor_constraints)
+
+
+def test_reference_device_array_attributes():
+    """x / v / c_lambda / c_active ... are None before the first flush and
+    read the device state after it, as the reference's Warp arrays."""
+    S = particle_solver_class()
+    s = S()
+    assert s.x is None and s.c_lambda is None
+    b = s.add_particle((0.0, 1.0, 0.0), 1.0)
+    s.add_pin(b, (0.0, 1.0, 0.0))
+    s.step()
+    np.testing.assert_array_equal(s.x.numpy(), s.positions())
+    np.testing.assert_array_equal(s.c_lambda.numpy(), s.lambdas())
+    assert s.c_active.numpy().tolist() == [1, 1, 1]
+    assert s.v.numpy().shape == (1, 3) and s.c_penalty.numpy().shape == (3,)
