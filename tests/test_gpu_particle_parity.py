@@ -222,3 +222,12 @@ def broken_pair_hook(s, f):
 def test_fracture_then_contact_lockstep():
     sg, _ = _lockstep(broken_pair, 120, hook=broken_pair_hook, dt=1 / 60, iterations=25)
     assert int((sg.c_type.numpy() == 5).sum()) >= 0
+
+
+def test_bench_scene_two_steps_bit_exact():
+    """The measured workload at full size: 256 soft blocks (87,808
+    particles, 642,048 rows, 331,776 tets), 2 steps against the checker."""
+    def bench_scene(s):
+        PS.soft_blocks(s, 256)
+    sg, _ = _lockstep(bench_scene, 2, dt=1 / 60, iterations=20)
+    assert len(sg._bodies_x) == 87808
