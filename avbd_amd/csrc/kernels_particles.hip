@@ -12,10 +12,10 @@
 // color-bucketed order), so a color's bodies are one contiguous index range
 // and the primal indexes x, mass and its item lists without an indirection;
 // rows, tets and items name bodies by device index, and the host-facing
-// reads and writes map between the two orders. A row is one 64-byte record (PRow) holding its static fields and
-// its state, so the primal's gather of a listed row touches one half cache
-// line; a tet is an int4 of vertices plus a 16-byte state record. Rows keep
-// the reference's indices. A contact's friction rows follow their normal
+// reads and writes map between the two orders. A row is one 64-byte record
+// (PRow) holding its static fields and its state, so the primal's gather of
+// a listed row touches one half cache line; a tet is an int4 of vertices
+// plus a 16-byte state record. Rows keep the reference's indices. A contact's friction rows follow their normal
 // row and are updated in the normal row's lane after it, which is the order
 // a serial dual gives them (a friction row's bound reads its normal's new
 // lambda).
@@ -300,15 +300,63 @@ constexpr int kChunk = 64;
 //   tet: {(tet << 2) | slot, the other three vertices in slot order}
 // so the row record or tet state and the other bodies' positions are
 // gathered side by side, one dependent load after the item.
+// Dataflow sweep (all colors in one launch): a body's lower-color
+// neighbours are exactly the device indices below its color's first index;
+// it waits until each has published this sweep's stamp in `done` and reads
+// its position through the agent-coherent path. Higher-color neighbours
+// wait for this body before they move, so their positions are read as they
+// stood at the start of the sweep.
+struct PFlow {
+  int* done;
+  int stamp;
+  int first;
+  int* err;
+};
+constexpr int kPFlowSpinCap = 1 << 20;
+
+__device__ __forceinline__ float4 p_ld_agent(const float4* p) {
+  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+  const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long b =
+      __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float4(__uint_as_float((unsigned)a), __uint_as_float((unsigned)(a >> 32)),
+                     __uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
+}
+__device__ __forceinline__ void p_st_agent(float4* p, float4 v) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  __hip_atomic_store(q, (unsigned long long)__float_as_uint(v.x) |
+                            ((unsigned long long)__float_as_uint(v.y) << 32),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, (unsigned long long)__float_as_uint(v.z) |
+                                ((unsigned long long)__float_as_uint(v.w) << 32),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool FLOW>
+__device__ __forceinline__ v3 other_pos(const float4* __restrict__ x, int d, const PFlow& fl) {
+  if (!FLOW || d >= fl.first) return ld3(x, d);
+  int spins = 0;
+  while (__hip_atomic_load(fl.done + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fl.stamp) {
+    if (++spins > kPFlowSpinCap) {  // never resolved: flag it for the host and go on
+      atomicExch(fl.err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  const float4 t = p_ld_agent(x + d);
+  return mk3(t.x, t.y, t.z);
+}
+
+template <bool FLOW>
 __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
                                           const float4* __restrict__ offs,
                                           const float4* __restrict__ tst,
                                           const float4* __restrict__ x, int4 it, int i, v3 xi,
-                                          v3& J, float& f, float& kp) {
+                                          const PFlow& fl, v3& J, float& f, float& kp) {
   if (it.x < 0) {
     const int j = -it.x - 1;
     const PRow r = rows[j];
-    const v3 xo = ld3(x, it.y);
+    const v3 xo = other_pos<FLOW>(x, it.y, fl);
     const float lam_sib = it.z >= 0 ? rows[it.z].st.x : 0.0f;
     if (r.st.w == 0.0f) return false;
     const int t = r.meta.x;
@@ -324,7 +372,8 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
   }
   const int tid = it.x >> 2, slot = it.x & 3;
   const float4 ts = tst[tid];
-  const v3 o1 = ld3(x, it.y), o2 = ld3(x, it.z), o3 = ld3(x, it.w);
+  const v3 o1 = other_pos<FLOW>(x, it.y, fl), o2 = other_pos<FLOW>(x, it.z, fl),
+           o3 = other_pos<FLOW>(x, it.w, fl);
   if (ts.w == 0.0f) return false;
   v3 p[4];
   int n = 0;
@@ -334,18 +383,34 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
   return true;
 }
 
-template <int G>
+// FLOW: one launch for all colors; block t is tile t of the color whose
+// tile range holds it (tile_start / cstart, n_colors + 1 entries each), and
+// the sweep's stamp is read from `epoch`.
+template <int G, bool FLOW>
 __global__ void __launch_bounds__(256)
-k_p_primal_grp(int count, int first, const int* __restrict__ slot_start,
+k_p_primal_grp(int count_arg, int first_arg, const int* __restrict__ slot_start,
                const int4* __restrict__ items, float4* __restrict__ x,
                const float4* __restrict__ inertial, const float* __restrict__ mass,
                const PRow* __restrict__ rows, const float4* __restrict__ offs,
-               const float4* __restrict__ tst, float dt) {
+               const float4* __restrict__ tst, float dt, int n_colors,
+               const int* __restrict__ tile_start, const int* __restrict__ cstart,
+               int* __restrict__ done, const int* __restrict__ epoch, int* err) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
   __shared__ float acc[BPB][9];
   const int g = threadIdx.x % G, lb = threadIdx.x / G;
-  const int k = blockIdx.x * BPB + lb;
+  int count = count_arg, first = first_arg, tile = blockIdx.x;
+  PFlow fl{done, 0, 0, err};
+  if (FLOW) {
+    int c = 0;
+    while (c + 1 < n_colors && tile >= tile_start[c + 1]) ++c;
+    first = cstart[c];
+    count = cstart[c + 1] - first;
+    tile -= tile_start[c];
+    fl.stamp = *epoch;
+    fl.first = first;
+  }
+  const int k = tile * BPB + lb;
   const bool live = k < count;
   const int i = first + (live ? k : 0);
   const int s0 = live ? slot_start[i] : 0, s1 = live ? slot_start[i + 1] : 0;
@@ -366,7 +431,8 @@ k_p_primal_grp(int count, int first, const int* __restrict__ slot_start,
     for (int q = g; q < n_here; q += G) {
       v3 J;
       float f, kp;
-      const bool ok = slot_item(rows, offs, tst, x, items[s0 + base + q], i, xi, J, f, kp);
+      const bool ok = slot_item<FLOW>(rows, offs, tst, x, items[s0 + base + q], i, xi, fl, J,
+                                      f, kp);
       float* dst = terms[lb][q];
       if (ok) {
         const Terms tm = make_terms(J, f, kp);
@@ -397,8 +463,24 @@ k_p_primal_grp(int count, int first, const int* __restrict__ slot_start,
     lhs.m[2][0] = c[5];
     lhs.m[2][1] = c[7];
     lhs.m[2][2] = c[8];
-    x[i] = st3(sub(xi, mvmul(minverse(lhs), mk3(c[0], c[1], c[2]))));
+    const float4 xn = st3(sub(xi, mvmul(minverse(lhs), mk3(c[0], c[1], c[2]))));
+    if (FLOW)
+      p_st_agent(x + i, xn);
+    else
+      x[i] = xn;
   }
+  if (FLOW && live && g == 0) {
+    // publish after the position store has completed (a static body
+    // publishes its unchanged position)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(done + i, fl.stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_p_epoch(int* epoch) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) epoch[0] = epoch[0] + 1;
 }
 
 // dual_update of one row (kernels.py:611-701)
@@ -506,7 +588,8 @@ struct avbd_p_solver {
   // one step captured as a graph, replayed while the buffers and the
   // parameters stay the same (a flush or new parameters capture again)
   bool use_graph = true;
-  int primal_variant = 2;  // 0: lane per body, 1/2/3: 16/32/64 lanes per body
+  int primal_variant = 4;  // 0: lane per body, 1/2/3: 16/32/64 lanes per body,
+                           // 4/5: 32/16 lanes per body, one dataflow launch per sweep
   bool graph_ok = false;
   avbd_p_params gparams{};
   hipGraph_t graph = nullptr;
@@ -543,6 +626,10 @@ struct avbd_p_solver {
   std::vector<float4> h_ar;  // host copy of every row's {anchor, rest} (plane rows move)
   PBuf<int> con_start, con_idx;  // per device index: its rows (row indices)
   PBuf<int> slot_start;          // per device index: its range of `items`
+  // dataflow sweep: tile starts for 32 / 16 lanes per body, color starts,
+  // per-body stamps, sweep counter, error flag
+  PBuf<int> d_tile_start32, d_tile_start16, d_cstart, done, epoch, flow_err;
+  int n_tiles32 = 0, n_tiles16 = 0;
   PBuf<int4> items;      // per-body rows and tets in slot order (k_p_primal_grp)
   PBuf<int4> tv;
   PBuf<float4> tst;  // {rest volume, alpha C0, penalty, active}
@@ -634,7 +721,21 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   };
   for (int it = 0; it < I + post; ++it) {
     if (post && it == I) cache(0.0f);
-    for (int c = 0; c < s->n_colors; ++c) {
+    if (s->primal_variant >= 4 && s->nb > 0) {
+      double bytes = 0.0;
+      for (double b : s->color_bytes) bytes += b;
+      hipLaunchKernelGGL(k_p_epoch, dim3(1), dim3(64), 0, st, s->epoch.p);
+      const bool g32 = s->primal_variant == 4;
+      auto fkern = g32 ? k_p_primal_grp<32, true> : k_p_primal_grp<16, true>;
+      PCK(timed(0, bytes, [&] {
+        hipLaunchKernelGGL(fkern, dim3(g32 ? s->n_tiles32 : s->n_tiles16), dim3(kB), 0, st, 0, 0,
+                           s->slot_start.p, s->items.p, s->x.p, s->inertial.p, s->mass.p,
+                           s->rows.p, s->offs.p, s->tst.p, dt, s->n_colors,
+                           g32 ? s->d_tile_start32.p : s->d_tile_start16.p, s->d_cstart.p,
+                           s->done.p, s->epoch.p, s->flow_err.p);
+      }));
+    }
+    for (int c = 0; c < s->n_colors && s->primal_variant < 4; ++c) {
       const int cnt = s->cstart[c + 1] - s->cstart[c];
       if (cnt <= 0) continue;
       PCK(timed(0, s->color_bytes[c], [&] {
@@ -646,11 +747,12 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
         } else {
           const int G = s->primal_variant == 1 ? 16 : s->primal_variant == 2 ? 32 : 64;
           const int bpb = kB / G;
-          auto kern = G == 16 ? k_p_primal_grp<16> : G == 32 ? k_p_primal_grp<32>
-                                                              : k_p_primal_grp<64>;
+          auto kern = G == 16 ? k_p_primal_grp<16, false>
+                      : G == 32 ? k_p_primal_grp<32, false> : k_p_primal_grp<64, false>;
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
-                             s->cstart[c], s->slot_start.p, s->items.p, s->x.p, s->inertial.p, s->mass.p, s->rows.p,
-                             s->offs.p, s->tst.p, dt);
+                             s->cstart[c], s->slot_start.p, s->items.p, s->x.p, s->inertial.p,
+                             s->mass.p, s->rows.p, s->offs.p, s->tst.p, dt, 0, nullptr, nullptr,
+                             nullptr, nullptr, nullptr);
         }
       }));
     }
@@ -830,6 +932,23 @@ int avbd_p_flush(avbd_p_solver* s, const avbd_p_scene* sc) {
     PCK(s->offs.upload(off.data(), off.size()));
   }
   PCK(s->stage.upload(nullptr, nc));
+  {
+    std::vector<int> t32((size_t)sc->n_colors + 1, 0), t16((size_t)sc->n_colors + 1, 0);
+    for (int c = 0; c < sc->n_colors; ++c) {
+      const int cnt = s->cstart[c + 1] - s->cstart[c];
+      t32[c + 1] = t32[c] + (cnt + 7) / 8;
+      t16[c + 1] = t16[c] + (cnt + 15) / 16;
+    }
+    s->n_tiles32 = t32[sc->n_colors];
+    s->n_tiles16 = t16[sc->n_colors];
+    PCK(s->d_tile_start32.upload(t32.data(), t32.size()));
+    PCK(s->d_tile_start16.upload(t16.data(), t16.size()));
+    PCK(s->d_cstart.upload(s->cstart.data(), s->cstart.size()));
+    const std::vector<int> zero((size_t)std::max(nb, 1), 0);
+    PCK(s->done.upload(zero.data(), nb));
+    PCK(s->epoch.upload(zero.data(), 1));
+    PCK(s->flow_err.upload(zero.data(), 1));
+  }
   PCK(s->fracture.upload(sc->fracture, nc));
   // per-device-index row and tet lists (each body's lists in their order)
   auto regroup = [&](const int* start, const int* idx, PBuf<int>& d_start,
@@ -920,7 +1039,7 @@ int avbd_p_set_option(avbd_p_solver* s, int option, int value) {
   if (option == AVBD_P_OPT_GRAPH) {
     s->use_graph = value != 0;
   } else if (option == AVBD_P_OPT_PRIMAL_VARIANT) {
-    if (value < 0 || value > 3) return perr(AVBD_E_ARG, "primal variant must be 0 to 3");
+    if (value < 0 || value > 5) return perr(AVBD_E_ARG, "primal variant must be 0 to 5");
     s->primal_variant = value;
   } else {
     return perr(AVBD_E_ARG, "unknown option");
@@ -959,6 +1078,15 @@ int avbd_p_read(avbd_p_solver* s, float* x, float* v, float* prev_v, float* lamb
   if (!s || !s->flushed) return perr(AVBD_E_STATE, "read before flush");
   PHIP(hipSetDevice(s->device));
   PHIP(hipStreamSynchronize(s->st));
+  {
+    int e = 0;
+    PHIP(hipMemcpy(&e, s->flow_err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (e != 0) {
+      e = 0;
+      PHIP(hipMemcpy(s->flow_err.p, &e, sizeof(int), hipMemcpyHostToDevice));
+      return perr(AVBD_E_HIP, "a dataflow primal sweep waited past its spin limit");
+    }
+  }
   std::vector<float4> tmp((size_t)std::max(s->nb, 1));
   auto unpack = [&](const PBuf<float4>& b, float* out) -> int {
     if (!out || s->nb == 0) return AVBD_OK;

@@ -115,8 +115,10 @@ def iters_to_residual(frames: int = 30) -> dict:
 
 
 PARTICLE_TRAFFIC_FILE = ROOT / "profiles" / "k_p_primal_grp_softblocks256_pmc.md"
-PARTICLE_KERNELS = ["k_p_primal (lane per body)", "k_p_primal_grp<16>", "k_p_primal_grp<32>",
-                    "k_p_primal_grp<64>"]
+PARTICLE_KERNELS = ["k_p_primal (lane per body) per color", "k_p_primal_grp<16> per color",
+                    "k_p_primal_grp<32> per color", "k_p_primal_grp<64> per color",
+                    "k_p_primal_grp<32, flow> one launch per sweep",
+                    "k_p_primal_grp<16, flow> one launch per sweep"]
 
 
 def particle_traffic(blocks: int, variant: int) -> float | None:
@@ -143,7 +145,7 @@ def particle_bench(args) -> dict:
     s._flush()
     if args.primal_variant is not None:
         s._h.set_option(1, args.primal_variant)
-    variant = args.primal_variant if args.primal_variant is not None else 2
+    variant = args.primal_variant if args.primal_variant is not None else 4
     s._h.set_option(1, variant)
     nb = len(s._bodies_x)
     print(f"# scene: {nb} particles, {len(s._constraints)} rows, {len(s._tets)} tets, "
@@ -181,7 +183,7 @@ def particle_bench(args) -> dict:
                    "particles": nb, "rows": len(s._constraints), "tets": len(s._tets),
                    "colors": s.num_colors, "parallelism": "single GPU",
                    "primal_variant": variant},
-        "roofline": {"bound": "hbm", "kernel": PARTICLE_KERNELS[variant] + " (one launch per color)",
+        "roofline": {"bound": "hbm", "kernel": PARTICLE_KERNELS[variant],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": particle_traffic(args.blocks, variant),

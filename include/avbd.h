@@ -350,8 +350,10 @@ int avbd_p_write_rows(avbd_p_solver* s, int first, int n, const float* anchor, c
  * AVBD_P_OPT_GRAPH: 1 (default) captures a step once as a HIP graph and
  * replays it while the flushed buffers and the parameters stay the same;
  * 0 launches every kernel directly.
- * AVBD_P_OPT_PRIMAL_VARIANT: 0 one lane per body; 1, 2 (default), 3: 16,
- * 32, 64 lanes per body evaluating its rows and tets in parallel.
+ * AVBD_P_OPT_PRIMAL_VARIANT: 0 one lane per body; 1, 2, 3: 16, 32, 64
+ * lanes per body evaluating its rows and tets in parallel, one launch per
+ * color; 4 (default), 5: 32, 16 lanes per body, one dataflow launch per
+ * sweep (a body waits for its lower-color neighbours' per-sweep stamps).
  * Results are bit-identical for every setting. */
 #define AVBD_P_OPT_GRAPH 0
 #define AVBD_P_OPT_PRIMAL_VARIANT 1

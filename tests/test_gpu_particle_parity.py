@@ -58,8 +58,9 @@ def chain_fracture(s):
     PS.chain(s, 6, 0.5, fracture=30.0)
 
 
-def test_chain_with_fracture_lockstep():
-    sg, _ = _lockstep(chain_fracture, 150, dt=1 / 60, iterations=10)
+@pytest.mark.parametrize("variant", [2, 4])
+def test_chain_with_fracture_lockstep(variant):
+    sg, _ = _lockstep(chain_fracture, 150, variant=variant, dt=1 / 60, iterations=10)
     assert sg.active()[3:].min() == 0  # a link broke during the run
 
 
@@ -67,7 +68,7 @@ def soft_block(s):
     PS.soft_block(s, n=(3, 3, 3), size=(0.4, 0.4, 0.4), center=(0.0, 0.5, 0.0))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_soft_block_on_floor_lockstep(variant):
     """tets, soft distance rows, floor rows with friction rows, post-stab and
     the finalize after it (solver.py:1374-1390); every primal kernel."""
@@ -108,12 +109,13 @@ def test_plates_perturbations_and_reflush_lockstep():
     _lockstep(plates, 60, hook=plates_hook, dt=1 / 60, iterations=10)
 
 
-def test_many_soft_blocks_bit_exact():
+@pytest.mark.parametrize("variant", [2, 4])
+def test_many_soft_blocks_bit_exact(variant):
     """64 soft blocks of 6^3 cells (21,952 particles, 160,512 rows, 82,944
-    tets), 4 steps."""
+    tets), 4 steps, per color launches and the dataflow sweep."""
     def many(s):
         PS.soft_blocks(s, 64)
-    sg, _ = _lockstep(many, 4, dt=1 / 60, iterations=20)
+    sg, _ = _lockstep(many, 4, variant=variant, dt=1 / 60, iterations=20)
     assert np.isfinite(sg.positions()).all()
 
 
