@@ -119,8 +119,10 @@ __device__ __forceinline__ void row_offsets(const float4* __restrict__ offs, int
 __global__ void __launch_bounds__(256)
 k_p_predict(int nb, float4* __restrict__ x, const float4* __restrict__ v,
             const float4* __restrict__ prev_v, const float* __restrict__ mass, float dt, float gx,
-            float gy, float gz, float4* __restrict__ initial, float4* __restrict__ inertial) {
+            float gy, float gz, float4* __restrict__ initial, float4* __restrict__ inertial,
+            int* __restrict__ epoch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) epoch[0] = epoch[0] + 1;  // step counter of the dataflow sweeps' stamps
   if (i >= nb) return;
   const v3 xi = ld3(x, i);
   initial[i] = st3(xi);
@@ -313,6 +315,7 @@ struct PFlow {
   int* err;
 };
 constexpr int kPFlowSpinCap = 1 << 20;
+constexpr int kPSweepStride = 4096;  // more sweeps a step than this: per-color launches
 
 __device__ __forceinline__ float4 p_ld_agent(const float4* p) {
   const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
@@ -384,8 +387,12 @@ __device__ __forceinline__ bool slot_item(const PRow* __restrict__ rows,
 }
 
 // FLOW: one launch for all colors; block t is tile t of the color whose
-// tile range holds it (tile_start / cstart, n_colors + 1 entries each), and
-// the sweep's stamp is read from `epoch`.
+// tile range holds it (tile_start / cstart, n_colors + 1 entries each). The
+// sweep's stamp is step * kPSweepStride + sweep + 1 (mod 2^32), with the step
+// counter read from `epoch` (advanced by the step's predict launch): two
+// consecutive sweeps never share a stamp (fewer than kPSweepStride sweeps a
+// step), graph replays included, and every body publishes every sweep, so a
+// body's stamp from the previous sweep never passes for this one.
 template <int G, bool FLOW>
 __global__ void __launch_bounds__(256)
 k_p_primal_grp(int count_arg, int first_arg, const int* __restrict__ slot_start,
@@ -394,7 +401,7 @@ k_p_primal_grp(int count_arg, int first_arg, const int* __restrict__ slot_start,
                const PRow* __restrict__ rows, const float4* __restrict__ offs,
                const float4* __restrict__ tst, float dt, int n_colors,
                const int* __restrict__ tile_start, const int* __restrict__ cstart,
-               int* __restrict__ done, const int* __restrict__ epoch, int* err) {
+               int* __restrict__ done, const int* __restrict__ epoch, int sweep, int* err) {
   constexpr int BPB = 256 / G;  // bodies per block
   __shared__ float terms[BPB][kChunk][9];
   __shared__ float acc[BPB][9];
@@ -407,7 +414,7 @@ k_p_primal_grp(int count_arg, int first_arg, const int* __restrict__ slot_start,
     first = cstart[c];
     count = cstart[c + 1] - first;
     tile -= tile_start[c];
-    fl.stamp = *epoch;
+    fl.stamp = (int)((unsigned)*epoch * (unsigned)kPSweepStride + (unsigned)sweep + 1u);
     fl.first = first;
   }
   const int k = tile * BPB + lb;
@@ -477,10 +484,6 @@ k_p_primal_grp(int count_arg, int first_arg, const int* __restrict__ slot_start,
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __hip_atomic_store(done + i, fl.stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-}
-
-__global__ void k_p_epoch(int* epoch) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) epoch[0] = epoch[0] + 1;
 }
 
 // dual_update of one row (kernels.py:611-701)
@@ -691,7 +694,7 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   };
   hipLaunchKernelGGL(k_p_predict, grid(nb), dim3(kB), 0, st, nb, s->x.p, s->v.p, s->prev_v.p,
                      s->mass.p, dt, p->gravity[0], p->gravity[1], p->gravity[2], s->initial.p,
-                     s->inertial.p);
+                     s->inertial.p, s->epoch.p);
   const int post = p->post_stabilize ? 1 : 0;
   auto cache = [&](float alpha) {
     if (nc > 0)
@@ -721,10 +724,10 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
   };
   for (int it = 0; it < I + post; ++it) {
     if (post && it == I) cache(0.0f);
-    if (s->primal_variant >= 4 && s->nb > 0) {
+    const bool flow = s->primal_variant >= 4 && I + post < kPSweepStride;
+    if (flow) {
       double bytes = 0.0;
       for (double b : s->color_bytes) bytes += b;
-      hipLaunchKernelGGL(k_p_epoch, dim3(1), dim3(64), 0, st, s->epoch.p);
       const bool g32 = s->primal_variant == 4;
       auto fkern = g32 ? k_p_primal_grp<32, true> : k_p_primal_grp<16, true>;
       PCK(timed(0, bytes, [&] {
@@ -732,10 +735,10 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
                            s->slot_start.p, s->items.p, s->x.p, s->inertial.p, s->mass.p,
                            s->rows.p, s->offs.p, s->tst.p, dt, s->n_colors,
                            g32 ? s->d_tile_start32.p : s->d_tile_start16.p, s->d_cstart.p,
-                           s->done.p, s->epoch.p, s->flow_err.p);
+                           s->done.p, s->epoch.p, it, s->flow_err.p);
       }));
     }
-    for (int c = 0; c < s->n_colors && s->primal_variant < 4; ++c) {
+    for (int c = 0; c < s->n_colors && !flow; ++c) {
       const int cnt = s->cstart[c + 1] - s->cstart[c];
       if (cnt <= 0) continue;
       PCK(timed(0, s->color_bytes[c], [&] {
@@ -752,7 +755,7 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
           hipLaunchKernelGGL(kern, dim3((cnt + bpb - 1) / bpb), dim3(kB), 0, st, cnt,
                              s->cstart[c], s->slot_start.p, s->items.p, s->x.p, s->inertial.p,
                              s->mass.p, s->rows.p, s->offs.p, s->tst.p, dt, 0, nullptr, nullptr,
-                             nullptr, nullptr, nullptr);
+                             nullptr, nullptr, 0, nullptr);
         }
       }));
     }

@@ -19,6 +19,7 @@ penalty to the next step by its pair key (solver.py:1160-1185, 1392-1421).
 from __future__ import annotations
 
 import math
+import time
 import warnings
 from dataclasses import dataclass
 
@@ -155,6 +156,7 @@ class Solver:
         self.color_counts: dict[int, int] = {}
         self.body_color = self.c_type = self.c_body_a = self.c_body_b = None
         self.c_active = self.c_lambda = self.c_penalty = None
+        self.primal_variant = 4  # the library's default primal kernel
         self.x = self.v = self.prev_v = self.tet_lambda = self.tet_active = None
 
     # ---- scene building ------------------------------------------------------
@@ -527,6 +529,30 @@ class Solver:
         if self._h is None:
             return np.ones(len(self._constraints), np.int32)
         return self._h.read(("active",))["active"]
+
+    def autotune_primal(self, steps: int = 2, variants=(2, 4)) -> dict[int, float]:
+        """Time `steps` real steps with each primal kernel variant (one
+        launch per color, one dataflow launch per sweep, ...) and keep the
+        fastest; every variant gives bit-identical results, so the steps
+        are ordinary simulation steps. Returns ms per step by variant. Not
+        in the reference."""
+        self._flush()
+        if self._h is None or not self._bodies_x:
+            return {}
+        times = {}
+        for v in variants:
+            self._h.set_option(N.P_OPT_PRIMAL_VARIANT, int(v))
+            self.step()
+            self._h.read(("x",))
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                self.step()
+            self._h.read(("x",))
+            times[int(v)] = 1e3 * (time.perf_counter() - t0) / steps
+        best = min(times, key=times.get)
+        self._h.set_option(N.P_OPT_PRIMAL_VARIANT, best)
+        self.primal_variant = best
+        return times
 
     def state(self) -> dict[str, np.ndarray]:
         """Every device array the step updates (bodies, rows, tets)."""

@@ -143,16 +143,23 @@ def particle_bench(args) -> dict:
     s = Solver(dt=1 / 60, iterations=20)
     PS.soft_blocks(s, args.blocks)
     s._flush()
-    if args.primal_variant is not None:
-        s._h.set_option(1, args.primal_variant)
-    variant = args.primal_variant if args.primal_variant is not None else 4
-    s._h.set_option(1, variant)
     nb = len(s._bodies_x)
     print(f"# scene: {nb} particles, {len(s._constraints)} rows, {len(s._tets)} tets, "
           f"{s.num_colors} colors, build+flush {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.step()
+    tuned = None
+    if args.primal_variant is not None:
+        variant = args.primal_variant
+        s._h.set_option(1, variant)
+    else:
+        # per-color launches or the dataflow sweep, by timing real steps
+        # (bit-identical results, untimed)
+        tuned = s.autotune_primal(steps=3)
+        variant = s.primal_variant
+        print(f"# primal autotune (ms per step): {tuned} -> variant {variant}", file=sys.stderr,
+              flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -182,7 +189,9 @@ def particle_bench(args) -> dict:
                                f"dt 1/60, 20 iterations + post-stabilisation",
                    "particles": nb, "rows": len(s._constraints), "tets": len(s._tets),
                    "colors": s.num_colors, "parallelism": "single GPU",
-                   "primal_variant": variant},
+                   "primal_variant": variant,
+                   "primal_autotune_ms": ({str(k): round(v, 3) for k, v in tuned.items()}
+                                          if tuned else None)},
         "roofline": {"bound": "hbm", "kernel": PARTICLE_KERNELS[variant],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -233,3 +233,16 @@ def test_bench_scene_two_steps_bit_exact():
         PS.soft_blocks(s, 256)
     sg, _ = _lockstep(bench_scene, 2, dt=1 / 60, iterations=20)
     assert len(sg._bodies_x) == 87808
+
+
+def test_autotune_keeps_the_trajectory():
+    """Autotuning steps the scene with each variant; the result equals the
+    checker stepped the same number of times."""
+    sg, so = Solver(dt=1 / 60, iterations=10), particle_solver_class()(dt=1 / 60, iterations=10)
+    for s in (sg, so):
+        PS.soft_blocks(s, 4, n=(3, 3, 3))
+    times = sg.autotune_primal(steps=2, variants=(2, 4))
+    assert set(times) == {2, 4} and sg.primal_variant in (2, 4)
+    for _ in range(2 * (1 + 2)):
+        so.step()
+    _compare(sg.state(), so.state(), "after autotune")
