@@ -246,3 +246,10 @@ def test_autotune_keeps_the_trajectory():
     for _ in range(2 * (1 + 2)):
         so.step()
     _compare(sg.state(), so.state(), "after autotune")
+
+
+def test_many_sweeps_fall_back_to_per_color_launches():
+    """With 4,096 or more sweeps a step the dataflow stamps would repeat
+    within a step, so the step uses the per-color launches; the result
+    still matches the checker."""
+    _lockstep(chain_fracture, 1, variant=4, dt=1 / 60, iterations=4100)
