@@ -507,10 +507,10 @@ __device__ __forceinline__ void p_dual_one(int j, const float4* __restrict__ x,
 
 // dual_update (kernels.py:611-701): a non-friction row's lane updates it and
 // then the friction rows that follow it and name it as their sibling.
-__global__ void __launch_bounds__(256)
-k_p_dual(int nc, const float4* __restrict__ x, PRow* __restrict__ rows,
-         const float4* __restrict__ offs, const float* __restrict__ fracture, float beta) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void p_dual_lane(int j, int nc, const float4* __restrict__ x,
+                                            PRow* __restrict__ rows,
+                                            const float4* __restrict__ offs,
+                                            const float* __restrict__ fracture, float beta) {
   if (j >= nc || rows[j].meta.x == kRowTangent) return;
   p_dual_one(j, x, rows, offs, fracture, beta);
   for (int k = j + 1; k < nc; ++k) {
@@ -521,11 +521,11 @@ k_p_dual(int nc, const float4* __restrict__ x, PRow* __restrict__ rows,
 }
 
 // tet_dual_update (kernels.py:866-912)
-__global__ void __launch_bounds__(256)
-k_p_tet_dual(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
-             float4* __restrict__ tst, float* __restrict__ lam, const float* __restrict__ stiff,
-             const float* __restrict__ fracture, float beta) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void p_tet_dual_lane(int t, int nt, const float4* __restrict__ x,
+                                                const int4* __restrict__ tv,
+                                                float4* __restrict__ tst, float* __restrict__ lam,
+                                                const float* __restrict__ stiff,
+                                                const float* __restrict__ fracture, float beta) {
   if (t >= nt) return;
   const float4 ts = tst[t];
   if (ts.w == 0.0f) return;
@@ -541,6 +541,22 @@ k_p_tet_dual(int nt, const float4* __restrict__ x, const int4* __restrict__ tv,
   }
   const float upper = fmin_(kPenaltyMax, stiff[t]);
   tst[t].z = fmin_(ts.z + beta * fabs_(C), upper);
+}
+
+// Both duals of an iteration in one launch: the first row_blocks blocks take
+// the rows, the rest the tets. They touch disjoint state, so the reference's
+// two launches (solver.py:1318-1350) and this one give the same result.
+__global__ void __launch_bounds__(256)
+k_p_duals(int nc, int row_blocks, int nt, const float4* __restrict__ x, PRow* __restrict__ rows,
+          const float4* __restrict__ offs, const float* __restrict__ fracture,
+          const int4* __restrict__ tv, float4* __restrict__ tst, float* __restrict__ tet_lam,
+          const float* __restrict__ tet_stiff, const float* __restrict__ tet_fracture,
+          float beta) {
+  if ((int)blockIdx.x < row_blocks)
+    p_dual_lane(blockIdx.x * blockDim.x + threadIdx.x, nc, x, rows, offs, fracture, beta);
+  else
+    p_tet_dual_lane((blockIdx.x - row_blocks) * blockDim.x + threadIdx.x, nt, x, tv, tst, tet_lam,
+                    tet_stiff, tet_fracture, beta);
 }
 
 // finalize_velocity + cap_velocity (kernels.py:707-737)
@@ -759,16 +775,14 @@ int enqueue_step(avbd_p_solver* s, const avbd_p_params* p) {
         }
       }));
     }
-    if (nc > 0 && it < I)
+    if ((nc > 0 || nt > 0) && it < I) {
+      const int rb = (nc + kB - 1) / kB, tb = (nt + kB - 1) / kB;
       PCK(timed(1, 0.0, [&] {
-        hipLaunchKernelGGL(k_p_dual, grid(nc), dim3(kB), 0, st, nc, s->x.p, s->rows.p,
-                           s->offs.p, s->fracture.p, p->beta);
+        hipLaunchKernelGGL(k_p_duals, dim3(rb + tb), dim3(kB), 0, st, nc, rb, nt, s->x.p,
+                           s->rows.p, s->offs.p, s->fracture.p, s->tv.p, s->tst.p, s->tet_lam.p,
+                           s->tet_stiff.p, s->tet_fracture.p, p->beta);
       }));
-    if (nt > 0 && it < I)
-      PCK(timed(1, 0.0, [&] {
-        hipLaunchKernelGGL(k_p_tet_dual, grid(nt), dim3(kB), 0, st, nt, s->x.p, s->tv.p,
-                           s->tst.p, s->tet_lam.p, s->tet_stiff.p, s->tet_fracture.p, p->beta);
-      }));
+    }
     if (it == I - 1 && nt == 0) finalize();
   }
   if (nt > 0) finalize();
