@@ -524,7 +524,12 @@ void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int
 // pos: position of each body in `order` (JP); null means the identity order.
 // Each lane collects up to kPollBatch earlier neighbours in LDS and polls
 // all of them at once with relaxed loads (the published value is the only
-// data exchanged), so one dependency step costs one L2 round trip.
+// data exchanged), so one dependency step across wavefronts costs one L2
+// round trip. A neighbour of the same ticket (64 consecutive positions, e.g.
+// the previous body of a row) is read from the wavefront's copy of the
+// published values in LDS instead, and after every poll the lanes that wait
+// only on such neighbours resolve among themselves from LDS until none makes
+// progress, so a chain inside one ticket costs LDS accesses, not L2 trips.
 constexpr int kPollBatch = 16;
 constexpr int kSpinCap = 1 << 20;
 
@@ -533,13 +538,25 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
                const int* __restrict__ nbr_start, const int* __restrict__ nbr,
                const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err,
                int* clamped, const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
+  // nj_s: a neighbour j to poll, or ~lane when j is this ticket's body of
+  // that lane; lpub: this ticket's published values, by lane
+  // (lpub is accessed with workgroup-scope relaxed atomics: a volatile LDS
+  // access would be emitted as a flat access, which serialises the poll)
   __shared__ int nj_s[kPollBatch][64];
+  __shared__ int lpub[64];
+  auto lpub_ld = [&](int l) {
+    return __hip_atomic_load(&lpub[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto lpub_st = [&](int l, int v) {
+    __hip_atomic_store(&lpub[l], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
   const int lane = threadIdx.x;
   for (;;) {
     int base = 0;
     if (lane == 0) base = atomicAdd(ticket, 64);
     base = __shfl(base, 0, 64);
     if (base >= nb) return;
+    lpub_st(lane, -1);
     const int k = base + lane;
     const int i = k < nb ? order[k] : -1;
     // a body of a clean scene (batched) keeps its color and has no neighbour
@@ -552,8 +569,38 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
       q = nbr_start[i];
       q1 = nbr_start[i + 1];
     }
-    unsigned used = 0u, pend = 0u, gt = 0u;
+    // pend: batch entries not yet resolved; loc: entries read from lpub;
+    // gt (JP): entries whose body index exceeds i (their round counts +1)
+    unsigned used = 0u, pend = 0u, loc = 0u, gt = 0u;
     int rmax = 0, spins = 0;
+    // fold a resolved neighbour's published value into used / rmax
+    auto absorb = [&](int t, int vt) {
+      pend &= ~(1u << t);
+      if (!mode_jp) {
+        if (vt < kMaxColors) used |= 1u << vt;
+      } else {
+        const int cj = (vt & 255) - 1;
+        const int rj = (vt >> 8) + (int)(gt >> t & 1u);
+        if (cj >= 0) used |= 1u << cj;
+        rmax = rj > rmax ? rj : rmax;
+      }
+    };
+    auto publish = [&]() {
+      // all 32 colors taken (jacobi) or past the round limit (JP): the
+      // result may leave same-color neighbours
+      if (used == 0xffffffffu || (mode_jp && rmax >= kJpRounds)) atomicOr(clamped, 1);
+      int val;
+      if (!mode_jp) {
+        val = lowest_free_color(used);
+      } else {
+        const int r = rmax < kJpRounds ? rmax : kJpRounds;
+        const int c = rmax < kJpRounds ? lowest_free_color(used) : -1;
+        val = (r << 8) | (c + 1);
+      }
+      st_relaxed(&pub[i], val);
+      lpub_st(lane, val);
+      done = true;
+    };
     // Per-lane state machine: collect a batch of earlier neighbours, poll it
     // until resolved, repeat, publish. The loop exit is uniform over the
     // wavefront (a ballot), so a lane's publishing store stays inside the
@@ -565,8 +612,10 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
         if (pend == 0u && q < q1) {
           int m = 0;
           gt = 0u;
+          loc = 0u;
           for (; q < q1 && m < kPollBatch; ++q) {
             const int j = nbr[q];
+            int pj = j;  // position of j in the processing order
             if (!mode_jp) {
               const int cj = color_in[j];
               if (cj >= 0) {
@@ -575,52 +624,71 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
               }
               if (j >= i) continue;
             } else {
-              if ((pos ? pos[j] : j) >= my_pos) continue;
+              pj = pos ? pos[j] : j;
+              if (pj >= my_pos) continue;
               if (j > i) gt |= 1u << m;
             }
-            nj_s[m][lane] = j;
+            if (pj >= base) {
+              nj_s[m][lane] = ~(pj - base);
+              loc |= 1u << m;
+            } else {
+              nj_s[m][lane] = j;
+            }
             pend |= 1u << m;
             ++m;
           }
         }
         if (pend != 0u) {
-          int v[kPollBatch];
+          // global entries first, all loads in flight at once; the LDS
+          // entries in a loop of their own so no load mixes the two
+          int jj[kPollBatch], v[kPollBatch], w[kPollBatch];
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t) jj[t] = nj_s[t][lane];
 #pragma unroll
           for (int t = 0; t < kPollBatch; ++t)
-            v[t] = (pend >> t & 1u) ? ld_relaxed(&pub[nj_s[t][lane]]) : -1;
+            v[t] = ((pend & ~loc) >> t & 1u) ? ld_relaxed(&pub[jj[t]]) : -1;
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t)
+            w[t] = ((pend & loc) >> t & 1u) ? lpub_ld(~jj[t]) : -1;
 #pragma unroll
           for (int t = 0; t < kPollBatch; ++t) {
-            if (v[t] < 0) continue;
-            pend &= ~(1u << t);
-            if (!mode_jp) {
-              if (v[t] < kMaxColors) used |= 1u << v[t];
-            } else {
-              const int cj = (v[t] & 255) - 1;
-              const int rj = (v[t] >> 8) + (int)(gt >> t & 1u);
-              if (cj >= 0) used |= 1u << cj;
-              rmax = rj > rmax ? rj : rmax;
-            }
+            const int vt = v[t] > w[t] ? v[t] : w[t];  // at most one is read
+            if (vt >= 0) absorb(t, vt);
           }
           if (pend != 0u && ++spins > kSpinCap) {
             // A dependency never resolved: publish color 0 so waiters
             // drain, and flag the failure for the host.
             atomicExch(err, 1);
             st_relaxed(&pub[i], mode_jp ? 1 : 0);
+            lpub_st(lane, mode_jp ? 1 : 0);
             done = true;
           }
-        } else if (q >= q1) {
-          // all 32 colors taken (jacobi) or past the round limit (JP): the
-          // result may leave same-color neighbours
-          if (used == 0xffffffffu || (mode_jp && rmax >= kJpRounds)) atomicOr(clamped, 1);
-          if (!mode_jp) {
-            st_relaxed(&pub[i], lowest_free_color(used));
-          } else {
-            const int r = rmax < kJpRounds ? rmax : kJpRounds;
-            const int c = rmax < kJpRounds ? lowest_free_color(used) : -1;
-            st_relaxed(&pub[i], (r << 8) | (c + 1));
-          }
-          done = true;
         }
+        if (!done && pend == 0u && q >= q1) publish();
+      }
+      // Lanes waiting only on lanes of this ticket: resolve from LDS while
+      // any of them progresses (each round resolves at least one entry, so
+      // this ends after at most 64 x kPollBatch rounds).
+      for (;;) {
+        const bool local_only = !done && pend != 0u && (pend & ~loc) == 0u;
+        if (__builtin_amdgcn_ballot_w64(local_only) == 0) break;
+        bool moved = false;
+        if (local_only) {
+          int w[kPollBatch];
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t) w[t] = nj_s[t][lane];
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t) w[t] = (pend >> t & 1u) ? lpub_ld(~w[t]) : -1;
+#pragma unroll
+          for (int t = 0; t < kPollBatch; ++t) {
+            if (w[t] >= 0) {
+              absorb(t, w[t]);
+              moved = true;
+            }
+          }
+          if (pend == 0u && q >= q1) publish();
+        }
+        if (__builtin_amdgcn_ballot_w64(moved) == 0) break;
       }
       if (__builtin_amdgcn_ballot_w64(!done && pend != 0u) != 0) __builtin_amdgcn_s_sleep(1);
     }
