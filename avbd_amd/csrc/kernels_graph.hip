@@ -515,48 +515,55 @@ void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int
 // neighbours that come earlier in the order, which is what a serial loop over
 // the order produces (gpu_color_spec_assign executed serially,
 // kernels_6dof.py:3203-3243; Jones-Plassmann round numbers,
-// solver_6dof.py:770-806). One wavefront takes 64 consecutive positions per
-// ticket; tickets are handed out in order, so every position a wavefront
-// waits on belongs to a wavefront that already holds a ticket.
+// solver_6dof.py:770-806). One workgroup of W wavefronts takes 64 W
+// consecutive positions per ticket; tickets are handed out in order, so every
+// position a workgroup waits on belongs to a workgroup that already holds a
+// ticket.
 //
 // pub[i]: -1 until body i is resolved. Jacobi: the chosen color. JP:
 // (min(round, 24) << 8) | (color + 1), color = -1 when round >= 24.
 // pos: position of each body in `order` (JP); null means the identity order.
 // Each lane collects up to kPollBatch earlier neighbours in LDS and polls
 // all of them at once with relaxed loads (the published value is the only
-// data exchanged), so one dependency step across wavefronts costs one L2
-// round trip. A neighbour of the same ticket (64 consecutive positions, e.g.
-// the previous body of a row) is read from the wavefront's copy of the
-// published values in LDS instead, and after every poll the lanes that wait
-// only on such neighbours resolve among themselves from LDS until none makes
-// progress, so a chain inside one ticket costs LDS accesses, not L2 trips.
+// data exchanged), so one dependency step across tickets costs one memory
+// round trip. A neighbour of the same ticket (e.g. the previous body of a
+// row) is read from the workgroup's copy of the published values in LDS
+// instead, and after every poll the lanes of a wavefront that wait only on
+// such neighbours resolve among themselves from LDS until none makes
+// progress, so a chain inside one ticket costs LDS accesses.
 constexpr int kPollBatch = 16;
 constexpr int kSpinCap = 1 << 20;
 
-__global__ void __launch_bounds__(64)
+template <int W>
+__global__ void __launch_bounds__(64 * W)
 k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ pos,
                const int* __restrict__ nbr_start, const int* __restrict__ nbr,
                const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err,
                int* clamped, const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
-  // nj_s: a neighbour j to poll, or ~lane when j is this ticket's body of
-  // that lane; lpub: this ticket's published values, by lane
-  // (lpub is accessed with workgroup-scope relaxed atomics: a volatile LDS
-  // access would be emitted as a flat access, which serialises the poll)
-  __shared__ int nj_s[kPollBatch][64];
-  __shared__ int lpub[64];
+  // kT = 64 W positions per ticket. nj_s: a neighbour j to poll, or ~slot
+  // when j is this ticket's body of that slot; lpub: this ticket's
+  // published values, by slot (accessed with workgroup-scope relaxed
+  // atomics: a volatile LDS access would be emitted as a flat access, which
+  // serialises the poll)
+  constexpr int kT = 64 * W;
+  __shared__ int nj_s[kPollBatch][kT];
+  __shared__ int lpub[kT];
+  __shared__ int s_base;
   auto lpub_ld = [&](int l) {
     return __hip_atomic_load(&lpub[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
   auto lpub_st = [&](int l, int v) {
     __hip_atomic_store(&lpub[l], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;  // slot in the ticket
   for (;;) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(ticket, 64);
-    base = __shfl(base, 0, 64);
-    if (base >= nb) return;
+    if (lane == 0) s_base = atomicAdd(ticket, kT);
     lpub_st(lane, -1);
+    // the ticket and the cleared lpub are seen by every wavefront before any
+    // of them reads them
+    __syncthreads();
+    const int base = s_base;
+    if (base >= nb) return;
     const int k = base + lane;
     const int i = k < nb ? order[k] : -1;
     // a body of a clean scene (batched) keeps its color and has no neighbour
@@ -692,6 +699,9 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
       }
       if (__builtin_amdgcn_ballot_w64(!done && pend != 0u) != 0) __builtin_amdgcn_s_sleep(1);
     }
+    // every slot is resolved before the next ticket reuses nj_s, lpub and
+    // s_base
+    __syncthreads();
   }
 }
 
@@ -712,10 +722,14 @@ void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* po
   if (nb <= 0) return;
   launch_fill_int(st, pub, nb, -1);
   launch_fill_int(st, ticket, 1, 0);
-  // Enough wavefronts to cover the GPU; each loops over tickets.
-  const int waves = std::min((nb + 63) / 64, 4096);
-  hipLaunchKernelGGL(k_color_greedy, dim3(waves), dim3(64), 0, st, nb, order, pos, nbr_start, nbr,
-                     color_in_out, pub, ticket, mode_jp, err, clamped, scene, scene_dirty);
+  // Workgroups of 16 wavefronts (tickets of 1024 positions, so more of a
+  // chain of consecutive bodies stays inside one ticket), enough of them to
+  // cover the GPU; each loops over tickets.
+  constexpr int kW = 16;
+  const int blocks = std::min((nb + 64 * kW - 1) / (64 * kW), 4096 / kW);
+  hipLaunchKernelGGL(k_color_greedy<kW>, dim3(blocks), dim3(64 * kW), 0, st, nb, order, pos,
+                     nbr_start, nbr, color_in_out, pub, ticket, mode_jp, err, clamped, scene,
+                     scene_dirty);
   hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
                      mode_jp, scene, scene_dirty);
 }
