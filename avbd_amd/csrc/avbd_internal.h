@@ -127,7 +127,11 @@ void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const Group
 // ---- graph kernels (kernels_graph.hip) ----
 // Exclusive scan of n ints; returns the total through *total_dev (device int).
 size_t scan_workspace_bytes(int n);
-void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws);
+// Exclusive scan of n ints (in == out allowed). ws: scan_workspace_bytes(n),
+// zero when first used and not shared with another stream; err (may be null)
+// is set to 1 if a tile's look-back hit its spin cap.
+void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws,
+                    int* err);
 void launch_tcsr_pairs(hipStream_t st, int kept, const int* pa, const int* pb, const int* succ,
                        int* asb_cnt, int* asb_groups, int* start_a, int* end_a);
 void launch_tcsr_asb_fill(hipStream_t st, int kept, const int* pb, const int* succ,

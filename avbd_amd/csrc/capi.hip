@@ -198,14 +198,17 @@ namespace {
 int ensure_scan_ws(avbd_solver* s, size_t n) {
   if (n <= s->ws_scan_elems && s->ws_scan.p) return AVBD_OK;
   size_t want = std::max(n, (size_t)1024);
-  CK(s->ws_scan.alloc(scan_workspace_bytes((int)want)));
+  const size_t bytes = scan_workspace_bytes((int)want);
+  CK(s->ws_scan.alloc(bytes));
+  // the single-pass scan needs a zero workspace when first used
+  launch_fill_int(s->st, (int*)s->ws_scan.p, (int)(bytes / sizeof(int)), 0);
   s->ws_scan_elems = want;
   return AVBD_OK;
 }
 
 int scan(avbd_solver* s, const int* in, int* out, int n, int* total_dev) {
   CK(ensure_scan_ws(s, (size_t)n));
-  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p);
+  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p, s->dcnt.p + 8);
   return AVBD_OK;
 }
 
@@ -213,9 +216,11 @@ int readback(avbd_solver* s, int first, int count) {
   HIPCK(hipMemcpyAsync(s->hcnt + first, s->dcnt.p + first, count * sizeof(int),
                        hipMemcpyDeviceToHost, s->st));
   HIPCK(hipStreamSynchronize(s->st));
-  // dcnt[8]: set by a dataflow primal sweep whose wait hit its spin cap; it
-  // is copied at the end of every substep and checked at the next sync.
-  if (s->hcnt[8] != 0) return set_err(AVBD_E_STATE, "primal dataflow sweep did not complete");
+  // dcnt[8]: set by a dataflow primal sweep or a scan whose wait hit its
+  // spin cap; it is copied at the end of every substep and checked at the
+  // next sync.
+  if (s->hcnt[8] != 0)
+    return set_err(AVBD_E_STATE, "a dataflow wait (primal sweep or scan) did not complete");
   return AVBD_OK;
 }
 
