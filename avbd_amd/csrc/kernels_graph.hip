@@ -1,8 +1,10 @@
 // kernels_graph.hip — scans, radix sort, CSR and coloring for gfx950.
 //
-//  exclusive_scan       device-wide int scan (block scan + carry pass)
+//  exclusive_scan       device-wide int scan in one launch (decoupled
+//                       look-back over 1024-int tiles)
 //  radix_sort_pairs     LSD radix sort, 4-bit digits, stable ranks from
-//                       64-lane ballots (used for Morton codes and colors)
+//                       64-lane ballots (used for Morton codes and colors);
+//                       two launches per pass (histogram, scatter)
 //  tcsr_*               body -> contact-group lists in color-position order,
 //                       each body's groups ascending as a serial scatter
 //                       leaves them (gpu_csr_count/starts/scatter,
@@ -203,14 +205,50 @@ k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, int nblk,
   if (threadIdx.x < 16) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Each tile finds its 16 output offsets from the digit histogram itself
+// (hist[d * nblk + b], digit-major): offset(d, b) = keys of smaller digits
+// in all tiles + keys of digit d in tiles before b. Reading the whole
+// histogram per tile costs 16 x nblk loads, a few KB, and saves a scan
+// launch per pass.
 __global__ void __launch_bounds__(256)
 k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin, int n, int shift,
-                int nblk, const int* __restrict__ hscan, unsigned* __restrict__ kout,
+                int nblk, const int* __restrict__ hist, unsigned* __restrict__ kout,
                 int* __restrict__ vout) {
   __shared__ int running[16];
   __shared__ int wcnt[4][16];
+  __shared__ int part[2][16];  // per digit: keys in tiles before this one, keys in all tiles
   int lane = __lane_id(), wid = threadIdx.x >> 6;
-  if (threadIdx.x < 16) running[threadIdx.x] = hscan[threadIdx.x * nblk + blockIdx.x];
+  {
+    // wavefront w sums digits 4w .. 4w+3 over the tiles
+    const int d0 = wid * 4;
+    int before[4], all[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) before[j] = all[j] = 0;
+    for (int bb = lane; bb < nblk; bb += 64) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = hist[(d0 + j) * nblk + bb];
+        all[j] += c;
+        before[j] += bb < (int)blockIdx.x ? c : 0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int sb = wave_sum(before[j]), sa = wave_sum(all[j]);
+      if (lane == 0) {
+        part[0][d0 + j] = sb;
+        part[1][d0 + j] = sa;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int smaller = 0;  // keys of digits below d, all tiles
+      for (int d = 0; d < 16; ++d) {
+        running[d] = smaller + part[0][d];
+        smaller += part[1][d];
+      }
+    }
+  }
   __syncthreads();
   unsigned long long lt = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
   int base = blockIdx.x * kRadixTile;
@@ -247,51 +285,14 @@ k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin, i
 size_t radix_workspace_bytes(int n) {
   int nblk = (n + kRadixTile - 1) / kRadixTile;
   size_t h = (size_t)16 * nblk;
-  return 2 * a256(h * sizeof(int)) + 256;
-}
-
-// Exclusive scan of a digit histogram (16 x tiles ints, a few thousand) in
-// one workgroup: one launch per radix pass instead of a device-wide scan.
-__global__ void __launch_bounds__(1024)
-k_scan_single(const int* __restrict__ in, int* __restrict__ out, int n) {
-  __shared__ int wsum[16];
-  __shared__ int s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  const int lane = __lane_id(), wid = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < n; c0 += 4096) {
-    const int base = c0 + threadIdx.x * 4;
-    int v[4], sum = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      v[k] = (base + k < n) ? in[base + k] : 0;
-      sum += v[k];
-    }
-    const int inc = wave_incl_scan(sum);
-    if (lane == 63) wsum[wid] = inc;
-    __syncthreads();
-    int pre = s_carry + inc - sum, tot = 0;
-    for (int w = 0; w < 16; ++w) {
-      if (w < wid) pre += wsum[w];
-      tot += wsum[w];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (base + k < n) out[base + k] = pre;
-      pre += v[k];
-    }
-    // every thread has read wsum and s_carry before they change
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += tot;
-  }
+  return a256(h * sizeof(int)) + 256;
 }
 
 static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
                             int* vals_tmp, int n, int n_passes, void* ws) {
   if (n <= 1) return;
   int nblk = (n + kRadixTile - 1) / kRadixTile;
-  size_t h = (size_t)16 * nblk;
-  int* hist = (int*)ws;
-  int* hscan = (int*)((char*)ws + a256(h * sizeof(int)));
+  int* hist = (int*)ws;  // 16 x nblk, digit-major
   unsigned* ka = keys;
   int* va = vals;
   unsigned* kb = keys_tmp;
@@ -299,9 +300,8 @@ static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned*
   for (int pass = 0; pass < n_passes; ++pass) {
     int shift = 4 * pass;
     hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, nblk, hist);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, hist, hscan, (int)h);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, st, ka, va, n, shift, nblk,
-                       hscan, kb, vb);
+                       hist, kb, vb);
     unsigned* tk = ka; ka = kb; kb = tk;
     int* tv = va; va = vb; vb = tv;
   }
