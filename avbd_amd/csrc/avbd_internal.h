@@ -71,7 +71,7 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
                          int* done, int stamp, int* err, int bpw, int fuse_dual, float beta,
-                         GroupSt* gst_w);
+                         GroupSt* gst_w, int tail_groups);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta, float* resid);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,

@@ -181,7 +181,9 @@ struct avbd_solver {
 
   // options
   int primal_variant = 6;  // one dataflow launch per sweep
-  bool fuse_dual_opt = false;  // measured slower: the sweep is VALU-bound
+  // dual placement with the dataflow sweep: 0 separate launch, 1 inside the
+  // tiles (measured slower: the sweep is VALU-bound), 2 trailing workgroups
+  int fuse_dual_mode = 0;
 
   // profiling
   bool prof = false;
@@ -622,7 +624,9 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4
                       : s->primal_variant == 9 ? 8 : s->primal_variant == 10 ? 3 : 2;
   // the dual rides along in the sweep when no two neighbours share a color
-  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_opt && !to_resid;
+  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_mode == 1 && !to_resid;
+  // or in trailing workgroups of the sweep's launch (any coloring)
+  const bool tail = flow && s->fuse_dual_mode == 2 && !to_resid;
   for (int it = 0; it < I + post; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
     // One timing interval per sweep (all colors back to back): per-launch
@@ -633,9 +637,10 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
                           s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
                           s->gst.p, dt, post_pass, s->flow_done.p, ++s->flow_stamp,
-                          s->dcnt.p + 8, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p);
+                          s->dcnt.p + 8, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p,
+                          tail && it < I ? n_groups : 0);
       launched = nonempty;  // bodies of every color, in one launch
-      CK(prof_end(s, pbytes * launched, 1));
+      CK(prof_end(s, pbytes * launched + (tail && it < I ? dual_sweep_bytes(s) : 0.0), 1));
     } else {
       for (int c = 0; c < s->n_active_colors; ++c) {
         int cnt = s->cstart[c + 1] - s->cstart[c];
@@ -647,7 +652,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
       }
       CK(prof_end(s, pbytes * launched, launched));
     }
-    if (it < I && !fuse) {
+    if (it < I && !fuse && !tail) {
       float* rd = to_resid ? reinterpret_cast<float*>(s->dcnt.p + 10) : nullptr;
       if (rd) launch_fill_int(st, s->dcnt.p + 10, 1, 0);
       CK(prof_begin(s, kDual));
@@ -1316,7 +1321,8 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
       s->primal_variant = value;
       return AVBD_OK;
     case AVBD_OPT_FUSE_DUAL:
-      s->fuse_dual_opt = value != 0;
+      if (value < 0 || value > 2) return set_err(AVBD_E_ARG, "dual placement must be 0 to 2");
+      s->fuse_dual_mode = value;
       return AVBD_OK;
     case AVBD_OPT_LEARNED_SCOPE:
       if (value < 0 || value > 1) return set_err(AVBD_E_ARG, "learned scope must be 0 or 1");

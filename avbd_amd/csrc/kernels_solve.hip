@@ -642,6 +642,12 @@ struct SweepFlow {
   int fuse_dual;
   float beta;
   GroupSt* gst_w;
+  // Dual update in trailing workgroups of the same launch: workgroups past
+  // the last tile take 64 groups each, wait until both bodies of a group
+  // have published this sweep's pose, and update it. Only the tiles of a
+  // group's two bodies read its state, and both are done by then, so the
+  // result is the separate dual launch's for any coloring.
+  int tail_groups;
 };
 
 constexpr int kFlowSpinCap = 1 << 20;
@@ -661,6 +667,31 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
   __shared__ int b_i[BPW], b_id[BPW], b_off[BPW + 1];
   __shared__ float b_pose[BPW][7];
   const int lane = threadIdx.x;
+  if (kFlow && fl.tail_groups > 0 && (int)blockIdx.x >= fl.n_tiles) {
+    const int g = ((int)blockIdx.x - fl.n_tiles) * 64 + lane;
+    if (g < fl.tail_groups) {
+      const GroupGeo G = geo[g];
+      const bool box = g_kind(G.hdr.x) == kGBox;
+      const int a = G.hdr.y, b = box ? G.hdr.z : G.hdr.y;
+      int spins = 0;
+      while (__hip_atomic_load(fl.done + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fl.stamp ||
+             __hip_atomic_load(fl.done + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fl.stamp) {
+        if (++spins > kFlowSpinCap) {
+          atomicExch(fl.err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      const float4 xa = ld_agent(pose + 2 * a), qa = ld_agent(pose + 2 * a + 1);
+      float4 xb = xa, qb = qa;
+      if (box) {
+        xb = ld_agent(pose + 2 * b);
+        qb = ld_agent(pose + 2 * b + 1);
+      }
+      dual_group(G, gst[g], fl.gst_w + g, xa, qa, xb, qb, fl.beta);
+    }
+    return;
+  }
   int count = count_arg, base = base_arg, first = blockIdx.x * BPW, my_color = 0;
   if (kFlow) {
     const int t = blockIdx.x;
@@ -942,9 +973,10 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
                          int* done, int stamp, int* err, int bpw, int fuse_dual,
-                         float beta, GroupSt* gst_w) {
+                         float beta, GroupSt* gst_w, int tail_groups) {
   SweepFlow fl{};
   fl.fuse_dual = fuse_dual;
+  fl.tail_groups = tail_groups;
   fl.beta = beta;
   fl.gst_w = gst_w;
   fl.n_colors = n_colors;
@@ -962,20 +994,21 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   fl.stamp = stamp;
   fl.err = err;
   if (t <= 0) return;
+  const int grid = t + (tail_groups + 63) / 64;
   if (bpw == 1)
-    hipLaunchKernelGGL((k_primal_tiled<1, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+    hipLaunchKernelGGL((k_primal_tiled<1, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else if (bpw == 4)
-    hipLaunchKernelGGL((k_primal_tiled<4, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+    hipLaunchKernelGGL((k_primal_tiled<4, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else if (bpw == 8)
-    hipLaunchKernelGGL((k_primal_tiled<8, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+    hipLaunchKernelGGL((k_primal_tiled<8, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else if (bpw == 3)
-    hipLaunchKernelGGL((k_primal_tiled<3, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+    hipLaunchKernelGGL((k_primal_tiled<3, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else
-    hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(t), dim3(64), 0, st, 0, color_bodies, 0,
+    hipLaunchKernelGGL((k_primal_tiled<2, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
 }
 

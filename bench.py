@@ -236,7 +236,8 @@ def main() -> int:
                     help="only run 5 device-to-device copies of MB MiB (the reference point "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--fuse-dual", type=int, default=0,
-                    help="1: dual update inside the dataflow sweep; 0: separate launch (default)")
+                    help="dual update with the dataflow sweep: 0 separate launch (default), "
+                         "1 inside the sweep's tiles, 2 in trailing workgroups of its launch")
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
                          "5 one or two bodies per wavefront by color size; "

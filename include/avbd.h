@@ -203,8 +203,11 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
 #define AVBD_OPT_PRIMAL_VARIANT 0
 /* AVBD_OPT_FUSE_DUAL: 1 = with the dataflow sweep (variants 6-10) and a
  * coloring without same-color neighbours, each contact group's dual update
- * runs inside the sweep, in the tile of its higher-color body; 0 (default) =
- * a separate dual launch after every sweep. Both give identical results. */
+ * runs inside the sweep, in the tile of its higher-color body; 2 = with the
+ * dataflow sweep, the dual runs in trailing workgroups of the sweep's launch,
+ * each group once both of its bodies have moved (any coloring); 0 (default)
+ * = a separate dual launch after every sweep. All give identical results;
+ * the residual mode always uses the separate launch. */
 #define AVBD_OPT_FUSE_DUAL 1
 /* AVBD_OPT_LEARNED_SCOPE (learned warm start, mode 2): 0 (default) = every
  * box-box and floor group, 1 = only groups whose plain start is empty

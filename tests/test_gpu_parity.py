@@ -68,7 +68,10 @@ def _tiled_scene(name):
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, (6, "fused dual"),
-                                     (9, "fused dual"), (10, "fused dual")])
+                                     (9, "fused dual"), (10, "fused dual"), (6, "tail dual"),
+                                     (7, "tail dual"), (9, "tail dual")],
+                         ids=lambda v: (f"v{v[0]}-{v[1].split()[0]}" if isinstance(v, tuple)
+                                        else f"v{v}"))
 @pytest.mark.parametrize("scene", ["pyramid5", "pile", "pins"])
 def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     """The tiled primal kernels sum every system entry in the same order as
@@ -77,9 +80,9 @@ def test_tiled_primal_bit_identical_to_lane_per_body(variant, scene):
     a._flush()
     b._flush()
     a._h.set_option(0, 0)          # lane per body
-    if isinstance(variant, tuple):  # dataflow sweep with the dual inside it
+    if isinstance(variant, tuple):  # dataflow sweep with the dual inside its launch
         b._h.set_option(0, variant[0])
-        b._h.set_option(1, 1)
+        b._h.set_option(1, 1 if variant[1] == "fused dual" else 2)
     else:
         b._h.set_option(0, variant)
     n_sub = {"pyramid5": 24, "pile": 200, "pins": 60}[scene]
