@@ -26,7 +26,13 @@ size_t radix_workspace_bytes(int n);
 __device__ __forceinline__ v3 f3(float4 a) { return mk3(a.x, a.y, a.z); }
 __device__ __forceinline__ q4 f4q(float4 a) { return mk4(a.x, a.y, a.z, a.w); }
 __device__ __forceinline__ float4 p4(v3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
-__device__ __forceinline__ v3 colm(const m33& R, int k) { return mk3(R.m[0][k], R.m[1][k], R.m[2][k]); }
+// Column k of R, picked with selects (a runtime index into the matrix would
+// put it in scratch memory).
+__device__ __forceinline__ v3 colm(const m33& R, int k) {
+  return k == 0   ? mk3(R.m[0][0], R.m[1][0], R.m[2][0])
+         : k == 1 ? mk3(R.m[0][1], R.m[1][1], R.m[2][1])
+                  : mk3(R.m[0][2], R.m[1][2], R.m[2][2]);
+}
 
 // ---------------------------------------------------------------------------
 // AABB
@@ -569,14 +575,14 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
   mf_count[p] = 0;
   bool sep = false;
   for (int k = 0; k < 3 && !sep; ++k) {
-    v3 L = A[k];
+    v3 L = colm(RA, k);
     float ov = (comp(eA, k) + proj_radius(L, Bc[0], Bc[1], Bc[2], eB)) - fabs_(dot(t, L));
     float tdl = dot(t, L);
     if (ov < -margin) { sep = true; break; }
     if (ov < best) { best = ov; best_idx = k; best_axis = tdl > 0.0f ? neg(L) : L; }
   }
   for (int k = 0; k < 3 && !sep; ++k) {
-    v3 L = Bc[k];
+    v3 L = colm(RB, k);
     float ov = (proj_radius(L, A[0], A[1], A[2], eA) + comp(eB, k)) - fabs_(dot(t, L));
     float tdl = dot(t, L);
     if (ov < -margin) { sep = true; break; }
@@ -584,7 +590,7 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
   }
   for (int ie = 0; ie < 3 && !sep; ++ie) {
     for (int je = 0; je < 3; ++je) {
-      v3 L = cross(A[ie], Bc[je]);
+      v3 L = cross(colm(RA, ie), colm(RB, je));
       float nl = length(L);
       if (nl < 1.0e-6f) continue;
       L = divs(L, nl);
@@ -695,11 +701,22 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
   int iax1 = (inc_axis + 1) % 3, iax2 = (inc_axis + 2) % 3;
   v3 iu = colm(R_inc, iax1), iv = colm(R_inc, iax2);
   float ieu = comp(e_inc, iax1), iev = comp(e_inc, iax2);
-  v3 buf[2][8];
-  buf[0][0] = add(add(ifc, mul(iu, ieu)), mul(iv, iev));
-  buf[0][1] = add(sub(ifc, mul(iu, ieu)), mul(iv, iev));
-  buf[0][2] = sub(sub(ifc, mul(iu, ieu)), mul(iv, iev));
-  buf[0][3] = sub(add(ifc, mul(iu, ieu)), mul(iv, iev));
+  // the two clip polygons (up to 8 vertices each) live in LDS, one column
+  // per lane: indexed by a running count, they would go to scratch memory
+  __shared__ float clip[2][8][3][128];
+  const int tl = threadIdx.x;
+  auto bget = [&](int b, int k) {
+    return mk3(clip[b][k][0][tl], clip[b][k][1][tl], clip[b][k][2][tl]);
+  };
+  auto bset = [&](int b, int k, v3 v) {
+    clip[b][k][0][tl] = v.x;
+    clip[b][k][1][tl] = v.y;
+    clip[b][k][2][tl] = v.z;
+  };
+  bset(0, 0, add(add(ifc, mul(iu, ieu)), mul(iv, iev)));
+  bset(0, 1, add(sub(ifc, mul(iu, ieu)), mul(iv, iev)));
+  bset(0, 2, sub(sub(ifc, mul(iu, ieu)), mul(iv, iev)));
+  bset(0, 3, sub(add(ifc, mul(iu, ieu)), mul(iv, iev)));
   int cur = 0, len = 4;
   for (int pl = 0; pl < 4; ++pl) {
     v3 ppt, pn;
@@ -710,54 +727,59 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
     if (len == 0) return;
     int out = 0;
     for (int iv2 = 0; iv2 < len; ++iv2) {
-      v3 ap = buf[cur][iv2];
-      v3 bp = buf[cur][(iv2 + 1) % len];
+      v3 ap = bget(cur, iv2);
+      v3 bp = bget(cur, (iv2 + 1) % len);
       float da = dot(sub(ap, ppt), pn);
       float db = dot(sub(bp, ppt), pn);
       if (da <= 0.0f) {
-        if (out < 8) buf[1 - cur][out++] = ap;
+        if (out < 8) bset(1 - cur, out++, ap);
         if (db > 0.0f) {
           float tt = da / (da - db);
-          if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+          if (out < 8) bset(1 - cur, out++, add(ap, mul(sub(bp, ap), tt)));
         }
       } else if (db <= 0.0f) {
         float tt = da / (da - db);
-        if (out < 8) buf[1 - cur][out++] = add(ap, mul(sub(bp, ap), tt));
+        if (out < 8) bset(1 - cur, out++, add(ap, mul(sub(bp, ap), tt)));
       }
     }
     cur = 1 - cur;
     len = out;
   }
   if (len == 0) return;
-  float bd[4] = {-1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
-  int bi[4] = {-1, -1, -1, -1};
+  // the four deepest clip points, in LDS for the same reason as the polygons
+  __shared__ float bd_s[4][128];
+  __shared__ int bi_s[4][128];
+  for (int k = 0; k < 4; ++k) {
+    bd_s[k][tl] = -1.0e30f;
+    bi_s[k][tl] = -1;
+  }
   int nbest = 0;
   for (int iv2 = 0; iv2 < len; ++iv2) {
-    float ds = dot(sub(buf[cur][iv2], rfc), rfn);
+    float ds = dot(sub(bget(cur, iv2), rfc), rfn);
     if (ds < margin) {
       float depth = -ds;
       if (nbest < 4) {
-        bd[nbest] = depth; bi[nbest] = iv2; ++nbest;
+        bd_s[nbest][tl] = depth; bi_s[nbest][tl] = iv2; ++nbest;
       } else {
         int mp = 0;
-        float mv = bd[0];
-        for (int k = 1; k < 4; ++k) if (bd[k] < mv) { mv = bd[k]; mp = k; }
-        if (depth > mv) { bd[mp] = depth; bi[mp] = iv2; }
+        float mv = bd_s[0][tl];
+        for (int k = 1; k < 4; ++k) if (bd_s[k][tl] < mv) { mv = bd_s[k][tl]; mp = k; }
+        if (depth > mv) { bd_s[mp][tl] = depth; bi_s[mp][tl] = iv2; }
       }
     }
   }
   if (nbest == 0) return;
   for (int is = 0; is < nbest - 1; ++is) {
     int mp = is;
-    for (int k = is + 1; k < nbest; ++k) if (bd[k] > bd[mp]) mp = k;
+    for (int k = is + 1; k < nbest; ++k) if (bd_s[k][tl] > bd_s[mp][tl]) mp = k;
     if (mp != is) {
-      float td = bd[is]; bd[is] = bd[mp]; bd[mp] = td;
-      int ti = bi[is]; bi[is] = bi[mp]; bi[mp] = ti;
+      float td = bd_s[is][tl]; bd_s[is][tl] = bd_s[mp][tl]; bd_s[mp][tl] = td;
+      int ti = bi_s[is][tl]; bi_s[is][tl] = bi_s[mp][tl]; bi_s[mp][tl] = ti;
     }
   }
   m33 Rtr = mT(R_ref), Rti = mT(R_inc);
   for (int c = 0; c < nbest; ++c) {
-    v3 pinc = buf[cur][bi[c]];
+    v3 pinc = bget(cur, bi_s[c][tl]);
     float ds = dot(sub(pinc, rfc), rfn);
     v3 pref = sub(pinc, mul(rfn, ds));
     off[c] = p4(mvmul(Rtr, sub(pref, ref_c)));
