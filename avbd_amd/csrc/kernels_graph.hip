@@ -383,6 +383,8 @@ k_tcsr_count(int nb, const int* __restrict__ cvals, const int* __restrict__ st_c
   padc[k] = (cnt + 3) & ~3;
 }
 
+constexpr int kAsbLds = 16;
+
 __global__ void __launch_bounds__(256)
 k_tcsr_fill(int nb, const int* __restrict__ cvals, const int* __restrict__ st_start,
             const int* __restrict__ st_idx, const int* __restrict__ asb_start,
@@ -396,16 +398,30 @@ k_tcsr_fill(int nb, const int* __restrict__ cvals, const int* __restrict__ st_st
   int o = cpos[k];
   const int end = cpos[k + 1];
   for (int q = st_start[i]; q < st_start[i + 1]; ++q) csr[o++] = make_int2(st_idx[q], i);
-  // as-b pairs: a handful per body, sorted in place by pair index
-  const int l0 = asb_start[i], l1 = asb_start[i + 1];
-  for (int u = l0 + 1; u < l1; ++u) {
-    const int v = asb_list[u];
-    int w = u - 1;
-    while (w >= l0 && asb_list[w] > v) { asb_list[w + 1] = asb_list[w]; --w; }
-    asb_list[w + 1] = v;
+  // as-b pairs: a handful per body, taken in ascending pair index. Up to
+  // kAsbLds of them are sorted in an LDS column of this lane (a sort in
+  // global memory is a chain of dependent loads); longer lists in place.
+  __shared__ int srt[kAsbLds][256];
+  const int l0 = asb_start[i], l1 = asb_start[i + 1], nl = l1 - l0;
+  const int t = threadIdx.x;
+  if (nl <= kAsbLds) {
+    for (int u = 0; u < nl; ++u) srt[u][t] = asb_list[l0 + u];
+    for (int u = 1; u < nl; ++u) {
+      const int v = srt[u][t];
+      int w = u - 1;
+      while (w >= 0 && srt[w][t] > v) { srt[w + 1][t] = srt[w][t]; --w; }
+      srt[w + 1][t] = v;
+    }
+  } else {
+    for (int u = l0 + 1; u < l1; ++u) {
+      const int v = asb_list[u];
+      int w = u - 1;
+      while (w >= l0 && asb_list[w] > v) { asb_list[w + 1] = asb_list[w]; --w; }
+      asb_list[w + 1] = v;
+    }
   }
-  for (int u = l0; u < l1; ++u) {
-    const int p = asb_list[u];
+  for (int u = 0; u < nl; ++u) {
+    const int p = nl <= kAsbLds ? srt[u][t] : asb_list[l0 + u];
     const int g0 = g_static + sscan[p], n = succ[p], other = pa[p];
     for (int c = 0; c < n; ++c) csr[o++] = make_int2(g0 + c, other);
   }
