@@ -27,6 +27,7 @@ COLOR_JONES_PLASSMANN = 1
 OPT_PRIMAL_VARIANT = 0
 OPT_FUSE_DUAL = 1
 OPT_LEARNED_SCOPE = 2
+OPT_FLOW_GIVE_UP_TEST = 3
 
 
 class AvbdError(RuntimeError):

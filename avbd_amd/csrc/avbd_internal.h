@@ -60,7 +60,7 @@ void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose, fl
                           float4* poseI, float4* inert, const float4* vel, const float4* pvel,
                           const float4* bstat, const float* Ilocal, GroupGeo* geo, GroupSt* gst,
                           float dt, float gx, float gy, float gz, float alpha, float gamma,
-                          int post, int do_prelude);
+                          int post, int do_prelude, unsigned* epoch);
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
@@ -70,8 +70,9 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const int* color, float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int* done, int stamp, int* err, int bpw, int fuse_dual, float beta,
-                         GroupSt* gst_w, int tail_groups);
+                         int* done, const unsigned* epoch, int sweep, int* err, int bpw,
+                         int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
+                         const int* done_rd = nullptr, int spin_cap = 0);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta, float* resid);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,

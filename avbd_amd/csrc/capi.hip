@@ -89,6 +89,13 @@ struct DBuf {
 
 enum ProfClass { kPrimal = 0, kDual, kPrelude, kContacts, kColoring, kCsr, kOther, kNumProf };
 
+// Device counters (dcnt) by slot: 0 pair count, 1-3 emission totals, 4
+// conflict rows, 5 uncolored bodies, 6 coloring error, 8 dataflow error
+// (sticky until the next flush), 9 color clamp, 10 residual, 12 substep
+// epoch (stamps of the dataflow sweeps).
+constexpr int kErrSlot = 8;
+constexpr int kEpochSlot = 12;
+
 }  // namespace
 
 struct avbd_solver {
@@ -136,9 +143,9 @@ struct avbd_solver {
   DBuf<int> tc_work;        // 5 * nb: as-b pair count, as-b group count, as-a start, as-a end, cursor
   DBuf<int> asb_start, asb_list, padc, cpos, pos_of, pos_map;
   DBuf<int2> tcsr;
-  // dataflow primal sweep: per-body stamp of the last sweep that moved it
+  // dataflow primal sweep: per-body stamp of the last sweep that moved it;
+  // the stamps derive from the substep epoch in dcnt[kEpochSlot]
   DBuf<int> flow_done;
-  int flow_stamp = 0;
   bool colors_conflict_free = false;  // no two neighbours share a color
   bool conflicts_ready = false;  // this substep's conflict count is in hcnt[4]
   bool csr_built = false;       // a CSR has been built since the last flush
@@ -184,6 +191,10 @@ struct avbd_solver {
   // dual placement with the dataflow sweep: 0 separate launch, 1 inside the
   // tiles (measured slower: the sweep is VALU-bound), 2 trailing workgroups
   int fuse_dual_mode = 0;
+  // test of the dataflow give-up path (AVBD_OPT_FLOW_GIVE_UP_TEST): waits
+  // poll flow_never (never written) and give up after this many polls
+  int flow_test_cap = 0;
+  DBuf<int> flow_never;
 
   // profiling
   bool prof = false;
@@ -210,7 +221,18 @@ int ensure_scan_ws(avbd_solver* s, size_t n) {
 
 int scan(avbd_solver* s, const int* in, int* out, int n, int* total_dev) {
   CK(ensure_scan_ws(s, (size_t)n));
-  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p, s->dcnt.p + 8);
+  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p, s->dcnt.p + kErrSlot);
+  return AVBD_OK;
+}
+
+// dcnt[kErrSlot]: set by a dataflow primal sweep or a scan whose wait hit
+// its spin cap. It is copied to the host at the end of every substep and
+// checked after every later sync (readbacks and reads), and it stays set
+// until the next flush: the state it left behind is not the reference's.
+int check_flow_error(avbd_solver* s) {
+  if (s->hcnt[kErrSlot] != 0)
+    return set_err(AVBD_E_STATE,
+                   "a dataflow wait (primal sweep or scan) did not complete; flush to reset");
   return AVBD_OK;
 }
 
@@ -218,12 +240,7 @@ int readback(avbd_solver* s, int first, int count) {
   HIPCK(hipMemcpyAsync(s->hcnt + first, s->dcnt.p + first, count * sizeof(int),
                        hipMemcpyDeviceToHost, s->st));
   HIPCK(hipStreamSynchronize(s->st));
-  // dcnt[8]: set by a dataflow primal sweep or a scan whose wait hit its
-  // spin cap; it is copied at the end of every substep and checked at the
-  // next sync.
-  if (s->hcnt[8] != 0)
-    return set_err(AVBD_E_STATE, "a dataflow wait (primal sweep or scan) did not complete");
-  return AVBD_OK;
+  return check_flow_error(s);
 }
 
 // ---- profiling ----
@@ -604,7 +621,8 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   CK(prof_begin(s, kPrelude));
   launch_substep_begin(st, nb, n_groups, p0, pw, s->poseI.p, s->inert.p, s->vel.p, s->pvel.p,
                        s->bstat.p, s->Ilocal.p, s->geo.p, s->gst.p, dt, p->gravity[0],
-                       p->gravity[1], p->gravity[2], p->alpha, p->gamma, post, n_active > 0 ? 1 : 0);
+                       p->gravity[1], p->gravity[2], p->alpha, p->gamma, post, n_active > 0 ? 1 : 0,
+                       reinterpret_cast<unsigned*>(s->dcnt.p + kEpochSlot));
   CK(prof_end(s, (double)nb * 188.0 + (double)s->n_contacts * 124.0 + (double)s->n_floor_groups * 88.0));
 
   float max_lin = p->max_linear_speed, max_ang = p->max_angular_speed;
@@ -620,7 +638,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
   // variants 6-8: every color of a sweep in one dataflow launch with 2, 1
   // or 4 bodies per wavefront
-  const bool flow = s->primal_variant >= 6 && pos_map == nullptr;
+  const bool flow = s->primal_variant >= 6 && pos_map == nullptr && I + post < (int)4095;
   const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4
                       : s->primal_variant == 9 ? 8 : s->primal_variant == 10 ? 3 : 2;
   // the dual rides along in the sweep when no two neighbours share a color
@@ -636,9 +654,11 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
     if (flow) {
       launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
                           s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
-                          s->gst.p, dt, post_pass, s->flow_done.p, ++s->flow_stamp,
-                          s->dcnt.p + 8, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p,
-                          tail && it < I ? n_groups : 0);
+                          s->gst.p, dt, post_pass, s->flow_done.p,
+                          reinterpret_cast<const unsigned*>(s->dcnt.p + kEpochSlot), it,
+                          s->dcnt.p + kErrSlot, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p,
+                          tail && it < I ? n_groups : 0,
+                          s->flow_test_cap > 0 ? s->flow_never.p : nullptr, s->flow_test_cap);
       launched = nonempty;  // bodies of every color, in one launch
       CK(prof_end(s, pbytes * launched + (tail && it < I ? dual_sweep_bytes(s) : 0.0), 1));
     } else {
@@ -696,7 +716,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   }
   if (iters_used) *iters_used = I;
   if (resid_out) *resid_out = resid;
-  HIPCK(hipMemcpyAsync(s->hcnt + 8, s->dcnt.p + 8, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCK(hipMemcpyAsync(s->hcnt + kErrSlot, s->dcnt.p + kErrSlot, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipGetLastError());
   s->cur = 1 - s->cur;
   s->last_post = post;
@@ -935,7 +955,11 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   CK(s->pos_map.alloc(nbb));
   CK(s->flow_done.alloc(nbb));
   launch_fill_int(s->st, s->flow_done.p, nbb, 0);
-  s->flow_stamp = 0;
+  // a flush starts the epoch over and clears a dataflow error of the
+  // previous scene
+  launch_fill_int(s->st, s->dcnt.p + kErrSlot, 1, 0);
+  launch_fill_int(s->st, s->dcnt.p + kEpochSlot, 1, 0);
+  s->hcnt[kErrSlot] = 0;
   s->csr_built = false;
   s->csr_order_stale = false;
   CK(s->tcsr.alloc(std::max(s->st_total, 1) + 3 * (size_t)nbb));
@@ -956,6 +980,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   s->pair_count = s->pool_count = s->n_contacts = s->adj_pairs = 0;
   s->last_post = 0;
   s->n_scenes = 0;  // a flush replaces the scene; avbd_set_scenes splits it again
+  s->flow_test_cap = 0;
   s->flushed = true;
   CK(ensure_hage(s));
   HIPCK(hipStreamSynchronize(s->st));
@@ -990,6 +1015,7 @@ int avbd_read_bodies(avbd_solver* s, float* x, float* q, float* v, float* w, flo
   HIPCK(hipMemcpyAsync(b.data(), s->vel.p, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, s->st));
   HIPCK(hipMemcpyAsync(c.data(), s->pvel.p, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, s->st));
   HIPCK(hipStreamSynchronize(s->st));
+  CK(check_flow_error(s));
   for (int i = 0; i < nb; ++i) {
     if (x) { x[3 * i] = a[2 * i].x; x[3 * i + 1] = a[2 * i].y; x[3 * i + 2] = a[2 * i].z; }
     if (q) { q[4 * i] = a[2 * i + 1].x; q[4 * i + 1] = a[2 * i + 1].y; q[4 * i + 2] = a[2 * i + 1].z; q[4 * i + 3] = a[2 * i + 1].w; }
@@ -1060,6 +1086,7 @@ int avbd_read_rows(avbd_solver* s, int n, float* lam, float* pen, int* act, int*
   std::vector<GroupGeo> G;
   std::vector<GroupSt> S;
   CK(download_groups(s, ng, G, S));
+  CK(check_flow_error(s));
   for (int g = 0; g < ng; ++g) {
     int h = G[g].hdr.x, kind = g_kind(h), nr = g_nrows(h);
     int bits = f2i(S[g].s2.y);
@@ -1327,6 +1354,17 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
     case AVBD_OPT_LEARNED_SCOPE:
       if (value < 0 || value > 1) return set_err(AVBD_E_ARG, "learned scope must be 0 or 1");
       s->ml_scope = value;
+      return AVBD_OK;
+    case AVBD_OPT_FLOW_GIVE_UP_TEST:
+      if (value < 0 || value > 4096) return set_err(AVBD_E_ARG, "give-up test cap must be 0 to 4096");
+      if (value > 0) {
+        if (!s->flushed) return set_err(AVBD_E_STATE, "give-up test before flush");
+        HIPCK(hipSetDevice(s->device));
+        CK(s->flow_never.ensure(std::max(s->nb, 1)));
+        launch_fill_int(s->st, s->flow_never.p, std::max(s->nb, 1), 0);  // 0 is never a stamp
+        HIPCK(hipStreamSynchronize(s->st));
+      }
+      s->flow_test_cap = value;
       return AVBD_OK;
     default:
       return set_err(AVBD_E_ARG, "unknown option");

@@ -67,13 +67,15 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total) {
 // at once, waits until all of them carry this call's stamp, adds the totals
 // down to the nearest inclusive prefix, and steps down another 64 tiles
 // when there is none. Tile 0 publishes its prefix at once, so every look-back
-// ends. A tile waits only on tiles with smaller indices, which the
-// dispatcher starts first; a spin cap turns any violation into a flagged
-// error instead of a hang.
+// ends. A workgroup takes its tile index from a ticket counter, so a tile
+// waits only on tiles whose workgroups have already started, whatever order
+// the dispatcher starts workgroups in; a spin cap still turns a wait that
+// does not end into a flagged error instead of a hang.
 //
 // Workspace: hdr[0] the stamp of the last call, hdr[1] the count of tiles
-// done in this call (the last tile resets it and advances hdr[0]), then the
-// status words. A fresh workspace must be zero. One launch replaces the
+// done in this call, hdr[2] the tile ticket (the last tile to finish resets
+// both and advances hdr[0]), then the status words. A fresh workspace must
+// be zero. One launch replaces the
 // three of a reduce-then-scan, so two kernel boundaries (~10 us each on this
 // GPU) are saved per scan.
 constexpr int kScanSpinCap = 1 << 20;
@@ -97,15 +99,19 @@ __global__ void __launch_bounds__(256)
 k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* status, int* total,
           int* err) {
   __shared__ unsigned s_stamp;
-  __shared__ int s_carry;
-  const int b = blockIdx.x, nblk = gridDim.x;
+  __shared__ int s_carry, s_tile;
+  const int nblk = gridDim.x;
   // hdr[0] changes only after every tile of this call is done, so all tiles
   // read the same value; stamps run 1 .. 2^31 - 1 (a zero workspace holds 0)
   if (threadIdx.x == 0) {
     unsigned st = (__hip_atomic_load(&hdr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u) &
                   0x7fffffffu;
     s_stamp = st == 0u ? 1u : st;
+    // tiles are numbered in the order their workgroups start
+    s_tile = (int)atomicAdd(&hdr[2], 1u);
   }
+  __syncthreads();
+  const int b = s_tile;
   const int base = b * kScanTile + threadIdx.x * 4;
   int v[4], sum = 0;
 #pragma unroll
@@ -162,6 +168,8 @@ k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* sta
   if (threadIdx.x == 0) {
     const unsigned done = atomicAdd(&hdr[1], 1u);
     if (done == (unsigned)(nblk - 1)) {
+      // every workgroup of this call has taken its ticket before finishing
+      __hip_atomic_store(&hdr[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&hdr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&hdr[0], stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -184,8 +184,9 @@ k_substep_begin(int nb, int n_groups, const float4* __restrict__ pose_cur,
                 const float4* __restrict__ pvel, const float4* __restrict__ bstat,
                 const float* __restrict__ Ilocal, GroupGeo* __restrict__ geo,
                 GroupSt* __restrict__ gst, float dt, float gx, float gy, float gz, float alpha,
-                float gamma, int post, int do_prelude) {
+                float gamma, int post, int do_prelude, unsigned* __restrict__ epoch) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0 && epoch) *epoch += 1u;  // stamps of this substep's sweeps
   if (tid < nb) {
     predict_body(tid, pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, dt,
                  mk3(gx, gy, gz));
@@ -200,12 +201,12 @@ void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose_cur
                           float4* pose_work, float4* poseI, float4* inert, const float4* vel,
                           const float4* pvel, const float4* bstat, const float* Ilocal,
                           GroupGeo* geo, GroupSt* gst, float dt, float gx, float gy, float gz,
-                          float alpha, float gamma, int post, int do_prelude) {
+                          float alpha, float gamma, int post, int do_prelude, unsigned* epoch) {
   int total = nb + (do_prelude ? n_groups : 0);
   if (total <= 0) return;
   hipLaunchKernelGGL(k_substep_begin, dim3((total + 255) / 256), dim3(256), 0, st, nb, n_groups,
                      pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, geo, gst, dt,
-                     gx, gy, gz, alpha, gamma, post, do_prelude);
+                     gx, gy, gz, alpha, gamma, post, do_prelude, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -632,8 +633,19 @@ struct SweepFlow {
   int color_start[kMaxColors + 1];  // first position of each color
   const int* color;                 // body -> color
   int* done;                        // body -> stamp of the last sweep that moved it
-  int stamp;
+  // The sweep's stamp is epoch * kSweepStride + sweep + 1 (mod 2^32): the
+  // epoch is a device counter that k_substep_begin advances once per
+  // substep, so replays of a captured substep get fresh stamps, and two
+  // consecutive sweeps never share one. Every body publishes in every sweep,
+  // so a stamp only has to differ from the previous sweep's.
+  const unsigned* epoch;
+  int sweep;
   int* err;
+  // Waits poll done_rd (normally done) and give up after spin_cap polls,
+  // setting *err. A host test points done_rd at an array nobody writes, with
+  // a small cap, to run the give-up path without a hang.
+  const int* done_rd;
+  int spin_cap;
   // Dual update fused into the sweep: each group is updated by the tile of
   // its higher-color body (the body itself for floor and pin groups), after
   // that tile's solve, when both bodies' poses of this sweep are final and
@@ -651,6 +663,11 @@ struct SweepFlow {
 };
 
 constexpr int kFlowSpinCap = 1 << 20;
+constexpr unsigned kSweepStride = 4096;  // sweeps per substep are fewer
+
+__device__ __forceinline__ int sweep_stamp(const SweepFlow& fl) {
+  return (int)(*fl.epoch * kSweepStride + (unsigned)fl.sweep + 1u);
+}
 
 template <int BPW, bool kFlow>
 __global__ void __launch_bounds__(64)
@@ -667,6 +684,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
   __shared__ int b_i[BPW], b_id[BPW], b_off[BPW + 1];
   __shared__ float b_pose[BPW][7];
   const int lane = threadIdx.x;
+  const int stamp = kFlow ? sweep_stamp(fl) : 0;
   if (kFlow && fl.tail_groups > 0 && (int)blockIdx.x >= fl.n_tiles) {
     const int g = ((int)blockIdx.x - fl.n_tiles) * 64 + lane;
     if (g < fl.tail_groups) {
@@ -674,9 +692,9 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       const bool box = g_kind(G.hdr.x) == kGBox;
       const int a = G.hdr.y, b = box ? G.hdr.z : G.hdr.y;
       int spins = 0;
-      while (__hip_atomic_load(fl.done + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fl.stamp ||
-             __hip_atomic_load(fl.done + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fl.stamp) {
-        if (++spins > kFlowSpinCap) {
+      while (__hip_atomic_load(fl.done_rd + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != stamp ||
+             __hip_atomic_load(fl.done_rd + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != stamp) {
+        if (++spins > fl.spin_cap) {
           atomicExch(fl.err, 1);
           break;
         }
@@ -728,7 +746,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       oqq = pose[2 * ent.y + 1];
     } else {
       colq = fl.color[ent.y];
-      flagq = __hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      flagq = __hip_atomic_load(fl.done_rd + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if (lane <= BPW) {
@@ -801,14 +819,14 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       if (ent.x >= 0 && ent.y >= 0) {
         int bb = 0;
         while (bb + 1 < BPW && j >= b_off[bb + 1]) ++bb;
-        need = ent.y != b_id[bb] && colq < my_color && flagq != fl.stamp;
+        need = ent.y != b_id[bb] && colq < my_color && flagq != stamp;
       }
       int spins = 0;
       while (need) {
-        if (__hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-            fl.stamp) {
+        if (__hip_atomic_load(fl.done_rd + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            stamp) {
           need = false;
-        } else if (++spins > kFlowSpinCap) {
+        } else if (++spins > fl.spin_cap) {
           atomicExch(fl.err, 1);
           need = false;
         } else {
@@ -858,7 +876,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
         oqq = pose[2 * ent.y + 1];
       } else {
         colq = fl.color[ent.y];
-        flagq = __hip_atomic_load(fl.done + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flagq = __hip_atomic_load(fl.done_rd + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     __syncthreads();
@@ -934,7 +952,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __hip_atomic_store(fl.done + b_id[lane], fl.stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(fl.done + b_id[lane], stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (kFlow && fl.fuse_dual) {
     __syncthreads();  // final poses of the tile's bodies in b_pose
@@ -967,14 +985,18 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
 }
 
 // One dataflow launch for a whole sweep (all colors); cstart holds the
-// n_colors + 1 color starts; stamp must differ from every earlier sweep's.
+// n_colors + 1 color starts; sweep is the sweep's index in its substep
+// (below kSweepStride - 1).
 void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const int* color_bodies,
                          const int* color, float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
-                         int* done, int stamp, int* err, int bpw, int fuse_dual,
-                         float beta, GroupSt* gst_w, int tail_groups) {
+                         int* done, const unsigned* epoch, int sweep, int* err, int bpw,
+                         int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
+                         const int* done_rd, int spin_cap) {
   SweepFlow fl{};
+  fl.done_rd = done_rd ? done_rd : done;
+  fl.spin_cap = spin_cap > 0 ? spin_cap : kFlowSpinCap;
   fl.fuse_dual = fuse_dual;
   fl.tail_groups = tail_groups;
   fl.beta = beta;
@@ -991,7 +1013,8 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   fl.n_tiles = t;
   fl.color = color;
   fl.done = done;
-  fl.stamp = stamp;
+  fl.epoch = epoch;
+  fl.sweep = sweep;
   fl.err = err;
   if (t <= 0) return;
   const int grid = t + (tail_groups + 63) / 64;

@@ -15,6 +15,7 @@ Rendering, drag gizmos and the viser server are not part of this module.
 from __future__ import annotations
 
 import argparse
+import colorsys
 import time
 from dataclasses import dataclass
 
@@ -56,15 +57,9 @@ def build_scene(args, solver_cls=Solver6DOF) -> tuple[Solver6DOF, list[SceneBox]
     rng = np.random.default_rng(seed=args.seed)
 
     def color(saturation=0.7, value=0.85):
-        # HSV -> RGB from one hue draw (consumes the generator like the viewer)
-        h = float(rng.random())
-        i = int(h * 6)
-        f = h * 6 - i
-        p = value * (1 - saturation)
-        qv = value * (1 - f * saturation)
-        t = value * (1 - (1 - f) * saturation)
-        return [(value, t, p), (qv, value, p), (p, value, t),
-                (p, qv, value), (t, p, value), (value, p, qv)][i % 6]
+        # one hue draw per tower keeps the generator's draw order of the
+        # reference scene (viewer.py:125-126); the shade itself is cosmetic
+        return colorsys.hsv_to_rgb(float(rng.random()), saturation, value)
 
     # grid of cube towers
     stress = bool(getattr(args, "stress", False))

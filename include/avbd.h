@@ -214,6 +214,14 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * (normal lambda 0: new contacts, floor corners not pressed last substep);
  * groups with a cached lambda keep the plain start. */
 #define AVBD_OPT_LEARNED_SCOPE 2
+/* AVBD_OPT_FLOW_GIVE_UP_TEST (tests only; no counterpart in the reference):
+ * value N > 0 makes every wait of the dataflow sweep poll an array that is
+ * never written and give up after N polls, as a wait that does not end
+ * would at its cap; 0 (default) restores normal waits. A flush resets it.
+ * The give-up sets the solver's error flag: the next sync (substep
+ * readback, avbd_read_bodies, avbd_read_rows) returns AVBD_E_STATE, and the
+ * flag stays set until the next avbd_flush. */
+#define AVBD_OPT_FLOW_GIVE_UP_TEST 3
 int avbd_set_option(avbd_solver* s, int option, int value);
 
 /* ---- profiling ------------------------------------------------------- */

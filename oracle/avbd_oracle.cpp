@@ -1501,4 +1501,59 @@ int orc_probe_pair(const float* xa, const float* qa, const float* ea, const floa
   return AVBD_OK;
 }
 
+
+// Test probe: one primal_update_6dof body update (kernels_6dof.py:322-582) on
+// a hand-built state, so that tests can check it against an independent
+// float64 solve of the same linearised system. Body 0 is updated; the other
+// bodies are the far sides of box-box and tangent rows. bodies: n_bodies x
+// 24 floats [x 3, q 4 (xyzw), x_inertial 3, q_inertial 4, mass, I_local 9
+// row-major]; I_world = R I_local R^T as predict_inertial_6dof computes it.
+// Rows, all active and all listed for body 0 in the given order:
+// irow 4 per row [type, body_a, body_b, sibling], frow 19 per row
+// [anchor 3, off_a 3, off_b 3, stiffness, fmin, fmax, mu_d, mu_s, lambda,
+// penalty, alpha_C0, was_static (0 / 1), unused].
+int orc_probe_primal(float dt, int n_bodies, const float* bodies, int n_rows, const int* irow,
+                     const float* frow, float* x_out, float* q_out) {
+  if (n_bodies < 1 || n_rows < 0) return fail(AVBD_E_ARG, "probe needs a body");
+  Orc S;
+  S.nb = n_bodies;
+  S.x.resize(n_bodies); S.q.resize(n_bodies); S.xin.resize(n_bodies); S.qin.resize(n_bodies);
+  S.mass.resize(n_bodies); S.Il.resize(n_bodies); S.Iw.resize(n_bodies);
+  for (int i = 0; i < n_bodies; ++i) {
+    const float* b = bodies + 24 * (size_t)i;
+    S.x[i] = mk3(b[0], b[1], b[2]);
+    S.q[i] = mk4(b[3], b[4], b[5], b[6]);
+    S.xin[i] = mk3(b[7], b[8], b[9]);
+    S.qin[i] = mk4(b[10], b[11], b[12], b[13]);
+    S.mass[i] = b[14];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) S.Il[i].m[r][c] = b[15 + 3 * r + c];
+    m33 R = qtomat(S.q[i]);
+    S.Iw[i] = mmul(mmul(R, S.Il[i]), mT(R));
+  }
+  resize_rows(S, std::max(n_rows, 1));
+  for (int j = 0; j < n_rows; ++j) {
+    const int* ir = irow + 4 * (size_t)j;
+    const float* fr = frow + 19 * (size_t)j;
+    S.ctype[j] = ir[0]; S.ca[j] = ir[1]; S.cb[j] = ir[2]; S.csib[j] = ir[3];
+    if (S.ca[j] < 0 || S.ca[j] >= n_bodies || S.cb[j] >= n_bodies ||
+        S.csib[j] >= n_rows || (S.ctype[j] == kTangent && S.csib[j] < 0))
+      return fail(AVBD_E_ARG, "probe row index out of range");
+    S.canc[j] = mk3(fr[0], fr[1], fr[2]);
+    S.coffa[j] = mk3(fr[3], fr[4], fr[5]);
+    S.coffb[j] = mk3(fr[6], fr[7], fr[8]);
+    S.cstiff[j] = fr[9]; S.cfmin[j] = fr[10]; S.cfmax[j] = fr[11];
+    S.cfric[j] = fr[12]; S.cfrics[j] = fr[13]; S.clam[j] = fr[14]; S.cpen[j] = fr[15];
+    S.caC0[j] = fr[16]; S.cwas[j] = fr[17] != 0.0f ? 1 : 0;
+    S.cact[j] = 1;
+    S.cfrac[j] = kInf;
+  }
+  S.con_starts.assign(n_bodies + 1, n_rows);
+  S.con_starts[0] = 0;
+  for (int j = 0; j < n_rows; ++j) S.con_idx[j] = j;
+  primal_body(S, 0, dt);
+  st3(x_out, 0, S.x[0]);
+  st4(q_out, 0, S.q[0]);
+  return AVBD_OK;
+}
+
 }  // extern "C"
