@@ -266,6 +266,40 @@ def test_set_velocity_writes_buffer_in_place(S):
 
 
 # ---- tests/test_kernel_face_clip.py ----------------------------------------
+def test_kick_all_after_midsim_add_box_settles_under_gravity(S):
+    """tests/test_solver_6dof.py:531-580 with the reference's assertions."""
+    h = 0.12
+    s = S(dt=1 / 60, iterations=20, substeps=6, gravity=(0., -9.81, 0.),
+                   post_stabilize=True)
+    s.enable_self_collision(True, default_friction=0.5)
+    for ix in range(2):
+        for iz in range(2):
+            for k in range(3):
+                b = s.add_box((0.5 * ix, h + 2 * h * k, 0.5 * iz), (h, h, h), mass=1.0,
+                              friction=0.5)
+                s.add_floor_contact_box(b, friction=0.5)
+    for _ in range(80):
+        s.step()
+    nb = s.add_box((0.25, 2.0, 0.25), (h, h, h), mass=1.0, friction=0.5)
+    s.add_floor_contact_box(nb, friction=0.5)
+    s._flush()
+    for _ in range(80):
+        s.step()
+    rng = np.random.default_rng(0)
+    for i in range(s.positions().shape[0]):
+        body = type("B", (), {"index": i})()
+        v = s.velocities()[i]
+        w = s.angular_velocities()[i]
+        s.set_velocity(body, tuple(float(v[j] + rng.uniform(-2.5, 2.5)) for j in range(3)))
+        s.set_angular_velocity(body, tuple(float(w[j] + rng.uniform(-3., 3.)) for j in range(3)))
+    for _ in range(120):
+        s.step()
+    pos, vel = s.positions(), s.velocities()
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    assert float(pos[:, 1].max()) < 1.2
+    assert float(np.linalg.norm(vel, axis=1).mean()) < 0.6
+
+
 def test_kernel_face_clip_settles_3_cube_stack(S):
     h = 0.20
     s = S(dt=1.0 / 60.0, iterations=15, substeps=8)
