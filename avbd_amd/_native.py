@@ -28,6 +28,7 @@ OPT_PRIMAL_VARIANT = 0
 OPT_FUSE_DUAL = 1
 OPT_LEARNED_SCOPE = 2
 OPT_FLOW_GIVE_UP_TEST = 3
+OPT_GRAPH = 4
 
 
 class AvbdError(RuntimeError):
@@ -144,6 +145,8 @@ SIGNATURES = {
     "substep": ([C.c_void_p, C.POINTER(Params), C.c_int, C.POINTER(Report)], C.c_int),
     "substep_to_residual": ([C.c_void_p, C.POINTER(Params), C.c_int, C.c_float,
                              C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(Report)], C.c_int),
+    "step_frame": ([C.c_void_p, C.POINTER(Params), C.c_int, C.c_int, C.POINTER(Report)], C.c_int),
+    "sync_count": ([C.c_void_p, I64P, I64P], C.c_int),
     "read_bodies": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
     "write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
     "read_rows": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P, I32P, I32P, I32P, F32P], C.c_int),
@@ -281,6 +284,19 @@ class NativeHandle:
                                                    float(tol), C.byref(it), C.byref(r),
                                                    C.byref(rep)))
         return rep, int(it.value), float(r.value)
+
+    def step_frame(self, params: Params, n_substeps: int, color_dirty: bool) -> Report:
+        """One frame of n_substeps substeps (avbd_step_frame)."""
+        rep = Report()
+        self._check(self._fn("step_frame")(self.h, C.byref(params), int(n_substeps),
+                                           int(bool(color_dirty)), C.byref(rep)))
+        return rep
+
+    def sync_count(self) -> tuple[int, int]:
+        """(counter synchronisations, substep graph launches) since creation."""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        self._check(self._fn("sync_count")(self.h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def read_bodies(self, n: int) -> dict[str, np.ndarray]:
         out = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "w", "prev_v", "prev_w")}

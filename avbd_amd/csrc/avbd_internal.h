@@ -51,16 +51,22 @@ struct __align__(16) GroupSt {
   float4 s2;    // {ac0_b, bits (int), -, -}
 };
 
-struct LaunchCfg {
-  hipStream_t stream;
-};
+// Element count of a launch sized by capacity: with a device count nd
+// (substeps captured as a graph keep every count on the device) the kernel
+// covers min(n, *nd) elements, else n.
+__device__ __forceinline__ int dev_count(int n, const int* nd) {
+  if (nd == nullptr) return n;
+  const int m = *nd;
+  return m < n ? m : n;
+}
 
 // ---- solve kernels (kernels_solve.hip) ----
 void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose, float4* pose0,
                           float4* poseI, float4* inert, const float4* vel, const float4* pvel,
                           const float4* bstat, const float* Ilocal, GroupGeo* geo, GroupSt* gst,
                           float dt, float gx, float gy, float gz, float alpha, float gamma,
-                          int post, int do_prelude, unsigned* epoch);
+                          int post, int do_prelude, unsigned* epoch,
+                          const int* nd_groups = nullptr);
 void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int base,
                          float4* pose, const float4* poseI, const float4* inert,
                          const float4* bstat, const int* cpos, const int2* csr,
@@ -72,9 +78,11 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
                          int* done, const unsigned* epoch, int sweep, int* err, int bpw,
                          int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
-                         const int* done_rd = nullptr, int spin_cap = 0);
+                         const int* done_rd = nullptr, int spin_cap = 0,
+                         const int* meta_ncolors = nullptr, const int* meta_cstart = nullptr,
+                         const int* meta_tiles = nullptr, int max_tiles = 0);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
-                 GroupSt* gst, float beta, float* resid);
+                 GroupSt* gst, float beta, float* resid, const int* nd = nullptr);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,
                      float4* vel, float4* pvel, const float4* bstat, float dt, float max_lin,
                      float max_ang);
@@ -105,38 +113,50 @@ size_t radix_workspace_bytes(int n);
 void launch_narrowphase(hipStream_t st, int n_pairs, const float4* pose, const float4* bstat,
                         const int* pair_a, const int* pair_b, float margin, int* overlap,
                         int* sat_idx, float4* n_hat, float* depth, int* mf_count, int* mf_refa,
-                        float4* mf_off /* 8 per pair: 4 ref, 4 inc */, float4* mf_n);
+                        float4* mf_off /* 8 per pair: 4 ref, 4 inc */, float4* mf_n,
+                        const int* nd = nullptr);
 void launch_emit_demand(hipStream_t st, int n_pairs, const int* pair_a, const int* pair_b,
                         const int* mf_count, const int* mf_refa, const float* bfric,
-                        float default_mu, int* demand, int* rpc_out);
+                        float default_mu, int* demand, int* rpc_out, const int* nd = nullptr);
 void launch_emit_succ(hipStream_t st, int n_pairs, const int* mf_count, const int* rpc,
                       const int* demand_scan, int cap_rows_after_static, int* succ,
-                      int* reserve);
+                      int* reserve, const int* nd = nullptr);
 void launch_emit_write(hipStream_t st, int n_pairs, const int* pair_a, const int* pair_b,
                        const int* mf_refa, const float4* mf_n, const float4* mf_off,
                        const float* bfric, float default_mu, float mult, const int* rpc,
                        const int* succ, const int* demand_scan, const int* succ_scan,
                        int n_static_rows, int g_static, GroupGeo* geo, GroupSt* gst,
-                       int hash_cap, const int* hkeys, const float4* hstate, const int* hage);
-void launch_fill_int(hipStream_t st, int* p, int n, int v);
+                       int hash_cap, const int* hkeys, const float4* hstate, const int* hage,
+                       const int* nd = nullptr, const int* hash_cap_dev = nullptr);
+// p[0 .. min(n, *nd)) = v (all n without nd)
+void launch_fill_int(hipStream_t st, int* p, int n, int v, const int* nd = nullptr);
+void launch_copy_int(hipStream_t st, int* dst, const int* src, int n, const int* nd = nullptr);
 void launch_hash_row0(hipStream_t st, const GroupGeo* geo, const GroupSt* gst, int hash_cap,
                       int* hkeys, float4* hstate, int* hage);
 void launch_hash_collect(hipStream_t st, int n_entries, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner,
-                         int* slot_of, float4* hstate, int* hage);
+                         int* slot_of, float4* hstate, int* hage, const int* nd = nullptr,
+                         const int* hash_cap_dev = nullptr);
+// Device-side sizes of a graph-captured substep (slots of the solver's
+// device counters; see capi.hip): hash table size for `entries` live
+// contacts, derived counts after emission, the recolor gate.
+void launch_hash_size(hipStream_t st, const int* entries, int hash_cap, int* live);
 
 // ---- graph kernels (kernels_graph.hip) ----
 // Exclusive scan of n ints; returns the total through *total_dev (device int).
 size_t scan_workspace_bytes(int n);
 // Exclusive scan of n ints (in == out allowed). ws: scan_workspace_bytes(n),
 // zero when first used and not shared with another stream; err (may be null)
-// is set to 1 if a tile's look-back hit its spin cap.
+// is set to 1 if a tile's look-back hit its spin cap. nd: device count (the
+// first min(n, *nd) entries are scanned).
 void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws,
-                    int* err);
+                    int* err, const int* nd = nullptr);
 void launch_tcsr_pairs(hipStream_t st, int kept, const int* pa, const int* pb, const int* succ,
-                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a);
+                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a,
+                       const int* nd = nullptr);
 void launch_tcsr_asb_fill(hipStream_t st, int kept, const int* pb, const int* succ,
-                          const int* asb_start, int* asb_cur, int* asb_list);
+                          const int* asb_start, int* asb_cur, int* asb_list,
+                          const int* nd = nullptr);
 void launch_tcsr_count(hipStream_t st, int nb, const int* cvals, const int* st_count,
                        const int* asb_groups, const int* start_a, const int* end_a,
                        const int* sscan, const int* succ, int* padc, int* pos_of);
@@ -150,25 +170,38 @@ void launch_tcsr_fill(hipStream_t st, int nb, const int* cvals, const int* st_st
 // (batched scenes, else null): a conflict marks its scene.
 void launch_count_conflicts(hipStream_t st, int n_pairs, const int* pa, const int* pb,
                             const int* succ, const int* rpc, const int* color, int* out,
-                            const int* scene, int* scene_dirty);
+                            const int* scene, int* scene_dirty, const int* nd = nullptr);
 void launch_color_reset(hipStream_t st, int nb, int* color, const int* scene,
-                        const int* scene_dirty);
+                        const int* scene_dirty, const int* nd = nullptr);
 void launch_adj_count(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
                       int* counts);
 void launch_adj_fill(hipStream_t st, int n_dyn, int g_first, const GroupGeo* geo,
                      const int* starts, int* cursor, int* nbr);
 void launch_adj_count_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
-                            const int* succ, int* counts);
+                            const int* succ, int* counts, const int* nd = nullptr);
 void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
-                           const int* succ, const int* starts, int* cursor, int* nbr);
+                           const int* succ, const int* starts, int* cursor, int* nbr,
+                           const int* nd = nullptr);
 // scene_dirty (batched scenes, else null): bodies of clean scenes keep their color
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
                          int* err, int* ticket, int mode_jp, int* clamped, const int* scene,
-                         const int* scene_dirty);
+                         const int* scene_dirty, const int* nd = nullptr);
 void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
-                          const float* prio, const int* color, int* color_next, int* uncolored);
-void launch_color_finish(hipStream_t st, int nb, int* color, int* counts);
+                          const float* prio, const int* color, int* color_next, int* uncolored,
+                          const int* nd = nullptr);
+// gate (may be null): a device count; 0 leaves counts and colors as they are
+void launch_color_finish(hipStream_t st, int nb, int* color, int* counts,
+                         const int* gate = nullptr);
+// Jacobi rounds 2 .. MAX_COLORS of a recolor, on one workgroup, when round 1
+// left bodies uncolored (*uncolored > 0); a no-op otherwise.
+void launch_color_jacobi_tail(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
+                              const float* prio, int* color, int* color_next, int* uncolored,
+                              int* clamped, const int* gate);
+// Color starts, achieved color count and tiles of `bpw` bodies per color
+// from the per-color counts (the layout of the solver's device counters).
+void launch_color_meta(hipStream_t st, const int* counts, int bpw, int* cstart, int* n_active,
+                       int* tile_start);
 void launch_color_scatter(hipStream_t st, int nb, const int* color, const int* starts,
                           int* cursor, int* bodies);
 void launch_sort_segments(hipStream_t st, int nb, const int* starts, int* idx);

@@ -20,12 +20,15 @@ using namespace avbd;
 
 namespace avbd {
 void color_bucket(hipStream_t st, int nb, const int* color, unsigned* keys, int* vals,
-                  unsigned* keys_tmp, int* vals_tmp, void* ws);
+                  unsigned* keys_tmp, int* vals_tmp, void* ws, const int* gate = nullptr);
 }
 
 namespace {
 
 thread_local std::string g_err;
+// Bumped by every device (re)allocation of a solver buffer: a captured
+// substep graph holds raw pointers, so any reallocation retires it.
+unsigned long long g_alloc_gen = 0;
 
 int set_err(int code, const std::string& m) {
   g_err = m;
@@ -54,7 +57,10 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      ++g_alloc_gen;
+    }
     p = nullptr;
     n = 0;
   }
@@ -65,6 +71,7 @@ struct DBuf {
       p = nullptr;
       return set_err(AVBD_E_NOMEM, "hipMalloc of " + std::to_string(c * sizeof(T)) + " bytes failed");
     }
+    ++g_alloc_gen;
     n = c;
     return AVBD_OK;
   }
@@ -83,18 +90,33 @@ struct DBuf {
     release();
     p = q;
     n = count;
+    ++g_alloc_gen;
     return AVBD_OK;
   }
 };
 
 enum ProfClass { kPrimal = 0, kDual, kPrelude, kContacts, kColoring, kCsr, kOther, kNumProf };
 
-// Device counters (dcnt) by slot: 0 pair count, 1-3 emission totals, 4
-// conflict rows, 5 uncolored bodies, 6 coloring error, 8 dataflow error
-// (sticky until the next flush), 9 color clamp, 10 residual, 12 substep
-// epoch (stamps of the dataflow sweeps).
+// Device counters (dcnt) by slot: 0 pair count, 1-3 emission totals (row
+// demand, emitted groups, reserved rows), 4 conflict rows, 5 uncolored
+// bodies, 6 coloring error, 8 dataflow error (sticky until the next flush),
+// 9 color clamp, 10 residual, 12 substep epoch (stamps of the dataflow
+// sweeps). Substeps whose decisions stay on the device (graph capture) also
+// keep there: 13 kept pairs, 14 live groups, 15 cache entries, 16 slots of
+// the warm-start table as last built, 17 / 18 the recolor gate as a count
+// (all / the kept pairs, 0 when the coloring is kept), 19 recolors done, 20
+// achieved color count, 32.. per-color body counts, 64.. color starts,
+// 100.. first tile of each color. One copy of slots [0, kCounterSlots)
+// brings every count to the host.
+constexpr int kPairSlot = 0, kGroupsSlot = 2, kRowsSlot = 3, kConflictSlot = 4;
+constexpr int kUncoloredSlot = 5;
 constexpr int kErrSlot = 8;
 constexpr int kEpochSlot = 12;
+constexpr int kKeptSlot = 13, kNGroupsSlot = 14, kEntriesSlot = 15, kHashLiveSlot = 16;
+constexpr int kGateAllSlot = 17, kGateKeptSlot = 18, kRecolorsSlot = 19, kNColorsSlot = 20;
+constexpr int kCountsSlot = 32, kCstartSlot = 64, kTilesSlot = 100;
+constexpr int kCounterSlots = 136;
+constexpr int kDcntSize = 256;
 
 }  // namespace
 
@@ -152,7 +174,7 @@ struct avbd_solver {
   bool csr_order_stale = false; // recolored since that CSR was built
 
   // coloring
-  DBuf<int> color, color_next, nbr_count, nbr_start, nbr_cursor, nbr, pub, ticket, ccounts;
+  DBuf<int> color, color_next, nbr_count, nbr_start, nbr_cursor, nbr, pub, ticket;
   DBuf<unsigned> ckeys, ckeys_tmp;
   DBuf<int> cvals, cvals_tmp;
   int cstart[kMaxColors + 1] = {0};
@@ -196,6 +218,16 @@ struct avbd_solver {
   int flow_test_cap = 0;
   DBuf<int> flow_never;
 
+  // substeps with every count and decision on the device, captured as a HIP
+  // graph (AVBD_OPT_GRAPH): one executable graph per pose buffer (the
+  // substep alternates them), with the key of what it was captured from
+  bool graph_on = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<char> gkey[2];
+  int recolors_seen = 0;     // dcnt[kRecolorsSlot] at the last frame end
+  long long syncs = 0;       // device -> host synchronisations of counters
+  long long graph_launches = 0;
+
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -219,9 +251,10 @@ int ensure_scan_ws(avbd_solver* s, size_t n) {
   return AVBD_OK;
 }
 
-int scan(avbd_solver* s, const int* in, int* out, int n, int* total_dev) {
+int scan(avbd_solver* s, const int* in, int* out, int n, int* total_dev,
+         const int* nd = nullptr) {
   CK(ensure_scan_ws(s, (size_t)n));
-  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p, s->dcnt.p + kErrSlot);
+  exclusive_scan(s->st, in, out, n, total_dev, s->ws_scan.p, s->dcnt.p + kErrSlot, nd);
   return AVBD_OK;
 }
 
@@ -240,6 +273,7 @@ int readback(avbd_solver* s, int first, int count) {
   HIPCK(hipMemcpyAsync(s->hcnt + first, s->dcnt.p + first, count * sizeof(int),
                        hipMemcpyDeviceToHost, s->st));
   HIPCK(hipStreamSynchronize(s->st));
+  ++s->syncs;
   return check_flow_error(s);
 }
 
@@ -358,21 +392,23 @@ int recolor(avbd_solver* s, int mode) {
     launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
                         s->pub.p, s->dcnt.p + 6, s->ticket.p, 1, s->dcnt.p + 9, scene, dirty);
   }
-  launch_color_finish(st, nb, s->color.p, s->ccounts.p);
+  launch_color_finish(st, nb, s->color.p, s->dcnt.p + kCountsSlot);
   color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
                s->ws_color.p);
   CK(prof_end(s, 0.0));
-  HIPCK(hipMemcpyAsync(s->hcnt + 16, s->ccounts.p, kMaxColors * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCK(hipMemcpyAsync(s->hcnt + kCountsSlot, s->dcnt.p + kCountsSlot, kMaxColors * sizeof(int),
+                       hipMemcpyDeviceToHost, st));
   HIPCK(hipMemcpyAsync(s->hcnt + 6, s->dcnt.p + 6, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipMemcpyAsync(s->hcnt + 9, s->dcnt.p + 9, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipStreamSynchronize(st));
+  ++s->syncs;
   if (s->hcnt[6] != 0) return set_err(AVBD_E_STATE, "coloring dataflow did not converge");
   s->colors_conflict_free = s->hcnt[9] == 0;
   int mx = 0;
   s->cstart[0] = 0;
   for (int c = 0; c < kMaxColors; ++c) {
-    s->cstart[c + 1] = s->cstart[c] + s->hcnt[16 + c];
-    if (s->hcnt[16 + c] > 0) mx = c;
+    s->cstart[c + 1] = s->cstart[c] + s->hcnt[kCountsSlot + c];
+    if (s->hcnt[kCountsSlot + c] > 0) mx = c;
   }
   s->n_active_colors = std::max(1, std::min(mx + 1, kMaxColors));
   return AVBD_OK;
@@ -701,6 +737,7 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
     CK(prof_begin(s, kOther));
     int entries = std::min(s->n_contacts, s->pool_max);
     s->hash_live = live_hash_slots(s, pool_reset ? 1 : entries);
+    launch_fill_int(st, s->dcnt.p + kHashLiveSlot, 1, s->hash_live);
     launch_fill_int(st, s->hkeys.p, s->hash_live, kHashEmpty);
     launch_fill_int(st, s->howner.p, s->hash_live, -1);
     if (!pool_reset) {
@@ -721,6 +758,305 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
   s->cur = 1 - s->cur;
   s->last_post = post;
   CK(prof_collect(s));
+  return AVBD_OK;
+}
+
+// ===========================================================================
+// Substeps with every count and decision on the device
+// ===========================================================================
+// The substep of do_substep in the fixed-capacity mode, issued without any
+// host read: launches are sized by capacity and each kernel takes its count
+// from the solver's device counters; the recolor gate is a count that is 0
+// when the coloring is kept; the sweep takes the color layout from the
+// device. With the same inputs it performs the same operations in the same
+// order, so it gives the same bits, and it can be captured as a HIP graph.
+
+// Kept pairs, live groups and cache entries after the emission.
+__global__ void k_after_emission(int* d, int pair_cap, int g_static, int pool_max) {
+  if (threadIdx.x != 0) return;
+  const int pairs = d[kPairSlot], groups = d[kGroupsSlot];
+  d[kKeptSlot] = pairs < pair_cap ? pairs : pair_cap;
+  d[kNGroupsSlot] = g_static + groups;
+  d[kEntriesSlot] = groups < pool_max ? groups : pool_max;
+}
+
+// Kept pairs before the emission (the narrowphase and emission kernels'
+// count).
+__global__ void k_kept_pairs(int* d, int pair_cap) {
+  if (threadIdx.x != 0) return;
+  const int pairs = d[kPairSlot];
+  d[kKeptSlot] = pairs < pair_cap ? pairs : pair_cap;
+}
+
+// The recolor gate (solver_6dof.py:1333-1356 with the coloring clean): a
+// conflict row opens it.
+__global__ void k_recolor_gate(int* d) {
+  if (threadIdx.x != 0) return;
+  const bool need = d[kConflictSlot] > 0;
+  d[kGateAllSlot] = need ? 0x7fffffff : 0;
+  d[kGateKeptSlot] = need ? d[kKeptSlot] : 0;
+  if (need) d[kRecolorsSlot] += 1;
+}
+
+int primal_bpw(int variant) {
+  return variant == 7 ? 1 : variant == 8 ? 4 : variant == 9 ? 8 : variant == 10 ? 3 : 2;
+}
+
+// What a graph-captured substep needs: the fixed-capacity semantics, the
+// dataflow sweep with a separate dual, rows to solve, no learned warm start
+// and no profiling. Buffers a direct substep would size on the fly are sized
+// to capacity first (this allocates, so it runs before any capture).
+int graph_eligible(avbd_solver* s, const avbd_params* p, bool* ok) {
+  *ok = false;
+  if (!s->graph_on || s->nb == 0 || p->fixed_capacity == 0 || s->ml_mode != 0 || s->prof ||
+      s->primal_variant < 6 || s->primal_variant > 10 || s->fuse_dual_mode != 0 ||
+      s->flow_test_cap != 0 || s->row_cap <= 0 || p->iterations + p->post_stabilize >= 4095)
+    return AVBD_OK;
+  const bool emit = p->self_collide && s->nb >= 2;
+  if (emit) CK(ensure_pairs(s, std::max(256, 16 * s->nb)));
+  const int pcap = std::max(s->pair_cap, 1);
+  CK(s->nbr.ensure(2 * (size_t)pcap + 1));
+  CK(s->asb_list.ensure((size_t)pcap));
+  CK(s->tcsr.ensure((size_t)s->st_total + 2 * (size_t)(s->g_cap - s->g_static) + 3 * (size_t)s->nb + 1));
+  CK(ensure_scan_ws(s, std::max<size_t>((size_t)pcap, (size_t)s->nb)));
+  CK(s->pslots.ensure((size_t)s->nb * kPartnerSlots));
+  *ok = true;
+  return AVBD_OK;
+}
+
+void substep_device(avbd_solver* s, const avbd_params* p) {
+  const int nb = s->nb;
+  hipStream_t st = s->st;
+  int* d = s->dcnt.p;
+  const bool emit = p->self_collide && nb >= 2;
+  const int pcap = s->pair_cap;
+  const bool batched = s->n_scenes > 0;
+  const int* kept = d + kKeptSlot;
+  // ---- contacts (emit_contacts) ----
+  if (emit) {
+    float4* pc = s->pose[s->cur].p;
+    launch_aabb(st, nb, pc, s->bstat.p, p->margin, s->lo.p, s->hi.p);
+    const float4* tlo = s->lo.p;
+    const float4* thi = s->hi.p;
+    const int* scene = nullptr;
+    if (batched) {
+      launch_scene_shift(st, nb, s->lo.p, s->hi.p, s->scene_of.p, s->scene_grid, s->scene_pitch,
+                         s->slo.p, s->shi.p);
+      tlo = s->slo.p;
+      thi = s->shi.p;
+      scene = s->scene_of.p;
+    }
+    bvh_build(st, nb, tlo, thi, s->ws_bvh.p, s->ws_bvh.n);
+    bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
+                    s->bcount.p, s->pslots.p);
+    (void)scan(s, s->bcount.p, s->boff.p, nb, d + kPairSlot);
+    hipLaunchKernelGGL(k_kept_pairs, dim3(1), dim3(64), 0, st, d, pcap);
+    bvh_write_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
+                    s->bcount.p, s->pslots.p, s->boff.p, pcap, s->pa.p, s->pb.p);
+    launch_narrowphase(st, pcap, pc, s->bstat.p, s->pa.p, s->pb.p, p->margin, s->pov.p,
+                       s->psat.p, s->pn.p, s->pdepth.p, s->mfc.p, s->mfr.p, s->mfoff.p, s->mfn.p,
+                       kept);
+    launch_emit_demand(st, pcap, s->pa.p, s->pb.p, s->mfc.p, s->mfr.p, s->bfric.p,
+                       p->self_friction, s->demand.p, s->rpc.p, kept);
+    (void)scan(s, s->demand.p, s->dscan.p, pcap, d + 1, kept);
+    launch_emit_succ(st, pcap, s->mfc.p, s->rpc.p, s->dscan.p, s->row_cap - s->n_static_rows,
+                     s->succ.p, s->reserve.p, kept);
+    (void)scan(s, s->succ.p, s->sscan.p, pcap, d + kGroupsSlot, kept);
+    (void)scan(s, s->reserve.p, s->rscan.p, pcap, d + kRowsSlot, kept);
+    launch_emit_write(st, pcap, s->pa.p, s->pb.p, s->mfr.p, s->mfn.p, s->mfoff.p, s->bfric.p,
+                      p->self_friction, p->friction_static_mult, s->rpc.p, s->succ.p, s->dscan.p,
+                      s->sscan.p, s->n_static_rows, s->g_static, s->geo.p, s->gst.p, s->hash_cap,
+                      s->hkeys.p, s->hstate.p, nullptr, kept, d + kHashLiveSlot);
+    // the recolor gate's conflict count
+    launch_fill_int(st, d + kConflictSlot, 1, 0);
+    if (batched) launch_fill_int(st, s->scene_dirty.p, s->n_scenes, 0);
+    launch_count_conflicts(st, pcap, s->pa.p, s->pb.p, s->succ.p, s->rpc.p, s->color.p,
+                           d + kConflictSlot, batched ? s->scene_of.p : nullptr,
+                           batched ? s->scene_dirty.p : nullptr, kept);
+  } else {
+    launch_fill_int(st, d, kConflictSlot + 1, 0);
+    if (batched) launch_fill_int(st, s->scene_dirty.p, s->n_scenes, 0);
+  }
+  hipLaunchKernelGGL(k_after_emission, dim3(1), dim3(64), 0, st, d, std::max(pcap, 0), s->g_static,
+                     s->pool_max);
+  // ---- recolor (recolor), gated ----
+  hipLaunchKernelGGL(k_recolor_gate, dim3(1), dim3(64), 0, st, d);
+  const int* gA = d + kGateAllSlot;
+  const int* gK = d + kGateKeptSlot;
+  const int* scene = batched ? s->scene_of.p : nullptr;
+  const int* dirty = batched ? s->scene_dirty.p : nullptr;
+  launch_fill_int(st, s->nbr_count.p, nb, 0, gA);
+  launch_adj_count_pairs(st, pcap, s->pa.p, s->pb.p, s->succ.p, s->nbr_count.p, gK);
+  (void)scan(s, s->nbr_count.p, s->nbr_start.p, nb, s->nbr_start.p + nb, gA);
+  launch_fill_int(st, s->nbr_cursor.p, nb, 0, gA);
+  launch_adj_fill_pairs(st, pcap, s->pa.p, s->pb.p, s->succ.p, s->nbr_start.p, s->nbr_cursor.p,
+                        s->nbr.p, gK);
+  launch_color_reset(st, nb, s->color.p, scene, dirty, gA);
+  launch_fill_int(st, d + 6, 1, 0);
+  launch_fill_int(st, d + 9, 1, 0);
+  if (p->coloring_mode == AVBD_COLOR_JACOBI) {
+    launch_color_greedy(st, nb, s->order_id.p, nullptr, s->nbr_start.p, s->nbr.p, s->color.p,
+                        s->pub.p, d + 6, s->ticket.p, 0, d + 9, scene, dirty, gA);
+    launch_fill_int(st, d + kUncoloredSlot, 1, 0);
+    launch_color_resolve(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
+                         s->color_next.p, d + kUncoloredSlot, gA);
+    launch_copy_int(st, s->color.p, s->color_next.p, nb, gA);
+    launch_color_jacobi_tail(st, nb, s->nbr_start.p, s->nbr.p, s->prio.p, s->color.p,
+                             s->color_next.p, d + kUncoloredSlot, d + 9, gA);
+  } else {
+    launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
+                        s->pub.p, d + 6, s->ticket.p, 1, d + 9, scene, dirty, gA);
+  }
+  launch_color_finish(st, nb, s->color.p, d + kCountsSlot, gA);
+  color_bucket(st, nb, s->color.p, s->ckeys.p, s->cvals.p, s->ckeys_tmp.p, s->cvals_tmp.p,
+               s->ws_color.p, gA);
+  const int bpw = primal_bpw(s->primal_variant);
+  launch_color_meta(st, d + kCountsSlot, bpw, d + kCstartSlot, d + kNColorsSlot, d + kTilesSlot);
+  // ---- body -> group lists (build_csr) ----
+  int* asb_cnt = s->tc_work.p;
+  int* asb_groups = asb_cnt + nb;
+  int* start_a = asb_groups + nb;
+  int* end_a = start_a + nb;
+  int* asb_cur = end_a + nb;
+  launch_fill_int(st, s->tc_work.p, 5 * nb, 0);
+  launch_tcsr_pairs(st, pcap, s->pa.p, s->pb.p, s->succ.p, asb_cnt, asb_groups, start_a, end_a,
+                    kept);
+  (void)scan(s, asb_cnt, s->asb_start.p, nb, s->asb_start.p + nb);
+  launch_tcsr_asb_fill(st, pcap, s->pb.p, s->succ.p, s->asb_start.p, asb_cur, s->asb_list.p, kept);
+  launch_tcsr_count(st, nb, s->cvals.p, s->st_count.p, asb_groups, start_a, end_a, s->sscan.p,
+                    s->succ.p, s->padc.p, s->pos_of.p);
+  (void)scan(s, s->padc.p, s->cpos.p, nb, s->cpos.p + nb);
+  launch_tcsr_fill(st, nb, s->cvals.p, s->st_start.p, s->st_idx.p, s->asb_start.p, s->asb_list.p,
+                   start_a, end_a, s->sscan.p, s->succ.p, s->pa.p, s->pb.p, s->g_static, s->cpos.p,
+                   s->tcsr.p);
+  // ---- predict + prelude, iterations, finalize ----
+  float4* p0 = s->pose[s->cur].p;
+  float4* pw = s->pose[1 - s->cur].p;
+  const float dt = p->dt;
+  const int post = p->post_stabilize ? 1 : 0;
+  const int* ngroups = d + kNGroupsSlot;
+  launch_substep_begin(st, nb, s->g_cap, p0, pw, s->poseI.p, s->inert.p, s->vel.p, s->pvel.p,
+                       s->bstat.p, s->Ilocal.p, s->geo.p, s->gst.p, dt, p->gravity[0],
+                       p->gravity[1], p->gravity[2], p->alpha, p->gamma, post, 1,
+                       reinterpret_cast<unsigned*>(d + kEpochSlot), ngroups);
+  float max_lin = p->max_linear_speed, max_ang = p->max_angular_speed;
+  if (!(max_lin > 0.0f && max_lin < kInf && max_ang > 0.0f && max_ang < kInf)) {
+    max_lin = kInf;
+    max_ang = kInf;
+  }
+  const int I = p->iterations;
+  const int max_tiles = (nb + bpw - 1) / bpw + kMaxColors;
+  for (int it = 0; it < I + post; ++it) {
+    const int post_pass = (post && it == I) ? 1 : 0;
+    launch_primal_sweep(st, 0, s->cstart, s->cvals.p, s->color.p, pw, s->poseI.p, s->inert.p,
+                        s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p, s->gst.p, dt, post_pass,
+                        s->flow_done.p, reinterpret_cast<const unsigned*>(d + kEpochSlot), it,
+                        d + kErrSlot, bpw, 0, p->beta, s->gst.p, 0, nullptr, 0, d + kNColorsSlot,
+                        d + kCstartSlot, d + kTilesSlot, max_tiles);
+    if (it < I) launch_dual(st, s->g_cap, pw, s->geo.p, s->gst.p, p->beta, nullptr, ngroups);
+    if (it == I - 1)
+      launch_finalize(st, nb, pw, p0, s->vel.p, s->pvel.p, s->bstat.p, dt, max_lin, max_ang);
+  }
+  // ---- warm-start cache refresh ----
+  if (p->self_collide && s->hash_cap > 0) {
+    launch_hash_size(st, d + kEntriesSlot, s->hash_cap, d + kHashLiveSlot);
+    launch_fill_int(st, s->hkeys.p, s->hash_cap, kHashEmpty, d + kHashLiveSlot);
+    launch_fill_int(st, s->howner.p, s->hash_cap, -1, d + kHashLiveSlot);
+    launch_hash_collect(st, std::max(s->pool_max, 1), s->g_static, s->geo.p, s->gst.p,
+                        s->hash_cap, s->hkeys.p, s->howner.p, s->slot_of.p, s->hstate.p, nullptr,
+                        d + kEntriesSlot, d + kHashLiveSlot);
+  }
+}
+
+// Everything a captured substep depends on besides device memory contents.
+std::vector<char> graph_key(const avbd_solver* s, const avbd_params* p) {
+  std::vector<char> k;
+  auto put = [&k](const void* v, size_t n) {
+    const char* c = static_cast<const char*>(v);
+    k.insert(k.end(), c, c + n);
+  };
+  put(p, sizeof(*p));
+  const long long ints[] = {s->nb, s->cur, s->n_static_rows, s->g_static, s->g_cap, s->row_cap,
+                            s->pair_cap, s->pool_max, s->hash_cap, s->st_total,
+                            s->primal_variant, s->n_scenes, s->scene_grid,
+                            (long long)g_alloc_gen};
+  put(ints, sizeof(ints));
+  put(&s->scene_pitch, sizeof(s->scene_pitch));
+  const void* ptrs[] = {s->color.p, s->color_next.p, s->pose[0].p, s->pose[1].p};
+  put(ptrs, sizeof(ptrs));
+  return k;
+}
+
+void drop_graphs(avbd_solver* s) {
+  for (int c = 0; c < 2; ++c) {
+    if (s->gexec[c]) (void)hipGraphExecDestroy(s->gexec[c]);
+    s->gexec[c] = nullptr;
+    s->gkey[c].clear();
+  }
+}
+
+// One substep through the graph of the current pose buffer, captured first
+// when it is missing or stale.
+int substep_graph(avbd_solver* s, const avbd_params* p) {
+  const int c = s->cur;
+  std::vector<char> key = graph_key(s, p);
+  if (!s->gexec[c] || s->gkey[c] != key) {
+    if (s->gexec[c]) (void)hipGraphExecDestroy(s->gexec[c]);
+    s->gexec[c] = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    substep_device(s, p);
+    const hipError_t e_end = hipStreamEndCapture(s->st, &g);
+    if (e_end != hipSuccess || g == nullptr)
+      return set_err(AVBD_E_HIP, std::string("substep capture failed: ") + hipGetErrorString(e_end));
+    const hipError_t e_inst = hipGraphInstantiate(&s->gexec[c], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e_inst != hipSuccess) {
+      s->gexec[c] = nullptr;
+      return set_err(AVBD_E_HIP, std::string("substep graph: ") + hipGetErrorString(e_inst));
+    }
+    s->gkey[c] = key;
+  }
+  HIPCK(hipGraphLaunch(s->gexec[c], s->st));
+  ++s->graph_launches;
+  s->cur = 1 - c;
+  s->last_post = p->post_stabilize ? 1 : 0;
+  return AVBD_OK;
+}
+
+// The host's view after device-decided substeps: one copy of the counters.
+int frame_end(avbd_solver* s, avbd_report* rep, int recolored_direct) {
+  HIPCK(hipMemcpyAsync(s->hcnt, s->dcnt.p, kCounterSlots * sizeof(int), hipMemcpyDeviceToHost,
+                       s->st));
+  HIPCK(hipStreamSynchronize(s->st));
+  ++s->syncs;
+  CK(check_flow_error(s));
+  const int* h = s->hcnt;
+  if (h[6] != 0) return set_err(AVBD_E_STATE, "coloring dataflow did not converge");
+  s->pair_count = h[kPairSlot];
+  s->adj_pairs = std::min(h[kPairSlot], s->pair_cap);
+  s->n_contacts = h[kGroupsSlot];
+  s->pool_count = s->n_contacts;
+  s->n_active_rows = s->n_static_rows + h[kRowsSlot];
+  s->hash_live = h[kHashLiveSlot] > 0 ? h[kHashLiveSlot] : s->hash_live;
+  s->n_active_colors = std::max(1, std::min(h[kNColorsSlot], kMaxColors));
+  for (int c = 0; c <= kMaxColors; ++c) s->cstart[c] = h[kCstartSlot + c];
+  s->colors_conflict_free = h[9] == 0;
+  s->csr_built = true;
+  s->csr_order_stale = false;
+  const int recolors = h[kRecolorsSlot];
+  std::memset(rep, 0, sizeof(*rep));
+  rep->recolored = (recolors != s->recolors_seen || recolored_direct) ? 1 : 0;
+  s->recolors_seen = recolors;
+  rep->n_active_colors = s->n_active_colors;
+  rep->n_active_rows = s->n_active_rows;
+  rep->row_capacity = s->row_cap;
+  rep->new_row_capacity = s->row_cap;
+  rep->pair_count = s->pair_count;
+  rep->pair_capacity = s->pair_cap;
+  rep->pool_count = s->pool_count;
+  rep->n_contacts = std::min(s->n_contacts, s->pool_max);
+  rep->color_conflicts = h[kConflictSlot];
   return AVBD_OK;
 }
 
@@ -755,20 +1091,20 @@ int avbd_create(int device, avbd_solver** out) {
     delete s;
     return set_err(AVBD_E_HIP, "hipStreamCreate failed");
   }
-  if (hipHostMalloc((void**)&s->hcnt, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&s->hcnt, kDcntSize * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     (void)hipStreamDestroy(s->st);
     delete s;
     return set_err(AVBD_E_NOMEM, "hipHostMalloc failed");
   }
-  std::memset(s->hcnt, 0, 64 * sizeof(int));
-  int r = s->dcnt.alloc(64);
+  std::memset(s->hcnt, 0, kDcntSize * sizeof(int));
+  int r = s->dcnt.alloc(kDcntSize);
   if (r != AVBD_OK) {
     (void)hipHostFree(s->hcnt);
     (void)hipStreamDestroy(s->st);
     delete s;
     return r;
   }
-  launch_fill_int(s->st, s->dcnt.p, 64, 0);
+  launch_fill_int(s->st, s->dcnt.p, kDcntSize, 0);
   *out = s;
   return AVBD_OK;
 }
@@ -777,6 +1113,7 @@ int avbd_destroy(avbd_solver* s) {
   if (!s) return AVBD_OK;
   (void)hipSetDevice(s->device);
   (void)hipStreamSynchronize(s->st);
+  drop_graphs(s);
   for (auto e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->hcnt) (void)hipHostFree(s->hcnt);
   (void)hipStreamDestroy(s->st);
@@ -792,6 +1129,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
       sc->max_pairs < 0)
     return set_err(AVBD_E_ARG, "negative size in scene");
   HIPCK(hipStreamSynchronize(s->st));
+  drop_graphs(s);
   s->nb = nb;
   s->cur = 0;
   const size_t nbb = std::max(nb, 1);
@@ -949,6 +1287,9 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   HIPCK(hipMemsetAsync(s->hstate.p, 0, 2 * (size_t)std::max(s->hash_cap, 1) * sizeof(float4), s->st));
   s->hash_live = std::min(s->hash_cap, 4096);
   s->hage.release();
+  launch_fill_int(s->st, s->dcnt.p + kHashLiveSlot, 1, s->hash_live);
+  launch_fill_int(s->st, s->dcnt.p + kRecolorsSlot, 1, 0);
+  s->recolors_seen = 0;
   // ---- CSR + coloring ----
   CK(s->tc_work.alloc(5 * nbb)); CK(s->asb_start.alloc(nbb + 1)); CK(s->asb_list.alloc(1));
   CK(s->padc.alloc(nbb)); CK(s->cpos.alloc(nbb + 1)); CK(s->pos_of.alloc(nbb));
@@ -966,7 +1307,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   HIPCK(hipMemsetAsync(s->cpos.p, 0, (nbb + 1) * sizeof(int), s->st));
   CK(s->color.alloc(nbb)); CK(s->color_next.alloc(nbb)); CK(s->nbr_count.alloc(nbb));
   CK(s->nbr_start.alloc(nbb + 1)); CK(s->nbr_cursor.alloc(nbb)); CK(s->nbr.alloc(1));
-  CK(s->pub.alloc(nbb)); CK(s->ticket.alloc(1)); CK(s->ccounts.alloc(kMaxColors));
+  CK(s->pub.alloc(nbb)); CK(s->ticket.alloc(1));
   CK(s->ckeys.alloc(nbb)); CK(s->ckeys_tmp.alloc(nbb)); CK(s->cvals.alloc(nbb));
   CK(s->cvals_tmp.alloc(nbb));
   CK(s->ws_bvh.alloc(bvh_workspace_bytes((int)nbb)));
@@ -1002,6 +1343,45 @@ int avbd_substep_to_residual(avbd_solver* s, const avbd_params* p, int color_dir
   if (!(tol >= 0.0f)) return set_err(AVBD_E_ARG, "tolerance must be >= 0");
   HIPCK(hipSetDevice(s->device));
   return do_substep(s, p, color_dirty, out, tol, iterations_used, residual);
+}
+
+int avbd_step_frame(avbd_solver* s, const avbd_params* p, int n_substeps, int color_dirty,
+                    avbd_report* out) {
+  if (!s || !p || !out) return set_err(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
+  if (n_substeps < 1) return set_err(AVBD_E_ARG, "a frame needs at least one substep");
+  HIPCK(hipSetDevice(s->device));
+  std::memset(out, 0, sizeof(*out));
+  out->n_active_colors = s->n_active_colors;
+  if (s->nb == 0) return AVBD_OK;
+  bool ok = false;
+  CK(graph_eligible(s, p, &ok));
+  avbd_report rep{};
+  int recolored_direct = 0;
+  bool on_device = false;
+  for (int k = 0; k < n_substeps; ++k) {
+    const int dirty = (color_dirty && k == 0) ? 1 : 0;
+    if (ok && !dirty) {
+      // the device counters must describe the state the graph starts from:
+      // after direct substeps they do (they are the same counters)
+      CK(substep_graph(s, p));
+      on_device = true;
+    } else {
+      CK(do_substep(s, p, dirty, &rep));
+      recolored_direct |= rep.recolored;
+    }
+  }
+  if (on_device) return frame_end(s, out, recolored_direct);
+  *out = rep;
+  out->recolored = recolored_direct;
+  return AVBD_OK;
+}
+
+int avbd_sync_count(avbd_solver* s, long long* syncs, long long* graph_launches) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  if (syncs) *syncs = s->syncs;
+  if (graph_launches) *graph_launches = s->graph_launches;
+  return AVBD_OK;
 }
 
 int avbd_read_bodies(avbd_solver* s, float* x, float* q, float* v, float* w, float* pv,
@@ -1354,6 +1734,15 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
     case AVBD_OPT_LEARNED_SCOPE:
       if (value < 0 || value > 1) return set_err(AVBD_E_ARG, "learned scope must be 0 or 1");
       s->ml_scope = value;
+      return AVBD_OK;
+    case AVBD_OPT_GRAPH:
+      if (value < 0 || value > 1) return set_err(AVBD_E_ARG, "graph option must be 0 or 1");
+      s->graph_on = value != 0;
+      if (!s->graph_on) {
+        HIPCK(hipSetDevice(s->device));
+        HIPCK(hipStreamSynchronize(s->st));
+        drop_graphs(s);
+      }
       return AVBD_OK;
     case AVBD_OPT_FLOW_GIVE_UP_TEST:
       if (value < 0 || value > 4096) return set_err(AVBD_E_ARG, "give-up test cap must be 0 to 4096");

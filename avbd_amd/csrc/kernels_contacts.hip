@@ -575,9 +575,10 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
               const int* __restrict__ pair_a, const int* __restrict__ pair_b, float margin,
               int* __restrict__ overlap, int* __restrict__ sat_out, float4* __restrict__ n_out,
               float* __restrict__ depth_out, int* __restrict__ mf_count,
-              int* __restrict__ mf_refa, float4* __restrict__ mf_off, float4* __restrict__ mf_n) {
+              int* __restrict__ mf_refa, float4* __restrict__ mf_off, float4* __restrict__ mf_n,
+              const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+  if (p >= dev_count(n_pairs, nd)) return;
   int i = pair_a[p], j = pair_b[p];
   v3 cA = f3(pose[2 * i]), cB = f3(pose[2 * j]);
   float4 bsA = bstat[i], bsB = bstat[j];
@@ -810,11 +811,11 @@ k_narrowphase(int n_pairs, const float4* __restrict__ pose, const float4* __rest
 void launch_narrowphase(hipStream_t st, int n_pairs, const float4* pose, const float4* bstat,
                         const int* pair_a, const int* pair_b, float margin, int* overlap,
                         int* sat_idx, float4* n_hat, float* depth, int* mf_count, int* mf_refa,
-                        float4* mf_off, float4* mf_n) {
+                        float4* mf_off, float4* mf_n, const int* nd) {
   if (n_pairs <= 0) return;
   hipLaunchKernelGGL(k_narrowphase, dim3((n_pairs + 127) / 128), dim3(128), 0, st, n_pairs, pose,
                      bstat, pair_a, pair_b, margin, overlap, sat_idx, n_hat, depth, mf_count,
-                     mf_refa, mf_off, mf_n);
+                     mf_refa, mf_off, mf_n, nd);
 }
 
 // ---------------------------------------------------------------------------
@@ -830,9 +831,9 @@ __global__ void __launch_bounds__(256)
 k_emit_demand(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
               const int* __restrict__ mf_count, const int* __restrict__ mf_refa,
               const float* __restrict__ bfric, float default_mu, int* __restrict__ demand,
-              int* __restrict__ rpc_out) {
+              int* __restrict__ rpc_out, const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+  if (p >= dev_count(n_pairs, nd)) return;
   int nc = mf_count[p];
   int rpc = 3;
   if (nc > 0) {
@@ -847,10 +848,10 @@ k_emit_demand(int n_pairs, const int* __restrict__ pa, const int* __restrict__ p
 
 void launch_emit_demand(hipStream_t st, int n_pairs, const int* pa, const int* pb,
                         const int* mf_count, const int* mf_refa, const float* bfric,
-                        float default_mu, int* demand, int* rpc_out) {
+                        float default_mu, int* demand, int* rpc_out, const int* nd) {
   if (n_pairs <= 0) return;
   hipLaunchKernelGGL(k_emit_demand, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, pa, pb,
-                     mf_count, mf_refa, bfric, default_mu, demand, rpc_out);
+                     mf_count, mf_refa, bfric, default_mu, demand, rpc_out, nd);
 }
 
 // Serial reservation order restated: a contact (p, c) fits iff every earlier
@@ -859,9 +860,9 @@ void launch_emit_demand(hipStream_t st, int n_pairs, const int* pa, const int* p
 __global__ void __launch_bounds__(256)
 k_emit_succ(int n_pairs, const int* __restrict__ mf_count, const int* __restrict__ rpc,
             const int* __restrict__ dscan, int avail, int* __restrict__ succ,
-            int* __restrict__ reserve) {
+            int* __restrict__ reserve, const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+  if (p >= dev_count(n_pairs, nd)) return;
   int nc = mf_count[p], r = rpc[p];
   int s = 0;
   if (nc > 0) {
@@ -876,10 +877,10 @@ k_emit_succ(int n_pairs, const int* __restrict__ mf_count, const int* __restrict
 }
 
 void launch_emit_succ(hipStream_t st, int n_pairs, const int* mf_count, const int* rpc,
-                      const int* dscan, int avail, int* succ, int* reserve) {
+                      const int* dscan, int avail, int* succ, int* reserve, const int* nd) {
   if (n_pairs <= 0) return;
   hipLaunchKernelGGL(k_emit_succ, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, mf_count,
-                     rpc, dscan, avail, succ, reserve);
+                     rpc, dscan, avail, succ, reserve, nd);
 }
 
 __global__ void __launch_bounds__(256)
@@ -890,9 +891,11 @@ k_emit_write(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb
              const int* __restrict__ dscan, const int* __restrict__ sscan, int n_static_rows,
              int g_static, GroupGeo* __restrict__ geo, GroupSt* __restrict__ gst, int hash_cap,
              const int* __restrict__ hkeys, const float4* __restrict__ hstate,
-             const int* __restrict__ hage) {
+             const int* __restrict__ hage, const int* __restrict__ nd,
+             const int* __restrict__ hash_cap_dev) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+  if (p >= dev_count(n_pairs, nd)) return;
+  if (hash_cap_dev) hash_cap = *hash_cap_dev;  // the table as last built
   int ns = succ[p];
   if (ns == 0) return;
   int ref = mf_refa[p] == 1 ? pa[p] : pb[p];
@@ -952,32 +955,66 @@ void launch_emit_write(hipStream_t st, int n_pairs, const int* pa, const int* pb
                        const float* bfric, float default_mu, float mult, const int* rpc,
                        const int* succ, const int* dscan, const int* sscan, int n_static_rows,
                        int g_static, GroupGeo* geo, GroupSt* gst, int hash_cap, const int* hkeys,
-                       const float4* hstate, const int* hage) {
+                       const float4* hstate, const int* hage, const int* nd,
+                       const int* hash_cap_dev) {
   if (n_pairs <= 0) return;
   hipLaunchKernelGGL(k_emit_write, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs, pa, pb,
                      mf_refa, mf_n, mf_off, bfric, default_mu, mult, rpc, succ, dscan, sscan,
-                     n_static_rows, g_static, geo, gst, hash_cap, hkeys, hstate, hage);
+                     n_static_rows, g_static, geo, gst, hash_cap, hkeys, hstate, hage, nd,
+                     hash_cap_dev);
 }
 
 // ---------------------------------------------------------------------------
 // Warm-start cache refresh
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fill_int(int* p, int n, int v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+// Fills and copies are grid-stride loops over at most kFillBlocks workgroups
+// (a table of many millions of slots needs no more to reach full bandwidth).
+constexpr int kFillBlocks = 4096;
+
+__global__ void __launch_bounds__(256) k_fill_int(int* p, int n, int v, const int* nd) {
+  n = dev_count(n, nd);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-void launch_fill_int(hipStream_t st, int* p, int n, int v) {
+void launch_fill_int(hipStream_t st, int* p, int n, int v, const int* nd) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+  const int blocks = std::min((n + 255) / 256, kFillBlocks);
+  hipLaunchKernelGGL(k_fill_int, dim3(blocks), dim3(256), 0, st, p, n, v, nd);
+}
+
+__global__ void __launch_bounds__(256)
+k_copy_int(int* __restrict__ dst, const int* __restrict__ src, int n, const int* nd) {
+  n = dev_count(n, nd);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy_int(hipStream_t st, int* dst, const int* src, int n, const int* nd) {
+  if (n <= 0) return;
+  const int blocks = std::min((n + 255) / 256, kFillBlocks);
+  hipLaunchKernelGGL(k_copy_int, dim3(blocks), dim3(256), 0, st, dst, src, n, nd);
+}
+
+// Slots of the warm-start table for `entries` live contacts: about four
+// times the entries, at least 4096, at most the reference's capacity.
+__global__ void k_hash_size(const int* entries, int hash_cap, int* live) {
+  if (threadIdx.x != 0) return;
+  const long long want = 4LL * (long long)*entries + 1;
+  const long long w = want > 4096 ? want : 4096;
+  *live = (int)(w < (long long)hash_cap ? w : (long long)hash_cap);
+}
+
+void launch_hash_size(hipStream_t st, const int* entries, int hash_cap, int* live) {
+  hipLaunchKernelGGL(k_hash_size, dim3(1), dim3(64), 0, st, entries, hash_cap, live);
 }
 
 __global__ void __launch_bounds__(256)
 k_hash_insert(int n, int g_first, const GroupGeo* __restrict__ geo,
               const GroupSt* __restrict__ gst, int hash_cap, int* hkeys, int* howner,
-              int* slot_of) {
+              int* slot_of, const int* __restrict__ nd, const int* __restrict__ hash_cap_dev) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= dev_count(n, nd)) return;
+  if (hash_cap_dev) hash_cap = *hash_cap_dev;
   slot_of[e] = -1;
   GroupSt S = gst[g_first + e];
   int bits = __float_as_int(S.s2.y);
@@ -1002,9 +1039,9 @@ __global__ void __launch_bounds__(256)
 k_hash_store(int n, int g_first, const GroupGeo* __restrict__ geo,
              const GroupSt* __restrict__ gst, const int* __restrict__ howner,
              const int* __restrict__ slot_of, float4* __restrict__ hstate,
-             int* __restrict__ hage) {
+             int* __restrict__ hage, const int* __restrict__ nd) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= dev_count(n, nd)) return;
   int slot = slot_of[e];
   if (slot < 0 || howner[slot] != e) return;
   GroupSt S = gst[g_first + e];
@@ -1054,12 +1091,12 @@ void launch_hash_row0(hipStream_t st, const GroupGeo* geo, const GroupSt* gst, i
 
 void launch_hash_collect(hipStream_t st, int n, int g_first, const GroupGeo* geo,
                          const GroupSt* gst, int hash_cap, int* hkeys, int* howner, int* slot_of,
-                         float4* hstate, int* hage) {
+                         float4* hstate, int* hage, const int* nd, const int* hash_cap_dev) {
   if (n <= 0 || hash_cap <= 0) return;
   hipLaunchKernelGGL(k_hash_insert, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
-                     hash_cap, hkeys, howner, slot_of);
+                     hash_cap, hkeys, howner, slot_of, nd, hash_cap_dev);
   hipLaunchKernelGGL(k_hash_store, dim3((n + 255) / 256), dim3(256), 0, st, n, g_first, geo, gst,
-                     howner, slot_of, hstate, hage);
+                     howner, slot_of, hstate, hage, nd);
 }
 
 }  // namespace avbd

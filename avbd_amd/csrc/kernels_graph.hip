@@ -97,10 +97,14 @@ __device__ __forceinline__ int wave_sum(int v) {
 
 __global__ void __launch_bounds__(256)
 k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* status, int* total,
-          int* err) {
+          int* err, const int* nd) {
   __shared__ unsigned s_stamp;
   __shared__ int s_carry, s_tile;
+  // with a device count, only the first ceil(n / tile) tickets scan; the
+  // others just take part in the completion count
+  n = dev_count(n, nd);
   const int nblk = gridDim.x;
+  const int nwork = (n + kScanTile - 1) / kScanTile;
   // hdr[0] changes only after every tile of this call is done, so all tiles
   // read the same value; stamps run 1 .. 2^31 - 1 (a zero workspace holds 0)
   if (threadIdx.x == 0) {
@@ -112,58 +116,62 @@ k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* sta
   }
   __syncthreads();
   const int b = s_tile;
-  const int base = b * kScanTile + threadIdx.x * 4;
-  int v[4], sum = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
-    sum += v[k];
-  }
-  int tot;
-  int pre = block_excl_scan(sum, &tot);  // its barriers also publish s_stamp
   const unsigned stamp = s_stamp;
-  if (threadIdx.x == 0) st_status(&status[b], stamp, b == 0 ? 1 : 0, tot);
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int p = b - 1; p >= 0; p -= 64) {
-      const int idx = p - lane;
-      unsigned long long w = 0ull;
-      bool ready = idx < 0;
-      int spins = 0;
-      // the loop exit is uniform over the wavefront
-      while (__builtin_amdgcn_ballot_w64(!ready) != 0) {
-        if (!ready) {
-          w = ld_status(&status[idx]);
-          ready = (unsigned)(w >> 33) == stamp;
-        }
-        if (++spins > kScanSpinCap && !ready) {
-          if (err) atomicExch(err, 1);
-          ready = true;
-        }
-      }
-      const int val = idx >= 0 ? (int)(unsigned)w : 0;
-      const unsigned long long pm =
-          __builtin_amdgcn_ballot_w64(idx >= 0 && ((w >> 32) & 1ull) != 0ull);
-      if (pm != 0ull) {
-        const int l0 = __builtin_ctzll(pm);
-        carry += wave_sum(lane <= l0 ? val : 0);
-        break;
-      }
-      carry += wave_sum(val);
-    }
-    if (lane == 0) {
-      s_carry = carry;
-      if (b > 0) st_status(&status[b], stamp, 1, carry + tot);
-      if (b == nblk - 1 && total) *total = carry + tot;
-    }
-  }
-  __syncthreads();
-  pre += s_carry;
+  if (b < nwork) {
+    const int base = b * kScanTile + threadIdx.x * 4;
+    int v[4], sum = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = pre;
-    pre += v[k];
+    for (int k = 0; k < 4; ++k) {
+      v[k] = (base + k < n) ? in[base + k] : 0;
+      sum += v[k];
+    }
+    int tot;
+    int pre = block_excl_scan(sum, &tot);
+    if (threadIdx.x == 0) st_status(&status[b], stamp, b == 0 ? 1 : 0, tot);
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      int carry = 0;
+      for (int p = b - 1; p >= 0; p -= 64) {
+        const int idx = p - lane;
+        unsigned long long w = 0ull;
+        bool ready = idx < 0;
+        int spins = 0;
+        // the loop exit is uniform over the wavefront
+        while (__builtin_amdgcn_ballot_w64(!ready) != 0) {
+          if (!ready) {
+            w = ld_status(&status[idx]);
+            ready = (unsigned)(w >> 33) == stamp;
+          }
+          if (++spins > kScanSpinCap && !ready) {
+            if (err) atomicExch(err, 1);
+            ready = true;
+          }
+        }
+        const int val = idx >= 0 ? (int)(unsigned)w : 0;
+        const unsigned long long pm =
+            __builtin_amdgcn_ballot_w64(idx >= 0 && ((w >> 32) & 1ull) != 0ull);
+        if (pm != 0ull) {
+          const int l0 = __builtin_ctzll(pm);
+          carry += wave_sum(lane <= l0 ? val : 0);
+          break;
+        }
+        carry += wave_sum(val);
+      }
+      if (lane == 0) {
+        s_carry = carry;
+        if (b > 0) st_status(&status[b], stamp, 1, carry + tot);
+        if (b == nwork - 1 && total) *total = carry + tot;
+      }
+    }
+    __syncthreads();
+    pre += s_carry;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (base + k < n) out[base + k] = pre;
+      pre += v[k];
+    }
+  } else if (b == 0 && threadIdx.x == 0 && total) {
+    *total = 0;  // nothing to scan
   }
   if (threadIdx.x == 0) {
     const unsigned done = atomicAdd(&hdr[1], 1u);
@@ -181,7 +189,7 @@ size_t scan_workspace_bytes(int n) {
 }
 
 void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_dev, void* ws,
-                    int* err) {
+                    int* err, const int* nd) {
   if (n <= 0) {
     if (total_dev) launch_fill_int(st, total_dev, 1, 0);
     return;
@@ -190,7 +198,7 @@ void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_d
   unsigned* hdr = (unsigned*)ws;
   unsigned long long* status = (unsigned long long*)((char*)ws + 16);
   hipLaunchKernelGGL(k_scan_1p, dim3(nblk), dim3(256), 0, st, in, out, n, hdr, status, total_dev,
-                     err);
+                     err, nd);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,8 +208,9 @@ constexpr int kRadixTile = 1024;
 
 __global__ void __launch_bounds__(256)
 k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, int nblk,
-             int* __restrict__ hist) {
+             int* __restrict__ hist, const int* __restrict__ nd) {
   __shared__ int cnt[16];
+  n = dev_count(n, nd);
   if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
   __syncthreads();
   int base = blockIdx.x * kRadixTile;
@@ -221,7 +230,8 @@ k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, int nblk,
 __global__ void __launch_bounds__(256)
 k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin, int n, int shift,
                 int nblk, const int* __restrict__ hist, unsigned* __restrict__ kout,
-                int* __restrict__ vout) {
+                int* __restrict__ vout, const int* __restrict__ nd) {
+  n = dev_count(n, nd);
   __shared__ int running[16];
   __shared__ int wcnt[4][16];
   __shared__ int part[2][16];  // per digit: keys in tiles before this one, keys in all tiles
@@ -297,7 +307,8 @@ size_t radix_workspace_bytes(int n) {
 }
 
 static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
-                            int* vals_tmp, int n, int n_passes, void* ws) {
+                            int* vals_tmp, int n, int n_passes, void* ws,
+                            const int* nd = nullptr) {
   if (n <= 1) return;
   int nblk = (n + kRadixTile - 1) / kRadixTile;
   int* hist = (int*)ws;  // 16 x nblk, digit-major
@@ -307,9 +318,9 @@ static void radix_sort_bits(hipStream_t st, unsigned* keys, int* vals, unsigned*
   int* vb = vals_tmp;
   for (int pass = 0; pass < n_passes; ++pass) {
     int shift = 4 * pass;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, nblk, hist);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, nblk, hist, nd);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(256), 0, st, ka, va, n, shift, nblk,
-                       hist, kb, vb);
+                       hist, kb, vb, nd);
     unsigned* tk = ka; ka = kb; kb = tk;
     int* tv = va; va = vb; vb = tv;
   }
@@ -326,8 +337,8 @@ void radix_sort_pairs(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_
 }
 
 void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
-                      int* vals_tmp, int n, int key_bits, void* ws) {
-  radix_sort_bits(st, keys, vals, keys_tmp, vals_tmp, n, (key_bits + 3) / 4, ws);
+                      int* vals_tmp, int n, int key_bits, void* ws, const int* nd) {
+  radix_sort_bits(st, keys, vals, keys_tmp, vals_tmp, n, (key_bits + 3) / 4, ws, nd);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,8 +360,10 @@ void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_
 __global__ void __launch_bounds__(256)
 k_tcsr_pairs(int kept, const int* __restrict__ pa, const int* __restrict__ pb,
              const int* __restrict__ succ, int* __restrict__ asb_cnt,
-             int* __restrict__ asb_groups, int* __restrict__ start_a, int* __restrict__ end_a) {
+             int* __restrict__ asb_groups, int* __restrict__ start_a, int* __restrict__ end_a,
+             const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
+  kept = dev_count(kept, nd);
   if (p >= kept) return;
   const int a = pa[p], b = pb[p];
   if (p == 0 || pa[p - 1] != a) start_a[a] = p;
@@ -365,9 +378,9 @@ k_tcsr_pairs(int kept, const int* __restrict__ pa, const int* __restrict__ pb,
 __global__ void __launch_bounds__(256)
 k_tcsr_asb_fill(int kept, const int* __restrict__ pb, const int* __restrict__ succ,
                 const int* __restrict__ asb_start, int* __restrict__ asb_cur,
-                int* __restrict__ asb_list) {
+                int* __restrict__ asb_list, const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kept || succ[p] <= 0) return;
+  if (p >= dev_count(kept, nd) || succ[p] <= 0) return;
   const int b = pb[p];
   asb_list[asb_start[b] + atomicAdd(&asb_cur[b], 1)] = p;
 }
@@ -441,17 +454,17 @@ k_tcsr_fill(int nb, const int* __restrict__ cvals, const int* __restrict__ st_st
 }
 
 void launch_tcsr_pairs(hipStream_t st, int kept, const int* pa, const int* pb, const int* succ,
-                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a) {
+                       int* asb_cnt, int* asb_groups, int* start_a, int* end_a, const int* nd) {
   if (kept > 0)
     hipLaunchKernelGGL(k_tcsr_pairs, dim3((kept + 255) / 256), dim3(256), 0, st, kept, pa, pb, succ,
-                       asb_cnt, asb_groups, start_a, end_a);
+                       asb_cnt, asb_groups, start_a, end_a, nd);
 }
 
 void launch_tcsr_asb_fill(hipStream_t st, int kept, const int* pb, const int* succ,
-                          const int* asb_start, int* asb_cur, int* asb_list) {
+                          const int* asb_start, int* asb_cur, int* asb_list, const int* nd) {
   if (kept > 0)
     hipLaunchKernelGGL(k_tcsr_asb_fill, dim3((kept + 255) / 256), dim3(256), 0, st, kept, pb, succ,
-                       asb_start, asb_cur, asb_list);
+                       asb_start, asb_cur, asb_list, nd);
 }
 
 void launch_tcsr_count(hipStream_t st, int nb, const int* cvals, const int* st_count,
@@ -498,8 +511,9 @@ __global__ void __launch_bounds__(256)
 k_count_conflicts(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
                   const int* __restrict__ succ, const int* __restrict__ rpc,
                   const int* __restrict__ color, int* out, const int* __restrict__ scene,
-                  int* __restrict__ scene_dirty) {
+                  int* __restrict__ scene_dirty, const int* __restrict__ nd) {
   __shared__ int wsum[4];
+  n_pairs = dev_count(n_pairs, nd);
   int c = 0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     const int ns = succ[p];
@@ -522,28 +536,28 @@ k_count_conflicts(int n_pairs, const int* __restrict__ pa, const int* __restrict
 
 void launch_count_conflicts(hipStream_t st, int n_pairs, const int* pa, const int* pb,
                             const int* succ, const int* rpc, const int* color, int* out,
-                            const int* scene, int* scene_dirty) {
+                            const int* scene, int* scene_dirty, const int* nd) {
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_count_conflicts,
                        dim3(std::min((n_pairs + 255) / 256, kConflictBlocks)), dim3(256), 0, st,
-                       n_pairs, pa, pb, succ, rpc, color, out, scene, scene_dirty);
+                       n_pairs, pa, pb, succ, rpc, color, out, scene, scene_dirty, nd);
 }
 
 // Start of a recolor: every body uncolored, or (batched scenes) only the
 // bodies of scenes marked dirty; the others keep their colors.
 __global__ void __launch_bounds__(256)
 k_color_reset(int nb, int* __restrict__ color, const int* __restrict__ scene,
-              const int* __restrict__ scene_dirty) {
+              const int* __restrict__ scene_dirty, const int* __restrict__ nd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= dev_count(nb, nd)) return;
   if (scene_dirty == nullptr || scene_dirty[scene[i]]) color[i] = -1;
 }
 
 void launch_color_reset(hipStream_t st, int nb, int* color, const int* scene,
-                        const int* scene_dirty) {
+                        const int* scene_dirty, const int* nd) {
   if (nb > 0)
     hipLaunchKernelGGL(k_color_reset, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, scene,
-                       scene_dirty);
+                       scene_dirty, nd);
 }
 
 __global__ void __launch_bounds__(256)
@@ -598,9 +612,9 @@ __device__ __forceinline__ void st_relaxed(int* p, int v) {
 // describe, without the per-contact-point duplicates).
 __global__ void __launch_bounds__(256)
 k_adj_count_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
-                  const int* __restrict__ succ, int* counts) {
+                  const int* __restrict__ succ, int* counts, const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs || succ[p] <= 0) return;
+  if (p >= dev_count(n_pairs, nd) || succ[p] <= 0) return;
   int a = pa[p], b = pb[p];
   if (a < 0 || b < 0 || a == b) return;
   atomicAdd(&counts[a], 1);
@@ -610,9 +624,9 @@ k_adj_count_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict
 __global__ void __launch_bounds__(256)
 k_adj_fill_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict__ pb,
                  const int* __restrict__ succ, const int* __restrict__ starts, int* cursor,
-                 int* nbr) {
+                 int* nbr, const int* __restrict__ nd) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs || succ[p] <= 0) return;
+  if (p >= dev_count(n_pairs, nd) || succ[p] <= 0) return;
   int a = pa[p], b = pb[p];
   if (a < 0 || b < 0 || a == b) return;
   nbr[starts[a] + atomicAdd(&cursor[a], 1)] = b;
@@ -620,17 +634,18 @@ k_adj_fill_pairs(int n_pairs, const int* __restrict__ pa, const int* __restrict_
 }
 
 void launch_adj_count_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
-                            const int* succ, int* counts) {
+                            const int* succ, int* counts, const int* nd) {
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_adj_count_pairs, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
-                       pa, pb, succ, counts);
+                       pa, pb, succ, counts, nd);
 }
 
 void launch_adj_fill_pairs(hipStream_t st, int n_pairs, const int* pa, const int* pb,
-                           const int* succ, const int* starts, int* cursor, int* nbr) {
+                           const int* succ, const int* starts, int* cursor, int* nbr,
+                           const int* nd) {
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_adj_fill_pairs, dim3((n_pairs + 255) / 256), dim3(256), 0, st, n_pairs,
-                       pa, pb, succ, starts, cursor, nbr);
+                       pa, pb, succ, starts, cursor, nbr, nd);
 }
 
 // Greedy coloring in a fixed processing order, computed as a dataflow: the
@@ -662,7 +677,9 @@ __global__ void __launch_bounds__(64 * W)
 k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ pos,
                const int* __restrict__ nbr_start, const int* __restrict__ nbr,
                const int* __restrict__ color_in, int* pub, int* ticket, int mode_jp, int* err,
-               int* clamped, const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
+               int* clamped, const int* __restrict__ scene, const int* __restrict__ scene_dirty,
+               const int* __restrict__ nd) {
+  nb = dev_count(nb, nd);
   // kT = 64 W positions per ticket. nj_s: a neighbour j to poll, or ~slot
   // when j is this ticket's body of that slot; lpub: this ticket's
   // published values, by slot (accessed with workgroup-scope relaxed
@@ -830,9 +847,10 @@ k_color_greedy(int nb, const int* __restrict__ order, const int* __restrict__ po
 
 __global__ void __launch_bounds__(256)
 k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp,
-              const int* __restrict__ scene, const int* __restrict__ scene_dirty) {
+              const int* __restrict__ scene, const int* __restrict__ scene_dirty,
+              const int* __restrict__ nd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= dev_count(nb, nd)) return;
   if (scene_dirty != nullptr && !scene_dirty[scene[i]]) return;
   if (mode_jp) color[i] = (pub[i] & 255) - 1;
   else if (color[i] == -1) color[i] = pub[i];
@@ -841,9 +859,9 @@ k_color_merge(int nb, int* color, const int* __restrict__ pub, int mode_jp,
 void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* pos,
                          const int* nbr_start, const int* nbr, int* color_in_out, int* pub,
                          int* err, int* ticket, int mode_jp, int* clamped, const int* scene,
-                         const int* scene_dirty) {
+                         const int* scene_dirty, const int* nd) {
   if (nb <= 0) return;
-  launch_fill_int(st, pub, nb, -1);
+  launch_fill_int(st, pub, nb, -1, nd);
   launch_fill_int(st, ticket, 1, 0);
   // Workgroups of 16 wavefronts (tickets of 1024 positions, so more of a
   // chain of consecutive bodies stays inside one ticket), enough of them to
@@ -852,17 +870,17 @@ void launch_color_greedy(hipStream_t st, int nb, const int* order, const int* po
   const int blocks = std::min((nb + 64 * kW - 1) / (64 * kW), 4096 / kW);
   hipLaunchKernelGGL(k_color_greedy<kW>, dim3(blocks), dim3(64 * kW), 0, st, nb, order, pos,
                      nbr_start, nbr, color_in_out, pub, ticket, mode_jp, err, clamped, scene,
-                     scene_dirty);
+                     scene_dirty, nd);
   hipLaunchKernelGGL(k_color_merge, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color_in_out, pub,
-                     mode_jp, scene, scene_dirty);
+                     mode_jp, scene, scene_dirty, nd);
 }
 
 __global__ void __launch_bounds__(256)
 k_color_resolve(int nb, const int* __restrict__ nbr_start, const int* __restrict__ nbr,
                 const float* __restrict__ prio, const int* __restrict__ color,
-                int* __restrict__ color_next, int* uncolored) {
+                int* __restrict__ color_next, int* uncolored, const int* __restrict__ nd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= dev_count(nb, nd)) return;
   int ci = color[i];
   int out = ci;
   if (ci >= 0) {
@@ -880,16 +898,127 @@ k_color_resolve(int nb, const int* __restrict__ nbr_start, const int* __restrict
 }
 
 void launch_color_resolve(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
-                          const float* prio, const int* color, int* color_next, int* uncolored) {
+                          const float* prio, const int* color, int* color_next, int* uncolored,
+                          const int* nd) {
   if (nb > 0)
     hipLaunchKernelGGL(k_color_resolve, dim3((nb + 255) / 256), dim3(256), 0, st, nb, nbr_start, nbr,
-                       prio, color, color_next, uncolored);
+                       prio, color, color_next, uncolored, nd);
+}
+
+// Jacobi rounds after the first, for a substep whose decisions stay on the
+// device: the host-driven recolor repeats greedy + resolve until no body is
+// uncolored (at most MAX_COLORS rounds, solver_6dof.py:795-844), reading the
+// uncolored count back each round. Round 1 leaves bodies uncolored only when
+// some body saw all 32 colors among its neighbours, which no scene here
+// reaches, so the later rounds run on one workgroup: the bodies still
+// uncolored are gathered in index order and colored by one thread exactly as
+// the serial greedy does (lowest color not used by a colored neighbour, in
+// place), then resolved as k_color_resolve does.
+__global__ void __launch_bounds__(1024)
+k_color_jacobi_tail(int nb, const int* __restrict__ nbr_start, const int* __restrict__ nbr,
+                    const float* __restrict__ prio, int* color, int* color_next, int* uncolored,
+                    int* clamped, const int* __restrict__ gate) {
+  __shared__ int s_list[1024];
+  __shared__ int s_wcnt[16];
+  __shared__ int s_unc;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if ((gate != nullptr && *gate == 0) || *uncolored == 0) return;
+  for (int round = 1; round < kMaxColors; ++round) {
+    // assign, in index order, in chunks of 1024 bodies
+    for (int base = 0; base < nb; base += 1024) {
+      const int i = base + tid;
+      const bool u = i < nb && color[i] == -1;
+      const unsigned long long m = __ballot(u);
+      if (lane == 0) s_wcnt[wid] = __popcll(m);
+      __syncthreads();
+      int off = 0, tot = 0;
+      for (int w = 0; w < 16; ++w) {
+        if (w < wid) off += s_wcnt[w];
+        tot += s_wcnt[w];
+      }
+      if (u) s_list[off + __popcll(m & ((1ull << lane) - 1ull))] = i;
+      __syncthreads();
+      if (tid == 0) {
+        for (int k = 0; k < tot; ++k) {
+          const int b = s_list[k];
+          unsigned used = 0u;
+          for (int q = nbr_start[b]; q < nbr_start[b + 1]; ++q) {
+            const int cj = color[nbr[q]];
+            if (cj >= 0 && cj < kMaxColors) used |= 1u << cj;
+          }
+          if (used == 0xffffffffu) atomicOr(clamped, 1);
+          color[b] = lowest_free_color(used);
+        }
+      }
+      __syncthreads();
+    }
+    // resolve into color_next, then back into color
+    if (tid == 0) s_unc = 0;
+    __syncthreads();
+    for (int i = tid; i < nb; i += 1024) {
+      const int ci = color[i];
+      int out = ci;
+      if (ci >= 0) {
+        const float mp = prio[i];
+        for (int q = nbr_start[i]; q < nbr_start[i + 1]; ++q) {
+          const int j = nbr[q];
+          if (color[j] == ci && (prio[j] > mp || (prio[j] == mp && j < i))) {
+            out = -1;
+            break;
+          }
+        }
+      }
+      color_next[i] = out;
+      if (out == -1) atomicAdd(&s_unc, 1);
+    }
+    __syncthreads();
+    for (int i = tid; i < nb; i += 1024) color[i] = color_next[i];
+    __syncthreads();
+    if (s_unc == 0) break;
+  }
+  if (tid == 0) *uncolored = s_unc;
+}
+
+void launch_color_jacobi_tail(hipStream_t st, int nb, const int* nbr_start, const int* nbr,
+                              const float* prio, int* color, int* color_next, int* uncolored,
+                              int* clamped, const int* gate) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_color_jacobi_tail, dim3(1), dim3(1024), 0, st, nb, nbr_start, nbr, prio,
+                       color, color_next, uncolored, clamped, gate);
+}
+
+// Color starts (kMaxColors + 1), the achieved color count (the highest
+// non-empty color + 1, at least 1) and the first tile of each color for
+// tiles of `bpw` bodies, from the per-color body counts.
+__global__ void k_color_meta(const int* __restrict__ counts, int bpw, int* cstart, int* n_active,
+                             int* tile_start) {
+  if (threadIdx.x != 0) return;
+  int c0 = 0, t = 0, mx = 0;
+  for (int c = 0; c < kMaxColors; ++c) {
+    const int n = counts[c];
+    cstart[c] = c0;
+    tile_start[c] = t;
+    c0 += n;
+    t += (n + bpw - 1) / bpw;
+    if (n > 0) mx = c;
+  }
+  cstart[kMaxColors] = c0;
+  tile_start[kMaxColors] = t;
+  *n_active = mx + 1;
+}
+
+void launch_color_meta(hipStream_t st, const int* counts, int bpw, int* cstart, int* n_active,
+                       int* tile_start) {
+  hipLaunchKernelGGL(k_color_meta, dim3(1), dim3(64), 0, st, counts, bpw, cstart, n_active,
+                     tile_start);
 }
 
 // Uncolored bodies (-1) take color 0; per-color body counts go through a
 // block histogram in LDS, then one global atomic per color per block.
-__global__ void __launch_bounds__(256) k_color_finish(int nb, int* color, int* counts) {
+__global__ void __launch_bounds__(256)
+k_color_finish(int nb, int* color, int* counts, const int* __restrict__ nd) {
   __shared__ int h[kMaxColors];
+  nb = dev_count(nb, nd);
   if (threadIdx.x < kMaxColors) h[threadIdx.x] = 0;
   __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -902,30 +1031,33 @@ __global__ void __launch_bounds__(256) k_color_finish(int nb, int* color, int* c
   if (threadIdx.x < kMaxColors && h[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
-void launch_color_finish(hipStream_t st, int nb, int* color, int* counts) {
-  launch_fill_int(st, counts, kMaxColors, 0);
+void launch_color_finish(hipStream_t st, int nb, int* color, int* counts, const int* gate) {
+  launch_fill_int(st, counts, kMaxColors, 0, gate);
   if (nb > 0)
-    hipLaunchKernelGGL(k_color_finish, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, counts);
+    hipLaunchKernelGGL(k_color_finish, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, counts,
+                       gate);
 }
 
 __global__ void __launch_bounds__(256)
-k_color_keys(int nb, const int* __restrict__ color, unsigned* keys, int* vals) {
+k_color_keys(int nb, const int* __restrict__ color, unsigned* keys, int* vals,
+             const int* __restrict__ nd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= dev_count(nb, nd)) return;
   keys[i] = (unsigned)color[i];
   vals[i] = i;
 }
 
 void radix_sort_small(hipStream_t st, unsigned* keys, int* vals, unsigned* keys_tmp,
-                      int* vals_tmp, int n, int key_bits, void* ws);
+                      int* vals_tmp, int n, int key_bits, void* ws, const int* nd);
 
 // color_bodies: bodies bucketed by color, ascending within a color
 // (gpu_color_bodies_scatter, kernels_6dof.py:3182-3199, in serial order).
 void color_bucket(hipStream_t st, int nb, const int* color, unsigned* keys, int* vals,
-                  unsigned* keys_tmp, int* vals_tmp, void* ws) {
+                  unsigned* keys_tmp, int* vals_tmp, void* ws, const int* gate) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_color_keys, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, keys, vals);
-  radix_sort_small(st, keys, vals, keys_tmp, vals_tmp, nb, 8, ws);
+  hipLaunchKernelGGL(k_color_keys, dim3((nb + 255) / 256), dim3(256), 0, st, nb, color, keys, vals,
+                     gate);
+  radix_sort_small(st, keys, vals, keys_tmp, vals_tmp, nb, 8, ws, gate);
 }
 
 }  // namespace avbd

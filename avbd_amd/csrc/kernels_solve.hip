@@ -184,29 +184,39 @@ k_substep_begin(int nb, int n_groups, const float4* __restrict__ pose_cur,
                 const float4* __restrict__ pvel, const float4* __restrict__ bstat,
                 const float* __restrict__ Ilocal, GroupGeo* __restrict__ geo,
                 GroupSt* __restrict__ gst, float dt, float gx, float gy, float gz, float alpha,
-                float gamma, int post, int do_prelude, unsigned* __restrict__ epoch) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid == 0 && epoch) *epoch += 1u;  // stamps of this substep's sweeps
-  if (tid < nb) {
-    predict_body(tid, pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, dt,
-                 mk3(gx, gy, gz));
-    return;
+                float gamma, int post, int do_prelude, unsigned* __restrict__ epoch,
+                const int* __restrict__ nd_groups) {
+  int tid0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid0 == 0 && epoch) *epoch += 1u;  // stamps of this substep's sweeps
+  n_groups = dev_count(n_groups, nd_groups);
+  const int total = nb + (do_prelude ? n_groups : 0);
+  // grid-stride: a launch sized by capacity runs a bounded grid
+  for (int tid = tid0; tid < total; tid += gridDim.x * blockDim.x) {
+    if (tid < nb)
+      predict_body(tid, pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, dt,
+                   mk3(gx, gy, gz));
+    else
+      prelude_group(tid - nb, pose_cur, geo, gst, alpha, gamma, post);
   }
-  int g = tid - nb;
-  if (!do_prelude || g >= n_groups) return;
-  prelude_group(g, pose_cur, geo, gst, alpha, gamma, post);
 }
+
+// Workgroups of a launch sized by capacity with a device count (each thread
+// strides over the elements).
+constexpr int kCapBlocks = 8192;
 
 void launch_substep_begin(hipStream_t st, int nb, int n_groups, float4* pose_cur,
                           float4* pose_work, float4* poseI, float4* inert, const float4* vel,
                           const float4* pvel, const float4* bstat, const float* Ilocal,
                           GroupGeo* geo, GroupSt* gst, float dt, float gx, float gy, float gz,
-                          float alpha, float gamma, int post, int do_prelude, unsigned* epoch) {
+                          float alpha, float gamma, int post, int do_prelude, unsigned* epoch,
+                          const int* nd_groups) {
   int total = nb + (do_prelude ? n_groups : 0);
   if (total <= 0) return;
-  hipLaunchKernelGGL(k_substep_begin, dim3((total + 255) / 256), dim3(256), 0, st, nb, n_groups,
+  int blocks = (total + 255) / 256;
+  if (nd_groups) blocks = std::min(blocks, kCapBlocks);
+  hipLaunchKernelGGL(k_substep_begin, dim3(blocks), dim3(256), 0, st, nb, n_groups,
                      pose_cur, pose_work, poseI, inert, vel, pvel, bstat, Ilocal, geo, gst, dt,
-                     gx, gy, gz, alpha, gamma, post, do_prelude, epoch);
+                     gx, gy, gz, alpha, gamma, post, do_prelude, epoch, nd_groups);
 }
 
 // ---------------------------------------------------------------------------
@@ -646,6 +656,13 @@ struct SweepFlow {
   // a small cap, to run the give-up path without a hang.
   const int* done_rd;
   int spin_cap;
+  // Color layout read from the device (graph-captured substeps): when
+  // meta_ncolors is set, the color count, color starts and first tiles come
+  // from these arrays instead of the fields above, and workgroups past the
+  // last tile exit.
+  const int* meta_ncolors;
+  const int* meta_cstart;
+  const int* meta_tiles;
   // Dual update fused into the sweep: each group is updated by the tile of
   // its higher-color body (the body itself for floor and pin groups), after
   // that tile's solve, when both bodies' poses of this sweep are final and
@@ -711,7 +728,16 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     return;
   }
   int count = count_arg, base = base_arg, first = blockIdx.x * BPW, my_color = 0;
-  if (kFlow) {
+  if (kFlow && fl.meta_ncolors != nullptr) {
+    const int t = blockIdx.x, nc = *fl.meta_ncolors;
+    if (t >= fl.meta_tiles[nc]) return;
+    int c = 0;
+    while (c + 1 < nc && t >= fl.meta_tiles[c + 1]) ++c;
+    my_color = c;
+    base = fl.meta_cstart[c];
+    count = fl.meta_cstart[c + 1] - base;
+    first = (t - fl.meta_tiles[c]) * BPW;
+  } else if (kFlow) {
     const int t = blockIdx.x;
     int c = 0;
     while (c + 1 < fl.n_colors && t >= fl.tile_start[c + 1]) ++c;
@@ -993,8 +1019,12 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const GroupGeo* geo, const GroupSt* gst, float dt, int post_pass,
                          int* done, const unsigned* epoch, int sweep, int* err, int bpw,
                          int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
-                         const int* done_rd, int spin_cap) {
+                         const int* done_rd, int spin_cap, const int* meta_ncolors,
+                         const int* meta_cstart, const int* meta_tiles, int max_tiles) {
   SweepFlow fl{};
+  fl.meta_ncolors = meta_ncolors;
+  fl.meta_cstart = meta_cstart;
+  fl.meta_tiles = meta_tiles;
   fl.done_rd = done_rd ? done_rd : done;
   fl.spin_cap = spin_cap > 0 ? spin_cap : kFlowSpinCap;
   fl.fuse_dual = fuse_dual;
@@ -1016,6 +1046,11 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   fl.epoch = epoch;
   fl.sweep = sweep;
   fl.err = err;
+  if (meta_ncolors != nullptr) {
+    // the layout is on the device: launch the most tiles any coloring has
+    t = max_tiles;
+    fl.n_tiles = t;
+  }
   if (t <= 0) return;
   const int grid = t + (tail_groups + 63) / 64;
   if (bpw == 1)
@@ -1073,26 +1108,29 @@ void launch_primal_color(hipStream_t st, int count, const int* color_bodies, int
 
 __global__ void __launch_bounds__(256)
 k_dual(int n_groups, const float4* __restrict__ pose, const GroupGeo* __restrict__ geo,
-       GroupSt* __restrict__ gst, float beta, float* resid) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_groups) return;
-  const GroupGeo G = geo[g];
-  const int kind = g_kind(G.hdr.x);
-  const int a = G.hdr.y, b = G.hdr.z;
-  const float4 xa = pose[2 * a], qa = pose[2 * a + 1];
-  float4 xb = xa, qb = qa;
-  if (kind == kGBox) {
-    xb = pose[2 * b];
-    qb = pose[2 * b + 1];
+       GroupSt* __restrict__ gst, float beta, float* resid, const int* __restrict__ nd) {
+  const int n = dev_count(n_groups, nd);
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    const GroupGeo G = geo[g];
+    const int kind = g_kind(G.hdr.x);
+    const int a = G.hdr.y, b = G.hdr.z;
+    const float4 xa = pose[2 * a], qa = pose[2 * a + 1];
+    float4 xb = xa, qb = qa;
+    if (kind == kGBox) {
+      xb = pose[2 * b];
+      qb = pose[2 * b + 1];
+    }
+    dual_group(G, gst[g], gst + g, xa, qa, xb, qb, beta, resid);
   }
-  dual_group(G, gst[g], gst + g, xa, qa, xb, qb, beta, resid);
 }
 
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
-                 GroupSt* gst, float beta, float* resid) {
+                 GroupSt* gst, float beta, float* resid, const int* nd) {
   if (n_groups <= 0) return;
-  hipLaunchKernelGGL(k_dual, dim3((n_groups + 255) / 256), dim3(256), 0, st, n_groups, pose, geo,
-                     gst, beta, resid);
+  int blocks = (n_groups + 255) / 256;
+  if (nd) blocks = std::min(blocks, kCapBlocks);
+  hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(256), 0, st, n_groups, pose, geo, gst, beta, resid,
+                     nd);
 }
 
 // ---------------------------------------------------------------------------

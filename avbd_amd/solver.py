@@ -116,7 +116,7 @@ class Solver:
     def __init__(self, dt: float = 1.0 / 60.0, iterations: int = 10,
                  gravity: tuple[float, float, float] = (0.0, -9.81, 0.0),
                  alpha: float = 0.99, beta: float = 1.0e5, gamma: float = 0.99,
-                 post_stabilize: bool = True, device: str = "cuda:0",
+                 post_stabilize: bool = True, device: str = "cpu",
                  max_speed: float = 30.0):
         self._device_index = _parse_device(device)
         self.device = device

@@ -141,11 +141,14 @@ def _snap_group_size(g: int) -> int:
 
 
 def _parse_device(device) -> int:
+    """HIP device index of a device string. The substep always runs on a HIP
+    device: "cpu" (the reference's default) selects device 0 and keeps the
+    reference's CPU-device behaviour (growing pools, a readback per substep);
+    "cuda[:N]" / "hip[:N]" / "gpu[:N]" select device N with the CUDA-device
+    behaviour (the resident, fixed-capacity path when gpu_resident is set)."""
     d = str(device).strip().lower()
-    if d in ("cpu",) or d.startswith("cpu:"):
-        raise ValueError(
-            "avbd_amd runs the Solver6DOF substep on an AMD Instinct GPU; device='cpu' has no "
-            "implementation here. Use device='cuda' / 'cuda:N' (HIP device N).")
+    if d == "cpu" or d.startswith("cpu:"):
+        return 0
     for pre in ("cuda", "hip", "gpu"):
         if d == pre:
             return 0
@@ -191,7 +194,7 @@ class Solver6DOF:
         beta: float = 1.0e5,
         gamma: float = 0.99,
         post_stabilize: bool = True,
-        device: str = "cuda:0",
+        device: str = "cpu",
         max_linear_speed: float = 30.0,
         max_angular_speed: float = 50.0,
         substeps: int = 1,
@@ -214,6 +217,9 @@ class Solver6DOF:
         self._recolor_every_substep = bool(recolor_every_substep)
         self._resident_recolor_rounds = 6
         self._unsafe_fixed_capacity = bool(unsafe_fixed_capacity)
+        # not in the reference: fixed-capacity frames replay a captured HIP
+        # graph for their substeps after the first (False: launch directly)
+        self.hip_graph = True
         self._coloring_mode = None
         self.coloring_mode = coloring_mode
         self.dt = float(dt)
@@ -244,6 +250,7 @@ class Solver6DOF:
         self._self_friction = 0.0
 
         self._h = None
+        self._h_graph_state = None
         self.x = self.q = self.v = self.omega = None
         self.prev_v = self.prev_omega = None
         self.body_color = None
@@ -393,9 +400,19 @@ class Solver6DOF:
         self._recolor_every_substep = bool(on)
 
     def _resident_on(self, dev) -> bool:
-        """The stale-coloring double-buffered path of the reference's CUDA
-        device is not used: the substep always runs the conflict-checked path."""
-        return False
+        """Whether the readback-free resident loop runs (solver_6dof.py:663-672):
+        a GPU device string, gpu_resident, the fused shuffled group primal.
+        It implies fixed-capacity pools, and each frame runs as one call whose
+        substeps after the first replay a captured HIP graph, with one counter
+        readback per frame. Unlike the reference's double-buffered resident
+        primal, coloring stays conflict-checked on the device, so the resident
+        loop gives the same bits as the per-substep path."""
+        d = str(dev).strip().lower()
+        return (self._gpu_resident
+                and d.startswith(("cuda", "hip", "gpu"))
+                and self._primal_shuffle
+                and self._primal_fused
+                and self._primal_group_size > 1)
 
     def _fixed_capacity_mode(self, dev) -> bool:
         return self._unsafe_fixed_capacity or self._resident_on(dev)
@@ -615,6 +632,10 @@ class Solver6DOF:
     # ---- step (solver_6dof.py:1178-1464) --------------------------------------
     def step(self) -> None:
         fixed_cap = self._fixed_capacity_mode(self.device)
+        if fixed_cap:
+            self._step_frame()
+            self._check_fixed_cap_overflow()
+            return
         if self.substeps <= 1:
             self._step_one()
             if fixed_cap:
@@ -738,6 +759,23 @@ class Solver6DOF:
                 "Increase the pair budget or disable gpu_resident/unsafe_fixed_capacity.",
                 RuntimeWarning, stacklevel=2)
             self._pair_overflow_warned = True
+
+    def _step_frame(self) -> None:
+        """All substeps of a frame in one library call (avbd_step_frame): in
+        fixed-capacity mode no host decision depends on a count read back
+        within the frame."""
+        if self._dirty:
+            self._flush()
+        n_b = len(self._x)
+        if n_b == 0:
+            return
+        want = (id(self._h), bool(self.hip_graph))
+        if self._h_graph_state != want:
+            self._h.set_option(N.OPT_GRAPH, int(want[1]))
+            self._h_graph_state = want
+        sub_dt = self.dt / self.substeps if self.substeps > 1 else self.dt
+        rep = self._h.step_frame(self._params(sub_dt), self.substeps, self._color_dirty)
+        self._apply_report(rep, n_b)
 
     def _step_one(self) -> None:
         if self._dirty:

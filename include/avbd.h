@@ -141,6 +141,24 @@ int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_rep
 int avbd_substep_to_residual(avbd_solver* s, const avbd_params* p, int color_dirty, float tol,
                              int* iterations_used, float* residual, avbd_report* out);
 
+/* Solver6DOF.step (solver_6dof.py:1178-1230): n_substeps substeps of
+ * p->dt each. With fixed_capacity set and AVBD_OPT_GRAPH on, every substep
+ * after the first one of a dirty coloring keeps its counts and decisions on
+ * the device (launches sized by capacity, the recolor gated on the device)
+ * and runs as a captured HIP graph, replayed while the scene layout and the
+ * parameters stay the same; the counters are read back once, at the end of
+ * the frame (the reference's once-per-frame overflow check, :1199-1230).
+ * Results are bit-identical to n_substeps avbd_substep calls. *out reports
+ * the frame: counts of its last substep, recolored = any substep recolored. */
+int avbd_step_frame(avbd_solver* s, const avbd_params* p, int n_substeps, int color_dirty,
+                    avbd_report* out);
+/* Device -> host synchronisations of the solver's counters since creation
+ * (readbacks inside substeps and at frame ends; reads requested through the
+ * read entry points are not counted) and substep graph launches. No
+ * counterpart in the reference (its tests count .numpy() calls,
+ * tests/test_robustness.py:335-395). */
+int avbd_sync_count(avbd_solver* s, long long* syncs, long long* graph_launches);
+
 /* ---- readback / write-back ------------------------------------------ */
 /* positions()/orientations()/velocities()/angular_velocities()
  * (solver_6dof.py:1920-1938); any pointer may be NULL. */
@@ -222,6 +240,11 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * readback, avbd_read_bodies, avbd_read_rows) returns AVBD_E_STATE, and the
  * flag stays set until the next avbd_flush. */
 #define AVBD_OPT_FLOW_GIVE_UP_TEST 3
+/* AVBD_OPT_GRAPH: 1 = avbd_step_frame runs fixed-capacity substeps as a
+ * captured HIP graph (see avbd_step_frame); 0 (default) = direct launches
+ * with the per-substep readbacks. No counterpart in the reference's API (its
+ * CUDA path captures the iteration loop, solver_6dof.py:1419-1447). */
+#define AVBD_OPT_GRAPH 4
 int avbd_set_option(avbd_solver* s, int option, int value);
 
 /* ---- profiling ------------------------------------------------------- */

@@ -36,7 +36,9 @@ def load() -> C.CDLL:
                                                                         "pack_state",
                                                                         "set_option",
                                                                         "profile_enable",
-                                                                        "profile_read")
+                                                                        "profile_read",
+                                                                        "step_frame",
+                                                                        "sync_count")
                                    and k not in N.ML_NAMES and k not in N.P_GPU_ONLY])
         lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.orc_create.restype = C.c_int
@@ -92,6 +94,19 @@ class OracleHandle(N.NativeHandle):
 
     def set_option(self, option, value):
         pass
+
+    def step_frame(self, params, n_substeps, color_dirty):
+        """A frame as consecutive substeps, the first one taking the dirty
+        coloring flag (the library's avbd_step_frame contract)."""
+        rep, recolored = None, 0
+        for k in range(max(1, int(n_substeps))):
+            rep = self.substep(params, bool(color_dirty) and k == 0)
+            recolored |= rep.recolored
+        rep.recolored = recolored
+        return rep
+
+    def sync_count(self):
+        return 0, 0
 
 
 def oracle_solver_class():

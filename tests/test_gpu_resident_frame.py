@@ -1,0 +1,113 @@
+"""The resident frame (solver_6dof.py:663-678, 1178-1230) on the HIP library.
+
+On a GPU device string with gpu_resident set, pools are fixed at flush and a
+frame's substeps run in one library call; the substeps after the first replay
+a captured HIP graph whose counts (pairs, rows, groups, colors, hash size)
+stay on the device, and the host reads one counter block per frame.
+
+* every frame is bit-identical to the checker stepped the same way and to the
+  library with the graph switched off (pyramid, pile with new contacts and
+  recolors, Jones-Plassmann stack, a batch of scenes);
+* a steady-state frame synchronises with the host at most twice
+  (tests/test_robustness.py:335-395: "<= 2 host readbacks per frame").
+"""
+import numpy as np
+import pytest
+
+from avbd_amd import scenes
+from avbd_amd.batched import BatchedSolver6DOF
+from avbd_amd.solver_6dof import Solver6DOF
+from oracle.oracle import oracle_solver_class
+from tests.parity import assert_bit_exact, diff_report
+
+pytestmark = pytest.mark.gpu
+
+
+def _jp(s):
+    s.coloring_mode = "jones_plassmann"
+    return s
+
+
+def _batch(solver_cls, **kw):
+    # the checker steps one scene per solver: a batch is compared with the
+    # library's own direct launches only (each scene of a batch against the
+    # scene alone is covered by the batched tests)
+    if solver_cls is not Solver6DOF:
+        return None
+    parts = [scenes.random_pile(n, seed=40 + k, lattice=(5, 5, 4), iterations=6,
+                                solver_cls=Solver6DOF, **kw)
+             for k, n in enumerate((9, 40, 17, 64))]
+    return BatchedSolver6DOF(parts)
+
+
+SCENES = {
+    "pyramid": lambda solver_cls, **kw: scenes.pyramid(5, seed=3, iterations=8,
+                                                       solver_cls=solver_cls, **kw),
+    "pile": lambda solver_cls, **kw: scenes.random_pile(120, seed=5, lattice=(6, 6, 4),
+                                                        iterations=6, solver_cls=solver_cls,
+                                                        **kw),
+    "stack_jp": lambda solver_cls, **kw: _jp(scenes.cube_stack(6, seed=2, iterations=8,
+                                                               solver_cls=solver_cls, **kw)),
+    "batch": _batch,
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_resident_frames_match_checker_and_direct(name):
+    build = SCENES[name]
+    g = build(Solver6DOF, device="cuda:0")
+    d = build(Solver6DOF, device="cuda:0")
+    d.hip_graph = False
+    o = build(oracle_solver_class(), device="cuda:0")
+    assert g._resident_on(g.device) and g._fixed_capacity_mode(g.device)
+    frames = 14 if name == "pile" else 8
+    for f in range(frames):
+        print(f"{name}: frame {f}", flush=True)
+        g.step()
+        d.step()
+        if o is not None:
+            o.step()
+            assert_bit_exact(diff_report(g, o), f"{name} frame {f} (graph vs checker)")
+        assert_bit_exact(diff_report(g, d), f"{name} frame {f} (graph vs direct)")
+    # the first substep (coloring dirty after the flush) runs directly, every
+    # later one replays the graph
+    assert g._h.sync_count()[1] == frames * g.substeps - 1
+    assert d._h.sync_count()[1] == 0
+    assert np.isfinite(g.positions()).all()
+
+
+def _stable_stack(**kw):
+    # tests/test_robustness.py:79-87
+    s = Solver6DOF(gravity=(0, -9.81, 0), iterations=20, substeps=4, dt=1 / 60,
+                   device="cuda:0", **kw)
+    s.enable_self_collision(True, default_friction=0.5)
+    for k in range(4):
+        b = s.add_box(position=(0.0, 0.13 + 0.24 * k, 0.0), half_extents=(0.12, 0.12, 0.12),
+                      mass=1.0, friction=0.5)
+        s.add_floor_contact_box(b, friction=0.5)
+    return s
+
+
+@pytest.mark.parametrize("recolor_every", [False, True])
+def test_resident_frame_syncs_at_most_twice(recolor_every):
+    s = _stable_stack(primal_group_size=16, primal_shuffle=True, primal_fused=True,
+                      gpu_resident=True, recolor_every_substep=recolor_every)
+    for _ in range(8):   # flush, first recolor, capture
+        s.step()
+    s.positions()
+    syncs0, launches0 = s._h.sync_count()
+    s.step()
+    syncs1, launches1 = s._h.sync_count()
+    assert syncs1 - syncs0 <= 2, f"{syncs1 - syncs0} host synchronisations in one frame"
+    assert launches1 - launches0 == s.substeps   # every substep replayed the graph
+    xr = s.positions()
+    assert np.isfinite(xr).all() and xr[:, 1].min() > -0.05 and np.abs(xr).max() < 3.0
+
+
+def test_not_resident_on_cpu_string_or_without_flag():
+    a = _stable_stack(gpu_resident=False)
+    b = Solver6DOF(device="cpu")
+    assert not a._resident_on(a.device) and not b._resident_on(b.device)
+    a.step()
+    a.step()
+    assert a._h.sync_count()[1] == 0

@@ -55,3 +55,19 @@ def test_growing_path_regrows_instead():
     for _ in range(4):
         s.step()
     assert int(s.n_active_rows.numpy()[0]) <= s._gpu_pool_n_capacity
+
+
+def test_resident_device_string_implies_fixed_capacity():
+    # solver_6dof.py:663-678: a GPU device string with gpu_resident and the
+    # fused shuffled group primal selects the resident, fixed-capacity loop
+    cls = oracle_solver_class()
+    assert cls(device="cuda:0")._fixed_capacity_mode("cuda:0")
+    assert not cls()._fixed_capacity_mode("cpu")
+    for kw in ({"gpu_resident": False}, {"primal_group_size": 1}, {"primal_fused": False},
+               {"primal_shuffle": False}):
+        s = cls(device="cuda:0", **kw)
+        assert not s._resident_on(s.device), kw
+    s = _dense_cluster(False)
+    s.device = "cuda:0"
+    s.step()   # a whole frame through step_frame on the checker
+    assert np.isfinite(s.positions()).all()

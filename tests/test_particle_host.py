@@ -317,9 +317,14 @@ def test_default_friction_backfill_and_floor_rows():
     assert (rows[0].fmin, rows[0].fmax) == (-math.inf, 0.0)
 
 
-def test_cpu_device_raises():
+def test_device_strings():
+    # the reference's default "cpu" runs on HIP device 0; GPU strings name one
+    from avbd_amd.solver_6dof import _parse_device
+    assert PSol.Solver().device == "cpu"
+    assert _parse_device("cpu") == 0 and _parse_device("cuda") == 0
+    assert _parse_device("cuda:3") == 3 and _parse_device("hip:1") == 1
     with pytest.raises(ValueError):
-        PSol.Solver(device="cpu")
+        PSol.Solver(device="tpu")
 
 
 def test_box_lattice_is_consistent():
