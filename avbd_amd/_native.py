@@ -92,6 +92,20 @@ class Report(C.Structure):
     ]
 
 
+class Views(C.Structure):
+    """avbd_views: device addresses of the live state."""
+    _fields_ = [
+        ("n_bodies", C.c_int),
+        ("x", C.c_void_p), ("q", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+        ("body_stride", C.c_int),
+        ("n_groups", C.c_int),
+        ("lam", C.c_void_p), ("penalty", C.c_void_p), ("state_bits", C.c_void_p),
+        ("group_stride", C.c_int),
+        ("header", C.c_void_p),
+        ("header_stride", C.c_int),
+    ]
+
+
 class PParams(C.Structure):
     """avbd_p_params: one particle Solver.step's settings."""
     _fields_ = [
@@ -147,6 +161,15 @@ SIGNATURES = {
                              C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(Report)], C.c_int),
     "step_frame": ([C.c_void_p, C.POINTER(Params), C.c_int, C.c_int, C.POINTER(Report)], C.c_int),
     "sync_count": ([C.c_void_p, I64P, I64P], C.c_int),
+    "stage_contacts": ([C.c_void_p, C.POINTER(Params), C.POINTER(Report)], C.c_int),
+    "stage_coloring": ([C.c_void_p, C.POINTER(Params), C.c_int, C.POINTER(Report)], C.c_int),
+    "stage_csr": ([C.c_void_p], C.c_int),
+    "stage_prelude": ([C.c_void_p, C.POINTER(Params)], C.c_int),
+    "stage_sweep": ([C.c_void_p, C.POINTER(Params), C.c_int], C.c_int),
+    "stage_end": ([C.c_void_p, C.POINTER(Params)], C.c_int),
+    "get_stream": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
+    "device_views": ([C.c_void_p, C.POINTER(Views)], C.c_int),
     "read_bodies": ([C.c_void_p, F32P, F32P, F32P, F32P, F32P, F32P], C.c_int),
     "write_bodies": ([C.c_void_p, C.c_int, C.c_int, C.c_int, F32P], C.c_int),
     "read_rows": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P, I32P, I32P, I32P, F32P], C.c_int),
@@ -222,6 +245,11 @@ def load_library() -> C.CDLL:
                 f"avbd_amd: native library {LIB_PATH} is missing. Build it with "
                 "`python avbd_amd/build.py` (or __graft_entry__.build()). There is no "
                 "CPU fallback for the solver.")
+        # torch first: the library then binds to the HIP runtime torch
+        # loaded, so device addresses and streams are shared with torch
+        # (with the library loaded first, torch would load a second runtime
+        # that cannot open the GPU beside it)
+        import torch  # noqa: F401
         _LIB = bind(C.CDLL(str(LIB_PATH)), "avbd_")
     return _LIB
 
@@ -297,6 +325,43 @@ class NativeHandle:
         a, b = C.c_longlong(0), C.c_longlong(0)
         self._check(self._fn("sync_count")(self.h, C.byref(a), C.byref(b)))
         return int(a.value), int(b.value)
+
+    # ---- one substep stage by stage (avbd_stage_*) ----
+    def stage_contacts(self, params: Params) -> Report:
+        rep = Report()
+        self._check(self._fn("stage_contacts")(self.h, C.byref(params), C.byref(rep)))
+        return rep
+
+    def stage_coloring(self, params: Params, color_dirty: bool, rep: Report) -> Report:
+        self._check(self._fn("stage_coloring")(self.h, C.byref(params), int(bool(color_dirty)),
+                                               C.byref(rep)))
+        return rep
+
+    def stage_csr(self) -> None:
+        self._check(self._fn("stage_csr")(self.h))
+
+    def stage_prelude(self, params: Params) -> None:
+        self._check(self._fn("stage_prelude")(self.h, C.byref(params)))
+
+    def stage_sweep(self, params: Params, sweep: int) -> None:
+        self._check(self._fn("stage_sweep")(self.h, C.byref(params), int(sweep)))
+
+    def stage_end(self, params: Params) -> None:
+        self._check(self._fn("stage_end")(self.h, C.byref(params)))
+
+    # ---- streams and device views ----
+    def get_stream(self) -> int:
+        st = C.c_void_p()
+        self._check(self._fn("get_stream")(self.h, C.byref(st)))
+        return int(st.value or 0)
+
+    def set_stream(self, stream: int | None) -> None:
+        self._check(self._fn("set_stream")(self.h, C.c_void_p(stream or None)))
+
+    def device_views(self) -> Views:
+        v = Views()
+        self._check(self._fn("device_views")(self.h, C.byref(v)))
+        return v
 
     def read_bodies(self, n: int) -> dict[str, np.ndarray]:
         out = {k: np.zeros((n, 3), np.float32) for k in ("x", "v", "w", "prev_v", "prev_w")}

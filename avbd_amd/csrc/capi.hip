@@ -122,7 +122,8 @@ constexpr int kDcntSize = 256;
 
 struct avbd_solver {
   int device = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;      // the stream every launch and copy goes to
+  hipStream_t own_st = nullptr;  // the solver's own stream (st unless a caller's is set)
   bool flushed = false;
 
   // bodies
@@ -221,6 +222,28 @@ struct avbd_solver {
   // substeps with every count and decision on the device, captured as a HIP
   // graph (AVBD_OPT_GRAPH): one executable graph per pose buffer (the
   // substep alternates them), with the key of what it was captured from
+  // state handed from one stage of a direct substep to the next (the stage
+  // entry points run them one call at a time); `stage` is the next stage
+  struct StageCtx {
+    int stage = 0;
+    int n_active = 0;
+    bool pool_reset = false;
+    int n_ml = 0;
+    bool recolor = false;
+    const int* pos_map = nullptr;
+    int n_groups = 0;
+    int iterations = 0;   // may shrink when iterating to a residual
+    int post = 0;
+    float tol = -1.0f;
+    float resid = 0.0f;
+    bool flow = false, fuse = false, tail = false;
+    int flow_bpw = 2;
+    int nonempty = 0;
+    double pbytes = 0.0;
+    float max_lin = 0.0f, max_ang = 0.0f;
+    int sweeps_done = 0;
+    bool finalized = false;
+  } sx;
   bool graph_on = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<char> gkey[2];
@@ -568,27 +591,34 @@ int ml_begin(avbd_solver* s, const avbd_params* p, int n) {
 // tol >= 0: iterate until the dual's residual max |lambda_new - lambda| / k
 // is <= tol, with p->iterations as the cap (one readback per iteration); the
 // result equals a fixed-iteration substep with that many iterations.
-int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep,
-               float tol = -1.0f, int* iters_used = nullptr, float* resid_out = nullptr) {
+// The stages of a direct substep, in the reference's launch order
+// (solver_6dof.py:1269-1464): contacts, coloring, body->group lists,
+// prelude, (primal sweep, dual) x iterations with the finalize after the
+// last dual, post-stabilisation sweep, cache refresh. Each stage leaves what
+// the next needs in s->sx.
+enum { kStContacts = 0, kStColoring, kStCsr, kStPrelude, kStSweeps, kStEnd };
+int primal_bpw(int variant);
+
+int stage_contacts(avbd_solver* s, const avbd_params* p, avbd_report* rep) {
   std::memset(rep, 0, sizeof(*rep));
   rep->n_active_colors = s->n_active_colors;
-  if (s->nb == 0) return AVBD_OK;
+  auto& x = s->sx;
+  x = avbd_solver::StageCtx{};
   const int nb = s->nb;
   const bool fixed = p->fixed_capacity != 0;
-  hipStream_t st = s->st;
   s->n_active_rows = s->n_static_rows;
   s->pool_count = 0;
   s->pair_count = 0;
   s->n_contacts = 0;
   s->adj_pairs = 0;
   s->conflicts_ready = false;
-  int n_active = s->n_static_rows;
-  bool pool_reset = false;  // the pool grew this substep (see the cache refresh)
+  x.n_active = s->n_static_rows;
+  x.pool_reset = false;  // the pool grew this substep (see the cache refresh)
   rep->row_capacity = s->row_cap;
   if (p->self_collide && nb >= 2) {
     CK(emit_contacts(s, p, fixed, rep));
     if (fixed) {
-      n_active = s->row_cap;
+      x.n_active = s->row_cap;
     } else {
       int raw = s->n_active_rows, cap = s->row_cap;
       if (raw > cap) {
@@ -596,25 +626,31 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
         const int pool_before = s->pool_max;
         CK(grow_row_pool(s, std::max(2 * s->n_dyn_cap, (raw - s->n_static_rows) * 2),
                          s->g_static + s->n_contacts));
-        pool_reset = s->pool_max > pool_before;
+        x.pool_reset = s->pool_max > pool_before;
       }
-      n_active = std::min(raw, cap);
+      x.n_active = std::min(raw, cap);
     }
   }
   // learned warm start: features of the emitted contact groups, then either
   // the network's initial dual state (mode 2) or nothing more until the
   // targets at the end of the substep (mode 1)
   // groups: static (floor, pins) first, then this substep's box-box contacts
-  const int n_ml = s->ml_mode == 0 ? 0
-                   : s->g_static + (p->self_collide ? std::min(s->n_contacts, s->pool_max) : 0);
+  x.n_ml = s->ml_mode == 0 ? 0
+           : s->g_static + (p->self_collide ? std::min(s->n_contacts, s->pool_max) : 0);
   s->ml_n = 0;
-  if (n_ml > 0) CK(ml_begin(s, p, n_ml));
+  if (x.n_ml > 0) CK(ml_begin(s, p, x.n_ml));
   rep->n_active_rows = s->n_active_rows;
   rep->new_row_capacity = s->row_cap;
   rep->pool_count = s->pool_count;
   rep->pair_capacity = s->pair_cap;
   rep->n_contacts = std::min(s->n_contacts, s->pool_max);
+  x.stage = kStColoring;
+  return AVBD_OK;
+}
 
+int stage_coloring(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep) {
+  auto& x = s->sx;
+  hipStream_t st = s->st;
   bool need = color_dirty != 0;
   if (!need) {
     if (s->conflicts_ready) {
@@ -636,111 +672,144 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
     s->colors_conflict_free = true;  // the conflict count just read is 0
   }
   rep->n_active_colors = s->n_active_colors;
+  x.recolor = need;
+  x.stage = kStCsr;
+  (void)p;
+  return AVBD_OK;
+}
 
+int stage_csr(avbd_solver* s) {
+  auto& x = s->sx;
   // Without live rows the reference keeps the previous body->constraint
   // lists (solver_6dof.py:1366); if the color order changed since they were
   // built, map each color position to the list's old position.
-  const int* pos_map = nullptr;
-  if (need) s->csr_order_stale = true;
-  if (n_active > 0) {
+  x.pos_map = nullptr;
+  if (x.recolor) s->csr_order_stale = true;
+  if (x.n_active > 0) {
     CK(build_csr(s));
   } else if (s->csr_built && s->csr_order_stale) {
-    launch_tcsr_posmap(st, nb, s->cvals.p, s->pos_of.p, s->pos_map.p);
-    pos_map = s->pos_map.p;
+    launch_tcsr_posmap(s->st, s->nb, s->cvals.p, s->pos_of.p, s->pos_map.p);
+    x.pos_map = s->pos_map.p;
   }
+  x.stage = kStPrelude;
+  return AVBD_OK;
+}
 
-  const int n_groups = s->g_static + s->n_contacts;
-  float4* p0 = s->pose[s->cur].p;
-  float4* pw = s->pose[1 - s->cur].p;
-  const float dt = p->dt;
-  const int post = p->post_stabilize ? 1 : 0;
+int stage_prelude(avbd_solver* s, const avbd_params* p, float tol) {
+  auto& x = s->sx;
+  const int nb = s->nb;
+  x.n_groups = s->g_static + s->n_contacts;
+  x.post = p->post_stabilize ? 1 : 0;
   CK(prof_begin(s, kPrelude));
-  launch_substep_begin(st, nb, n_groups, p0, pw, s->poseI.p, s->inert.p, s->vel.p, s->pvel.p,
-                       s->bstat.p, s->Ilocal.p, s->geo.p, s->gst.p, dt, p->gravity[0],
-                       p->gravity[1], p->gravity[2], p->alpha, p->gamma, post, n_active > 0 ? 1 : 0,
+  launch_substep_begin(s->st, nb, x.n_groups, s->pose[s->cur].p, s->pose[1 - s->cur].p,
+                       s->poseI.p, s->inert.p, s->vel.p, s->pvel.p, s->bstat.p, s->Ilocal.p,
+                       s->geo.p, s->gst.p, p->dt, p->gravity[0], p->gravity[1], p->gravity[2],
+                       p->alpha, p->gamma, x.post, x.n_active > 0 ? 1 : 0,
                        reinterpret_cast<unsigned*>(s->dcnt.p + kEpochSlot));
   CK(prof_end(s, (double)nb * 188.0 + (double)s->n_contacts * 124.0 + (double)s->n_floor_groups * 88.0));
-
-  float max_lin = p->max_linear_speed, max_ang = p->max_angular_speed;
-  if (!(max_lin > 0.0f && max_lin < kInf && max_ang > 0.0f && max_ang < kInf)) {
-    max_lin = kInf;
-    max_ang = kInf;
+  x.max_lin = p->max_linear_speed;
+  x.max_ang = p->max_angular_speed;
+  if (!(x.max_lin > 0.0f && x.max_lin < kInf && x.max_ang > 0.0f && x.max_ang < kInf)) {
+    x.max_lin = kInf;
+    x.max_ang = kInf;
   }
-  int I = p->iterations;
+  x.iterations = p->iterations;
+  x.tol = tol;
+  x.resid = 0.0f;
   const bool to_resid = tol >= 0.0f;
-  float resid = 0.0f;
-  int nonempty = 0;
-  for (int c = 0; c < s->n_active_colors; ++c) nonempty += (s->cstart[c + 1] > s->cstart[c]);
-  const double pbytes = primal_sweep_bytes(s) / std::max(1, nonempty);
-  // variants 6-8: every color of a sweep in one dataflow launch with 2, 1
-  // or 4 bodies per wavefront
-  const bool flow = s->primal_variant >= 6 && pos_map == nullptr && I + post < (int)4095;
-  const int flow_bpw = s->primal_variant == 7 ? 1 : s->primal_variant == 8 ? 4
-                      : s->primal_variant == 9 ? 8 : s->primal_variant == 10 ? 3 : 2;
+  x.nonempty = 0;
+  for (int c = 0; c < s->n_active_colors; ++c) x.nonempty += (s->cstart[c + 1] > s->cstart[c]);
+  x.pbytes = primal_sweep_bytes(s) / std::max(1, x.nonempty);
+  // variants 6-10: every color of a sweep in one dataflow launch
+  x.flow = s->primal_variant >= 6 && x.pos_map == nullptr && x.iterations + x.post < (int)4095;
+  x.flow_bpw = primal_bpw(s->primal_variant);
   // the dual rides along in the sweep when no two neighbours share a color
-  const bool fuse = flow && s->colors_conflict_free && s->fuse_dual_mode == 1 && !to_resid;
+  x.fuse = x.flow && s->colors_conflict_free && s->fuse_dual_mode == 1 && !to_resid;
   // or in trailing workgroups of the sweep's launch (any coloring)
-  const bool tail = flow && s->fuse_dual_mode == 2 && !to_resid;
-  for (int it = 0; it < I + post; ++it) {
-    const int post_pass = (post && it == I) ? 1 : 0;
-    // One timing interval per sweep (all colors back to back): per-launch
-    // events would add their own gaps between the color launches.
-    CK(prof_begin(s, kPrimal));
-    int launched = 0;
-    if (flow) {
-      launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
-                          s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
-                          s->gst.p, dt, post_pass, s->flow_done.p,
-                          reinterpret_cast<const unsigned*>(s->dcnt.p + kEpochSlot), it,
-                          s->dcnt.p + kErrSlot, flow_bpw, fuse && it < I ? 1 : 0, p->beta, s->gst.p,
-                          tail && it < I ? n_groups : 0,
-                          s->flow_test_cap > 0 ? s->flow_never.p : nullptr, s->flow_test_cap);
-      launched = nonempty;  // bodies of every color, in one launch
-      CK(prof_end(s, pbytes * launched + (tail && it < I ? dual_sweep_bytes(s) : 0.0), 1));
-    } else {
-      for (int c = 0; c < s->n_active_colors; ++c) {
-        int cnt = s->cstart[c + 1] - s->cstart[c];
-        if (cnt <= 0) continue;
-        launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
-                            s->bstat.p, s->cpos.p, s->tcsr.p, pos_map, s->geo.p, s->gst.p, dt,
-                            post_pass, s->primal_variant);
-        ++launched;
-      }
-      CK(prof_end(s, pbytes * launched, launched));
+  x.tail = x.flow && s->fuse_dual_mode == 2 && !to_resid;
+  x.sweeps_done = 0;
+  x.finalized = false;
+  x.stage = kStSweeps;
+  return AVBD_OK;
+}
+
+// Sweep `it` of the substep: the primal sweep, then (for it < iterations)
+// the dual update unless it rode along in the sweep, then the finalize
+// after the last dual.
+int stage_sweep(avbd_solver* s, const avbd_params* p, int it) {
+  auto& x = s->sx;
+  hipStream_t st = s->st;
+  const int I = x.iterations;
+  const float dt = p->dt;
+  float4* pw = s->pose[1 - s->cur].p;
+  const int post_pass = (x.post && it == I) ? 1 : 0;
+  // One timing interval per sweep (all colors back to back): per-launch
+  // events would add their own gaps between the color launches.
+  CK(prof_begin(s, kPrimal));
+  int launched = 0;
+  if (x.flow) {
+    launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
+                        s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
+                        s->gst.p, dt, post_pass, s->flow_done.p,
+                        reinterpret_cast<const unsigned*>(s->dcnt.p + kEpochSlot), it,
+                        s->dcnt.p + kErrSlot, x.flow_bpw, x.fuse && it < I ? 1 : 0, p->beta,
+                        s->gst.p, x.tail && it < I ? x.n_groups : 0,
+                        s->flow_test_cap > 0 ? s->flow_never.p : nullptr, s->flow_test_cap);
+    launched = x.nonempty;  // bodies of every color, in one launch
+    CK(prof_end(s, x.pbytes * launched + (x.tail && it < I ? dual_sweep_bytes(s) : 0.0), 1));
+  } else {
+    for (int c = 0; c < s->n_active_colors; ++c) {
+      int cnt = s->cstart[c + 1] - s->cstart[c];
+      if (cnt <= 0) continue;
+      launch_primal_color(st, cnt, s->cvals.p, s->cstart[c], pw, s->poseI.p, s->inert.p,
+                          s->bstat.p, s->cpos.p, s->tcsr.p, x.pos_map, s->geo.p, s->gst.p, dt,
+                          post_pass, s->primal_variant);
+      ++launched;
     }
-    if (it < I && !fuse && !tail) {
-      float* rd = to_resid ? reinterpret_cast<float*>(s->dcnt.p + 10) : nullptr;
-      if (rd) launch_fill_int(st, s->dcnt.p + 10, 1, 0);
-      CK(prof_begin(s, kDual));
-      launch_dual(st, n_groups, pw, s->geo.p, s->gst.p, p->beta, rd);
-      CK(prof_end(s, dual_sweep_bytes(s)));
-      if (rd) {
-        CK(readback(s, 10, 1));
-        std::memcpy(&resid, s->hcnt + 10, sizeof(float));
-        if (resid <= tol) I = it + 1;  // this was the last dual
-      }
-    }
-    if (it == I - 1) {
-      CK(prof_begin(s, kOther));
-      launch_finalize(st, nb, pw, p0, s->vel.p, s->pvel.p, s->bstat.p, dt, max_lin, max_ang);
-      CK(prof_end(s, (double)nb * 132.0));
+    CK(prof_end(s, x.pbytes * launched, launched));
+  }
+  if (it < I && !x.fuse && !x.tail) {
+    const bool to_resid = x.tol >= 0.0f;
+    float* rd = to_resid ? reinterpret_cast<float*>(s->dcnt.p + 10) : nullptr;
+    if (rd) launch_fill_int(st, s->dcnt.p + 10, 1, 0);
+    CK(prof_begin(s, kDual));
+    launch_dual(st, x.n_groups, pw, s->geo.p, s->gst.p, p->beta, rd);
+    CK(prof_end(s, dual_sweep_bytes(s)));
+    if (rd) {
+      CK(readback(s, 10, 1));
+      std::memcpy(&x.resid, s->hcnt + 10, sizeof(float));
+      if (x.resid <= x.tol) x.iterations = it + 1;  // this was the last dual
     }
   }
-  if (n_ml > 0 && s->ml_mode == 1) {
+  if (it == x.iterations - 1) {
     CK(prof_begin(s, kOther));
-    launch_contact_targets(st, n_ml, 0, s->geo.p, s->gst.p, s->ml_feat.p,
+    launch_finalize(st, s->nb, pw, s->pose[s->cur].p, s->vel.p, s->pvel.p, s->bstat.p, dt,
+                    x.max_lin, x.max_ang);
+    CK(prof_end(s, (double)s->nb * 132.0));
+    x.finalized = true;
+  }
+  x.sweeps_done = it + 1;
+  return AVBD_OK;
+}
+
+int stage_end(avbd_solver* s, const avbd_params* p) {
+  auto& x = s->sx;
+  hipStream_t st = s->st;
+  if (x.n_ml > 0 && s->ml_mode == 1) {
+    CK(prof_begin(s, kOther));
+    launch_contact_targets(st, x.n_ml, 0, s->geo.p, s->gst.p, s->ml_feat.p,
                            p->alpha * p->gamma, p->gamma, s->ml_targ.p, s->ml_mask.p);
-    CK(prof_end(s, (double)n_ml * (96.0 + 48.0 + 16.0 + 20.0)));
-    s->ml_n = n_ml;
+    CK(prof_end(s, (double)x.n_ml * (96.0 + 48.0 + 16.0 + 20.0)));
+    s->ml_n = x.n_ml;
   }
   if (p->self_collide && s->hash_cap > 0) {
     CK(prof_begin(s, kOther));
     int entries = std::min(s->n_contacts, s->pool_max);
-    s->hash_live = live_hash_slots(s, pool_reset ? 1 : entries);
+    s->hash_live = live_hash_slots(s, x.pool_reset ? 1 : entries);
     launch_fill_int(st, s->dcnt.p + kHashLiveSlot, 1, s->hash_live);
     launch_fill_int(st, s->hkeys.p, s->hash_live, kHashEmpty);
     launch_fill_int(st, s->howner.p, s->hash_live, -1);
-    if (!pool_reset) {
+    if (!x.pool_reset) {
       launch_hash_collect(st, entries, s->g_static, s->geo.p, s->gst.p, s->hash_live, s->hkeys.p,
                           s->howner.p, s->slot_of.p, s->hstate.p,
                           s->ml_mode ? s->hage.p : nullptr);
@@ -751,14 +820,28 @@ int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_repor
     }
     CK(prof_end(s, 0.0));
   }
-  if (iters_used) *iters_used = I;
-  if (resid_out) *resid_out = resid;
   HIPCK(hipMemcpyAsync(s->hcnt + kErrSlot, s->dcnt.p + kErrSlot, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCK(hipGetLastError());
   s->cur = 1 - s->cur;
-  s->last_post = post;
+  s->last_post = x.post;
   CK(prof_collect(s));
+  x.stage = kStContacts;
   return AVBD_OK;
+}
+
+int do_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* rep,
+               float tol = -1.0f, int* iters_used = nullptr, float* resid_out = nullptr) {
+  std::memset(rep, 0, sizeof(*rep));
+  rep->n_active_colors = s->n_active_colors;
+  if (s->nb == 0) return AVBD_OK;
+  CK(stage_contacts(s, p, rep));
+  CK(stage_coloring(s, p, color_dirty, rep));
+  CK(stage_csr(s));
+  CK(stage_prelude(s, p, tol));
+  for (int it = 0; it < s->sx.iterations + s->sx.post; ++it) CK(stage_sweep(s, p, it));
+  if (iters_used) *iters_used = s->sx.iterations;
+  if (resid_out) *resid_out = s->sx.resid;
+  return stage_end(s, p);
 }
 
 // ===========================================================================
@@ -1004,9 +1087,18 @@ int substep_graph(avbd_solver* s, const avbd_params* p) {
     if (s->gexec[c]) (void)hipGraphExecDestroy(s->gexec[c]);
     s->gexec[c] = nullptr;
     hipGraph_t g = nullptr;
-    HIPCK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    // captured on the solver's own stream (a caller's stream may be the null
+    // stream, which cannot capture) and launched on the current one
+    const hipStream_t launch_st = s->st;
+    s->st = s->own_st;
+    const hipError_t e_begin = hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal);
+    if (e_begin != hipSuccess) {
+      s->st = launch_st;
+      return set_err(AVBD_E_HIP, std::string("substep capture: ") + hipGetErrorString(e_begin));
+    }
     substep_device(s, p);
     const hipError_t e_end = hipStreamEndCapture(s->st, &g);
+    s->st = launch_st;
     if (e_end != hipSuccess || g == nullptr)
       return set_err(AVBD_E_HIP, std::string("substep capture failed: ") + hipGetErrorString(e_end));
     const hipError_t e_inst = hipGraphInstantiate(&s->gexec[c], g, nullptr, nullptr, 0);
@@ -1091,6 +1183,7 @@ int avbd_create(int device, avbd_solver** out) {
     delete s;
     return set_err(AVBD_E_HIP, "hipStreamCreate failed");
   }
+  s->own_st = s->st;
   if (hipHostMalloc((void**)&s->hcnt, kDcntSize * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     (void)hipStreamDestroy(s->st);
     delete s;
@@ -1116,7 +1209,7 @@ int avbd_destroy(avbd_solver* s) {
   drop_graphs(s);
   for (auto e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->hcnt) (void)hipHostFree(s->hcnt);
-  (void)hipStreamDestroy(s->st);
+  (void)hipStreamDestroy(s->own_st);
   delete s;
   return AVBD_OK;
 }
@@ -1323,6 +1416,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
   s->n_scenes = 0;  // a flush replaces the scene; avbd_set_scenes splits it again
   s->flow_test_cap = 0;
   s->flushed = true;
+  s->sx = avbd_solver::StageCtx{};
   CK(ensure_hage(s));
   HIPCK(hipStreamSynchronize(s->st));
   return AVBD_OK;
@@ -1331,6 +1425,7 @@ int avbd_flush(avbd_solver* s, const avbd_scene* sc) {
 int avbd_substep(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out) {
   if (!s || !p || !out) return set_err(AVBD_E_ARG, "null argument");
   if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
+  if (s->sx.stage != kStContacts) return set_err(AVBD_E_STATE, "a staged substep is in progress");
   HIPCK(hipSetDevice(s->device));
   return do_substep(s, p, color_dirty, out);
 }
@@ -1341,6 +1436,7 @@ int avbd_substep_to_residual(avbd_solver* s, const avbd_params* p, int color_dir
     return set_err(AVBD_E_ARG, "null argument");
   if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
   if (!(tol >= 0.0f)) return set_err(AVBD_E_ARG, "tolerance must be >= 0");
+  if (s->sx.stage != kStContacts) return set_err(AVBD_E_STATE, "a staged substep is in progress");
   HIPCK(hipSetDevice(s->device));
   return do_substep(s, p, color_dirty, out, tol, iterations_used, residual);
 }
@@ -1350,6 +1446,7 @@ int avbd_step_frame(avbd_solver* s, const avbd_params* p, int n_substeps, int co
   if (!s || !p || !out) return set_err(AVBD_E_ARG, "null argument");
   if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
   if (n_substeps < 1) return set_err(AVBD_E_ARG, "a frame needs at least one substep");
+  if (s->sx.stage != kStContacts) return set_err(AVBD_E_STATE, "a staged substep is in progress");
   HIPCK(hipSetDevice(s->device));
   std::memset(out, 0, sizeof(*out));
   out->n_active_colors = s->n_active_colors;
@@ -1381,6 +1478,131 @@ int avbd_sync_count(avbd_solver* s, long long* syncs, long long* graph_launches)
   if (!s) return set_err(AVBD_E_ARG, "null solver");
   if (syncs) *syncs = s->syncs;
   if (graph_launches) *graph_launches = s->graph_launches;
+  return AVBD_OK;
+}
+
+// ---- one substep stage by stage -------------------------------------------
+static int stage_guard(avbd_solver* s, const void* p, int want) {
+  if (!s || !p) return set_err(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return set_err(AVBD_E_STATE, "substep before flush");
+  if (s->sx.stage != want) return set_err(AVBD_E_STATE, "substep stage out of order");
+  HIPCK(hipSetDevice(s->device));
+  return AVBD_OK;
+}
+
+int avbd_stage_contacts(avbd_solver* s, const avbd_params* p, avbd_report* out) {
+  CK(stage_guard(s, out ? (const void*)p : nullptr, kStContacts));
+  if (s->nb == 0) {
+    std::memset(out, 0, sizeof(*out));
+    out->n_active_colors = s->n_active_colors;
+    s->sx = avbd_solver::StageCtx{};
+    s->sx.stage = kStColoring;
+    return AVBD_OK;
+  }
+  const int r = stage_contacts(s, p, out);
+  if (r != AVBD_OK) s->sx.stage = kStContacts;
+  return r;
+}
+
+int avbd_stage_coloring(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out) {
+  CK(stage_guard(s, out ? (const void*)p : nullptr, kStColoring));
+  if (s->nb == 0) {
+    s->sx.stage = kStCsr;
+    return AVBD_OK;
+  }
+  const int r = stage_coloring(s, p, color_dirty, out);
+  if (r != AVBD_OK) s->sx.stage = kStContacts;
+  return r;
+}
+
+int avbd_stage_csr(avbd_solver* s) {
+  CK(stage_guard(s, s, kStCsr));
+  if (s->nb == 0) {
+    s->sx.stage = kStPrelude;
+    return AVBD_OK;
+  }
+  const int r = stage_csr(s);
+  if (r != AVBD_OK) s->sx.stage = kStContacts;
+  return r;
+}
+
+int avbd_stage_prelude(avbd_solver* s, const avbd_params* p) {
+  CK(stage_guard(s, p, kStPrelude));
+  if (s->nb == 0) {
+    s->sx.iterations = p->iterations;
+    s->sx.post = p->post_stabilize ? 1 : 0;
+    s->sx.sweeps_done = 0;
+    s->sx.stage = kStSweeps;
+    return AVBD_OK;
+  }
+  const int r = stage_prelude(s, p, -1.0f);
+  if (r != AVBD_OK) s->sx.stage = kStContacts;
+  return r;
+}
+
+int avbd_stage_sweep(avbd_solver* s, const avbd_params* p, int sweep) {
+  CK(stage_guard(s, p, kStSweeps));
+  const int total = s->sx.iterations + s->sx.post;
+  if (sweep != s->sx.sweeps_done || sweep >= total)
+    return set_err(AVBD_E_ARG, "sweeps run in order, 0 .. iterations + post_stabilize - 1");
+  if (s->nb == 0) {
+    s->sx.sweeps_done = sweep + 1;
+    return AVBD_OK;
+  }
+  const int r = stage_sweep(s, p, sweep);
+  if (r != AVBD_OK) s->sx.stage = kStContacts;
+  return r;
+}
+
+int avbd_stage_end(avbd_solver* s, const avbd_params* p) {
+  CK(stage_guard(s, p, kStSweeps));
+  if (s->sx.sweeps_done != s->sx.iterations + s->sx.post)
+    return set_err(AVBD_E_STATE, "substep ended before its last sweep");
+  if (s->nb == 0) {
+    s->sx.stage = kStContacts;
+    return AVBD_OK;
+  }
+  const int r = stage_end(s, p);
+  s->sx.stage = kStContacts;
+  return r;
+}
+
+// ---- streams and device views ---------------------------------------------
+int avbd_get_stream(avbd_solver* s, void** stream) {
+  if (!s || !stream) return set_err(AVBD_E_ARG, "null argument");
+  *stream = reinterpret_cast<void*>(s->st);
+  return AVBD_OK;
+}
+
+int avbd_set_stream(avbd_solver* s, void* stream) {
+  if (!s) return set_err(AVBD_E_ARG, "null solver");
+  if (s->sx.stage != kStContacts) return set_err(AVBD_E_STATE, "a staged substep is in progress");
+  HIPCK(hipSetDevice(s->device));
+  HIPCK(hipStreamSynchronize(s->st));
+  s->st = stream ? reinterpret_cast<hipStream_t>(stream) : s->own_st;
+  return AVBD_OK;
+}
+
+int avbd_device_views(avbd_solver* s, avbd_views* out) {
+  if (!s || !out) return set_err(AVBD_E_ARG, "null argument");
+  if (!s->flushed) return set_err(AVBD_E_STATE, "views before flush");
+  std::memset(out, 0, sizeof(*out));
+  out->n_bodies = s->nb;
+  float* pose = reinterpret_cast<float*>(s->pose[s->cur].p);
+  float* vel = reinterpret_cast<float*>(s->vel.p);
+  out->x = pose;
+  out->q = pose ? pose + 4 : nullptr;
+  out->v = vel;
+  out->w = vel ? vel + 4 : nullptr;
+  out->body_stride = 8;
+  out->n_groups = s->g_static + std::min(s->n_contacts, s->pool_max);
+  float* gst = reinterpret_cast<float*>(s->gst.p);
+  out->lambda = gst;
+  out->penalty = gst ? gst + 4 : nullptr;
+  out->state_bits = gst ? reinterpret_cast<int*>(gst + 9) : nullptr;
+  out->group_stride = (int)(sizeof(GroupSt) / sizeof(float));
+  out->header = reinterpret_cast<int*>(s->geo.p);
+  out->header_stride = (int)(sizeof(GroupGeo) / sizeof(int));
   return AVBD_OK;
 }
 

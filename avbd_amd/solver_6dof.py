@@ -842,6 +842,81 @@ class Solver6DOF:
         return self._h.count_color_conflicts()
 
     # ---- readback (solver_6dof.py:1920-2052) ----------------------------------
+    # ---- device-side boundary (not in the reference's API) -------------------
+    def step_staged(self) -> None:
+        """One frame like step() on the per-substep path, each substep issued
+        stage by stage through the library's stage entry points in the
+        reference's launch order (solver_6dof.py:1269-1464); the same bits as
+        step() with the same settings."""
+        if self._dirty:
+            self._flush()
+        n_b = len(self._x)
+        if n_b == 0:
+            return
+        sub_dt = self.dt / self.substeps if self.substeps > 1 else self.dt
+        for _ in range(max(1, self.substeps)):
+            params = self._params(sub_dt)
+            rep = self._h.stage_contacts(params)
+            self._h.stage_coloring(params, self._color_dirty, rep)
+            self._h.stage_csr()
+            self._h.stage_prelude(params)
+            for k in range(params.iterations + params.post_stabilize):
+                self._h.stage_sweep(params, k)
+            self._h.stage_end(params)
+            self._apply_report(rep, n_b)
+        if self._fixed_capacity_mode(self.device):
+            self._check_fixed_cap_overflow()
+
+    @property
+    def stream(self) -> int:
+        """The hipStream_t (as an int) the solver launches on."""
+        self._flush()
+        return self._h.get_stream()
+
+    def set_stream(self, stream) -> None:
+        """Launch on `stream` from now on: a torch.cuda.Stream, a raw
+        hipStream_t int, or None for the solver's own stream. Work queued on
+        the previous stream is waited for first."""
+        self._flush()
+        raw = getattr(stream, "cuda_stream", stream)
+        self._h.set_stream(int(raw) if raw is not None else None)
+
+    def torch_views(self) -> dict:
+        """Zero-copy torch tensors over the live device state, valid until
+        the next substep or flush: x [n, 3], q [n, 4] (xyzw), v [n, 3],
+        omega [n, 3]; per constraint group (static rows' groups, then this
+        substep's contacts) lam [g, 3], penalty [g, 3], state_bits [g]
+        (bit r: row r active, bit 3 + r: row r sticking), header [g, 4] =
+        (kind | (rows - 1) << 2, body_a, body_b, first flat row)."""
+        import torch
+        self._flush()
+        v = self._h.device_views()
+        dev = torch.device("cuda", self._device_index)
+
+        def view(ptr, shape, strides, typestr):
+            class _Mem:
+                __cuda_array_interface__ = {
+                    "data": (int(ptr or 0), False), "shape": tuple(shape),
+                    "strides": tuple(4 * st for st in strides), "typestr": typestr,
+                    "version": 2}
+            if 0 in shape or not ptr:
+                return torch.empty(shape, dtype=torch.float32 if typestr == "<f4"
+                                   else torch.int32, device=dev)
+            return torch.as_tensor(_Mem(), device=dev)
+
+        n, g = v.n_bodies, v.n_groups
+        bs, gs, hs = v.body_stride, v.group_stride, v.header_stride
+        return {
+            "x": view(v.x, (n, 3), (bs, 1), "<f4"),
+            "q": view(v.q, (n, 4), (bs, 1), "<f4"),
+            "v": view(v.v, (n, 3), (bs, 1), "<f4"),
+            "omega": view(v.w, (n, 3), (bs, 1), "<f4"),
+            "lam": view(v.lam, (g, 3), (gs, 1), "<f4"),
+            "penalty": view(v.penalty, (g, 3), (gs, 1), "<f4"),
+            "state_bits": view(v.state_bits, (g,), (gs,), "<i4"),
+            "header": view(v.header, (g, 4), (hs, 1), "<i4"),
+        }
+
     def positions(self) -> np.ndarray:
         if self.x is None:
             return np.array(self._x, dtype=np.float32).reshape(-1, 3)

@@ -59,7 +59,8 @@ def measured_traffic(layers: int, variant: int) -> float | None:
 
 def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
     """The C++ restatement of the reference substep (one host thread) on the
-    same scene, timed over `substeps_sample` substeps from the initial state."""
+    same scene: one untimed substep from the initial state, then
+    `substeps_sample` timed substeps."""
     from avbd_amd import scenes
     from oracle.oracle import oracle_solver_class
     from tests.parity import one_substep
@@ -67,6 +68,7 @@ def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
     s = scenes.pyramid(layers, seed=seed, solver_cls=oracle_solver_class())
     s._flush()
     n_b = len(s._x)
+    one_substep(s)   # warm-up: first recolor, first touch of every array
     t0 = time.perf_counter()
     for _ in range(substeps_sample):
         one_substep(s)
@@ -76,8 +78,9 @@ def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
         "unit": "body-substeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{substeps_sample} substep(s) of the {n_b}-cube pyramid from its initial "
-                   f"state ({dt:.1f} s), C++ restatement of the reference substep, 1 thread"),
+        "sample": (f"substeps 2-{substeps_sample + 1} of the {n_b}-cube pyramid from its "
+                   f"initial state ({dt:.1f} s; substep 1 untimed), C++ restatement of the "
+                   f"reference substep on 1 thread of a {os.cpu_count()}-thread host"),
     }
 
 
@@ -231,7 +234,13 @@ def main() -> int:
     ap.add_argument("--layers", type=int, default=66)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-substeps", type=int, default=1)
+    ap.add_argument("--cpu-substeps", type=int, default=2,
+                    help="timed substeps of the CPU baseline (after one untimed one)")
+    ap.add_argument("--direct", type=int, default=0,
+                    help="1: direct launches per substep (the CPU-device semantics) instead "
+                         "of resident frames replayed as a HIP graph")
+    ap.add_argument("--profile-frames", type=int, default=2,
+                    help="frames after the timed region with per-launch event timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB MiB (the reference point "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -271,6 +280,10 @@ def main() -> int:
         s, sizes = batched_scene(args.scenes, args.seed)
     else:
         s = scenes.pyramid(args.layers, seed=args.seed)
+    # "cuda:0": the resident frame (fixed-capacity pools, substeps replayed
+    # as a captured HIP graph, one counter readback per frame); "cpu": the
+    # reference's CPU-device semantics, a direct launch sequence per substep
+    s.device = "cpu" if args.direct else "cuda:0"
     s._flush()
     variant = args.primal_variant
     if variant is not None:
@@ -292,17 +305,29 @@ def main() -> int:
         variant = s.primal_variant
         print(f"# primal autotune (ms of primal sweeps per frame): {tuned} -> variant {variant}",
               file=sys.stderr, flush=True)
-    s._h.read_counters()
-    s.positions()   # synchronises the stream
-    s._h.profile_enable(True)
+        s.step()   # untimed: captures the graph of the chosen kernel
+    import torch
+    syncs0, graphs0 = s._h.sync_count()
+    torch.cuda.synchronize()
+    frame_ms = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        s.step()
-    s.positions()
+        tf = time.perf_counter()
+        s.step()   # a resident frame ends on its counter readback
+        frame_ms.append(1e3 * (time.perf_counter() - tf))
+    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    syncs1, graphs1 = s._h.sync_count()
+    assert bool(__import__("numpy").isfinite(s.positions()).all()), "state diverged"
+    n_rows, n_pairs, n_contacts = s._h.read_counters()
+    dropped = n_pairs > s._bp_max_pairs or n_rows > s._gpu_pool_n_capacity
+    # kernel times: a few more frames with per-launch events (direct
+    # launches, since event timing is not captured into the graph)
+    s._h.profile_enable(True)
+    for _ in range(args.profile_frames):
+        s.step()
     ms, launches, nbytes = s._h.profile_read(7)
     s._h.profile_enable(False)
-    n_rows, n_pairs, n_contacts = s._h.read_counters()
     substeps = s.substeps * args.steps
     value = n_b * substeps / elapsed
     prim_ms, prim_n, prim_b = float(ms[0]), int(launches[0]), float(nbytes[0])
@@ -310,8 +335,18 @@ def main() -> int:
     classes = ["primal", "dual", "prelude", "contacts", "coloring", "csr", "other"]
     breakdown = {c: {"ms": round(float(ms[i]), 3), "launches": int(launches[i])}
                  for i, c in enumerate(classes)}
-    print(f"# kernel time by class over the timed region: {breakdown}", file=sys.stderr,
-          flush=True)
+    print(f"# kernel time by class over {args.profile_frames} profiled frames: {breakdown}",
+          file=sys.stderr, flush=True)
+    fs = sorted(frame_ms)
+    pct = lambda q: fs[min(len(fs) - 1, int(round(q * (len(fs) - 1))))]
+    timing = {"timed_frames": args.steps,
+              "launch": ("direct launches per substep" if args.direct else
+                         "resident frames: substeps replayed from a captured HIP graph"),
+              "graph_substeps": graphs1 - graphs0, "host_syncs": syncs1 - syncs0,
+              "frame_ms_p50": round(pct(0.5), 3), "frame_ms_p99": round(pct(0.99), 3),
+              "frame_ms_max": round(fs[-1], 3),
+              "contacts_dropped_by_fixed_pools": bool(dropped),
+              "profiled_frames": args.profile_frames}
     if args.workload == "batched":
         workload = (f"{args.scenes} independent piles of 8-1024 boxes (log-uniform, "
                     f"{n_b} boxes in all) stepped as one batch, dt 1/60, {s.substeps} substeps, "
@@ -354,10 +389,15 @@ def main() -> int:
             "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
         },
         "kernel_ms": breakdown,
+        "timing": timing,
     }
     if args.workload == "batched":
         out["config"]["scenes"] = args.scenes
         out["config"]["scene_bodies_min_max"] = [int(sizes.min()), int(sizes.max())]
+        # every scene of the batch advances in the same launches, so a
+        # scene's frame latency is the batch's frame time
+        out["per_scene_latency_ms"] = {"p50": timing["frame_ms_p50"],
+                                       "p99": timing["frame_ms_p99"]}
     if args.residual and args.workload == "pyramid":
         t0 = time.perf_counter()
         out["iters_to_residual"] = iters_to_residual()

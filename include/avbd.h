@@ -159,6 +159,66 @@ int avbd_step_frame(avbd_solver* s, const avbd_params* p, int n_substeps, int co
  * tests/test_robustness.py:335-395). */
 int avbd_sync_count(avbd_solver* s, long long* syncs, long long* graph_launches);
 
+/* ---- one substep, stage by stage -------------------------------------------
+ * The stages of avbd_substep as separate calls, in the reference's launch
+ * order (solver_6dof.py:1269-1464): contact generation (:1277-1309),
+ * coloring (:1333-1356), body->constraint lists (:1360-1366), prelude
+ * (predict_inertial_6dof + substep_prelude_6dof + cache_alpha_C0_6dof,
+ * :1368-1394), then sweeps 0 .. iterations + post_stabilize - 1 (each: the
+ * primal sweep over every color, :1397-1424, the dual update after sweeps
+ * < iterations, :1425-1432, and the finalize after the last dual, :1434-1440),
+ * then the end (contact cache refresh, :1442-1464). Called in that order with
+ * the same params they give the bits of one avbd_substep; out of order they
+ * return AVBD_E_STATE and the substep must start again from the contacts.
+ * The coloring stage adds its fields to the report of the contacts stage. */
+int avbd_stage_contacts(avbd_solver* s, const avbd_params* p, avbd_report* out);
+int avbd_stage_coloring(avbd_solver* s, const avbd_params* p, int color_dirty, avbd_report* out);
+int avbd_stage_csr(avbd_solver* s);
+int avbd_stage_prelude(avbd_solver* s, const avbd_params* p);
+int avbd_stage_sweep(avbd_solver* s, const avbd_params* p, int sweep);
+int avbd_stage_end(avbd_solver* s, const avbd_params* p);
+
+/* ---- streams and device memory ---------------------------------------------
+ * Every launch and copy of the solver goes to one HIP stream: its own
+ * (created with the solver) or one the caller sets (a hipStream_t, NULL to go
+ * back to the solver's own). Setting waits for the work queued on the
+ * previous stream. The caller keeps a stream it set alive while the solver
+ * uses it. The reference's arrays live on its Warp device
+ * (solver_6dof.py:904-1174) and its launches go to Warp's current stream. */
+int avbd_get_stream(avbd_solver* s, void** stream);
+int avbd_set_stream(avbd_solver* s, void* stream);
+
+/* Device addresses of the live state (solver_6dof.py:213-225: x, q, v, omega,
+ * c_lambda, c_penalty, c_active), valid until the next substep or flush
+ * (the pose buffer alternates between substeps). Element strides in 4-byte
+ * words:
+ *   body i:   x[i*body_stride + 0..2], q[i*body_stride + 0..3] (xyzw),
+ *             v[i*body_stride + 0..2], w[i*body_stride + 0..2];
+ *   group g:  lambda[g*group_stride + r], penalty[g*group_stride + r] for its
+ *             rows r < nrows; state_bits[g*group_stride]: bit r = row r
+ *             active, bit 3 + r = row r static (sticking);
+ *             header[g*header_stride + 0..3] = {kind | (nrows - 1) << 2 |
+ *             ..., body_a, body_b, first row}: row j of the reference's flat
+ *             row arrays is (g, r) with j = first row + r; kind 0 floor,
+ *             1 box-box, 2 pin.
+ * Groups: the static rows' groups first, then this substep's contacts. */
+typedef struct avbd_views {
+  int n_bodies;
+  float* x;
+  float* q;
+  float* v;
+  float* w;
+  int body_stride;
+  int n_groups;
+  float* lambda;
+  float* penalty;
+  int* state_bits;
+  int group_stride;
+  int* header;
+  int header_stride;
+} avbd_views;
+int avbd_device_views(avbd_solver* s, avbd_views* out);
+
 /* ---- readback / write-back ------------------------------------------ */
 /* positions()/orientations()/velocities()/angular_velocities()
  * (solver_6dof.py:1920-1938); any pointer may be NULL. */

@@ -38,7 +38,11 @@ def load() -> C.CDLL:
                                                                         "profile_enable",
                                                                         "profile_read",
                                                                         "step_frame",
-                                                                        "sync_count")
+                                                                        "sync_count",
+                                                                        "get_stream",
+                                                                        "set_stream",
+                                                                        "device_views")
+                                   and not k.startswith("stage_")
                                    and k not in N.ML_NAMES and k not in N.P_GPU_ONLY])
         lib.orc_create.argtypes = [C.POINTER(C.c_void_p)]
         lib.orc_create.restype = C.c_int

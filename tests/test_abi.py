@@ -39,3 +39,6 @@ def test_struct_layout_matches_header():
     # 1 int + 11 ptrs + 1 int + 5 ptrs + 11 ptrs + 2 ptrs + 4 ints (with padding)
     ptr = ctypes.sizeof(ctypes.c_void_p)
     assert ctypes.sizeof(N.Scene) == 8 + 11 * ptr + 8 + 18 * ptr + 16
+    # avbd_views: int + 4 ptrs + 2 ints + 3 ptrs + int + ptr + int (with padding)
+    assert ctypes.sizeof(N.Views) == 8 + 4 * ptr + 8 + 3 * ptr + 8 + ptr + 8
+    assert N.Views.lam.offset == 8 + 4 * ptr + 8 and N.Views.header.offset == 24 + 7 * ptr
