@@ -252,9 +252,12 @@ __device__ __forceinline__ void row_values(RowVals& o, bool& gflag, v3 jl, v3 ja
   float f = clampf_(lam_plus, f_lo, f_hi);
   float k_lhs = kp;
   float absC = fabs_(C);
-  if (absC > 1.0e-12f) {
-    if (lam_plus < f_lo) k_lhs = fabs_(f_lo - lam_plus) / absC;
-    else if (lam_plus > f_hi) k_lhs = fabs_(f_hi - lam_plus) / absC;
+  // Eq. 14: the bound that clipped lambda+ over |C| (one division for
+  // either bound)
+  const bool below = lam_plus < f_lo;
+  if (absC > 1.0e-12f && (below || lam_plus > f_hi)) {
+    const float bound = below ? f_lo : f_hi;
+    k_lhs = fabs_(bound - lam_plus) / absC;
   }
   o.jl = jl;
   o.ja = ja;
@@ -687,7 +690,7 @@ __device__ __forceinline__ int sweep_stamp(const SweepFlow& fl) {
 }
 
 template <int BPW, bool kFlow>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BPW >= 4 ? 3 : 1)))
 k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
                const float4* __restrict__ inert, const float4* __restrict__ bstat,
@@ -753,13 +756,27 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     rows[kRowFields * r + kFieldNegZero][lane] = -0.0f;
   }
   // Tile slots: the CSR entries of the tile's positions, contiguous and
-  // padded per body to a multiple of four.
+  // padded per body to a multiple of four. A pass gives each body a chunk of
+  // kChunk consecutive slots of its own list (lanes [b * kChunk, (b + 1) *
+  // kChunk) for body b), so that every body's slots are evaluated and
+  // accumulated at the same pace and the accumulation of a pass costs
+  // kChunk / 4 steps whatever the bodies' list lengths.
+  constexpr int kChunk = (64 / BPW) & ~3;
   const int e0 = cpos[base + first];
   const int total = cpos[base + (first + BPW < count ? first + BPW : count)] - e0;
+  const int pb = lane / kChunk, pq = lane % kChunk;  // this lane's body and slot in the chunk
+  int q_lo = 0, q_hi = 0;                            // its body's CSR range
+  if (pb < BPW) {
+    const int s0 = first + pb < count ? first + pb : count;
+    const int s1 = first + pb + 1 < count ? first + pb + 1 : count;
+    q_lo = cpos[base + s0];
+    q_hi = cpos[base + s1];
+  }
   // Loads of the first pass go out before the body set-up below, so that
   // their latency overlaps it: list entry, group records and (when the
   // neighbour cannot move during this launch) the other body's pose.
-  int2 ent = lane < total ? csr[e0 + lane] : make_int2(-1, -1);
+  int slot = q_lo + pq;
+  int2 ent = slot < q_hi ? csr[slot] : make_int2(-1, -1);
   GroupGeo Gq;
   GroupSt Sq;
   float4 oxq, oqq;
@@ -780,10 +797,10 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     b_off[lane] = cpos[base + q] - e0;
   }
   if (lane < BPW) {
-    int slot = first + lane;
+    const int pos = first + lane;
     int i = -1, id = -1;
-    if (slot < count) {
-      id = i = color_bodies[base + slot];
+    if (pos < count) {
+      id = i = color_bodies[base + pos];
       if (!(bstat[i].x > 0.0f)) i = -1;
     }
     b_i[lane] = i;
@@ -814,39 +831,56 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
     }
   }
   __syncthreads();
-  // Accumulator lanes: kLanes lanes per body, each owning kEpl entries
-  // (entry ac + k * kLanes, k < kEpl, when below 30).
-  constexpr int kLanes = 64 / BPW;
+  // passes: the longest list of the tile in chunks
+  int longest = 0;
+#pragma unroll
+  for (int b = 0; b < BPW; ++b) {
+    const int nb_slots = b_off[b + 1] - b_off[b];
+    longest = nb_slots > longest ? nb_slots : longest;
+  }
+  const int n_pass = (longest + kChunk - 1) / kChunk;
+  // Accumulator lanes: kChunk lanes per body, each owning kEpl entries
+  // (entry ac + k * kChunk, k < kEpl, when below 30).
+  constexpr int kLanes = kChunk;
   constexpr int kEpl = (kComps + kLanes - 1) / kLanes;
-  const int ab = lane / kLanes, al = lane % kLanes;
+  const int ab = pb, al = pq;
   const bool body_ok = ab < BPW && b_i[ab < BPW ? ab : 0] >= 0;
-  int fa[kEpl], fb[kEpl], fz[kEpl], fg[kEpl];
+  const int ab_slots = ab < BPW ? b_off[ab + 1] - b_off[ab] : 0;
+  // With an even number of entries per lane, the first half of a lane's
+  // entries are A and B entries (0-14), which take no geometric-stiffness
+  // term (their g is the constant -0, and x + -0 == x), so their additions
+  // of g are left out; the second half are D, r_lin and r_ang (15-29).
+  constexpr bool kSplit = (kEpl % 2) == 0;
+  constexpr int kHalf = kEpl / 2;
+  int fa[kEpl], fb[kEpl], fz[kEpl], fg[kEpl], acs[kEpl];
   bool own[kEpl];
   float accv[kEpl];
+  bool is_acc = false;
 #pragma unroll
   for (int k = 0; k < kEpl; ++k) {
-    const int ac = al + k * kLanes;
-    own[k] = body_ok && ac < kComps;
-    entry_fields(ac < kComps ? ac : 0, fa[k], fb[k], fz[k], fg[k]);
-    accv[k] = own[k] ? sys[ab][ac] : 0.0f;
+    const int ac = !kSplit ? al + k * kLanes
+                           : (k < kHalf ? al + k * kLanes : 15 + al + (k - kHalf) * kLanes);
+    const int lim = (kSplit && k < kHalf) ? 15 : kComps;
+    own[k] = body_ok && ac < lim;
+    acs[k] = ac < lim ? ac : 0;
+    entry_fields(acs[k], fa[k], fb[k], fz[k], fg[k]);
+    accv[k] = own[k] ? sys[ab][acs[k]] : 0.0f;
+    is_acc = is_acc || own[k];
   }
-  const bool is_acc = own[0];
-  for (int p0 = 0; p0 < total; p0 += 64) {
+  const bool self_ok = pb < BPW && b_i[pb < BPW ? pb : 0] >= 0;
+  const int self_id = pb < BPW ? b_id[pb] : -1;
+  for (int p = 0; p < n_pass; ++p) {
     // ---- A: row values of one incidence per lane ----
-    const int j = p0 + lane;
+    const bool live = slot < q_hi;
     // list entry of this lane's slot in the next pass
-    const int jn = j + 64;
-    const int2 ent_next = jn < total ? csr[e0 + jn] : make_int2(-1, -1);
+    const int slot_next = slot + kChunk;
+    const int2 ent_next = slot_next < q_hi ? csr[slot_next] : make_int2(-1, -1);
     if (kFlow) {
       // wait for this pass's lower-color neighbours (their color and stamp
       // were loaded with the group records), then read their poses through
       // the coherent path, after the stamp was seen
-      bool need = false;
-      if (ent.x >= 0 && ent.y >= 0) {
-        int bb = 0;
-        while (bb + 1 < BPW && j >= b_off[bb + 1]) ++bb;
-        need = ent.y != b_id[bb] && colq < my_color && flagq != stamp;
-      }
+      bool need = ent.x >= 0 && ent.y >= 0 && ent.y != self_id && colq < my_color &&
+                  flagq != stamp;
       int spins = 0;
       while (need) {
         if (__hip_atomic_load(fl.done_rd + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
@@ -864,15 +898,13 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
         oqq = ld_agent(pose + 2 * ent.y + 1);
       }
     }
-    if (j < total) {
-      int b = 0;
-      while (b + 1 < BPW && j >= b_off[b + 1]) ++b;
+    if (live) {
       RowVals rv[3];
       int mask = 0, gmask = 0;
-      if (ent.x >= 0 && b_i[b] >= 0) {
-        v3 xi = mk3(b_pose[b][0], b_pose[b][1], b_pose[b][2]);
-        q4 qi = mk4(b_pose[b][3], b_pose[b][4], b_pose[b][5], b_pose[b][6]);
-        mask = group_rows(rv, gmask, b_i[b], xi, qi, Gq, Sq, oxq, oqq, post_pass);
+      if (ent.x >= 0 && self_ok) {
+        v3 xi = mk3(b_pose[pb][0], b_pose[pb][1], b_pose[pb][2]);
+        q4 qi = mk4(b_pose[pb][3], b_pose[pb][4], b_pose[pb][5], b_pose[pb][6]);
+        mask = group_rows(rv, gmask, b_i[pb], xi, qi, Gq, Sq, oxq, oqq, post_pass);
       }
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
@@ -893,6 +925,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       }
     }
     // loads of the next pass, in flight during this pass's accumulation
+    slot = slot_next;
     ent = ent_next;
     if (ent.x >= 0) {
       Gq = geo[ent.x];
@@ -906,12 +939,13 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       }
     }
     __syncthreads();
-    // ---- B: ordered accumulation, four slots per step ----
+    // ---- B: ordered accumulation of the body's chunk, four slots per step ----
     if (is_acc) {
-      const int s0 = b_off[ab] > p0 ? b_off[ab] - p0 : 0;
-      const int s1 = b_off[ab + 1] - p0 < 64 ? b_off[ab + 1] - p0 : 64;
+      const int s0 = ab * kChunk;
+      const int left = ab_slots - p * kChunk;
+      const int s1 = s0 + (left < kChunk ? (left > 0 ? left : 0) : kChunk);
       for (int sb = s0; sb < s1; sb += 4) {
-        __builtin_assume((sb & 3) == 0);  // body slot ranges start on multiples of 4
+        __builtin_assume((sb & 3) == 0);  // chunks and list lengths are multiples of 4
 #pragma unroll
         for (int k = 0; k < kEpl; ++k) {
           float4 X[3], Y[3], Z[3], Gd[3];
@@ -924,6 +958,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
             Gd[r] = *reinterpret_cast<const float4*>(&rows[rb + fg[k]][sb]);
           }
           float acc = accv[k];
+          const bool no_g = kSplit && k < kHalf;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
 #pragma unroll
@@ -932,7 +967,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
               const float y = u == 0 ? Y[r].x : u == 1 ? Y[r].y : u == 2 ? Y[r].z : Y[r].w;
               const float z = u == 0 ? Z[r].x : u == 1 ? Z[r].y : u == 2 ? Z[r].z : Z[r].w;
               const float g = u == 0 ? Gd[r].x : u == 1 ? Gd[r].y : u == 2 ? Gd[r].z : Gd[r].w;
-              acc = (acc + (x * y) * z) + g;
+              acc = no_g ? acc + (x * y) * z : (acc + (x * y) * z) + g;
             }
           }
           accv[k] = acc;
@@ -943,7 +978,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
   }
 #pragma unroll
   for (int k = 0; k < kEpl; ++k)
-    if (own[k]) sys[ab][al + k * kLanes] = accv[k];
+    if (own[k]) sys[ab][acs[k]] = accv[k];
   __syncthreads();
   // ---- C: solve ----
   if (lane < BPW && b_i[lane] >= 0) {
