@@ -196,6 +196,8 @@ SIGNATURES = {
     "ml_read_samples": ([C.c_void_p, C.c_int, F32P, F32P, I32P, I32P], C.c_int),
     "mlp_forward": ([C.c_int, C.c_int, F32P, F32P, F32P], C.c_int),
     "mlp_grad": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_float, F32P, F32P], C.c_int),
+    "mlp_grad_noalloc": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_float, F32P, F32P],
+                         C.c_int),
     "mlp_train": ([C.c_int, C.c_int, F32P, F32P, I32P, F32P, C.c_int, C.c_int, C.c_float,
                    C.c_float, C.c_uint, F32P], C.c_int),
     # 3-DOF particle solver (Solver, src/avbd3d/solver.py)
@@ -214,7 +216,7 @@ SIGNATURES = {
 
 # entry points of batched scenes and the learned warm start (the checker has none)
 ML_NAMES = ("set_scenes", "mlp_num_params", "ml_set_mode", "ml_set_params", "ml_read_samples", "mlp_forward",
-            "mlp_grad", "mlp_train")
+            "mlp_grad", "mlp_grad_noalloc", "mlp_train")
 
 # entry points of the particle solver; the checker has all but the last
 # three (graph replay and launch timing are properties of the GPU path)

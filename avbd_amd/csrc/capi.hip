@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -2127,6 +2128,50 @@ struct MlpJob {
 
 }  // namespace
 
+// ---- avbd_mlp_grad without allocation ------------------------------------
+// Staging for avbd_mlp_grad_noalloc in one static device array per GPU (in
+// floats): samples x [kNoAllocMaxSamples][24], targets [..][4], masks [..],
+// parameters, the gradient and the loss, each region 256-byte aligned.
+namespace {
+constexpr int kNoAllocMaxSamples = 4096;
+constexpr int kNoAllocMaxParams = 8192;
+constexpr size_t kScrX = 0;
+constexpr size_t kScrY = kScrX + (size_t)kNoAllocMaxSamples * 24;
+constexpr size_t kScrM = kScrY + (size_t)kNoAllocMaxSamples * 4;
+constexpr size_t kScrP = kScrM + (size_t)kNoAllocMaxSamples;
+constexpr size_t kScrG = kScrP + kNoAllocMaxParams;
+constexpr size_t kScrL = kScrG + kNoAllocMaxParams;
+constexpr size_t kScrFloats = kScrL + 64;   // 135,232 floats = 528.25 KiB
+static_assert(kScrY % 64 == 0 && kScrM % 64 == 0 && kScrP % 64 == 0 && kScrG % 64 == 0 &&
+              kScrL % 64 == 0, "scratch regions stay 256-byte aligned");
+}  // namespace
+
+// The name `scratch` is a host-side stand-in in host code: its device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) float scratch[kScrFloats];
+
+namespace {
+constexpr int kScrMaxDevices = 64;
+std::mutex g_scr_mu;
+float* g_scr_ptr[kScrMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current): looked
+// up at the first call on that device and kept.
+int scratch_ptr(int device, float** out) {
+  std::lock_guard<std::mutex> lock(g_scr_mu);
+  if (device < 0 || device >= kScrMaxDevices) return set_err(AVBD_E_ARG, "device ordinal out of range");
+  if (g_scr_ptr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return set_err(AVBD_E_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scr_ptr[device] = static_cast<float*>(p);
+  }
+  *out = g_scr_ptr[device];
+  return AVBD_OK;
+}
+}  // namespace
+
 int avbd_mlp_forward(int device, int n, const float* x, const float* params, float* y) {
   if (!y) return set_err(AVBD_E_ARG, "null output");
   MlpJob j;
@@ -2155,6 +2200,53 @@ int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* 
   HIPCK(hipMemcpyAsync(grad, j.G.p, nw * sizeof(float), hipMemcpyDeviceToHost, j.st));
   HIPCK(hipMemcpyAsync(loss, j.L.p, sizeof(float), hipMemcpyDeviceToHost, j.st));
   HIPCK(hipStreamSynchronize(j.st));
+  return AVBD_OK;
+}
+
+int avbd_mlp_grad_noalloc(int device, int n, const float* x, const float* y, const int* mask,
+                          const float* params, float huber, float* grad, float* loss) {
+  if (!x || !y || !params || !grad || !loss) return set_err(AVBD_E_ARG, "null argument");
+  if (n <= 0) return set_err(AVBD_E_ARG, "empty sample set");
+  if (n > kNoAllocMaxSamples)
+    return set_err(AVBD_E_ARG, "avbd_mlp_grad_noalloc: " + std::to_string(n) +
+                                   " samples exceed its static scratch limit of " +
+                                   std::to_string(kNoAllocMaxSamples) +
+                                   " (avbd_mlp_grad takes any count)");
+  const int np = mlp_num_params(), nw = mlp_num_weights();
+  if (np > kNoAllocMaxParams || nw > kNoAllocMaxParams)
+    return set_err(AVBD_E_STATE, "network larger than the static scratch layout");
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
+    return set_err(AVBD_E_HIP, "no HIP device available");
+  if (device < 0 || device >= nd) return set_err(AVBD_E_ARG, "device ordinal out of range");
+  HIPCK(hipSetDevice(device));
+  float* base = nullptr;
+  CK(scratch_ptr(device, &base));
+  float* X = base + kScrX;
+  float* Y = base + kScrY;
+  int* M = reinterpret_cast<int*>(base + kScrM);
+  float* P = base + kScrP;
+  float* G = base + kScrG;
+  float* L = base + kScrL;
+  // the null stream: creating a stream would allocate; the copies below are
+  // ordered with it
+  hipStream_t st = nullptr;
+  HIPCK(hipMemcpy(X, x, (size_t)n * 24 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(Y, y, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(P, params, (size_t)np * sizeof(float), hipMemcpyHostToDevice));
+  if (mask) {
+    HIPCK(hipMemcpy(M, mask, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  } else {
+    launch_fill_int(st, M, n, 15);
+  }
+  // the gradient and the loss are accumulated: they start at zero on every
+  // call (the array keeps the previous call's contents)
+  launch_fill_int(st, reinterpret_cast<int*>(G), nw, 0);
+  launch_fill_int(st, reinterpret_cast<int*>(L), 1, 0);
+  launch_mlp_train_tile(st, n, nullptr, X, Y, M, P, 1.0f / (float)n, huber, G, L);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpy(grad, G, (size_t)nw * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(loss, L, sizeof(float), hipMemcpyDeviceToHost));
   return AVBD_OK;
 }
 

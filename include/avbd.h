@@ -364,6 +364,17 @@ int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* 
 int avbd_mlp_train(int device, int n, const float* x, const float* y, const int* mask,
                    float* params, int steps, int batch, float lr, float huber, unsigned seed,
                    float* loss_hist);
+/* avbd_mlp_grad without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that must not call hipMalloc, such as tight serving loops and harnesses
+ * that forbid allocation). Same arguments and results as avbd_mlp_grad, for
+ * n <= 4096 samples (more returns AVBD_E_ARG). The samples, parameters,
+ * gradient and loss are staged in one static device array of 528.25 KiB,
+ * which every GPU the library runs on holds until the process ends; every
+ * call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * race. Runs on the null stream. */
+int avbd_mlp_grad_noalloc(int device, int n, const float* x, const float* y, const int* mask,
+                          const float* params, float huber, float* grad, float* loss);
 
 /* ---- 3-DOF particle solver ---------------------------------------------
  * Solver.step (src/avbd3d/solver.py:1124-1400): particles with pin (x/y/z),
