@@ -45,14 +45,33 @@ PRIMAL_KERNELS = ["k_primal_color (one launch per color)", "k_primal_tiled<1> pe
 TRAFFIC_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_pmc.md"
 
 
+# The sources the primal sweep kernel is built from: a traffic figure
+# measured on other sources does not describe this build.
+KERNEL_SOURCES = ["avbd_amd/csrc/kernels_solve.hip", "avbd_amd/csrc/avbd_internal.h",
+                  "avbd_amd/csrc/avbd_fp.h"]
+
+
+def kernel_source_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update((ROOT / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(layers: int, variant: int) -> float | None:
     """Bytes per primal launch from the committed PMC summary, when it was
-    taken on this workload and kernel variant; None otherwise."""
+    taken on this workload, kernel variant and kernel sources; None
+    otherwise."""
     if layers != 66 or not TRAFFIC_FILE.exists():
         return None
-    fields = dict(line.split(":", 1) for line in TRAFFIC_FILE.read_text().splitlines()
-                  if line.startswith(("traffic_bytes_per_launch:", "primal_variant:")))
+    keys = ("traffic_bytes_per_launch:", "primal_variant:", "kernel_source_sha256:")
+    fields = {k.strip(): v.strip() for k, v in
+              (line.split(":", 1) for line in TRAFFIC_FILE.read_text().splitlines()
+               if line.startswith(keys))}
     if int(fields.get("primal_variant", "-1")) != variant:
+        return None
+    if fields.get("kernel_source_sha256") != kernel_source_hash():
         return None
     return float(fields["traffic_bytes_per_launch"])
 
