@@ -12,10 +12,11 @@
 //  k_mlp_infer          one wavefront per 16 contacts: three layers as
 //                       v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32
 //                       accumulation), weights staged in LDS, SiLU in fp32
-//  k_mlp_train_tile     forward + backward of 64 samples per workgroup in
-//                       fp32 on the same bf16-rounded weights and layer
-//                       inputs the inference kernel uses; weight gradients
-//                       summed per workgroup, then one atomic add each
+//  k_mlp_train_tile     forward + backward of 64 samples per workgroup on
+//                       v_mfma_f32_16x16x32_bf16 (bf16 weights, layer
+//                       inputs and gradients, fp32 accumulation); weight
+//                       gradients summed per workgroup, then one atomic
+//                       add each
 //  k_adam               Adam step on the fp32 master weights
 //
 // Network: x (24) -> 64 SiLU -> 64 SiLU -> 4. Parameters (fp32, in order):
@@ -342,130 +343,247 @@ void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* par
 }
 
 // ---------------------------------------------------------------------------
-// training: fp32 forward/backward on bf16-rounded weights and layer inputs
+// training: forward and backward on MFMA (bf16 operands, fp32 accumulation)
 // ---------------------------------------------------------------------------
-// One workgroup (256 lanes) per tile of 64 samples; sample s of the batch is
-// row idx[s] of X, Y, ymask (row s when idx is null). Loss = sum over samples
-// and used outputs of h(yhat - y) * inv_total, y normalised, with
+// One workgroup (4 wavefronts) per tile of 64 samples; sample s of the batch
+// is row idx[s] of X, Y, ymask (row s when idx is null). Loss = sum over
+// samples and used outputs of h(yhat - y) * inv_total, y normalised, with
 // h(d) = d^2 (huber <= 0) or the Huber loss d^2 for |d| <= huber and
 // huber (2|d| - huber) beyond; mask bit o of ymask says whether output o is
 // used.
+//
+// Every product runs as v_mfma_f32_16x16x32_bf16 except the 4-wide layer-3
+// backward (VALU). Rounded to bf16 as MFMA operands: the weights, the
+// normalised input, both hidden activations (as in inference), the output
+// error dy and the pre-activation gradients dz2, dz1. The silu'
+// factors, the bias gradients (sums of the unrounded dy, dz2, dz1) and all
+// accumulation stay fp32.
+//
+// MFMA 16x16x32 operand layout, lane l = 16 kq + c (kq = l / 16, c = l % 16):
+// A [16 x 32]: a[j] = A[c][8 kq + j]; B [32 x 16]: b[j] = B[8 kq + j][c];
+// D [16 x 16]: d[r] = D[4 kq + r][c].
+__device__ __forceinline__ floatx4 mfma16(const bf16x8& a, const bf16x8& b, floatx4 acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+
 __global__ void __launch_bounds__(256)
 k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X,
                  const float* __restrict__ Y, const int* __restrict__ ymask,
                  const float* __restrict__ params, float inv_total, float huber,
-                 float* __restrict__ grad,
-                 float* __restrict__ loss) {
-  __shared__ float W1[kIn * kHid], W2[kHid * kHid], W3[kHid * kOut];
-  __shared__ float B1[kHid], B2[kHid], B3[kOut];
-  __shared__ float xin[64][kIn];
-  __shared__ float z1[64][kHid], a1[64][kHid], z2[64][kHid], a2[64][kHid];
-  __shared__ float dy[64][kOut];
-  __shared__ float lsum;
-  const int tid = threadIdx.x;
-  for (int k = tid; k < kIn * kHid; k += 256) W1[k] = bf16r(params[kOffW1 + k]);
-  for (int k = tid; k < kHid * kHid; k += 256) W2[k] = bf16r(params[kOffW2 + k]);
-  for (int k = tid; k < kHid * kOut; k += 256) W3[k] = bf16r(params[kOffW3 + k]);
-  for (int k = tid; k < kHid; k += 256) { B1[k] = params[kOffB1 + k]; B2[k] = params[kOffB2 + k]; }
-  if (tid < kOut) B3[tid] = params[kOffB3 + tid];
-  if (tid == 0) lsum = 0.0f;
-  const int s0 = blockIdx.x * 64;
+                 float* __restrict__ grad, float* __restrict__ loss) {
+  __shared__ __bf16 w1[32][64];    // W1 [in][hid], rows 24-31 zero
+  __shared__ __bf16 w2[64][64];    // W2 [hid][hid]
+  __shared__ __bf16 w3[64][16];    // W3 [hid][out], columns 4-15 zero
+  __shared__ float bias[64 + 64 + 4];
+  __shared__ __bf16 xs[64][40];    // layer-1 input [sample][in], columns 24-39 zero
+  __shared__ __bf16 a1s[64][72];   // layer-2 input [sample][hid]
+  __shared__ __bf16 a2s[64][72];   // layer-3 input [sample][hid]
+  __shared__ float dyf[64][4];     // dy [sample][out], fp32
+  __shared__ __bf16 dyb[64][16];   // dy as an MFMA operand, columns 4-15 zero
+  __shared__ __bf16 dz2b[64][72];  // dz2 [sample][hid]
+  __shared__ __bf16 dz1b[64][72];  // dz1 [sample][hid]
   __shared__ int rows[64];
-  if (tid < 64) rows[tid] = s0 + tid < n ? (idx ? idx[s0 + tid] : s0 + tid) : -1;
+  __shared__ float lsum;
+  const int tid = threadIdx.x, wave = tid / 64, l = tid % 64;
+  const int c = l % 16, kq = l / 16;
+  for (int k = tid; k < 32 * 64; k += 256) {
+    const int r = k / 64, col = k % 64;
+    w1[r][col] = (__bf16)(r < kIn ? params[kOffW1 + r * kHid + col] : 0.0f);
+  }
+  for (int k = tid; k < 64 * 64; k += 256) w2[k / 64][k % 64] = (__bf16)params[kOffW2 + k];
+  for (int k = tid; k < 64 * 16; k += 256) {
+    const int r = k / 16, col = k % 16;
+    w3[r][col] = (__bf16)(col < kOut ? params[kOffW3 + r * kOut + col] : 0.0f);
+  }
+  for (int k = tid; k < 132; k += 256)
+    bias[k] = k < 64 ? params[kOffB1 + k] : k < 128 ? params[kOffB2 + k - 64] : params[kOffB3 + k - 128];
+  const int t0 = blockIdx.x * 64;
+  if (tid < 64) rows[tid] = t0 + tid < n ? (idx ? idx[t0 + tid] : t0 + tid) : -1;
+  if (tid == 0) lsum = 0.0f;
+  for (int k = tid; k < 64 * 16; k += 256) dyb[k / 16][k % 16] = (__bf16)0.0f;
   __syncthreads();
-  for (int k = tid; k < 64 * kIn; k += 256) {
-    const int s = k / kIn, c = k % kIn;
+  for (int k = tid; k < 64 * 40; k += 256) {
+    const int s = k / 40, col = k % 40;
     float v = 0.0f;
-    if (rows[s] >= 0) v = (X[(size_t)rows[s] * kIn + c] - params[kOffXm + c]) / params[kOffXs + c];
-    xin[s][c] = bf16r(v);
+    if (col < kIn && rows[s] >= 0)
+      v = (X[(size_t)rows[s] * kIn + col] - params[kOffXm + col]) / params[kOffXs + col];
+    xs[s][col] = (__bf16)v;
   }
   __syncthreads();
-  // forward
-  for (int k = tid; k < 64 * kHid; k += 256) {
-    const int s = k / kHid, u = k % kHid;
-    float z = 0.0f;
-    for (int i = 0; i < kIn; ++i) z += xin[s][i] * W1[i * kHid + u];
-    z += B1[u];
-    z1[s][u] = z;
-    a1[s][u] = bf16r(silu(z));
-  }
-  __syncthreads();
-  for (int k = tid; k < 64 * kHid; k += 256) {
-    const int s = k / kHid, u = k % kHid;
-    float z = 0.0f;
-    for (int i = 0; i < kHid; ++i) z += a1[s][i] * W2[i * kHid + u];
-    z += B2[u];
-    z2[s][u] = z;
-    a2[s][u] = bf16r(silu(z));
-  }
-  __syncthreads();
-  for (int k = tid; k < 64 * kOut; k += 256) {
-    const int s = k / kOut, o = k % kOut;
-    float d = 0.0f;
-    if (rows[s] >= 0 && (ymask[rows[s]] >> o & 1)) {
-      float yh = B3[o];
-      for (int i = 0; i < kHid; ++i) yh += a2[s][i] * W3[i * kOut + o];
-      const float yt = (Y[(size_t)rows[s] * kOut + o] - params[kOffYm + o]) / params[kOffYs + o];
-      d = yh - yt;
-      if (huber > 0.0f && fabsf(d) > huber) {
-        atomicAdd(&lsum, huber * (2.0f * fabsf(d) - huber) * inv_total);
-        d = copysignf(huber, d);
-      } else {
-        atomicAdd(&lsum, d * d * inv_total);
+  const int s0 = wave * 16;  // this wavefront's samples in the forward pass
+  // ---- forward, layer 1: [16 x 32] x [32 x 64] ----
+  floatx4 sg1[4], sg2[4];  // silu'(z) of this lane's outputs D[4 kq + r][16 t + c]
+  {
+    bf16x8 a;
+    for (int j = 0; j < 8; ++j) a[j] = xs[s0 + c][8 * kq + j];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      bf16x8 b;
+      for (int j = 0; j < 8; ++j) b[j] = w1[8 * kq + j][16 * t + c];
+      floatx4 z = mfma16(a, b, floatx4{0.f, 0.f, 0.f, 0.f});
+      for (int r = 0; r < 4; ++r) {
+        const float zz = z[r] + bias[16 * t + c];
+        a1s[s0 + 4 * kq + r][16 * t + c] = (__bf16)silu(zz);
+        sg1[t][r] = silu_grad(zz);
       }
     }
-    dy[s][o] = 2.0f * d * inv_total;
   }
   __syncthreads();
-  // backward: grads of layer 3
-  for (int k = tid; k < kHid * kOut; k += 256) {
-    const int i = k / kOut, o = k % kOut;
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += a2[s][i] * dy[s][o];
-    atomicAdd(&grad[kOffW3 + k], g);
-  }
-  if (tid < kOut) {
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += dy[s][tid];
-    atomicAdd(&grad[kOffB3 + tid], g);
-  }
-  // dz2 (reuse z2)
-  for (int k = tid; k < 64 * kHid; k += 256) {
-    const int s = k / kHid, u = k % kHid;
-    float g = 0.0f;
-    for (int o = 0; o < kOut; ++o) g += dy[s][o] * W3[u * kOut + o];
-    z2[s][u] = g * silu_grad(z2[s][u]);
-  }
-  __syncthreads();
-  for (int k = tid; k < kHid * kHid; k += 256) {
-    const int i = k / kHid, u = k % kHid;
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += a1[s][i] * z2[s][u];
-    atomicAdd(&grad[kOffW2 + k], g);
-  }
-  for (int k = tid; k < kHid; k += 256) {
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += z2[s][k];
-    atomicAdd(&grad[kOffB2 + k], g);
-  }
-  // dz1 (reuse z1)
-  for (int k = tid; k < 64 * kHid; k += 256) {
-    const int s = k / kHid, u = k % kHid;
-    float g = 0.0f;
-    for (int v = 0; v < kHid; ++v) g += z2[s][v] * W2[u * kHid + v];
-    z1[s][u] = g * silu_grad(z1[s][u]);
+  // ---- layer 2: [16 x 64] x [64 x 64] ----
+  {
+    floatx4 z[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f},
+                    {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a;
+      for (int j = 0; j < 8; ++j) a[j] = a1s[s0 + c][32 * ks + 8 * kq + j];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 b;
+        for (int j = 0; j < 8; ++j) b[j] = w2[32 * ks + 8 * kq + j][16 * t + c];
+        z[t] = mfma16(a, b, z[t]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      for (int r = 0; r < 4; ++r) {
+        const float zz = z[t][r] + bias[64 + 16 * t + c];
+        a2s[s0 + 4 * kq + r][16 * t + c] = (__bf16)silu(zz);
+        sg2[t][r] = silu_grad(zz);
+      }
   }
   __syncthreads();
-  for (int k = tid; k < kIn * kHid; k += 256) {
-    const int i = k / kHid, u = k % kHid;
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += xin[s][i] * z1[s][u];
-    atomicAdd(&grad[kOffW1 + k], g);
+  // ---- layer 3: [16 x 64] x [64 x 16] (4 used), the loss and dy ----
+  {
+    floatx4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = a2s[s0 + c][32 * ks + 8 * kq + j];
+        b[j] = w3[32 * ks + 8 * kq + j][c];
+      }
+      y = mfma16(a, b, y);
+    }
+    if (c < kOut) {
+      for (int r = 0; r < 4; ++r) {
+        const int s = s0 + 4 * kq + r, row = rows[s];
+        float d = 0.0f;
+        if (row >= 0 && (ymask[row] >> c & 1)) {
+          const float yh = y[r] + bias[128 + c];
+          const float yt = (Y[(size_t)row * kOut + c] - params[kOffYm + c]) / params[kOffYs + c];
+          d = yh - yt;
+          if (huber > 0.0f && fabsf(d) > huber) {
+            atomicAdd(&lsum, huber * (2.0f * fabsf(d) - huber) * inv_total);
+            d = copysignf(huber, d);
+          } else {
+            atomicAdd(&lsum, d * d * inv_total);
+          }
+        }
+        const float g = 2.0f * d * inv_total;
+        dyf[s][c] = g;
+        dyb[s][c] = (__bf16)g;
+      }
+    }
   }
-  for (int k = tid; k < kHid; k += 256) {
-    float g = 0.0f;
-    for (int s = 0; s < 64; ++s) g += z1[s][k];
-    atomicAdd(&grad[kOffB1 + k], g);
+  __syncthreads();
+  // ---- dW3 = a2^T dy: [64 x 64 samples] x [64 samples x 16]; block of wavefront w: rows 16w.. ----
+  {
+    floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = a2s[32 * ks + 8 * kq + j][16 * wave + c];
+        b[j] = dyb[32 * ks + 8 * kq + j][c];
+      }
+      d = mfma16(a, b, d);
+    }
+    if (c < kOut)
+      for (int r = 0; r < 4; ++r) atomicAdd(&grad[kOffW3 + (16 * wave + 4 * kq + r) * kOut + c], d[r]);
+    if (wave == 0 && l < kOut) {
+      float g = 0.0f;
+      for (int s = 0; s < 64; ++s) g += dyf[s][l];
+      atomicAdd(&grad[kOffB3 + l], g);
+    }
+  }
+  // ---- dz2 = (dy W3^T) * silu'(z2) for this wavefront's samples (VALU, K = 4) ----
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int u = 16 * t + c;
+    float bsum = 0.0f;
+    for (int r = 0; r < 4; ++r) {
+      const int s = s0 + 4 * kq + r;
+      float h = 0.0f;
+      for (int o = 0; o < kOut; ++o) h += dyf[s][o] * (float)w3[u][o];
+      const float dz = h * sg2[t][r];
+      dz2b[s][u] = (__bf16)dz;
+      bsum += dz;
+    }
+    atomicAdd(&grad[kOffB2 + u], bsum);
+  }
+  __syncthreads();
+  // ---- dW2 = a1^T dz2: row block 16w, column blocks 0-3 ----
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = a1s[32 * ks + 8 * kq + j][16 * wave + c];
+        b[j] = dz2b[32 * ks + 8 * kq + j][16 * t + c];
+      }
+      d = mfma16(a, b, d);
+    }
+    for (int r = 0; r < 4; ++r)
+      atomicAdd(&grad[kOffW2 + (16 * wave + 4 * kq + r) * kHid + 16 * t + c], d[r]);
+  }
+  // ---- dz1 = (dz2 W2^T) * silu'(z1) for this wavefront's samples ----
+  {
+    floatx4 h[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f},
+                    {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a;
+      for (int j = 0; j < 8; ++j) a[j] = dz2b[s0 + c][32 * ks + 8 * kq + j];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 b;  // B[u][i] = W2[i][u]
+        for (int j = 0; j < 8; ++j) b[j] = w2[16 * t + c][32 * ks + 8 * kq + j];
+        h[t] = mfma16(a, b, h[t]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = 16 * t + c;
+      float bsum = 0.0f;
+      for (int r = 0; r < 4; ++r) {
+        const float dz = h[t][r] * sg1[t][r];
+        dz1b[s0 + 4 * kq + r][i] = (__bf16)dz;
+        bsum += dz;
+      }
+      atomicAdd(&grad[kOffB1 + i], bsum);
+    }
+  }
+  __syncthreads();
+  // ---- dW1 = x^T dz1: column block w, row blocks 0-1 (rows 24-31 dropped) ----
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib) {
+    floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = xs[32 * ks + 8 * kq + j][16 * ib + c];
+        b[j] = dz1b[32 * ks + 8 * kq + j][16 * wave + c];
+      }
+      d = mfma16(a, b, d);
+    }
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * ib + 4 * kq + r;
+      if (i < kIn) atomicAdd(&grad[kOffW1 + i * kHid + 16 * wave + c], d[r]);
+    }
   }
   __syncthreads();
   if (tid == 0) atomicAdd(loss, lsum);

@@ -177,8 +177,11 @@ def loss_grad_np(params: np.ndarray, X: np.ndarray, Y: np.ndarray, mask: np.ndar
     averaged over samples): squared error d^2, or with huber > 0 the Huber
     loss (d^2 for |d| <= huber, huber (2|d| - huber) beyond), and its
     gradient w.r.t. the weights (N_WEIGHTS values, parameter order). With
-    bf16 the roundings are treated as identities in the backward pass, as the
-    training kernel does."""
+    bf16 the backward rounds what the training kernel feeds its MFMA
+    products to bfloat16 (the output error dy and the pre-activation
+    gradients dz2, dz1, besides the forward's weights and activations), and
+    the roundings are treated as identities for differentiation; bias
+    gradients sum the unrounded values, as the kernel does."""
     P = unpack(params)
     n = len(X)
     _, c = forward_np(params, X, bf16)
@@ -195,14 +198,20 @@ def loss_grad_np(params: np.ndarray, X: np.ndarray, Y: np.ndarray, mask: np.ndar
     else:
         loss = float(np.sum(d * d) / n)
     dy = 2.0 * d / n
+    if bf16:
+        def rb(a):
+            return bf16_round(np.asarray(a, np.float32)).astype(np.float64)
+    else:
+        def rb(a):
+            return a
     g = {}
-    g["W3"] = c["a2"].T @ dy
+    g["W3"] = c["a2"].T @ rb(dy)
     g["b3"] = dy.sum(axis=0)
     dz2 = (dy @ c["W3"].T) * _silu_grad(c["z2"])
-    g["W2"] = c["a1"].T @ dz2
+    g["W2"] = c["a1"].T @ rb(dz2)
     g["b2"] = dz2.sum(axis=0)
-    dz1 = (dz2 @ c["W2"].T) * _silu_grad(c["z1"])
-    g["W1"] = c["x0"].T @ dz1
+    dz1 = (rb(dz2) @ c["W2"].T) * _silu_grad(c["z1"])
+    g["W1"] = c["x0"].T @ rb(dz1)
     g["b1"] = dz1.sum(axis=0)
     out = np.zeros(N_WEIGHTS)
     for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
