@@ -219,9 +219,13 @@ void launch_contact_targets(hipStream_t st, int n_dyn, int g_first, const GroupG
 void launch_mlp_infer(hipStream_t st, int n, const float* feat, const float* params, int g_first,
                       GroupSt* gst, const GroupGeo* geo, float ag, float gamma, int mode,
                       float* yout);
+// Gradient and loss of n samples: `part` holds ceil(n / 64) rows of
+// mlp_part_stride() floats (one per workgroup); grad and *loss are
+// overwritten with the rows' sums.
+int mlp_part_stride();
 void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
                            const int* ymask, const float* params, float inv_total, float huber,
-                           float* grad, float* loss);
+                           float* part, float* grad, float* loss);
 void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,
                  float b1, float b2, float c1, float c2, float eps);
 

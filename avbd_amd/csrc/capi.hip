@@ -2090,7 +2090,7 @@ namespace {
 // device, stream and the sample arrays of one standalone network call
 struct MlpJob {
   hipStream_t st = nullptr;
-  DBuf<float> X, Y, P, G, L;
+  DBuf<float> X, Y, P, G, L, Q;  // Q: per-workgroup gradient rows
   DBuf<int> M, I;
   ~MlpJob() {
     if (st) {
@@ -2131,7 +2131,8 @@ struct MlpJob {
 // ---- avbd_mlp_grad without allocation ------------------------------------
 // Staging for avbd_mlp_grad_noalloc in one static device array per GPU (in
 // floats): samples x [kNoAllocMaxSamples][24], targets [..][4], masks [..],
-// parameters, the gradient and the loss, each region 256-byte aligned.
+// parameters, the gradient, the loss and the per-workgroup gradient rows,
+// each region 256-byte aligned.
 namespace {
 constexpr int kNoAllocMaxSamples = 4096;
 constexpr int kNoAllocMaxParams = 8192;
@@ -2141,9 +2142,11 @@ constexpr size_t kScrM = kScrY + (size_t)kNoAllocMaxSamples * 4;
 constexpr size_t kScrP = kScrM + (size_t)kNoAllocMaxSamples;
 constexpr size_t kScrG = kScrP + kNoAllocMaxParams;
 constexpr size_t kScrL = kScrG + kNoAllocMaxParams;
-constexpr size_t kScrFloats = kScrL + 64;   // 135,232 floats = 528.25 KiB
+constexpr size_t kScrQ = kScrL + 64;        // per-workgroup gradient rows
+constexpr size_t kScrFloats = kScrQ + (size_t)(kNoAllocMaxSamples / 64) * kNoAllocMaxParams;
+// 659,520 floats = 2,638,080 bytes
 static_assert(kScrY % 64 == 0 && kScrM % 64 == 0 && kScrP % 64 == 0 && kScrG % 64 == 0 &&
-              kScrL % 64 == 0, "scratch regions stay 256-byte aligned");
+              kScrL % 64 == 0 && kScrQ % 64 == 0, "scratch regions stay 256-byte aligned");
 }  // namespace
 
 // The name `scratch` is a host-side stand-in in host code: its device
@@ -2192,10 +2195,9 @@ int avbd_mlp_grad(int device, int n, const float* x, const float* y, const int* 
   const int nw = mlp_num_weights();
   CK(j.G.alloc(nw));
   CK(j.L.alloc(1));
-  launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
-  launch_fill_int(j.st, reinterpret_cast<int*>(j.L.p), 1, 0);
-  launch_mlp_train_tile(j.st, n, nullptr, j.X.p, j.Y.p, j.M.p, j.P.p, 1.0f / (float)n, huber, j.G.p,
-                        j.L.p);
+  CK(j.Q.alloc((size_t)((n + 63) / 64) * mlp_part_stride()));
+  launch_mlp_train_tile(j.st, n, nullptr, j.X.p, j.Y.p, j.M.p, j.P.p, 1.0f / (float)n, huber, j.Q.p,
+                        j.G.p, j.L.p);
   HIPCK(hipGetLastError());
   HIPCK(hipMemcpyAsync(grad, j.G.p, nw * sizeof(float), hipMemcpyDeviceToHost, j.st));
   HIPCK(hipMemcpyAsync(loss, j.L.p, sizeof(float), hipMemcpyDeviceToHost, j.st));
@@ -2213,7 +2215,7 @@ int avbd_mlp_grad_noalloc(int device, int n, const float* x, const float* y, con
                                    std::to_string(kNoAllocMaxSamples) +
                                    " (avbd_mlp_grad takes any count)");
   const int np = mlp_num_params(), nw = mlp_num_weights();
-  if (np > kNoAllocMaxParams || nw > kNoAllocMaxParams)
+  if (np > kNoAllocMaxParams || nw > kNoAllocMaxParams || mlp_part_stride() > kNoAllocMaxParams)
     return set_err(AVBD_E_STATE, "network larger than the static scratch layout");
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
@@ -2228,6 +2230,7 @@ int avbd_mlp_grad_noalloc(int device, int n, const float* x, const float* y, con
   float* P = base + kScrP;
   float* G = base + kScrG;
   float* L = base + kScrL;
+  float* Q = base + kScrQ;
   // the null stream: creating a stream would allocate; the copies below are
   // ordered with it
   hipStream_t st = nullptr;
@@ -2239,11 +2242,10 @@ int avbd_mlp_grad_noalloc(int device, int n, const float* x, const float* y, con
   } else {
     launch_fill_int(st, M, n, 15);
   }
-  // the gradient and the loss are accumulated: they start at zero on every
-  // call (the array keeps the previous call's contents)
-  launch_fill_int(st, reinterpret_cast<int*>(G), nw, 0);
-  launch_fill_int(st, reinterpret_cast<int*>(L), 1, 0);
-  launch_mlp_train_tile(st, n, nullptr, X, Y, M, P, 1.0f / (float)n, huber, G, L);
+  // every value read below is written first in this call (the array keeps
+  // the previous call's contents): the rows of the ceil(n / 64) workgroups,
+  // then the gradient and the loss, which are overwritten with their sums
+  launch_mlp_train_tile(st, n, nullptr, X, Y, M, P, 1.0f / (float)n, huber, Q, G, L);
   HIPCK(hipGetLastError());
   HIPCK(hipMemcpy(grad, G, (size_t)nw * sizeof(float), hipMemcpyDeviceToHost));
   HIPCK(hipMemcpy(loss, L, sizeof(float), hipMemcpyDeviceToHost));
@@ -2283,6 +2285,7 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
   CK(m.alloc(nw));
   CK(v.alloc(nw));
   CK(j.L.alloc(std::max(steps, 1)));
+  CK(j.Q.alloc((size_t)((b + 63) / 64) * mlp_part_stride()));
   launch_fill_int(j.st, reinterpret_cast<int*>(m.p), nw, 0);
   launch_fill_int(j.st, reinterpret_cast<int*>(v.p), nw, 0);
   launch_fill_int(j.st, reinterpret_cast<int*>(j.L.p), std::max(steps, 1), 0);
@@ -2291,9 +2294,8 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
   for (int t = 0; t < steps; ++t) {
     p1 *= b1;
     p2 *= b2;
-    launch_fill_int(j.st, reinterpret_cast<int*>(j.G.p), nw, 0);
     launch_mlp_train_tile(j.st, b, j.I.p + (size_t)t * b, j.X.p, j.Y.p, j.M.p, j.P.p,
-                          1.0f / (float)b, huber, j.G.p, j.L.p + t);
+                          1.0f / (float)b, huber, j.Q.p, j.G.p, j.L.p + t);
     launch_adam(j.st, nw, j.P.p, j.G.p, m.p, v.p, lr, b1, b2, (float)(1.0 - p1), (float)(1.0 - p2),
                 eps);
   }

@@ -14,9 +14,9 @@
 //                       accumulation), weights staged in LDS, SiLU in fp32
 //  k_mlp_train_tile     forward + backward of 64 samples per workgroup on
 //                       v_mfma_f32_16x16x32_bf16 (bf16 weights, layer
-//                       inputs and gradients, fp32 accumulation); weight
-//                       gradients summed per workgroup, then one atomic
-//                       add each
+//                       inputs and gradients, fp32 accumulation); each
+//                       workgroup stores its gradient row, and
+//  k_mlp_part_reduce    sums the rows in a fixed order
 //  k_adam               Adam step on the fp32 master weights
 //
 // Network: x (24) -> 64 SiLU -> 64 SiLU -> 4. Parameters (fp32, in order):
@@ -366,11 +366,18 @@ __device__ __forceinline__ floatx4 mfma16(const bf16x8& a, const bf16x8& b, floa
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
 }
 
+// Each workgroup writes its gradient and loss, with plain stores, to its own
+// row of `part` (kPartStride floats: the weight gradient, then the loss);
+// k_mlp_part_reduce then sums the rows in workgroup order, so the result does
+// not depend on scheduling.
+constexpr int kPartStride = ((kNumWeights + 1 + 63) / 64) * 64;
+int mlp_part_stride() { return kPartStride; }
+
 __global__ void __launch_bounds__(256)
 k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X,
                  const float* __restrict__ Y, const int* __restrict__ ymask,
                  const float* __restrict__ params, float inv_total, float huber,
-                 float* __restrict__ grad, float* __restrict__ loss) {
+                 float* __restrict__ part) {
   __shared__ __bf16 w1[32][64];    // W1 [in][hid], rows 24-31 zero
   __shared__ __bf16 w2[64][64];    // W2 [hid][hid]
   __shared__ __bf16 w3[64][16];    // W3 [hid][out], columns 4-15 zero
@@ -383,7 +390,9 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
   __shared__ __bf16 dz2b[64][72];  // dz2 [sample][hid]
   __shared__ __bf16 dz1b[64][72];  // dz1 [sample][hid]
   __shared__ int rows[64];
-  __shared__ float lsum;
+  __shared__ float bpart[2][16][64];  // bias-gradient partials of b2, b1 per (wavefront, kq)
+  __shared__ float lpart[16][4];      // loss partials per (wavefront, kq) and output
+  float* __restrict__ grad = part + (size_t)blockIdx.x * kPartStride;
   const int tid = threadIdx.x, wave = tid / 64, l = tid % 64;
   const int c = l % 16, kq = l / 16;
   for (int k = tid; k < 32 * 64; k += 256) {
@@ -399,7 +408,6 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
     bias[k] = k < 64 ? params[kOffB1 + k] : k < 128 ? params[kOffB2 + k - 64] : params[kOffB3 + k - 128];
   const int t0 = blockIdx.x * 64;
   if (tid < 64) rows[tid] = t0 + tid < n ? (idx ? idx[t0 + tid] : t0 + tid) : -1;
-  if (tid == 0) lsum = 0.0f;
   for (int k = tid; k < 64 * 16; k += 256) dyb[k / 16][k % 16] = (__bf16)0.0f;
   __syncthreads();
   for (int k = tid; k < 64 * 40; k += 256) {
@@ -466,6 +474,7 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
       y = mfma16(a, b, y);
     }
     if (c < kOut) {
+      float lp = 0.0f;
       for (int r = 0; r < 4; ++r) {
         const int s = s0 + 4 * kq + r, row = rows[s];
         float d = 0.0f;
@@ -474,16 +483,17 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
           const float yt = (Y[(size_t)row * kOut + c] - params[kOffYm + c]) / params[kOffYs + c];
           d = yh - yt;
           if (huber > 0.0f && fabsf(d) > huber) {
-            atomicAdd(&lsum, huber * (2.0f * fabsf(d) - huber) * inv_total);
+            lp += huber * (2.0f * fabsf(d) - huber) * inv_total;
             d = copysignf(huber, d);
           } else {
-            atomicAdd(&lsum, d * d * inv_total);
+            lp += d * d * inv_total;
           }
         }
         const float g = 2.0f * d * inv_total;
         dyf[s][c] = g;
         dyb[s][c] = (__bf16)g;
       }
+      lpart[4 * wave + kq][c] = lp;
     }
   }
   __syncthreads();
@@ -500,11 +510,11 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
       d = mfma16(a, b, d);
     }
     if (c < kOut)
-      for (int r = 0; r < 4; ++r) atomicAdd(&grad[kOffW3 + (16 * wave + 4 * kq + r) * kOut + c], d[r]);
+      for (int r = 0; r < 4; ++r) grad[kOffW3 + (16 * wave + 4 * kq + r) * kOut + c] = d[r];
     if (wave == 0 && l < kOut) {
       float g = 0.0f;
       for (int s = 0; s < 64; ++s) g += dyf[s][l];
-      atomicAdd(&grad[kOffB3 + l], g);
+      grad[kOffB3 + l] = g;
     }
   }
   // ---- dz2 = (dy W3^T) * silu'(z2) for this wavefront's samples (VALU, K = 4) ----
@@ -520,7 +530,7 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
       dz2b[s][u] = (__bf16)dz;
       bsum += dz;
     }
-    atomicAdd(&grad[kOffB2 + u], bsum);
+    bpart[0][4 * wave + kq][u] = bsum;
   }
   __syncthreads();
   // ---- dW2 = a1^T dz2: row block 16w, column blocks 0-3 ----
@@ -536,8 +546,7 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
       }
       d = mfma16(a, b, d);
     }
-    for (int r = 0; r < 4; ++r)
-      atomicAdd(&grad[kOffW2 + (16 * wave + 4 * kq + r) * kHid + 16 * t + c], d[r]);
+    for (int r = 0; r < 4; ++r) grad[kOffW2 + (16 * wave + 4 * kq + r) * kHid + 16 * t + c] = d[r];
   }
   // ---- dz1 = (dz2 W2^T) * silu'(z1) for this wavefront's samples ----
   {
@@ -563,7 +572,7 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
         dz1b[s0 + 4 * kq + r][i] = (__bf16)dz;
         bsum += dz;
       }
-      atomicAdd(&grad[kOffB1 + i], bsum);
+      bpart[1][4 * wave + kq][i] = bsum;
     }
   }
   __syncthreads();
@@ -582,11 +591,44 @@ k_mlp_train_tile(int n, const int* __restrict__ idx, const float* __restrict__ X
     }
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * ib + 4 * kq + r;
-      if (i < kIn) atomicAdd(&grad[kOffW1 + i * kHid + 16 * wave + c], d[r]);
+      if (i < kIn) grad[kOffW1 + i * kHid + 16 * wave + c] = d[r];
     }
   }
+  // bias gradients and the loss: partials summed in a fixed order
+  if (tid < 128) {
+    const int which = tid / 64, u = tid % 64;
+    float g = 0.0f;
+    for (int q = 0; q < 16; ++q) g += bpart[which][q][u];
+    grad[(which == 0 ? kOffB2 : kOffB1) + u] = g;
+  } else if (tid == 128) {
+    float lsum = 0.0f;
+    for (int q = 0; q < 16; ++q)
+      for (int o = 0; o < kOut; ++o) lsum += lpart[q][o];
+    grad[kNumWeights] = lsum;
+  }
+}
+
+// grad[i] = sum of the workgroups' rows (i < kNumWeights); *loss = sum of
+// their losses. A workgroup takes 64 entries; its four wavefronts sum the
+// rows q, q + 4, q + 8, ... (q = wavefront), then wavefront 0 adds the four
+// partial sums in order: a fixed order for every call.
+__global__ void __launch_bounds__(256)
+k_mlp_part_reduce(int n_blocks, const float* __restrict__ part, float* __restrict__ grad,
+                  float* __restrict__ loss) {
+  __shared__ float acc[4][64];
+  const int q = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int i = blockIdx.x * 64 + l;
+  float g = 0.0f;
+  if (i <= kNumWeights) {
+#pragma unroll 8
+    for (int b = q; b < n_blocks; b += 4) g += part[(size_t)b * kPartStride + i];
+  }
+  acc[q][l] = g;
   __syncthreads();
-  if (tid == 0) atomicAdd(loss, lsum);
+  if (q != 0 || i > kNumWeights) return;
+  const float s = ((acc[0][l] + acc[1][l]) + acc[2][l]) + acc[3][l];
+  if (i < kNumWeights) grad[i] = s;
+  else *loss = s;
 }
 
 __global__ void __launch_bounds__(256)
@@ -604,10 +646,13 @@ k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restr
 
 void launch_mlp_train_tile(hipStream_t st, int n, const int* idx, const float* X, const float* Y,
                            const int* ymask, const float* params, float inv_total, float huber,
-                           float* grad, float* loss) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_mlp_train_tile, dim3((n + 63) / 64), dim3(256), 0, st, n, idx, X, Y,
-                       ymask, params, inv_total, huber, grad, loss);
+                           float* part, float* grad, float* loss) {
+  if (n <= 0) return;
+  const int blocks = (n + 63) / 64;
+  hipLaunchKernelGGL(k_mlp_train_tile, dim3(blocks), dim3(256), 0, st, n, idx, X, Y, ymask,
+                     params, inv_total, huber, part);
+  hipLaunchKernelGGL(k_mlp_part_reduce, dim3((kNumWeights + 1 + 63) / 64), dim3(256), 0, st,
+                     blocks, part, grad, loss);
 }
 
 void launch_adam(hipStream_t st, int n, float* w, const float* g, float* m, float* v, float lr,

@@ -331,13 +331,17 @@ def log_trace(solver, frames: int, tol: float = 1e-4, max_iters: int = 200,
 
 def iterations_to_residual(solver, frames: int, mode: str = "plain", params=None,
                            tol: float = 1e-4, max_iters: int = 200,
-                           scope: str = "all") -> np.ndarray:
+                           scope: str = "all", residuals: list | None = None) -> np.ndarray:
     """Iterations each substep needed to reach the residual tolerance
-    (capped at max_iters), under the plain or the learned warm start."""
+    (capped at max_iters), under the plain or the learned warm start. The
+    residual each substep ended at is appended to `residuals` if given."""
     solver.set_learned_warm_start(mode, params, scope)
     its = []
     for _ in range(frames):
-        its.extend(i for i, _r in solver.step_to_residual(tol, max_iters))
+        for i, r in solver.step_to_residual(tol, max_iters):
+            its.append(i)
+            if residuals is not None:
+                residuals.append(r)
     return np.asarray(its, np.int64)
 
 
@@ -354,11 +358,23 @@ def undo_decay_params(like: np.ndarray, alpha: float = 0.99, gamma: float = 0.99
 
 
 def compare(builder, frames: int, params, tol: float = 1e-4, max_iters: int = 200,
-            scope: str = "all") -> dict:
-    """Iterations to residual from the same scene, plain vs learned warm start."""
-    plain = iterations_to_residual(builder(), frames, "plain", None, tol, max_iters)
-    learned = iterations_to_residual(builder(), frames, "learned", params, tol, max_iters, scope)
+            scope: str = "all", settle_frames: int = 0) -> dict:
+    """Iterations to residual from the same scene, plain vs learned warm
+    start. With settle_frames, both copies first run that many ordinary
+    frames (fixed iterations, plain warm start), so the measurement starts
+    from the same settled state."""
+    def settled():
+        s = builder()
+        for _ in range(settle_frames):
+            s.step()
+        return s
+    rp, rl = [], []
+    plain = iterations_to_residual(settled(), frames, "plain", None, tol, max_iters, "all", rp)
+    learned = iterations_to_residual(settled(), frames, "learned", params, tol, max_iters, scope,
+                                     rl)
     return {"plain_mean": float(plain.mean()), "learned_mean": float(learned.mean()),
+            # where the substeps stopped: below tol, or the residual at the cap
+            "plain_resid_median": float(np.median(rp)), "learned_resid_median": float(np.median(rl)),
             "plain_total": int(plain.sum()), "learned_total": int(learned.sum()),
             "plain_capped": int((plain >= max_iters).sum()),
             "learned_capped": int((learned >= max_iters).sum()),

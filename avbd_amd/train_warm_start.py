@@ -37,7 +37,8 @@ def gather(args, rng) -> L.Trace:
     cube stacks."""
     xs, ys, ms = [], [], []
     sizes = np.exp(rng.uniform(np.log(args.min_boxes), np.log(args.max_boxes), args.train_piles))
-    jobs = [(f"pile {n}", lambda n=int(n), k=k: L.pile(n, seed=1000 + k), args.frames)
+    jobs = [(f"pile {n}", lambda n=int(n), k=k: _settled(L.pile(n, seed=1000 + k),
+                                                          args.settle_frames), args.frames)
             for k, n in enumerate(np.round(sizes).astype(int))]
     jobs += [(f"pyramid {lay}", lambda lay=lay, k=k: scenes.pyramid(lay, seed=2000 + k),
               args.pyramid_frames) for k, lay in enumerate(args.train_pyramids)]
@@ -52,6 +53,14 @@ def gather(args, rng) -> L.Trace:
         _log(f"trace {k + 1}/{len(jobs)}: {name}, {len(tr.X)} samples, "
              f"{time.perf_counter() - t0:.1f} s")
     return L.Trace(np.concatenate(xs), np.concatenate(ys), np.concatenate(ms))
+
+
+def _settled(s, frames: int):
+    """The scene after `frames` ordinary frames (fixed iterations, plain
+    warm start): a dropped pile has landed before its samples are logged."""
+    for _ in range(frames):
+        s.step()
+    return s
 
 
 def _log_capped(s, args, frames, seed) -> L.Trace:
@@ -125,24 +134,31 @@ def train(tr: L.Trace, args, rng):
 
 def evaluate(p, args) -> list[dict]:
     out = []
-    cases = [(f"pile{n}", (lambda n=n, sd=sd: L.pile(n, seed=sd)))
+    # (name, builder, settle frames, measured frames)
+    cases = [(f"pile{n}", (lambda n=n, sd=sd: L.pile(n, seed=sd)), args.settle_frames,
+              args.eval_frames)
              for n, sd in zip(args.eval_sizes, range(9000, 9000 + len(args.eval_sizes)))]
     if args.eval_pyramid > 0:
         cases.append((f"pyramid{args.eval_pyramid}",
-                      lambda: scenes.pyramid(args.eval_pyramid, seed=9500)))
-    for name, builder in cases:
+                      lambda: scenes.pyramid(args.eval_pyramid, seed=9500), 0, args.eval_frames))
+    if args.eval_configs:
+        # the 10,000-box pile (landed) and the 98,021-cube pyramid
+        cases.append(("pile10000", lambda: scenes.random_pile(10_000, seed=0),
+                      args.eval_settle_frames, args.eval_frames))
+        cases.append(("pyramid66", lambda: scenes.pyramid(66, seed=0), 0,
+                      args.eval_frames_large))
+    for name, builder, settle, frames in cases:
         t0 = time.perf_counter()
-        r = L.compare(builder, args.eval_frames, p, args.tol, args.max_iters, args.scope)
+        r = L.compare(builder, frames, p, args.tol, args.max_iters, args.scope, settle)
         r = {k: v for k, v in r.items() if not isinstance(v, np.ndarray)}
         r["scene"] = name
-        u = L.iterations_to_residual(builder(), args.eval_frames, "learned",
-                                     L.undo_decay_params(p), args.tol, args.max_iters)
-        r["undo_decay_mean"] = float(u.mean())
+        r["settle_frames"] = settle
+        r["frames"] = frames
         out.append(r)
         _log(f"eval {name}: iterations/substep plain {r['plain_mean']:.2f} learned "
-             f"{r['learned_mean']:.2f} undo-decay {r['undo_decay_mean']:.2f} "
-             f"(capped {r['plain_capped']} / {r['learned_capped']} of "
-             f"{r['substeps']}), {time.perf_counter() - t0:.1f} s")
+             f"{r['learned_mean']:.2f} (capped {r['plain_capped']} / {r['learned_capped']} of "
+             f"{r['substeps']}; median final residual {r['plain_resid_median']:.3g} / "
+             f"{r['learned_resid_median']:.3g}), {time.perf_counter() - t0:.1f} s")
     return out
 
 
@@ -151,7 +167,9 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--train-piles", type=int, default=40)
     p.add_argument("--min-boxes", type=int, default=1000)
     p.add_argument("--max-boxes", type=int, default=10000)
-    p.add_argument("--frames", type=int, default=120)
+    p.add_argument("--frames", type=int, default=120, help="logged frames per pile")
+    p.add_argument("--settle-frames", type=int, default=0,
+                   help="ordinary frames a pile runs before it is logged or measured")
     p.add_argument("--keep-cap", type=int, default=500)
     p.add_argument("--train-pyramids", type=int, nargs="*", default=[6, 8, 10, 12, 14])
     p.add_argument("--train-stacks", type=int, nargs="*", default=[3, 5, 8])
@@ -172,6 +190,12 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--eval-sizes", type=int, nargs="*", default=[1500, 4000])
     p.add_argument("--eval-pyramid", type=int, default=9)
     p.add_argument("--eval-frames", type=int, default=120)
+    p.add_argument("--eval-configs", action="store_true",
+                   help="also measure the 10,000-box pile and the 98,021-cube pyramid")
+    p.add_argument("--eval-settle-frames", type=int, default=600,
+                   help="ordinary frames the 10,000-box pile runs before it is measured")
+    p.add_argument("--eval-frames-large", type=int, default=5,
+                   help="measured frames of the 98,021-cube pyramid")
     p.add_argument("--out", type=Path, default=L.WEIGHTS_PATH)
     p.add_argument("--report", type=Path, default=None)
     p.add_argument("--trace-cache", type=Path, default=None,
@@ -198,7 +222,9 @@ def main(argv=None) -> int:
     _log(f"held-out loss of the plain warm start (zero correction): {base:.4f}")
     ev = evaluate(p, args)
     meta = {"train_piles": args.train_piles, "boxes": [args.min_boxes, args.max_boxes],
-            "frames": args.frames, "samples": int(len(tr.X)), "tol": args.tol,
+            "frames": args.frames, "settle_frames": args.settle_frames,
+            "keep_cap": args.keep_cap, "train_pyramids": args.train_pyramids,
+            "train_stacks": args.train_stacks, "samples": int(len(tr.X)), "tol": args.tol,
             "max_iters": args.max_iters, "steps": args.steps, "batch": args.batch,
             "lr": args.lr, "scope": args.scope, "huber": args.huber, "clip": args.clip, "history": hist, "plain_val_loss": base, "eval": ev,
             "seconds": time.perf_counter() - t0}

@@ -7,11 +7,15 @@ dt = 1/60, 4 substeps, 20 iterations, FP32. One step = one frame
 sweeps, finalize, cache refresh). Metric: body-substeps/s = bodies x
 substeps x frames / wall time, with the scene resident in HBM before the
 timed region starts. The line also carries iterations to the dual residual
-1e-4 (cap 200), plain vs learned warm start, on a 285-cube pyramid.
+1e-4 (cap 200), plain vs learned warm start, on the landed 10,000-box pile,
+the 98,021-cube pyramid and a 285-cube pyramid.
 
 --workload batched: 2048 independent piles of n = round(exp(U(ln 8,
 ln 1024))) boxes each (the 10k-box pile's generator per scene), 10
 iterations, stepped as one batch; same metric over all scenes.
+
+--workload train: Adam steps of the warm-start network's MFMA training
+kernels on 2,097,152 synthetic samples, batch 16,384. Metric: samples/s.
 
 --workload particles: the 3-DOF particle Solver on 256 soft blocks (tet
 lattices of 6^3 cells: 87,808 particles, 642,048 rows, 331,776 tets) dropped
@@ -121,19 +125,36 @@ def batched_scene(n_scenes: int, seed: int):
     return BatchedSolver6DOF(parts), sizes
 
 
-def iters_to_residual(frames: int = 30) -> dict:
-    """Iterations per substep to the dual residual 1e-4 (cap 200) on a
-    9-layer pyramid, plain warm start vs the shipped learned one."""
+def iters_to_residual() -> dict:
+    """Iterations per substep to the dual residual 1e-4 (cap 200), plain vs
+    the shipped learned warm start, from identical states: the landed
+    10,000-box pile (600 ordinary frames first, then 10 measured frames),
+    the 98,021-cube pyramid (3 frames) and a 285-cube pyramid (30 frames).
+    Where every substep stops at the cap, the median residual it stopped at
+    says which start got further."""
     from avbd_amd import learned, scenes
 
     p = learned.load_params()
     scope = learned.load_meta().get("scope", "all")
-    build = lambda: scenes.pyramid(9, seed=9500)
-    r = learned.compare(build, frames, p, 1e-4, 200, scope)
-    return {"scene": "pyramid of 285 cubes (9 layers), 30 frames x 4 substeps",
-            "tol": 1e-4, "cap": 200, "plain": r["plain_mean"], "learned": r["learned_mean"],
+    cases = [("pile of 10,000 boxes, landed (600 frames), 10 frames x 4 substeps",
+              lambda: scenes.random_pile(10_000, seed=0), 600, 10),
+             ("pyramid of 98,021 cubes (66 layers), 3 frames x 4 substeps",
+              lambda: scenes.pyramid(66, seed=0), 0, 3),
+             ("pyramid of 285 cubes (9 layers), 30 frames x 4 substeps",
+              lambda: scenes.pyramid(9, seed=9500), 0, 30)]
+    out = {"tol": 1e-4, "cap": 200, "scope": scope, "scenes": []}
+    for name, build, settle, frames in cases:
+        t0 = time.perf_counter()
+        r = learned.compare(build, frames, p, 1e-4, 200, scope, settle)
+        out["scenes"].append({
+            "scene": name, "plain": r["plain_mean"], "learned": r["learned_mean"],
             "plain_capped": r["plain_capped"], "learned_capped": r["learned_capped"],
-            "scope": scope}
+            "substeps": r["substeps"], "plain_resid_median": r["plain_resid_median"],
+            "learned_resid_median": r["learned_resid_median"]})
+        print(f"# iterations to residual, {name}: plain {r['plain_mean']:.2f} learned "
+              f"{r['learned_mean']:.2f} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
+    return out
 
 
 PARTICLE_TRAFFIC_FILE = ROOT / "profiles" / "k_p_primal_grp_softblocks256_pmc.md"
@@ -239,11 +260,55 @@ def particle_bench(args) -> dict:
     return out
 
 
+TRAIN_FLOP_PER_SAMPLE = (2 * (24 * 64 + 64 * 64 + 64 * 4)                        # forward
+                         + 2 * (64 * 4 + 64 * 4 + 64 * 64 + 64 * 64 + 24 * 64))  # backward
+
+
+def train_bench(args) -> dict:
+    """The warm-start network's training kernels (avbd_mlp_train: one MFMA
+    forward+backward launch and one Adam launch per step) on synthetic
+    samples. The timed call includes copying the sample set to the device
+    and drawing the minibatch order, so `value` is a lower bound on the
+    kernels' own rate; rocprof gives the kernel durations."""
+    import numpy as np
+    from avbd_amd import learned as L
+
+    rng = np.random.default_rng(args.seed)
+    n = args.train_samples
+    X = rng.normal(0.0, 1.0, (n, L.N_IN)).astype(np.float32)
+    Y = np.stack([np.tanh(X[:, 0] + 0.5 * X[:, 1]), np.sin(X[:, 2]) * X[:, 3],
+                  0.3 * X[:, 4] ** 2, X[:, 5] - X[:, 6]], axis=1).astype(np.float32)
+    M = np.full(n, 15, np.int32)
+    p = L.init_params(*L.normalisation(X[:65536], Y[:65536], M[:65536]), seed=1)
+    L.train_gpu(p, X[:65536], Y[:65536], M[:65536], steps=max(1, args.warmup), batch=args.batch)
+    t0 = time.perf_counter()
+    p1, hist = L.train_gpu(p, X, Y, M, steps=args.steps, batch=args.batch, lr=1e-3, seed=2)
+    el = time.perf_counter() - t0
+    assert np.all(np.isfinite(hist)), "training diverged"
+    samples = args.steps * min(args.batch, n)
+    return {
+        "metric": "warm-start network training samples/s (MFMA forward+backward+Adam)",
+        "value": samples / el, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": f"24-64-64-4 SiLU MLP, {args.steps} Adam steps of {args.batch} "
+                               f"samples from {n} synthetic samples (host arrays copied in)",
+                   "flop_per_sample": TRAIN_FLOP_PER_SAMPLE,
+                   "parallelism": "single GPU"},
+        "achieved_tflops_wall": samples * TRAIN_FLOP_PER_SAMPLE / el / 1e12,
+        "loss_first_last": [float(hist[0]), float(hist[-1])],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("pyramid", "batched", "particles"),
+    ap.add_argument("--workload", choices=("pyramid", "batched", "particles", "train"),
                     default="pyramid")
+    ap.add_argument("--train-samples", type=int, default=2_097_152,
+                    help="train: synthetic samples")
+    ap.add_argument("--batch", type=int, default=16_384, help="train: minibatch")
     ap.add_argument("--blocks", type=int, default=256, help="particles: number of soft blocks")
     ap.add_argument("--scenes", type=int, default=2048, help="batched: number of scenes")
     ap.add_argument("--residual", type=int, default=1,
@@ -290,6 +355,9 @@ def main() -> int:
 
     if args.workload == "particles":
         print(json.dumps(particle_bench(args)), flush=True)
+        return 0
+    if args.workload == "train":
+        print(json.dumps(train_bench(args)), flush=True)
         return 0
 
     from avbd_amd import scenes
@@ -418,10 +486,7 @@ def main() -> int:
         out["per_scene_latency_ms"] = {"p50": timing["frame_ms_p50"],
                                        "p99": timing["frame_ms_p99"]}
     if args.residual and args.workload == "pyramid":
-        t0 = time.perf_counter()
         out["iters_to_residual"] = iters_to_residual()
-        print(f"# iterations to residual: {out['iters_to_residual']} "
-              f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     if args.cpu_baseline and args.workload == "pyramid":
         out["cpu_baseline"] = cpu_baseline(args.layers, args.seed, args.cpu_substeps)
     print(json.dumps(out), flush=True)

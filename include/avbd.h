@@ -369,7 +369,8 @@ int avbd_mlp_train(int device, int n, const float* x, const float* y, const int*
  * that must not call hipMalloc, such as tight serving loops and harnesses
  * that forbid allocation). Same arguments and results as avbd_mlp_grad, for
  * n <= 4096 samples (more returns AVBD_E_ARG). The samples, parameters,
- * gradient and loss are staged in one static device array of 528.25 KiB,
+ * gradient, loss and per-workgroup gradient rows are staged in one static
+ * device array of 2,638,080 bytes (2.52 MiB),
  * which every GPU the library runs on holds until the process ends; every
  * call on a GPU shares that GPU's array, so calls that overlap on one GPU
  * race. Runs on the null stream. */
