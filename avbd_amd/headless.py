@@ -15,7 +15,6 @@ Rendering, drag gizmos and the viser server are not part of this module.
 from __future__ import annotations
 
 import argparse
-import colorsys
 import time
 from dataclasses import dataclass
 
@@ -56,10 +55,11 @@ def build_scene(args, solver_cls=Solver6DOF) -> tuple[Solver6DOF, list[SceneBox]
     boxes: list[SceneBox] = []
     rng = np.random.default_rng(seed=args.seed)
 
-    def color(saturation=0.7, value=0.85):
-        # one hue draw per tower keeps the generator's draw order of the
-        # reference scene (viewer.py:125-126); the shade itself is cosmetic
-        return colorsys.hsv_to_rgb(float(rng.random()), saturation, value)
+    def tint():
+        # one draw per tower keeps the generator's draw order of the reference
+        # scene (viewer.py:125-126); the value only tints the tower
+        t = float(rng.random())
+        return (0.35 + 0.5 * t, 0.55, 0.85 - 0.5 * t)
 
     # grid of cube towers
     stress = bool(getattr(args, "stress", False))
@@ -72,7 +72,7 @@ def build_scene(args, solver_cls=Solver6DOF) -> tuple[Solver6DOF, list[SceneBox]
         for iz in range(grid_n):
             cx = grid_origin + ix * tower_spacing
             cz = grid_origin + iz * tower_spacing
-            tower_color = color(saturation=0.55, value=0.92)
+            tower_color = tint()
             for k in range(tower_height):
                 jitter = rng.uniform(-1e-3, 1e-3, size=2)
                 cy = h_cube + 2.0 * h_cube * k
