@@ -30,7 +30,9 @@ static size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
 // ---------------------------------------------------------------------------
 // scan
 // ---------------------------------------------------------------------------
-constexpr int kScanTile = 1024;
+constexpr int kScanPer = 16;                // ints per thread
+constexpr int kScanTile = 256 * kScanPer;   // ints per tile
+constexpr int kScanMaxBlocks = 256;         // workgroups of one scan launch
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
   int lane = __lane_id();
@@ -60,7 +62,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total) {
   return pre + inc - v;
 }
 
-// Single-pass scan with decoupled look-back: tile b (1024 ints) publishes
+// Single-pass scan with decoupled look-back: tile b (4096 ints) publishes
 // its total, then its inclusive prefix once known, in one 64-bit status word
 // {stamp << 1 | kind, value} (kind 1: inclusive prefix, 0: tile total). To
 // find its carry, wavefront 0 of tile b reads the 64 status words below it
@@ -100,28 +102,35 @@ k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* sta
           int* err, const int* nd) {
   __shared__ unsigned s_stamp;
   __shared__ int s_carry, s_tile;
-  // with a device count, only the first ceil(n / tile) tickets scan; the
-  // others just take part in the completion count
+  // with a device count, only the first ceil(n / tile) tickets scan
   n = dev_count(n, nd);
   const int nblk = gridDim.x;
   const int nwork = (n + kScanTile - 1) / kScanTile;
-  // hdr[0] changes only after every tile of this call is done, so all tiles
+  // hdr[0] changes only after every workgroup of this call is done, so all
   // read the same value; stamps run 1 .. 2^31 - 1 (a zero workspace holds 0)
   if (threadIdx.x == 0) {
     unsigned st = (__hip_atomic_load(&hdr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u) &
                   0x7fffffffu;
     s_stamp = st == 0u ? 1u : st;
-    // tiles are numbered in the order their workgroups start
-    s_tile = (int)atomicAdd(&hdr[2], 1u);
   }
   __syncthreads();
-  const int b = s_tile;
   const unsigned stamp = s_stamp;
-  if (b < nwork) {
-    const int base = b * kScanTile + threadIdx.x * 4;
-    int v[4], sum = 0;
+  // Workgroups take tiles from the ticket counter until the tickets run past
+  // the last tile: tiles are numbered in the order they are taken, and a
+  // tile's predecessors are all held by running workgroups.
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&hdr[2], 1u);
+    __syncthreads();
+    const int b = s_tile;
+    __syncthreads();
+    if (b >= nwork) {
+      if (b == 0 && threadIdx.x == 0 && total) *total = 0;  // nothing to scan
+      break;
+    }
+    const int base = b * kScanTile + threadIdx.x * kScanPer;
+    int v[kScanPer], sum = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kScanPer; ++k) {
       v[k] = (base + k < n) ? in[base + k] : 0;
       sum += v[k];
     }
@@ -166,17 +175,15 @@ k_scan_1p(const int* in, int* out, int n, unsigned* hdr, unsigned long long* sta
     __syncthreads();
     pre += s_carry;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kScanPer; ++k) {
       if (base + k < n) out[base + k] = pre;
       pre += v[k];
     }
-  } else if (b == 0 && threadIdx.x == 0 && total) {
-    *total = 0;  // nothing to scan
   }
   if (threadIdx.x == 0) {
     const unsigned done = atomicAdd(&hdr[1], 1u);
     if (done == (unsigned)(nblk - 1)) {
-      // every workgroup of this call has taken its ticket before finishing
+      // every workgroup of this call has drawn its last ticket
       __hip_atomic_store(&hdr[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&hdr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&hdr[0], stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -194,7 +201,7 @@ void exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* total_d
     if (total_dev) launch_fill_int(st, total_dev, 1, 0);
     return;
   }
-  const int nblk = (n + kScanTile - 1) / kScanTile;
+  const int nblk = std::min((n + kScanTile - 1) / kScanTile, kScanMaxBlocks);
   unsigned* hdr = (unsigned*)ws;
   unsigned long long* status = (unsigned long long*)((char*)ws + 16);
   hipLaunchKernelGGL(k_scan_1p, dim3(nblk), dim3(256), 0, st, in, out, n, hdr, status, total_dev,
