@@ -260,6 +260,21 @@ def particle_bench(args) -> dict:
     return out
 
 
+def substep_roofline(n_b: int, n_c: int, n_f: int, iters: int, ms_per_substep: float) -> dict:
+    """The whole substep against HBM peak: the minimal-record byte count of
+    one substep (prelude, per-contact prelude, iters + 1 primal sweeps, iters
+    duals, post-stabilisation pass) for n_b bodies, n_c box contacts and n_f
+    floor corners, over the measured time per substep."""
+    b = (n_b * (188 + 132) + n_c * 124 + n_f * 88
+         + (iters + 1) * (n_b * 120 + 2 * n_c * 112 + n_f * 76)
+         + iters * (n_c * 136 + n_f * 100)
+         + n_c * 80 + n_f * 44)
+    gbs = b / (ms_per_substep * 1e-3) / 1e9
+    return {"algorithmic_bytes_per_substep": b, "ms_per_substep": ms_per_substep,
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "body_substeps_per_s_at_peak": n_b / (b / (HBM_PEAK_GBS * 1e9))}
+
+
 TRAIN_FLOP_PER_SAMPLE = (2 * (24 * 64 + 64 * 64 + 64 * 4)                        # forward
                          + 2 * (64 * 4 + 64 * 4 + 64 * 64 + 64 * 64 + 24 * 64))  # backward
 
@@ -476,6 +491,8 @@ def main() -> int:
             "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
         },
         "kernel_ms": breakdown,
+        "substep_roofline": substep_roofline(n_b, n_contacts, len(s._rows) // 3, s.iterations,
+                                             1e3 * elapsed / substeps),
         "timing": timing,
     }
     if args.workload == "batched":
