@@ -345,6 +345,72 @@ def iterations_to_residual(solver, frames: int, mode: str = "plain", params=None
     return np.asarray(its, np.int64)
 
 
+def _state(solver) -> dict:
+    """Everything of a solver that carries from one substep to the next."""
+    h, nb = solver._h, len(solver._x)
+    keys, st = h.export_hash()
+    return {"bodies": h.read_bodies(nb), "rows": h.read_rows(solver._gpu_pool_n_static),
+            "keys": keys, "hash": st, "colors": h.read_colors(nb),
+            "n_colors": solver._n_active_colors, "dirty": solver._color_dirty}
+
+
+def _load(solver, bodies, rows, keys, hash_state, colors, n_colors, dirty) -> None:
+    h = solver._h
+    for field, key in enumerate(("x", "q", "v", "w", "prev_v", "prev_w")):
+        h.write_bodies(field, 0, bodies[key])
+    h.write_static_rows(rows["lambda"], rows["penalty"], rows["active"], rows["was_static"])
+    h.import_hash(keys, hash_state)
+    h.write_colors(colors, n_colors)
+    solver._color_dirty = dirty
+    solver._n_active_colors = n_colors
+
+
+def perfect_start_bound(solver, frames: int, tol: float = 1e-4, max_iters: int = 200) -> dict:
+    """What any warm start could gain, measured: every substep of `frames`
+    frames runs plain to the residual tolerance (cap max_iters), then again
+    from the same starting state with the state that run ended at written in
+    as its warm start (static rows' and cached contacts' lambda, penalty and
+    stick state; penalties divided by the prelude's gamma, lambdas by its
+    alpha * gamma when it decays them). The trajectory continues from the
+    plain run. Returns the iterations of both runs per substep."""
+    if solver._dirty:
+        solver._flush()
+    n_b = len(solver._x)
+    sub_dt = solver.dt / solver.substeps if solver.substeps > 1 else solver.dt
+    decay_lam = 1.0 if solver.post_stabilize else solver.alpha * solver.gamma
+    plain, perfect, rp, rq = [], [], [], []
+    for _ in range(frames):
+        for _k in range(max(1, solver.substeps)):
+            s0 = _state(solver)
+            params = solver._params(sub_dt)
+            params.iterations = int(max_iters)
+            rep, it0, r0 = solver._h.substep_to_residual(params, solver._color_dirty, tol)
+            solver._apply_report(rep, n_b)
+            s1 = _state(solver)
+            rows = dict(s1["rows"])
+            rows["lambda"] = (rows["lambda"] / np.float32(decay_lam)).astype(np.float32)
+            rows["penalty"] = (rows["penalty"] / np.float32(solver.gamma)).astype(np.float32)
+            st = s1["hash"].copy()
+            st[:, 0:3] /= np.float32(decay_lam)
+            st[:, 3:6] /= np.float32(solver.gamma)
+            _load(solver, s0["bodies"], rows, s1["keys"], st, s0["colors"], s0["n_colors"],
+                  s0["dirty"])
+            rep2, it1, r1 = solver._h.substep_to_residual(params, solver._color_dirty, tol)
+            plain.append(it0)
+            perfect.append(it1)
+            rp.append(r0)
+            rq.append(r1)
+            # continue from the plain run's end state
+            _load(solver, s1["bodies"], s1["rows"], s1["keys"], s1["hash"], s1["colors"],
+                  s1["n_colors"], s1["dirty"])
+    plain, perfect = np.asarray(plain), np.asarray(perfect)
+    return {"plain_mean": float(plain.mean()), "perfect_mean": float(perfect.mean()),
+            "plain_capped": int((plain >= max_iters).sum()),
+            "perfect_capped": int((perfect >= max_iters).sum()), "substeps": int(len(plain)),
+            "plain_resid_median": float(np.median(rp)),
+            "perfect_resid_median": float(np.median(rq)), "plain": plain, "perfect": perfect}
+
+
 def undo_decay_params(like: np.ndarray, alpha: float = 0.99, gamma: float = 0.99) -> np.ndarray:
     """A constant network: every normal-row correction is log(alpha gamma),
     i.e. the start is the cached lambda without the prelude's decay (to

@@ -159,6 +159,18 @@ def evaluate(p, args) -> list[dict]:
              f"{r['learned_mean']:.2f} (capped {r['plain_capped']} / {r['learned_capped']} of "
              f"{r['substeps']}; median final residual {r['plain_resid_median']:.3g} / "
              f"{r['learned_resid_median']:.3g}), {time.perf_counter() - t0:.1f} s")
+        if args.headroom_frames > 0:
+            t0 = time.perf_counter()
+            sv = builder()
+            for _ in range(settle):
+                sv.step()
+            b = L.perfect_start_bound(sv, args.headroom_frames, args.tol, args.max_iters)
+            r["perfect_start"] = {k: v for k, v in b.items() if not isinstance(v, np.ndarray)}
+            _log(f"bound {name}: iterations/substep plain {b['plain_mean']:.2f}, from the "
+                 f"state the plain run ends at {b['perfect_mean']:.2f} (capped "
+                 f"{b['plain_capped']} / {b['perfect_capped']} of {b['substeps']}; median final "
+                 f"residual {b['plain_resid_median']:.3g} / {b['perfect_resid_median']:.3g}), "
+                 f"{time.perf_counter() - t0:.1f} s")
     return out
 
 
@@ -192,12 +204,16 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--eval-frames", type=int, default=120)
     p.add_argument("--eval-configs", action="store_true",
                    help="also measure the 10,000-box pile and the 98,021-cube pyramid")
+    p.add_argument("--headroom-frames", type=int, default=0,
+                   help="frames of the perfect-start bound on each evaluation scene (0 = off)")
     p.add_argument("--eval-settle-frames", type=int, default=600,
                    help="ordinary frames the 10,000-box pile runs before it is measured")
     p.add_argument("--eval-frames-large", type=int, default=5,
                    help="measured frames of the 98,021-cube pyramid")
     p.add_argument("--out", type=Path, default=L.WEIGHTS_PATH)
     p.add_argument("--report", type=Path, default=None)
+    p.add_argument("--eval-only", type=Path, default=None,
+                   help="evaluate the network in this npz (no traces, no training)")
     p.add_argument("--trace-cache", type=Path, default=None,
                    help="npz of logged samples: loaded if present, else written")
     return p
@@ -207,6 +223,16 @@ def main(argv=None) -> int:
     args = make_parser().parse_args(argv)
     rng = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
+    if args.eval_only is not None:
+        # no traces, no training: evaluate the network in the given file
+        p = L.load_params(args.eval_only)
+        ev = evaluate(p, args)
+        if args.report is not None:
+            args.report.parent.mkdir(parents=True, exist_ok=True)
+            args.report.write_text(json.dumps({"params": str(args.eval_only), "eval": ev,
+                                               "tol": args.tol, "max_iters": args.max_iters,
+                                               "scope": args.scope}, indent=1))
+        return 0
     if args.trace_cache is not None and args.trace_cache.exists():
         with np.load(args.trace_cache, allow_pickle=False) as z:
             tr = L.Trace(z["X"], z["Y"], z["mask"])
