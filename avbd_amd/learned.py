@@ -360,7 +360,8 @@ def _load(solver, bodies, rows, keys, hash_state, colors, n_colors, dirty) -> No
         h.write_bodies(field, 0, bodies[key])
     h.write_static_rows(rows["lambda"], rows["penalty"], rows["active"], rows["was_static"])
     h.import_hash(keys, hash_state)
-    h.write_colors(colors, n_colors)
+    if not dirty:   # a dirty coloring is recomputed by the next substep
+        h.write_colors(colors, n_colors)
     solver._color_dirty = dirty
     solver._n_active_colors = n_colors
 

@@ -144,3 +144,31 @@ def test_undo_decay_network_outputs_log_alpha_gamma():
     p = L.undo_decay_params(_params(X, Y, M))
     y, _ = L.forward_np(p, X, bf16=False)
     assert np.allclose(y[:, 0], np.log(0.99 * 0.99), rtol=1e-6) and np.all(y[:, 1:] == 0)
+
+
+def test_perfect_start_bound_keeps_the_plain_trajectory():
+    """The warm-start bound on the checker (pyramid of 10 cubes from rest, 2
+    frames, first substep with a dirty coloring): its plain iteration counts
+    and the state carried to the next substep (bodies, static rows, contact
+    cache) are those of an ordinary run to the residual, bit for bit, and so
+    are the frames after it; the converged start needs fewer iterations in
+    total. The contact rows of the pool are rebuilt from the cache by every
+    substep and are not compared."""
+    from avbd_amd import scenes
+    from oracle.oracle import oracle_solver_class
+    O = oracle_solver_class()
+    a = scenes.pyramid(4, seed=3, solver_cls=O)
+    b = scenes.pyramid(4, seed=3, solver_cls=O)
+    r = L.perfect_start_bound(a, 2, 1e-4, 200)
+    its = L.iterations_to_residual(b, 2)
+    assert np.array_equal(r["plain"], its)
+    assert np.array_equal(a.positions(), b.positions())
+    n_static = a._gpu_pool_n_static
+    assert np.array_equal(a.lambdas()[:n_static], b.lambdas()[:n_static])
+    for x, y in zip(a._h.export_hash(), b._h.export_hash()):
+        assert np.array_equal(x, y)
+    for _ in range(2):
+        a.step()
+        b.step()
+    assert np.array_equal(a.positions(), b.positions())
+    assert r["substeps"] == 8 and r["perfect"].sum() < r["plain"].sum()
