@@ -400,6 +400,7 @@ def main() -> int:
         print(f"# warmup frame {w}: {1e3 * (time.perf_counter() - t0):.1f} ms", file=sys.stderr,
               flush=True)
     tuned = None
+    frames_before = args.warmup  # frames the scene has advanced before the timed ones
     if variant is None:
         # the primal sweep kernel for this scene, by timing one frame with
         # each (identical results, so these are ordinary untimed frames)
@@ -408,6 +409,7 @@ def main() -> int:
         print(f"# primal autotune (ms of primal sweeps per frame): {tuned} -> variant {variant}",
               file=sys.stderr, flush=True)
         s.step()   # untimed: captures the graph of the chosen kernel
+        frames_before += len(tuned) + 1
     import torch
     syncs0, graphs0 = s._h.sync_count()
     torch.cuda.synchronize()
@@ -442,6 +444,9 @@ def main() -> int:
     fs = sorted(frame_ms)
     pct = lambda q: fs[min(len(fs) - 1, int(round(q * (len(fs) - 1))))]
     timing = {"timed_frames": args.steps,
+              # 1-based frames of the scene's life: warm-up, autotune and
+              # graph-capture frames come first
+              "timed_frame_range": [frames_before + 1, frames_before + args.steps],
               "launch": ("direct launches per substep" if args.direct else
                          "resident frames: substeps replayed from a captured HIP graph"),
               "graph_substeps": graphs1 - graphs0, "host_syncs": syncs1 - syncs0,
