@@ -482,3 +482,40 @@ def test_fresh_contact_starts_cold():
         np.testing.assert_array_equal(R["penalty"][[n, t, bt]],
                                       np.array([1e6, 1.0, 1.0], np.float32))
         assert R["was_static"][t] == 0 and R["was_static"][bt] == 0
+
+
+# ---------------------------------------------------------------------------
+# speed caps of the velocity update (finalize_and_cap_6dof, kernels_6dof.py:1871-1901)
+# ---------------------------------------------------------------------------
+def _free_box(v0, w0, max_lin, max_ang):
+    """A lone box in free fall (no floor, no contacts): after one substep its
+    pose is the inertial prediction, so the velocity update sees
+    v = v0 + g dt and the rotation of w0 dt before any cap."""
+    s = O.oracle_solver_class()(dt=1 / 60, iterations=4, substeps=1,
+                                max_linear_speed=max_lin, max_angular_speed=max_ang)
+    s.add_box((0., 5.0, 0.), (0.2, 0.2, 0.2), mass=1.0, velocity=v0, angular_velocity=w0)
+    s._flush()
+    one_substep(s)
+    return s.velocities()[0].astype(np.float64), s.angular_velocities()[0].astype(np.float64)
+
+
+def test_speed_caps_scale_to_the_cap_along_the_velocity():
+    """|v| above max_linear_speed is scaled to the cap along its own direction
+    (and |w| above max_angular_speed likewise); below the caps v = v0 + g dt
+    and w = w0 (a free rotation about a principal axis)."""
+    dt = 1 / 60
+    v_free = np.array([20.0, -9.81 * dt, 0.0])
+    w0 = (0.0, 60.0, 0.0)
+    v, w = _free_box((20.0, 0.0, 0.0), w0, INF, INF)
+    np.testing.assert_allclose(v, v_free, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(w, w0, rtol=1e-5, atol=1e-5)
+    v, w = _free_box((20.0, 0.0, 0.0), w0, 5.0, 50.0)
+    assert abs(np.linalg.norm(v) - 5.0) < 5.0 * 1e-6
+    np.testing.assert_allclose(v / np.linalg.norm(v), v_free / np.linalg.norm(v_free),
+                               rtol=1e-6, atol=1e-6)
+    assert abs(np.linalg.norm(w) - 50.0) < 50.0 * 1e-6
+    np.testing.assert_allclose(w / np.linalg.norm(w), (0.0, 1.0, 0.0), atol=1e-6)
+    # caps above the speeds change nothing
+    v2, w2 = _free_box((20.0, 0.0, 0.0), w0, 30.0, 70.0)
+    np.testing.assert_allclose(v2, v_free, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(w2, w0, rtol=1e-5, atol=1e-5)
