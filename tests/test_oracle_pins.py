@@ -519,3 +519,54 @@ def test_speed_caps_scale_to_the_cap_along_the_velocity():
     v2, w2 = _free_box((20.0, 0.0, 0.0), w0, 30.0, 70.0)
     np.testing.assert_allclose(v2, v_free, rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(w2, w0, rtol=1e-5, atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# frames of the rotation update and the broad phase's contact margin
+# ---------------------------------------------------------------------------
+def _qexp(w):
+    th = float(np.linalg.norm(w))
+    if th == 0.0:
+        return np.array([0.0, 0.0, 0.0, 1.0])
+    a = np.asarray(w, np.float64) / th
+    return np.concatenate([a * math.sin(th / 2), [math.cos(th / 2)]])
+
+
+def test_rotation_update_is_in_the_world_frame():
+    """Angular velocity is a world-frame vector (kernels_6dof.py:122-187,
+    1871-1901): the inertial rotation of a free box is q = exp(w dt) (x) q0,
+    not q0 (x) exp(w dt), and the velocity update recovers the same world w
+    from q (x) conj(q0). A box turned 30 deg about z spinning about world x
+    tells the two orders apart (they differ by ~0.1 in q)."""
+    dt = 1 / 60
+    q0 = np.array([0.0, 0.0, math.sin(math.pi / 12), math.cos(math.pi / 12)])
+    w0 = np.array([5.0, 0.0, 0.0])
+    s = O.oracle_solver_class()(dt=dt, iterations=4, substeps=1, gravity=(0.0, 0.0, 0.0))
+    s.add_box((0., 5.0, 0.), (0.2, 0.3, 0.4), mass=1.0, orientation=tuple(q0),
+              angular_velocity=tuple(w0))
+    s._flush()
+    one_substep(s)
+    q = s.orientations()[0].astype(np.float64)
+    want = qmul(_qexp(w0 * dt), q0)
+    wrong = qmul(q0, _qexp(w0 * dt))
+    assert np.linalg.norm(want - wrong) > 0.01
+    q = q if np.dot(q, want) >= 0 else -q
+    np.testing.assert_allclose(q, want, atol=2e-6)
+    np.testing.assert_allclose(s.angular_velocities()[0], w0, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("gap,pairs", [(0.003, 1), (0.009, 1), (0.011, 0)])
+def test_broad_phase_pairs_boxes_within_twice_the_margin(gap, pairs):
+    """Every box's AABB is grown by the 5 mm contact margin on each side
+    (kernels_6dof.py:1939-1941, margin from solver_6dof.py:1789), so two
+    axis-aligned boxes side by side are a broad-phase pair while their face
+    gap is under 10 mm."""
+    h = 0.2
+    s = O.oracle_solver_class()(dt=1 / 60, iterations=4, substeps=1, gravity=(0.0, 0.0, 0.0))
+    s.add_box((0., 1.0, 0.), (h, h, h), mass=1.0, friction=0.5)
+    s.add_box((2 * h + gap, 1.0, 0.), (h, h, h), mass=1.0, friction=0.5)
+    s.enable_self_collision(True, default_friction=0.5)
+    s._flush()
+    one_substep(s)
+    _, n_pairs, _ = s._h.read_counters()
+    assert n_pairs == pairs
