@@ -887,13 +887,14 @@ int primal_bpw(int variant) {
 }
 
 // What a graph-captured substep needs: the fixed-capacity semantics, the
-// dataflow sweep with a separate dual, rows to solve, no learned warm start
+// dataflow sweep with a separate dual or the dual in trailing workgroups of
+// its launch, rows to solve, no learned warm start
 // and no profiling. Buffers a direct substep would size on the fly are sized
 // to capacity first (this allocates, so it runs before any capture).
 int graph_eligible(avbd_solver* s, const avbd_params* p, bool* ok) {
   *ok = false;
   if (!s->graph_on || s->nb == 0 || p->fixed_capacity == 0 || s->ml_mode != 0 || s->prof ||
-      s->primal_variant < 6 || s->primal_variant > 10 || s->fuse_dual_mode != 0 ||
+      s->primal_variant < 6 || s->primal_variant > 10 || s->fuse_dual_mode == 1 ||
       s->flow_test_cap != 0 || s->row_cap <= 0 || p->iterations + p->post_stabilize >= 4095)
     return AVBD_OK;
   const bool emit = p->self_collide && s->nb >= 2;
@@ -1030,14 +1031,19 @@ void substep_device(avbd_solver* s, const avbd_params* p) {
   }
   const int I = p->iterations;
   const int max_tiles = (nb + bpw - 1) / bpw + kMaxColors;
+  // the dual in trailing workgroups of the sweep's launch (sized by the group
+  // capacity, the live count read on the device), or a launch of its own
+  const bool tail = s->fuse_dual_mode == 2;
   for (int it = 0; it < I + post; ++it) {
     const int post_pass = (post && it == I) ? 1 : 0;
     launch_primal_sweep(st, 0, s->cstart, s->cvals.p, s->color.p, pw, s->poseI.p, s->inert.p,
                         s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p, s->gst.p, dt, post_pass,
                         s->flow_done.p, reinterpret_cast<const unsigned*>(d + kEpochSlot), it,
-                        d + kErrSlot, bpw, 0, p->beta, s->gst.p, 0, nullptr, 0, d + kNColorsSlot,
-                        d + kCstartSlot, d + kTilesSlot, max_tiles);
-    if (it < I) launch_dual(st, s->g_cap, pw, s->geo.p, s->gst.p, p->beta, nullptr, ngroups);
+                        d + kErrSlot, bpw, 0, p->beta, s->gst.p, tail && it < I ? s->g_cap : 0,
+                        nullptr, 0, d + kNColorsSlot, d + kCstartSlot, d + kTilesSlot, max_tiles,
+                        ngroups);
+    if (it < I && !tail)
+      launch_dual(st, s->g_cap, pw, s->geo.p, s->gst.p, p->beta, nullptr, ngroups);
     if (it == I - 1)
       launch_finalize(st, nb, pw, p0, s->vel.p, s->pvel.p, s->bstat.p, dt, max_lin, max_ang);
   }
@@ -1063,7 +1069,7 @@ std::vector<char> graph_key(const avbd_solver* s, const avbd_params* p) {
   const long long ints[] = {s->nb, s->cur, s->n_static_rows, s->g_static, s->g_cap, s->row_cap,
                             s->pair_cap, s->pool_max, s->hash_cap, s->st_total,
                             s->primal_variant, s->n_scenes, s->scene_grid,
-                            (long long)g_alloc_gen};
+                            s->fuse_dual_mode, (long long)g_alloc_gen};
   put(ints, sizeof(ints));
   put(&s->scene_pitch, sizeof(s->scene_pitch));
   const void* ptrs[] = {s->color.p, s->color_next.p, s->pose[0].p, s->pose[1].p};

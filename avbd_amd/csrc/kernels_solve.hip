@@ -678,8 +678,11 @@ struct SweepFlow {
   // the last tile take 64 groups each, wait until both bodies of a group
   // have published this sweep's pose, and update it. Only the tiles of a
   // group's two bodies read its state, and both are done by then, so the
-  // result is the separate dual launch's for any coloring.
+  // result is the separate dual launch's for any coloring. tail_nd
+  // (graph-captured substeps): the live group count on the device, of which
+  // tail_groups is the capacity.
   int tail_groups;
+  const int* tail_nd;
 };
 
 constexpr int kFlowSpinCap = 1 << 20;
@@ -707,7 +710,7 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
   const int stamp = kFlow ? sweep_stamp(fl) : 0;
   if (kFlow && fl.tail_groups > 0 && (int)blockIdx.x >= fl.n_tiles) {
     const int g = ((int)blockIdx.x - fl.n_tiles) * 64 + lane;
-    if (g < fl.tail_groups) {
+    if (g < dev_count(fl.tail_groups, fl.tail_nd)) {
       const GroupGeo G = geo[g];
       const bool box = g_kind(G.hdr.x) == kGBox;
       const int a = G.hdr.y, b = box ? G.hdr.z : G.hdr.y;
@@ -1055,8 +1058,10 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          int* done, const unsigned* epoch, int sweep, int* err, int bpw,
                          int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
                          const int* done_rd, int spin_cap, const int* meta_ncolors,
-                         const int* meta_cstart, const int* meta_tiles, int max_tiles) {
+                         const int* meta_cstart, const int* meta_tiles, int max_tiles,
+                         const int* tail_nd) {
   SweepFlow fl{};
+  fl.tail_nd = tail_nd;
   fl.meta_ncolors = meta_ncolors;
   fl.meta_cstart = meta_cstart;
   fl.meta_tiles = meta_tiles;

@@ -111,6 +111,11 @@ def run_headless(args, out=print, solver_cls=Solver6DOF) -> dict:
     out(f"coloring mode:    {solver.coloring_mode}")
     for _ in range(args.headless_warmup):
         solver.step()
+    if getattr(args, "autotune", False) and hasattr(solver, "autotune_dual"):
+        tuned = solver.autotune_primal(frames=1)
+        out(f"primal autotune:  {tuned} -> {solver.primal_variant}")
+        tuned = solver.autotune_dual(frames=3)
+        out(f"dual autotune:    {tuned} -> {solver.dual_placement}")
     out(f"colors (cap):     {solver.num_colors}")
     out(f"colors (active):  {solver.num_active_colors}")
     out(f"total rows (cap): {solver._gpu_pool_n_capacity}")
@@ -166,6 +171,9 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--fixed-capacity", action="store_true")
     p.add_argument("--headless-warmup", type=int, default=5)
     p.add_argument("--headless-frames", type=int, default=30)
+    p.add_argument("--autotune", action="store_true",
+                   help="after the warm-up, pick the primal sweep kernel and the dual's "
+                        "placement by timing frames (identical results; not in the reference)")
     return p
 
 

@@ -283,6 +283,7 @@ class Solver6DOF:
         self._warm_start_params: np.ndarray | None = None
         self._warm_start_scope = 0
         self.primal_variant = 6            # the library's default primal sweep kernel
+        self.dual_placement = 0            # the dual in a launch of its own (autotune_dual)
 
     # ---- scene building (solver_6dof.py:446-578) ----------------------------
     def add_box(self, position, half_extents, mass: float, orientation=(0.0, 0.0, 0.0, 1.0),
@@ -700,6 +701,36 @@ class Solver6DOF:
         best = min(times, key=times.get)
         self._h.set_option(N.OPT_PRIMAL_VARIANT, best)
         self.primal_variant = best
+        return times
+
+    DUAL_PLACEMENTS = (0, 2)
+
+    def autotune_dual(self, frames: int = 3) -> dict:
+        """Pick where the dual update of each iteration runs by timing real
+        frames (wall clock, each frame synchronised by a read of the poses):
+        in a launch of its own after the sweep (0), or in trailing workgroups
+        of the sweep's launch, each group once both of its bodies have moved
+        (2). Both give bit-identical results, so the frames spent timing are
+        ordinary simulation frames; the trailing placement saves one launch
+        per iteration, which pays on small scenes. Returns {placement: ms per
+        frame} and keeps the fastest. No counterpart in the reference."""
+        import time
+        if self._dirty:
+            self._flush()
+        n = max(1, int(frames))
+        times = {}
+        for mode in self.DUAL_PLACEMENTS:
+            self._h.set_option(N.OPT_FUSE_DUAL, int(mode))
+            self.step()          # untimed: captures the graph of this placement
+            self.positions()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                self.step()
+            self.positions()
+            times[int(mode)] = (time.perf_counter() - t0) * 1e3 / n
+        best = min(times, key=times.get)
+        self._h.set_option(N.OPT_FUSE_DUAL, best)
+        self.dual_placement = best
         return times
 
     # ---- learned warm start (not in the reference) ---------------------------

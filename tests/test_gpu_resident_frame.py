@@ -52,12 +52,19 @@ SCENES = {
 }
 
 
+@pytest.mark.parametrize("dual", [0, 2], ids=["dual-launch", "dual-trailing"])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_resident_frames_match_checker_and_direct(name):
+def test_resident_frames_match_checker_and_direct(name, dual):
+    """dual: the dual update in a launch of its own after each sweep (0) or in
+    trailing workgroups of the sweep's launch (2), in both the graph and the
+    direct solver."""
     build = SCENES[name]
     g = build(Solver6DOF, device="cuda:0")
     d = build(Solver6DOF, device="cuda:0")
     d.hip_graph = False
+    for s in (g, d):
+        s._flush()
+        s._h.set_option(1, dual)
     o = build(oracle_solver_class(), device="cuda:0")
     assert g._resident_on(g.device) and g._fixed_capacity_mode(g.device)
     frames = 14 if name == "pile" else 8
