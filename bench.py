@@ -80,6 +80,33 @@ def measured_traffic(layers: int, variant: int) -> float | None:
     return float(fields["traffic_bytes_per_launch"])
 
 
+SQ_FILE = ROOT / "profiles" / "k_primal_sweep_pyramid66_sq.md"
+SIMDS, SCLK_GHZ = 1024, 2.4   # MI355X: 256 CUs x 4 SIMDs, peak engine clock
+
+
+def valu_issue(layers: int, variant: int, avg_launch_us: float | None) -> dict | None:
+    """VALU issue of the primal sweep: its VALU instructions per launch, from
+    the committed SQ counter summary when it was taken on this workload,
+    kernel variant and kernel sources, against this run's launch time. A
+    wave64 FP32 instruction without packed math holds its SIMD for 4 cycles,
+    so the issue limit is SIMDS x clock / 4 wavefront instructions per
+    second. None when the summary does not match."""
+    if layers != 66 or not avg_launch_us or not SQ_FILE.exists():
+        return None
+    keys = ("valu_instructions_per_launch:", "primal_variant:", "kernel_source_sha256:")
+    fields = {k.strip(): v.strip() for k, v in
+              (line.split(":", 1) for line in SQ_FILE.read_text().splitlines()
+               if line.startswith(keys))}
+    if int(fields.get("primal_variant", "-1")) != variant:
+        return None
+    if fields.get("kernel_source_sha256") != kernel_source_hash():
+        return None
+    instr = float(fields["valu_instructions_per_launch"])
+    limit_us = instr * 4.0 / (SIMDS * SCLK_GHZ * 1e3)
+    return {"instructions_per_launch": instr, "issue_limit_us": limit_us,
+            "frac": limit_us / avg_launch_us, "source": SQ_FILE.name}
+
+
 def cpu_baseline(layers: int, seed: int, substeps_sample: int) -> dict:
     """The C++ restatement of the reference substep (one host thread) on the
     same scene: one untimed substep from the initial state, then
@@ -494,6 +521,9 @@ def main() -> int:
                     else None),
             "avg_launch_us": 1e3 * prim_ms / prim_n if prim_n else None,
             "algorithmic_bytes_per_launch": prim_b / prim_n if prim_n else None,
+            # the bound the sweep actually meets: VALU issue
+            "valu": (valu_issue(args.layers, variant, 1e3 * prim_ms / prim_n if prim_n else None)
+                     if args.workload == "pyramid" else None),
         },
         "kernel_ms": breakdown,
         "substep_roofline": substep_roofline(n_b, n_contacts, len(s._rows) // 3, s.iterations,
