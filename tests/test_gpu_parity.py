@@ -151,6 +151,30 @@ def test_autotuned_solver_keeps_the_trajectory():
             assert np.array_equal(ba[key], bb[key]), key
 
 
+def test_dual_autotuned_resident_solver_keeps_the_trajectory():
+    """autotune_dual times resident frames with the dual in its own launch and
+    in trailing workgroups of the sweep's launch (graph replays of each); both
+    give identical results, so the tuned solver stays bit-identical to one
+    that was not tuned."""
+    from avbd_amd.solver_6dof import Solver6DOF
+    a = scenes.random_pile(150, seed=9, lattice=(6, 6, 5), solver_cls=Solver6DOF,
+                           device="cuda:0")
+    b = scenes.random_pile(150, seed=9, lattice=(6, 6, 5), solver_cls=Solver6DOF,
+                           device="cuda:0")
+    times = b.autotune_dual(frames=2)
+    print("ms per frame by dual placement", times, "picked", b.dual_placement, flush=True)
+    assert set(times) == {0, 2} and times[b.dual_placement] == min(times.values())
+    for _ in range(2 * 3):    # the frames autotune_dual stepped
+        a.step()
+    for _ in range(5):
+        a.step()
+        b.step()
+        ba, bb = a._h.read_bodies(len(a._x)), b._h.read_bodies(len(b._x))
+        for key in ("x", "q", "v", "w"):
+            assert np.array_equal(ba[key], bb[key]), key
+    assert b._h.sync_count()[1] > 0   # the frames replayed the captured substep
+
+
 def test_ten_cube_stack_full_run_lockstep():
     """The 10-cube stack configuration run to its full length: 300 frames of
     4 substeps (10 iterations), bit-exact against the checker throughout.
