@@ -98,7 +98,7 @@ size_t bvh_workspace_bytes(int nb);
 // copies); scene: per-body scene index or null (one scene)
 // slots: nb x kPartnerSlots ints, each body's first partners sorted by the
 // count pass and copied by the write pass
-constexpr int kPartnerSlots = 16;
+constexpr int kPartnerSlots = 64;
 void bvh_count_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
                      const void* ws, int* counts, int* slots);
