@@ -10,6 +10,11 @@ timed region starts. The line also carries iterations to the dual residual
 1e-4 (cap 200), plain vs learned warm start, on the landed 10,000-box pile,
 the 98,021-cube pyramid and a 285-cube pyramid.
 
+--workload pile: the random pile of 10,000 boxes (half-extents U[0.1, 0.4] m,
+random orientations, 22 x 22 x 21 lattice, mu_d 0.4 / mu_s 0.5), dt 1/60,
+4 substeps, 10 iterations; the timed frames follow --warmup frames of the
+fall (use --warmup 600 for the landed pile of the configuration).
+
 --workload batched: 2048 independent piles of n = round(exp(U(ln 8,
 ln 1024))) boxes each (the 10k-box pile's generator per scene), 10
 iterations, stepped as one batch; same metric over all scenes.
@@ -346,8 +351,9 @@ def train_bench(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("pyramid", "batched", "particles", "train"),
+    ap.add_argument("--workload", choices=("pyramid", "pile", "batched", "particles", "train"),
                     default="pyramid")
+    ap.add_argument("--boxes", type=int, default=10_000, help="pile: number of boxes")
     ap.add_argument("--train-samples", type=int, default=2_097_152,
                     help="train: synthetic samples")
     ap.add_argument("--batch", type=int, default=16_384, help="train: minibatch")
@@ -407,6 +413,8 @@ def main() -> int:
     t_build = time.perf_counter()
     if args.workload == "batched":
         s, sizes = batched_scene(args.scenes, args.seed)
+    elif args.workload == "pile":
+        s = scenes.random_pile(args.boxes, seed=args.seed)
     else:
         s = scenes.pyramid(args.layers, seed=args.seed)
     # "cuda:0": the resident frame (fixed-capacity pools, substeps replayed
@@ -486,6 +494,11 @@ def main() -> int:
                     f"{n_b} boxes in all) stepped as one batch, dt 1/60, {s.substeps} substeps, "
                     f"{s.iterations} iterations")
         metric = f"body-substeps/s, {args.scenes} batched scenes"
+    elif args.workload == "pile":
+        workload = (f"random pile of {n_b} boxes, dt 1/60, {s.substeps} substeps, "
+                    f"{s.iterations} iterations, frames {frames_before + 1}-"
+                    f"{frames_before + args.steps} after the drop")
+        metric = "body-substeps/s, 10k-box pile"
     else:
         workload = (f"pyramid of {n_b} cubes ({args.layers} layers), dt 1/60, "
                     f"{s.substeps} substeps, {s.iterations} iterations")
