@@ -29,6 +29,7 @@ OPT_FUSE_DUAL = 1
 OPT_LEARNED_SCOPE = 2
 OPT_FLOW_GIVE_UP_TEST = 3
 OPT_GRAPH = 4
+OPT_BROAD_LANES = 5
 
 
 class AvbdError(RuntimeError):

@@ -98,10 +98,13 @@ size_t bvh_workspace_bytes(int nb);
 // copies); scene: per-body scene index or null (one scene)
 // slots: nb x kPartnerSlots ints, each body's first partners sorted by the
 // count pass and copied by the write pass
+// lanes: traversal lanes per body (1 = one lane walking the escape links; 8,
+// 16, 64 = a group sharing an LDS stack; 0 = bvh_count_lanes(nb))
 constexpr int kPartnerSlots = 64;
 void bvh_count_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, int* counts, int* slots);
+                     const void* ws, int* counts, int* slots, int lanes);
+int bvh_count_lanes(int nb);
 void bvh_write_pairs(hipStream_t st, int nb, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
                      const void* ws, const int* counts, const int* slots, const int* offsets,

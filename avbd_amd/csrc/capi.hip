@@ -215,6 +215,7 @@ struct avbd_solver {
   // dual placement with the dataflow sweep: 0 separate launch, 1 inside the
   // tiles (measured slower: the sweep is VALU-bound), 2 trailing workgroups
   int fuse_dual_mode = 0;
+  int broad_lanes = 0;  // AVBD_OPT_BROAD_LANES
   // test of the dataflow give-up path (AVBD_OPT_FLOW_GIVE_UP_TEST): waits
   // poll flow_never (never written) and give up after this many polls
   int flow_test_cap = 0;
@@ -509,7 +510,7 @@ int emit_contacts(avbd_solver* s, const avbd_params* p, bool fixed, avbd_report*
   bvh_build(st, nb, tlo, thi, s->ws_bvh.p, s->ws_bvh.n);
   CK(s->pslots.ensure((size_t)nb * kPartnerSlots));
   bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p, s->bcount.p,
-                  s->pslots.p);
+                  s->pslots.p, s->broad_lanes);
   CK(scan(s, s->bcount.p, s->boff.p, nb, s->dcnt.p + 0));
   CK(prof_end(s, 0.0));
   CK(readback(s, 0, 1));
@@ -933,7 +934,7 @@ void substep_device(avbd_solver* s, const avbd_params* p) {
     }
     bvh_build(st, nb, tlo, thi, s->ws_bvh.p, s->ws_bvh.n);
     bvh_count_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
-                    s->bcount.p, s->pslots.p);
+                    s->bcount.p, s->pslots.p, s->broad_lanes);
     (void)scan(s, s->bcount.p, s->boff.p, nb, d + kPairSlot);
     hipLaunchKernelGGL(k_kept_pairs, dim3(1), dim3(64), 0, st, d, pcap);
     bvh_write_pairs(st, nb, s->lo.p, s->hi.p, tlo, thi, scene, s->bstat.p, s->ws_bvh.p,
@@ -1069,7 +1070,7 @@ std::vector<char> graph_key(const avbd_solver* s, const avbd_params* p) {
   const long long ints[] = {s->nb, s->cur, s->n_static_rows, s->g_static, s->g_cap, s->row_cap,
                             s->pair_cap, s->pool_max, s->hash_cap, s->st_total,
                             s->primal_variant, s->n_scenes, s->scene_grid,
-                            s->fuse_dual_mode, (long long)g_alloc_gen};
+                            s->fuse_dual_mode, s->broad_lanes, (long long)g_alloc_gen};
   put(ints, sizeof(ints));
   put(&s->scene_pitch, sizeof(s->scene_pitch));
   const void* ptrs[] = {s->color.p, s->color_next.p, s->pose[0].p, s->pose[1].p};
@@ -1972,6 +1973,11 @@ int avbd_set_option(avbd_solver* s, int option, int value) {
         HIPCK(hipStreamSynchronize(s->st));
         drop_graphs(s);
       }
+      return AVBD_OK;
+    case AVBD_OPT_BROAD_LANES:
+      if (value != 0 && value != 1 && value != 8 && value != 16 && value != 64)
+        return set_err(AVBD_E_ARG, "broad-phase lanes must be 0, 1, 8, 16 or 64");
+      s->broad_lanes = value;
       return AVBD_OK;
     case AVBD_OPT_FLOW_GIVE_UP_TEST:
       if (value < 0 || value > 4096) return set_err(AVBD_E_ARG, "give-up test cap must be 0 to 4096");

@@ -482,6 +482,109 @@ k_bvh_count(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
   counts[i] = cnt;
 }
 
+// The same traversal with G lanes of one wavefront per body. The group's lanes
+// share a stack in LDS: every round each lane pops one node (the group takes
+// up to G from the top), tests it, and an overlapping internal node pushes
+// both children; the group's pushes are placed by a prefix count over its
+// ballot bits. Every leaf whose ancestors all overlap the query is reached
+// exactly once, as in the stackless walk, so the partner set is the same; a
+// body's partners land in an LDS list (order of discovery) and are written to
+// its slots by rank, ascending. When a push would not fit in the stack, the
+// lane walks both children's subtrees itself with the escape links: from the
+// left child until the parent's escape, the next node after the parent's
+// subtree. Groups never leave their wavefront, so the shared arrays need no
+// workgroup barrier: LDS instructions of one wavefront complete in order.
+template <int G>
+__global__ void __launch_bounds__(256)
+k_bvh_count_grp(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
+                int* __restrict__ counts, int* __restrict__ slots) {
+  static_assert(G >= 2 && G <= 64 && (64 % G) == 0, "group within a wavefront");
+  constexpr int kGroups = 256 / G;
+  constexpr int kStack = G >= 64 ? 256 : (G >= 16 ? 192 : 128);
+  __shared__ int stk[kGroups][kStack];
+  __shared__ int found[kGroups][kPartnerSlots];
+  __shared__ int nfound[kGroups];
+  const int grp = threadIdx.x / G, r = threadIdx.x % G;
+  const int i = blockIdx.x * kGroups + grp;
+  if (i >= n) return;  // whole groups: G divides the wavefront
+  int* seg = slots + (size_t)i * kPartnerSlots;
+  if (n == 1) {
+    if (r == 0) counts[i] = 0;
+    return;
+  }
+  const int lane = threadIdx.x & 63, base = lane - r;
+  const unsigned long long gmask =
+      G == 64 ? ~0ull : (((1ull << G) - 1) << base);
+  const unsigned long long below = (1ull << lane) - 1;
+  const float4 qlo = B.lo[i], qhi = B.hi[i], tqlo = B.tlo[i], tqhi = B.thi[i];
+  const bool st_i = bstat[i].x <= 0.0f;
+  if (r == 0) {
+    stk[grp][0] = 0;
+    nfound[grp] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  int top = 1;  // the same in every lane of the group
+  auto hit = [&](int j) {
+    const int at = atomicAdd(&nfound[grp], 1);
+    if (at < kPartnerSlots) found[grp][at] = j;
+  };
+  while (top > 0) {
+    const int take = top < G ? top : G;
+    const int node = r < take ? stk[grp][top - 1 - r] : -1;
+    top -= take;
+    int kid = -1, rkid = -1, esc = -1;
+    if (node >= n - 1) {
+      const int j = w.lf[node - (n - 1)].x;
+      if (j > i && leaf_pair(B, i, j, qlo, qhi, st_i, bstat)) hit(j);
+    } else if (node >= 0) {
+      const float4 a = w.pk[2 * node], b = w.pk[2 * node + 1];
+      const int rk = w.right[node];
+      if (box_overlap(tqlo, tqhi, a, b)) {
+        kid = __float_as_int(a.w);
+        rkid = rk;
+        esc = __float_as_int(b.w);
+      }
+    }
+    const unsigned long long m = __ballot(kid >= 0) & gmask;
+    const int before = __popcll(m & below), pushes = __popcll(m);
+    const int room = (kStack - top) / 2;
+    __builtin_amdgcn_wave_barrier();
+    if (kid >= 0) {
+      if (before < room) {
+        stk[grp][top + 2 * before] = rkid;
+        stk[grp][top + 2 * before + 1] = kid;
+      } else {
+        int v = kid;
+        int steps = 0;
+        const int max_steps = 4 * n + 8;
+        while (v != esc && v != -1 && ++steps <= max_steps) {
+          if (v >= n - 1) {
+            const int2 L = w.lf[v - (n - 1)];
+            if (L.x > i && leaf_pair(B, i, L.x, qlo, qhi, st_i, bstat)) hit(L.x);
+            v = L.y;
+          } else {
+            const float4 a = w.pk[2 * v], b = w.pk[2 * v + 1];
+            v = box_overlap(tqlo, tqhi, a, b) ? __float_as_int(a.w) : __float_as_int(b.w);
+          }
+        }
+      }
+    }
+    top += 2 * (pushes < room ? pushes : room);
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int cnt = nfound[grp];
+  if (r == 0) counts[i] = cnt;
+  if (cnt <= kPartnerSlots) {
+    // partners are distinct: a partner's rank is its position in ascending order
+    for (int a = r; a < cnt; a += G) {
+      const int v = found[grp][a];
+      int rank = 0;
+      for (int b = 0; b < cnt; ++b) rank += found[grp][b] < v ? 1 : 0;
+      seg[rank] = v;
+    }
+  }
+}
+
 // Smallest overlapping partner of body i that is greater than `after`.
 __device__ int bvh_next_partner(int n, int i, int after, const BroadBoxes& B,
                                 const float4* __restrict__ bstat, const BvhWs& w) {
@@ -546,12 +649,30 @@ k_bvh_write(int n, BroadBoxes B, const float4* __restrict__ bstat, BvhWs w,
 
 void bvh_count_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,
-                     const void* ws, int* counts, int* slots) {
+                     const void* ws, int* counts, int* slots, int lanes) {
   BvhWs w = bvh_carve(const_cast<void*>(ws), n);
   const BroadBoxes B{lo, hi, tlo, thi, scene};
-  hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w, counts,
-                     slots);
+  if (lanes == 0) lanes = bvh_count_lanes(n);
+  switch (lanes) {
+    case 8:
+      hipLaunchKernelGGL(k_bvh_count_grp<8>, dim3((n + 31) / 32), dim3(256), 0, st, n, B, bstat, w,
+                         counts, slots);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_bvh_count_grp<16>, dim3((n + 15) / 16), dim3(256), 0, st, n, B, bstat, w,
+                         counts, slots);
+      break;
+    case 64:
+      hipLaunchKernelGGL(k_bvh_count_grp<64>, dim3((n + 3) / 4), dim3(256), 0, st, n, B, bstat, w,
+                         counts, slots);
+      break;
+    default:
+      hipLaunchKernelGGL(k_bvh_count, dim3((n + 255) / 256), dim3(256), 0, st, n, B, bstat, w,
+                         counts, slots);
+  }
 }
+
+int bvh_count_lanes(int n) { return n <= 16384 ? 64 : (n <= 65536 ? 16 : 8); }
 
 void bvh_write_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,

@@ -379,6 +379,9 @@ def main() -> int:
     ap.add_argument("--fuse-dual", type=int, default=0,
                     help="dual update with the dataflow sweep: 0 separate launch (default), "
                          "1 inside the sweep's tiles, 2 in trailing workgroups of its launch")
+    ap.add_argument("--broad-lanes", type=int, default=0,
+                    help="broad-phase traversal lanes per body: 0 by body count (default), "
+                         "1, 8, 16 or 64")
     ap.add_argument("--primal-variant", type=int, default=None,
                     help="0 lane per body; 1-4 one wavefront per 1, 2, 4, 8 bodies; "
                          "5 one or two bodies per wavefront by color size; "
@@ -426,6 +429,7 @@ def main() -> int:
     if variant is not None:
         s._h.set_option(0, variant)
     s._h.set_option(1, args.fuse_dual)
+    s._h.set_option(5, args.broad_lanes)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
           f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
@@ -520,7 +524,7 @@ def main() -> int:
                    "bodies": n_b, "substeps_per_step": s.substeps, "iterations": s.iterations,
                    "contacts": n_contacts, "pairs": n_pairs, "rows": n_rows,
                    "colors": s.num_active_colors, "parallelism": "single GPU",
-                   "primal_variant": variant,
+                   "primal_variant": variant, "broad_lanes": args.broad_lanes,
                    "primal_autotune_ms": ({str(k): round(v, 3) for k, v in tuned.items()}
                                           if tuned else None)},
         "roofline": {
