@@ -672,7 +672,13 @@ void bvh_count_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
   }
 }
 
-int bvh_count_lanes(int n) { return n <= 16384 ? 64 : (n <= 65536 ? 16 : 8); }
+// Lanes per body by body count: few bodies fill few wavefronts, so their walks
+// are split; with many, the chip is full and lanes that find nothing cost
+// issue slots (measured: 10k-box pile best at 64, 98k pyramid at 8, 429k
+// batched bodies at 1).
+int bvh_count_lanes(int n) {
+  return n <= 16384 ? 64 : (n <= 65536 ? 16 : (n <= 262144 ? 8 : 1));
+}
 
 void bvh_write_pairs(hipStream_t st, int n, const float4* lo, const float4* hi,
                      const float4* tlo, const float4* thi, const int* scene, const float4* bstat,

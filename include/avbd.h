@@ -309,8 +309,8 @@ int avbd_read_manifold(avbd_solver* s, int cap, int* count, int* ref_is_a, float
  * (the reference's wp.bvh_query_aabb loop, kernels_6dof.py:1946-1978, walks
  * one thread per body): 1 = one lane per body; 8, 16, 64 = a group of that
  * many lanes sharing a stack, each lane testing one node per round; 0
- * (default) = 64 up to 16,384 bodies, 16 up to 65,536, 8 above. All give
- * identical pair lists. */
+ * (default) = 64 up to 16,384 bodies, 16 up to 65,536, 8 up to 262,144, 1
+ * above. All give identical pair lists. */
 #define AVBD_OPT_BROAD_LANES 5
 int avbd_set_option(avbd_solver* s, int option, int value);
 
