@@ -81,7 +81,7 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          const int* done_rd = nullptr, int spin_cap = 0,
                          const int* meta_ncolors = nullptr, const int* meta_cstart = nullptr,
                          const int* meta_tiles = nullptr, int max_tiles = 0,
-                         const int* tail_nd = nullptr);
+                         const int* tail_nd = nullptr, bool same_color_order = false);
 void launch_dual(hipStream_t st, int n_groups, const float4* pose, const GroupGeo* geo,
                  GroupSt* gst, float beta, float* resid, const int* nd = nullptr);
 void launch_finalize(hipStream_t st, int nb, const float4* pose, const float4* pose0,

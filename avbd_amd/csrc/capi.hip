@@ -402,6 +402,7 @@ int recolor(avbd_solver* s, int mode) {
   launch_color_reset(st, nb, s->color.p, scene, dirty);
   launch_fill_int(st, s->dcnt.p + 6, 1, 0);
   launch_fill_int(st, s->dcnt.p + 9, 1, 0);
+  bool leftover = false;  // bodies still uncolored after the last round take color 0
   if (mode == AVBD_COLOR_JACOBI) {
     for (int round = 0; round < kMaxColors; ++round) {
       launch_color_greedy(st, nb, s->order_id.p, nullptr, s->nbr_start.p, s->nbr.p, s->color.p,
@@ -411,7 +412,8 @@ int recolor(avbd_solver* s, int mode) {
                            s->color_next.p, s->dcnt.p + 5);
       std::swap(s->color.p, s->color_next.p);
       CK(readback(s, 5, 1));
-      if (s->hcnt[5] == 0) break;
+      leftover = s->hcnt[5] != 0;
+      if (!leftover) break;
     }
   } else {
     launch_color_greedy(st, nb, s->order_jp.p, s->pos_jp.p, s->nbr_start.p, s->nbr.p, s->color.p,
@@ -428,7 +430,8 @@ int recolor(avbd_solver* s, int mode) {
   HIPCK(hipStreamSynchronize(st));
   ++s->syncs;
   if (s->hcnt[6] != 0) return set_err(AVBD_E_STATE, "coloring dataflow did not converge");
-  s->colors_conflict_free = s->hcnt[9] == 0;
+  // dcnt[9]: a body found every color taken (or, JP, ran out of rounds)
+  s->colors_conflict_free = s->hcnt[9] == 0 && !leftover;
   int mx = 0;
   s->cstart[0] = 0;
   for (int c = 0; c < kMaxColors; ++c) {
@@ -750,13 +753,19 @@ int stage_sweep(avbd_solver* s, const avbd_params* p, int it) {
   CK(prof_begin(s, kPrimal));
   int launched = 0;
   if (x.flow) {
+    // A coloring with same-color neighbours (more than 32 colors needed):
+    // one body per tile, each waiting for its same-color neighbours of lower
+    // index, which is the serial order within a color.
+    const bool ordered = !s->colors_conflict_free;
     launch_primal_sweep(st, s->n_active_colors, s->cstart, s->cvals.p, s->color.p, pw,
                         s->poseI.p, s->inert.p, s->bstat.p, s->cpos.p, s->tcsr.p, s->geo.p,
                         s->gst.p, dt, post_pass, s->flow_done.p,
                         reinterpret_cast<const unsigned*>(s->dcnt.p + kEpochSlot), it,
-                        s->dcnt.p + kErrSlot, x.flow_bpw, x.fuse && it < I ? 1 : 0, p->beta,
-                        s->gst.p, x.tail && it < I ? x.n_groups : 0,
-                        s->flow_test_cap > 0 ? s->flow_never.p : nullptr, s->flow_test_cap);
+                        s->dcnt.p + kErrSlot, ordered ? 1 : x.flow_bpw,
+                        x.fuse && it < I ? 1 : 0, p->beta, s->gst.p,
+                        x.tail && it < I ? x.n_groups : 0,
+                        s->flow_test_cap > 0 ? s->flow_never.p : nullptr, s->flow_test_cap,
+                        nullptr, nullptr, nullptr, 0, nullptr, ordered);
     launched = x.nonempty;  // bodies of every color, in one launch
     CK(prof_end(s, x.pbytes * launched + (x.tail && it < I ? dual_sweep_bytes(s) : 0.0), 1));
   } else {

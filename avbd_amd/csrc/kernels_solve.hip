@@ -692,7 +692,11 @@ __device__ __forceinline__ int sweep_stamp(const SweepFlow& fl) {
   return (int)(*fl.epoch * kSweepStride + (unsigned)fl.sweep + 1u);
 }
 
-template <int BPW, bool kFlow>
+// kSameColorOrder (one body per tile): a body also waits for its same-color
+// neighbours of lower index, so a color that holds neighbours (a coloring
+// past its 32 colors) is updated in body order, as a serial loop over the
+// color does; tiles are in that order, so the waits still point backwards.
+template <int BPW, bool kFlow, bool kSameColorOrder = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BPW >= 4 ? 3 : 1)))
 k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg,
                float4* __restrict__ pose, const float4* __restrict__ poseI,
@@ -882,8 +886,9 @@ k_primal_tiled(int count_arg, const int* __restrict__ color_bodies, int base_arg
       // wait for this pass's lower-color neighbours (their color and stamp
       // were loaded with the group records), then read their poses through
       // the coherent path, after the stamp was seen
-      bool need = ent.x >= 0 && ent.y >= 0 && ent.y != self_id && colq < my_color &&
-                  flagq != stamp;
+      const bool before_me =
+          colq < my_color || (kSameColorOrder && colq == my_color && ent.y < self_id);
+      bool need = ent.x >= 0 && ent.y >= 0 && ent.y != self_id && before_me && flagq != stamp;
       int spins = 0;
       while (need) {
         if (__hip_atomic_load(fl.done_rd + ent.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
@@ -1059,7 +1064,7 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
                          int fuse_dual, float beta, GroupSt* gst_w, int tail_groups,
                          const int* done_rd, int spin_cap, const int* meta_ncolors,
                          const int* meta_cstart, const int* meta_tiles, int max_tiles,
-                         const int* tail_nd) {
+                         const int* tail_nd, bool same_color_order) {
   SweepFlow fl{};
   fl.tail_nd = tail_nd;
   fl.meta_ncolors = meta_ncolors;
@@ -1093,7 +1098,11 @@ void launch_primal_sweep(hipStream_t st, int n_colors, const int* cstart, const 
   }
   if (t <= 0) return;
   const int grid = t + (tail_groups + 63) / 64;
-  if (bpw == 1)
+  if (same_color_order && bpw == 1)
+    hipLaunchKernelGGL((k_primal_tiled<1, true, true>), dim3(grid), dim3(64), 0, st, 0,
+                       color_bodies, 0, pose, poseI, inert, bstat, cpos, csr, geo, gst, dt,
+                       post_pass, fl);
+  else if (bpw == 1)
     hipLaunchKernelGGL((k_primal_tiled<1, true>), dim3(grid), dim3(64), 0, st, 0, color_bodies, 0,
                        pose, poseI, inert, bstat, cpos, csr, geo, gst, dt, post_pass, fl);
   else if (bpw == 4)

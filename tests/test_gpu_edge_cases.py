@@ -77,12 +77,12 @@ def test_edge_case_lockstep(build, nsub):
         assert g._h.read_capacities()[0] > 16 * 40   # the pair pool was regrown
 
 
-def _crowd(cls):
+def _crowd(cls, mode="jacobi"):
     # 90 boxes in a 0.4 m ball: the contact graph needs more than the 32
-    # colors of the greedy coloring (MAX_COLORS, kernels_6dof.py), so its last
-    # color holds neighbours
+    # colors of the greedy coloring (MAX_COLORS, kernels_6dof.py), so a color
+    # holds neighbours, which a serial loop over the color updates in body order
     rng = np.random.default_rng(11)
-    s = cls(dt=1.0 / 60.0, iterations=4, substeps=2)
+    s = cls(dt=1.0 / 60.0, iterations=4, substeps=2, coloring_mode=mode)
     s.enable_self_collision(True, default_friction=0.5)
     for _ in range(90):
         p = rng.uniform(-0.2, 0.2, 3) + (0.0, 1.5, 0.0)
@@ -91,16 +91,19 @@ def _crowd(cls):
     return s
 
 
-@pytest.mark.xfail(strict=False, reason=(
-    "more than 32 colors needed: the checker updates same-color neighbours in body order, as "
-    "the reference's CPU device does; the dataflow sweep waits only for lower colors, so "
-    "same-color neighbours race, as in the reference's CUDA kernels"))
-def test_color_overflow_lockstep():
-    g, o = _pair(_crowd)
+@pytest.mark.parametrize("mode", ["jacobi", "jones_plassmann"])
+def test_color_overflow_lockstep(mode):
+    """Same-color neighbours are updated in body order (the sweep runs one body
+    per tile, each waiting for its lower-index same-color neighbours)."""
+    g, o = _crowd(Solver6DOF, mode), _crowd(oracle_solver_class(), mode)
+    most = 0
     for k in range(6):
         one_substep(g)
         one_substep(o)
         rep = diff_report(g, o)
-        assert rep["colors_equal"] and rep["pairs_equal"], k
-        assert rep["n_active_colors"][0] == 32
-        assert_bit_exact(rep, f"crowd substep {k}")
+        print(mode, k, rep["counters"], rep["n_active_colors"], flush=True)
+        most = max(most, rep["n_active_colors"][0])
+        assert_bit_exact(rep, f"crowd {mode} substep {k}")
+    # the speculative (Jacobi) coloring overflows here; Jones-Plassmann's
+    # priority order needs 27 colors for the same crowd
+    assert most == (32 if mode == "jacobi" else 27)
