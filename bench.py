@@ -378,7 +378,8 @@ def main() -> int:
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--fuse-dual", type=int, default=0,
                     help="dual update with the dataflow sweep: 0 separate launch (default), "
-                         "1 inside the sweep's tiles, 2 in trailing workgroups of its launch")
+                         "1 inside the sweep's tiles, 2 in trailing workgroups of its launch, "
+                         "-1 the faster of 0 and 2 by timing frames (Solver6DOF.autotune_dual)")
     ap.add_argument("--broad-lanes", type=int, default=0,
                     help="broad-phase traversal lanes per body: 0 by body count (default), "
                          "1, 8, 16 or 64")
@@ -428,7 +429,7 @@ def main() -> int:
     variant = args.primal_variant
     if variant is not None:
         s._h.set_option(0, variant)
-    s._h.set_option(1, args.fuse_dual)
+    s._h.set_option(1, max(args.fuse_dual, 0))
     s._h.set_option(5, args.broad_lanes)
     n_b = len(s._x)
     print(f"# scene: {n_b} bodies, {len(s._rows)} static rows, build+flush "
@@ -449,6 +450,13 @@ def main() -> int:
               file=sys.stderr, flush=True)
         s.step()   # untimed: captures the graph of the chosen kernel
         frames_before += len(tuned) + 1
+    dual_tuned = None
+    if args.fuse_dual < 0:
+        dual_tuned = s.autotune_dual(frames=3)
+        print(f"# dual placement autotune (ms per frame): {dual_tuned} -> {s.dual_placement}",
+              file=sys.stderr, flush=True)
+        s.step()   # untimed: captures the graph of the chosen placement
+        frames_before += len(dual_tuned) * 4 + 1
     import torch
     syncs0, graphs0 = s._h.sync_count()
     torch.cuda.synchronize()
@@ -525,6 +533,9 @@ def main() -> int:
                    "contacts": n_contacts, "pairs": n_pairs, "rows": n_rows,
                    "colors": s.num_active_colors, "parallelism": "single GPU",
                    "primal_variant": variant, "broad_lanes": args.broad_lanes,
+                   "dual_placement": (s.dual_placement if dual_tuned else args.fuse_dual),
+                   "dual_autotune_ms": ({str(k): round(v, 3) for k, v in dual_tuned.items()}
+                                        if dual_tuned else None),
                    "primal_autotune_ms": ({str(k): round(v, 3) for k, v in tuned.items()}
                                           if tuned else None)},
         "roofline": {
